@@ -1,0 +1,200 @@
+/*
+ * shenfun_amd: C ABI of the MI355X (gfx950) spectral-Galerkin Helmholtz hot path.
+ *
+ * Plain pointers and sizes only.  Every array argument is a DEVICE pointer
+ * (e.g. a PyTorch-ROCm tensor's data_ptr()), shapes and strides are int64 in
+ * ELEMENTS of the array's own dtype, `stream` is a hipStream_t passed as
+ * void* (NULL = default stream).  All calls are asynchronous on `stream`,
+ * return SFA_OK (0) or a negative status, and leave a message for
+ * sfa_last_error().  Arrays have ndim 1..3; `axis` is the transformed or
+ * solved axis.  dtype is SFA_F64 (float64) or SFA_C128 (complex128,
+ * interleaved re/im).
+ *
+ * Each entry point names the reference (shenfun 4.3.0) interface it replaces,
+ * as path:line inside the shenfun source tree.
+ */
+#ifndef SHENFUN_AMD_H
+#define SHENFUN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFA_F64 0
+#define SFA_C128 1
+
+#define SFA_OK 0
+#define SFA_ERR_ARG (-1)
+#define SFA_ERR_HIP (-2)
+#define SFA_ERR_UNSUPPORTED (-3)
+#define SFA_ERR_NOMEM (-4)
+
+/* Library version (major*10000 + minor*100 + patch). */
+int sfa_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char* sfa_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* Transforms along one axis (the FFTW planners' execute step)         */
+/* ------------------------------------------------------------------ */
+
+/* Complex-to-complex DFT along `axis`, out = scale * sum_j in_j e^{sign 2 pi i jk/n}.
+ * Replaces the fftw.fftn / ifftn planners of fourier/bases.py:515-533 (C2C);
+ * shenfun uses sign=-1, scale=1/N for the scalar product (:157-162) and
+ * sign=+1, scale=1 for backward (normalise_idft=False, :533).  in == out is allowed. */
+int sfa_fft_c2c(const void* in, void* out, int ndim, const int64_t* shape,
+                const int64_t* in_strides, const int64_t* out_strides, int axis,
+                int sign, double scale, void* stream);
+
+/* Real-to-complex DFT along `axis` (real length shape[axis] = N, even),
+ * out has N/2+1 entries along axis.  Replaces fftw.rfftn of
+ * fourier/bases.py:249-265 (R2C scalar product, scale = 1/N). */
+int sfa_fft_r2c(const double* in, void* out, int ndim, const int64_t* real_shape,
+                const int64_t* in_strides, const int64_t* out_strides, int axis,
+                double scale, void* stream);
+
+/* Complex-to-real inverse DFT along `axis` (real length real_shape[axis] = N,
+ * even; in has N/2+1 entries, imaginary parts of entries 0 and N/2 ignored).
+ * Replaces fftw.irfftn of fourier/bases.py:380-407 (normalise_idft=False, scale 1). */
+int sfa_fft_c2r(const void* in, double* out, int ndim, const int64_t* real_shape,
+                const int64_t* in_strides, const int64_t* out_strides, int axis,
+                double scale, void* stream);
+
+/* Chebyshev-Gauss scalar product along `axis`: out_k = pi/(2N) REDFT10(in)_k,
+ * then, if K2 != NULL (device, N-2 entries), the composite stencil
+ * out_k += K2[k] out_{k+2} for k < N-2.  Replaces
+ * chebyshev/bases.py:340-353 (Orthogonal._evaluate_scalar_product, DCTWrap
+ * :100-115) and spectralbase.py:1462-1474 (CompositeBase stencil matvec). */
+int sfa_cheb_scalar_product(const void* in, void* out, int dtype, int ndim,
+                            const int64_t* shape, const int64_t* in_strides,
+                            const int64_t* out_strides, int axis, const double* K2,
+                            void* stream);
+
+/* Chebyshev-Gauss synthesis along `axis`: if K2 != NULL first t = to_ortho(in)
+ * (t_k = in_k + K2[k-2] in_{k-2}, BC slots N-2, N-1 ignored), then
+ * out_j = sum_k t_k T_k(x_j) = 0.5 REDFT01(t)_j + t_0/2.  Replaces
+ * chebyshev/bases.py:315-338 and spectralbase.py:1450-1460, 1616-1634. */
+int sfa_cheb_backward(const void* in, void* out, int dtype, int ndim,
+                      const int64_t* shape, const int64_t* in_strides,
+                      const int64_t* out_strides, int axis, const double* K2,
+                      void* stream);
+
+/* Build the dense Legendre transform matrix on the device (the plan step of
+ * the recursive transforms, spectralbase.py:484-491, 550-558).
+ * analysis=1: M[k][j] = w_j (L_k(x_j) + K2[k] L_{k+2}(x_j))  (k < N-2 when
+ *             K2 != NULL; rows N-2, N-1 keep w_j L_k(x_j)), shape (N, Nx).
+ * analysis=0: M[j][k] = L_k(x_j) + K2[k] L_{k+2}(x_j) (k < N-2) for the
+ *             composite synthesis (columns N-2, N-1 zero when K2 != NULL;
+ *             plain L_k(x_j) when K2 == NULL), shape (Nx, N).
+ * x, w (w unused for analysis=0), K2: device arrays.  L_k by the three-term
+ * recurrence of transforms.pyx:219-323. */
+int sfa_legendre_matrix(const double* x, const double* w, int64_t Nx, int64_t N,
+                        const double* K2, int analysis, double* M, void* stream);
+
+/* out(.., k, ..) = sum_j M[k][j] in(.., j, ..) along `axis` (M row-major
+ * nout x shape[axis], FP64).  With the matrices above this replaces the
+ * recursive Legendre kernels transforms.pyx:175-190 (scalar_product) and
+ * :199-216 (evaluate_expansion_all).  in != out. */
+int sfa_line_matmul(const void* in, void* out, int dtype, int ndim,
+                    const int64_t* in_shape, const int64_t* in_strides,
+                    const int64_t* out_strides, int axis, int64_t nout,
+                    const double* M, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Banded solves (la.py + optimization/cython/la.pyx)                  */
+/* ------------------------------------------------------------------ */
+
+/* In-place LU of nbatch TDMA systems, data (nbatch, 3, n) DIA rows
+ * (offsets -2, 0, 2 as la.py:256-258).  Replaces TDMA_LU la.pyx:243-252. */
+int sfa_tdma_lu(double* data, int64_t n, int64_t nbatch, void* stream);
+
+/* In-place LU of nbatch PDMA systems, data (nbatch, 5, n) (offsets
+ * -4,-2,0,2,4).  Replaces PDMA_LU la.pyx:213-241. */
+int sfa_pdma_lu(double* data, int64_t n, int64_t nbatch, void* stream);
+
+/* Solve in place along `axis` with LU'd DIA data.  Line l (C order over the
+ * non-axis dims) uses data + l*data_line_stride (0 = one system for all
+ * lines, TDMA_Solve la.pyx:77-91 / Solve_axis_2D/3D :429-460); lines
+ * l < skip_lines are left untouched (SolverGeneric1ND_solve_data
+ * la.pyx:371-426 with is_zero_index).  Only entries 0..n-1 along axis are
+ * touched (n <= shape[axis]).  Kernels: TDMA_inner_solve_ptr :520-532,
+ * PDMA_inner_solve_ptr :496-514, DiagMA_inner_solve_ptr :586-590. */
+int sfa_tdma_solve(void* u, int dtype, int ndim, const int64_t* shape,
+                   const int64_t* strides, int axis, const double* data, int64_t n,
+                   int64_t data_line_stride, int64_t skip_lines, void* stream);
+int sfa_pdma_solve(void* u, int dtype, int ndim, const int64_t* shape,
+                   const int64_t* strides, int axis, const double* data, int64_t n,
+                   int64_t data_line_stride, int64_t skip_lines, void* stream);
+int sfa_diagma_solve(void* u, int dtype, int ndim, const int64_t* shape,
+                     const int64_t* strides, int axis, const double* d, int64_t n,
+                     int64_t data_line_stride, int64_t skip_lines, void* stream);
+
+/* Chebyshev Helmholtz LU (alfa*A + beta*B), N basis size (M = N-2 unknowns).
+ * A0 (M), A2 (M-2), A4 (M-4): stiffness diagonals 0, +2, +4; Bm2 (M-2), B0 (M),
+ * B2 (M-2): mass diagonals (unscaled; the Neumann 1/j^2 column scaling and
+ * k=0 gauge of la.pyx:735-749 are applied inside).  alfa/beta: per line
+ * (ab_stride 1, C order over the non-axis dims) or one value (ab_stride 0).
+ * d0, d1, d2, L: real arrays of `shape` with `f_strides` (shape[axis] = N),
+ * entries beyond the reference's loop ranges are not written.
+ * Replaces LU_Helmholtz la.pyx:708-843 (LU_Helmholtz_1D :717-763). */
+int sfa_helmholtz_lu(int neumann, int64_t N, const double* A0, const double* A2,
+                     const double* A4, const double* Bm2, const double* B0,
+                     const double* B2, const double* alfa, const double* beta,
+                     int64_t ab_stride, double* d0, double* d1, double* d2, double* L,
+                     int ndim, const int64_t* shape, const int64_t* f_strides, int axis,
+                     void* stream);
+
+/* Chebyshev Helmholtz solve along `axis`: u = H^{-1} b using the factors above.
+ * b may alias u.  Entries N-2, N-1 of u along axis are not written.
+ * Replaces Solve_Helmholtz la.pyx:845-973 (Solve_Helmholtz_1D_ptr :871-912). */
+int sfa_helmholtz_solve(const void* b, void* u, int dtype, int ndim, const int64_t* shape,
+                        const int64_t* b_strides, const int64_t* u_strides, int axis,
+                        int neumann, const double* d0, const double* d1,
+                        const double* d2, const double* L, const int64_t* f_strides,
+                        void* stream);
+
+/* c = (alfa*A + beta*B) v along `axis` for the Chebyshev SD/SN Galerkin
+ * matrices: A given by A0 (M) and its row-constant upper tail At (M-2,
+ * At[k] = A[k][k+2j] for all j >= 1 after the SN column scaling), B by Bm2,
+ * B0, B2.  Entries N-2, N-1 of c are set to 0.  Replaces Helmholtz_matvec
+ * Matvec.pyx:656-720 and Helmholtz_Neumann_matvec :722-818.  v != c. */
+int sfa_helmholtz_matvec(const void* v, void* c, int dtype, int ndim, const int64_t* shape,
+                         const int64_t* v_strides, const int64_t* c_strides, int axis,
+                         int neumann, const double* A0, const double* At,
+                         const double* Bm2, const double* B0, const double* B2,
+                         const double* alfa, const double* beta, int64_t ab_stride,
+                         void* stream);
+
+/* Chebyshev Dirichlet Poisson A u = b (alpha = 0), A0 (M) diagonal, A2 (M-2)
+ * row-constant tail; u /= scale unless |scale-1| < 1e-8.  b may alias u.
+ * Replaces Poisson_Solve_ADD la.pyx:614-706 (chebyshev/la.py:11-63). */
+int sfa_poisson_add_solve(const void* b, void* u, int dtype, int ndim, const int64_t* shape,
+                          const int64_t* b_strides, const int64_t* u_strides, int axis,
+                          const double* A0, const double* A2, double scale, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Fused pipelines                                                     */
+/* ------------------------------------------------------------------ */
+
+/* (alpha - lap) u = f on Chebyshev(SD or SN) x C2C x R2C, f and u physical
+ * (Nx, Ny, Nz) C-contiguous float64: scalar product -> Helmholtz solve
+ * (alfa = -1, beta = alpha + ky^2 + kz^2, LU computed on the fly) ->
+ * backward.  Equivalent to the reference sequence
+ * TensorProductSpace.scalar_product (tensorproductspace.py:146-190) ->
+ * chebyshev.la.Helmholtz.__call__ (chebyshev/la.py:288-317) ->
+ * TensorProductSpace.backward (tensorproductspace.py:46-96).
+ * precision: 0 = FP64 throughout; 1 = mixed (complex64 intermediates, FP64
+ * solve and FP64 residual refinement, `refine` sweeps).
+ * work: device workspace of at least sfa_helmholtz3d_workspace() bytes. */
+int64_t sfa_helmholtz3d_workspace(int64_t Nx, int64_t Ny, int64_t Nz, int precision);
+int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
+                    double alpha, int neumann, int precision, int refine, void* work,
+                    int64_t work_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHENFUN_AMD_H */

@@ -1,0 +1,525 @@
+"""1D function spaces of the hot path with device (HIP) transforms.
+
+Mirrors the per-axis SpectralBase interface (spectralbase.py:200-313):
+``scalar_product``, ``forward`` and ``backward`` with
+``(input_array=None, output_array=None, kind=None)``, acting along
+``self.axis`` of torch tensors on the GPU.  Spaces:
+
+* Chebyshev ``Orthogonal`` (GC), ``ShenDirichlet``, ``ShenNeumann``
+  (chebyshev/bases.py:118-465, 476-554, 863-945);
+* Legendre ``Orthogonal`` (LG), ``ShenDirichlet``, ``ShenNeumann``
+  (legendre/bases.py:76-313, 319-384, 449-510);
+* Fourier ``R2C`` and ``C2C`` (fourier/bases.py:54-641).
+
+Compute runs only through the C ABI of libshenfun_amd.so.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .config import config
+
+PI = np.pi
+
+
+def _dev(device=None):
+    return torch.device(device) if device is not None else torch.device("cuda")
+
+
+def _geom(t):
+    return _lib.arr(t.shape), _lib.arr(t.stride())
+
+
+def _check(t, name, dtype=None):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA (HIP) torch tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"{name} must have dtype {dtype}, got {t.dtype}")
+    if t.dim() < 1 or t.dim() > 3:
+        raise ValueError(f"{name} must have 1..3 dims")
+
+
+class SpectralBase:
+    """Common part of the 1D spaces."""
+
+    def __init__(self, N, dtype=np.float64, device=None):
+        self.N = int(N)
+        self.dtype = np.dtype(dtype)
+        self.axis = 0
+        self.dimensions = 1
+        self.tensorproductspace = None
+        self.device = _dev(device)
+        self._cache = {}
+
+    # ---------------------------------------------------------------- info
+    def family(self):
+        raise NotImplementedError
+
+    def boundary_condition(self):
+        return None
+
+    def dim(self):
+        return self.N
+
+    def slice(self):
+        return slice(0, self.dim())
+
+    def shape(self, forward_output=True):
+        return self.N
+
+    def mesh(self):
+        return self.points_and_weights()[0]
+
+    @property
+    def is_composite(self):
+        return False
+
+    def _kind(self, kind):
+        if kind is None:
+            kind = config["transforms"]["kind"][self.family()]
+        if kind not in ("fast", "recursive", "vandermonde", "hip"):
+            raise ValueError(f"unknown transform kind {kind!r}")
+        return kind
+
+    def _out(self, inp, output_array, shape_axis, dtype):
+        shape = list(inp.shape)
+        shape[self.axis] = shape_axis
+        if output_array is None:
+            return torch.empty(shape, dtype=dtype, device=inp.device)
+        _check(output_array, "output_array", dtype)
+        if list(output_array.shape) != shape:
+            raise ValueError(f"output_array shape {tuple(output_array.shape)} != {tuple(shape)}")
+        return output_array
+
+    def _tensor(self, a):
+        if isinstance(a, np.ndarray):
+            return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+        return a
+
+    def _dev_array(self, key, fn):
+        if key not in self._cache:
+            self._cache[key] = torch.from_numpy(np.ascontiguousarray(fn())).to(self.device)
+        return self._cache[key]
+
+
+class _PolyBase(SpectralBase):
+    """Chebyshev/Legendre orthogonal or composite (Shen) space."""
+
+    bc_kind = None  # None, 'D' or 'N'
+
+    @property
+    def is_composite(self):
+        return self.bc_kind is not None
+
+    def dim(self):
+        return self.N - 2 if self.is_composite else self.N
+
+    def boundary_condition(self):
+        return {None: None, "D": "Dirichlet", "N": "Neumann"}[self.bc_kind]
+
+    def short_name(self):
+        return {None: "T" if self.family() == "chebyshev" else "L", "D": "SD", "N": "SN"}[self.bc_kind]
+
+    def stencil_k2(self):
+        """K2(k), k < N-2: phi_k = P_k + K2(k) P_{k+2}."""
+        k = np.arange(self.N - 2, dtype=float)
+        if self.bc_kind == "D":
+            return -np.ones(self.N - 2)
+        if self.family() == "chebyshev":
+            return -(k / (k + 2)) ** 2  # chebyshev/bases.py:937
+        return -k * (k + 1) / (k * k + 5 * k + 6)  # legendre/bases.py:502
+
+    def _K2(self):
+        if not self.is_composite:
+            return None
+        return self._dev_array("K2", self.stencil_k2)
+
+    def _dtype_code(self, t):
+        if t.dtype == torch.complex128:
+            return _lib.C128
+        if t.dtype == torch.float64:
+            return _lib.F64
+        raise TypeError(f"unsupported dtype {t.dtype} (float64 / complex128 only)")
+
+    # dense per-line contraction: 'recursive' / 'vandermonde' kinds
+    def _matmul(self, inp, out, M):
+        shp, sin = _geom(inp)
+        _, sout = _geom(out)
+        _lib.call("sfa_line_matmul", _lib.ptr(inp), _lib.ptr(out), self._dtype_code(inp), inp.dim(),
+                  shp, sin, sout, self.axis, int(M.shape[0]), _lib.ptr(M), _lib.stream_ptr())
+        return out
+
+    def _analysis_matrix(self):
+        raise NotImplementedError
+
+    def _synthesis_matrix(self):
+        raise NotImplementedError
+
+    # ------------------------------------------------------------ transforms
+    def scalar_product(self, input_array=None, output_array=None, kind=None):
+        kind = self._kind(kind)
+        inp = self._tensor(input_array)
+        _check(inp, "input_array")
+        out = self._out(inp, output_array, self.N, inp.dtype)
+        self._scalar_product(inp, out, kind)
+        return out
+
+    def backward(self, input_array=None, output_array=None, kind=None):
+        kind = self._kind(kind)
+        inp = self._tensor(input_array)
+        _check(inp, "input_array")
+        if inp.shape[self.axis] != self.N:
+            raise ValueError("input_array has the wrong length along axis")
+        out = self._out(inp, output_array, self.N, inp.dtype)
+        if out.data_ptr() == inp.data_ptr():
+            raise ValueError("backward cannot run in place")
+        self._backward(inp, out, kind)
+        return out
+
+    def forward(self, input_array=None, output_array=None, kind=None):
+        """Projection: scalar product, then the inverse mass matrix along axis
+        (spectralbase.py:235-263, apply_inverse_mass :560-577).  BC slots
+        N-2, N-1 of a composite space are set to 0 (homogeneous BCs,
+        tensorproductspace.py:2091-2106)."""
+        out = self.scalar_product(input_array, output_array, kind)
+        if self.is_composite:
+            sl = [slice(None)] * out.dim()
+            sl[self.axis] = slice(self.N - 2, self.N)
+            out[tuple(sl)] = 0
+        self._mass_solver()(out, axis=self.axis)
+        return out
+
+    def _mass_solver(self):
+        if "mass" not in self._cache:
+            from . import la
+            from .matrices import mass_matrix, SparseMatrix
+            if self.is_composite:
+                B = mass_matrix(self)
+            else:
+                B = SparseMatrix({0: self.norm_sq()}, (self.N, self.N))
+            self._cache["mass"] = B.get_solver()(B)
+        return self._cache["mass"]
+
+
+class ChebyshevBase(_PolyBase):
+    """Chebyshev-Gauss space; fast transforms are the LDS DCT kernels."""
+
+    def family(self):
+        return "chebyshev"
+
+    def points_and_weights(self, N=None):
+        """chebyshev/bases.py:213-216: x_j = cos(pi(2j+1)/(2N)) (descending), w = pi/N."""
+        N = self.N if N is None else N
+        j = np.arange(N, dtype=float)
+        return np.cos(PI * (2 * j + 1) / (2 * N)), np.full(N, PI / N)
+
+    def norm_sq(self):
+        h = np.full(self.N, PI / 2)
+        h[0] = PI
+        return h
+
+    def _vander(self):
+        x, _ = self.points_and_weights()
+        j = np.arange(self.N)
+        return np.cos(np.outer(np.arccos(x), j))  # T_k(x_j) = cos(k theta_j)
+
+    def _analysis_matrix(self):
+        def build():
+            V = self._vander()
+            _, w = self.points_and_weights()
+            M = (V * w[:, None]).T.copy()  # M[k, j] = w_j T_k(x_j)
+            if self.is_composite:
+                K2 = self.stencil_k2()
+                M[:-2] = M[:-2] + K2[:, None] * M[2:]
+            return M
+        return self._dev_array("Ma", build)
+
+    def _synthesis_matrix(self):
+        def build():
+            V = self._vander()
+            if self.is_composite:
+                K2 = self.stencil_k2()
+                S = np.zeros_like(V)
+                S[:, :-2] = V[:, :-2] + K2[None, :] * V[:, 2:]
+                return S
+            return V
+        return self._dev_array("Ms", build)
+
+    def _scalar_product(self, inp, out, kind):
+        if kind in ("fast", "hip"):
+            shp, sin = _geom(inp)
+            _, sout = _geom(out)
+            _lib.call("sfa_cheb_scalar_product", _lib.ptr(inp), _lib.ptr(out), self._dtype_code(inp),
+                      inp.dim(), shp, sin, sout, self.axis, _lib.ptr(self._K2()), _lib.stream_ptr())
+        else:
+            if out.data_ptr() == inp.data_ptr():
+                raise ValueError("dense transform cannot run in place")
+            self._matmul(inp, out, self._analysis_matrix())
+
+    def _backward(self, inp, out, kind):
+        if kind in ("fast", "hip"):
+            shp, sin = _geom(inp)
+            _, sout = _geom(out)
+            _lib.call("sfa_cheb_backward", _lib.ptr(inp), _lib.ptr(out), self._dtype_code(inp),
+                      inp.dim(), shp, sin, sout, self.axis, _lib.ptr(self._K2()), _lib.stream_ptr())
+        else:
+            self._matmul(inp, out, self._synthesis_matrix())
+
+
+class LegendreBase(_PolyBase):
+    """Legendre-Gauss space; transforms are dense per-line contractions with
+    matrices built on the device from the three-term recurrence (the default
+    'recursive' kind, spectralbase.py:484-491, 550-558)."""
+
+    def family(self):
+        return "legendre"
+
+    def points_and_weights(self, N=None):
+        """Legendre-Gauss nodes (ascending) and weights (legendre/bases.py:121-126)."""
+        N = self.N if N is None else N
+        key = ("pw", N)
+        if key not in self._cache:
+            self._cache[key] = leggauss(N)
+        return self._cache[key]
+
+    def norm_sq(self):
+        k = np.arange(self.N, dtype=float)
+        return 2.0 / (2 * k + 1)
+
+    def _xw(self):
+        x, w = self.points_and_weights()
+        return (self._dev_array("x", lambda: x), self._dev_array("w", lambda: w))
+
+    def _analysis_matrix(self):
+        if "Ma" not in self._cache:
+            x, w = self._xw()
+            M = torch.empty((self.N, self.N), dtype=torch.float64, device=self.device)
+            _lib.call("sfa_legendre_matrix", _lib.ptr(x), _lib.ptr(w), self.N, self.N,
+                      _lib.ptr(self._K2()), 1, _lib.ptr(M), _lib.stream_ptr())
+            self._cache["Ma"] = M
+        return self._cache["Ma"]
+
+    def _synthesis_matrix(self):
+        if "Ms" not in self._cache:
+            x, w = self._xw()
+            M = torch.empty((self.N, self.N), dtype=torch.float64, device=self.device)
+            _lib.call("sfa_legendre_matrix", _lib.ptr(x), _lib.ptr(w), self.N, self.N,
+                      _lib.ptr(self._K2()), 0, _lib.ptr(M), _lib.stream_ptr())
+            self._cache["Ms"] = M
+        return self._cache["Ms"]
+
+    def _scalar_product(self, inp, out, kind):
+        if out.data_ptr() == inp.data_ptr():
+            raise ValueError("Legendre transforms cannot run in place")
+        self._matmul(inp, out, self._analysis_matrix())
+
+    def _backward(self, inp, out, kind):
+        self._matmul(inp, out, self._synthesis_matrix())
+
+
+def _legendre_pd(n, x):
+    p0 = np.ones_like(x)
+    p1 = x.copy()
+    for k in range(2, n + 1):
+        p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+    dp = n * (x * p1 - p0) / (x * x - 1)
+    return p1, dp
+
+
+def leggauss(N):
+    """Legendre-Gauss nodes (ascending) and weights in float64.
+
+    Newton iteration on the three-term recurrence in extended precision
+    (np.longdouble) from a cosine initial guess, then rounded; symmetric.
+    """
+    if N == 1:
+        return np.array([0.0]), np.array([2.0])
+    ld = np.longdouble
+    k = np.arange(1, N + 1, dtype=ld)
+    x = np.cos(ld(np.pi) * (k - ld(0.25)) / (N + ld(0.5)))
+    for _ in range(100):
+        p, dp = _legendre_pd(N, x)
+        dx = p / dp
+        x = x - dx
+        if np.max(np.abs(dx)) < ld(1e-19):
+            break
+    _, dp = _legendre_pd(N, x)
+    w = 2 / ((1 - x * x) * dp * dp)
+    o = np.argsort(x)
+    x = x[o].astype(np.float64)
+    w = w[o].astype(np.float64)
+    return 0.5 * (x - x[::-1]), 0.5 * (w + w[::-1])
+
+
+class FourierBase(SpectralBase):
+    """Fourier exponentials on [0, 2 pi): scalar product = FFT/N, backward =
+    unnormalised inverse (fourier/bases.py:131-162, 380-407, 526-533)."""
+
+    def family(self):
+        return "fourier"
+
+    def boundary_condition(self):
+        return "Periodic"
+
+    def points_and_weights(self, N=None):
+        N = self.N if N is None else N
+        return np.arange(N, dtype=float) * 2 * PI / N, np.array([1.0 / N])
+
+    def forward(self, input_array=None, output_array=None, kind=None):
+        return self.scalar_product(input_array, output_array, kind)
+
+
+class R2C(FourierBase):
+    def __init__(self, N, device=None):
+        super().__init__(N, np.float64, device)
+
+    def short_name(self):
+        return "R2C"
+
+    def dim(self):
+        return self.N // 2 + 1
+
+    def shape(self, forward_output=True):
+        return self.N // 2 + 1 if forward_output else self.N
+
+    def wavenumbers(self):
+        return np.fft.rfftfreq(self.N, 1.0 / self.N).astype(int)  # fourier/bases.py:334-342
+
+    def scalar_product(self, input_array=None, output_array=None, kind=None):
+        inp = self._tensor(input_array)
+        _check(inp, "input_array", torch.float64)
+        out = self._out(inp, output_array, self.N // 2 + 1, torch.complex128)
+        shp, sin = _geom(inp)
+        _, sout = _geom(out)
+        _lib.call("sfa_fft_r2c", _lib.ptr(inp), _lib.ptr(out), inp.dim(), shp, sin, sout, self.axis,
+                  1.0 / self.N, _lib.stream_ptr())
+        return out
+
+    def backward(self, input_array=None, output_array=None, kind=None):
+        inp = self._tensor(input_array)
+        _check(inp, "input_array", torch.complex128)
+        out = self._out(inp, output_array, self.N, torch.float64)
+        shp = list(inp.shape)
+        shp[self.axis] = self.N
+        _, sin = _geom(inp)
+        _, sout = _geom(out)
+        _lib.call("sfa_fft_c2r", _lib.ptr(inp), _lib.ptr(out), inp.dim(), _lib.arr(shp), sin, sout,
+                  self.axis, 1.0, _lib.stream_ptr())
+        return out
+
+
+class C2C(FourierBase):
+    def __init__(self, N, device=None):
+        super().__init__(N, np.complex128, device)
+
+    def short_name(self):
+        return "C2C"
+
+    def wavenumbers(self):
+        return np.fft.fftfreq(self.N, 1.0 / self.N).astype(int)  # fourier/bases.py:535-543
+
+    def _c2c(self, input_array, output_array, sign, scale):
+        inp = self._tensor(input_array)
+        _check(inp, "input_array", torch.complex128)
+        out = self._out(inp, output_array, self.N, torch.complex128)
+        shp, sin = _geom(inp)
+        _, sout = _geom(out)
+        _lib.call("sfa_fft_c2c", _lib.ptr(inp), _lib.ptr(out), inp.dim(), shp, sin, sout, self.axis,
+                  sign, scale, _lib.stream_ptr())
+        return out
+
+    def scalar_product(self, input_array=None, output_array=None, kind=None):
+        return self._c2c(input_array, output_array, -1, 1.0 / self.N)
+
+    def backward(self, input_array=None, output_array=None, kind=None):
+        return self._c2c(input_array, output_array, +1, 1.0)
+
+
+# ---------------------------------------------------------------- factories
+def _poly(cls_base, bc_kind, name):
+    return type(name, (cls_base,), {"bc_kind": bc_kind})
+
+
+class _Init:
+    def __init__(self, N, quad=None, bc=None, dtype=np.float64, device=None, **kw):
+        SpectralBase.__init__(self, N, dtype, device)
+        self.quad = quad
+
+
+class ChebyshevOrthogonal(_Init, ChebyshevBase):
+    bc_kind = None
+
+
+class ChebyshevShenDirichlet(_Init, ChebyshevBase):
+    bc_kind = "D"
+
+
+class ChebyshevShenNeumann(_Init, ChebyshevBase):
+    bc_kind = "N"
+
+
+class LegendreOrthogonal(_Init, LegendreBase):
+    bc_kind = None
+
+
+class LegendreShenDirichlet(_Init, LegendreBase):
+    bc_kind = "D"
+
+
+class LegendreShenNeumann(_Init, LegendreBase):
+    bc_kind = "N"
+
+
+def _bc_kind(bc):
+    if bc is None:
+        return None
+    if isinstance(bc, str):
+        b = bc.lower()
+        if b in ("dirichlet", "d"):
+            return "D"
+        if b in ("neumann", "n"):
+            return "N"
+        raise ValueError(f"unsupported bc {bc!r}")
+    if isinstance(bc, (tuple, list)):
+        if len(bc) == 2 and all(float(v) == 0 for v in bc):
+            return "D"
+        raise NotImplementedError("only homogeneous boundary conditions are on the hot path")
+    if isinstance(bc, dict):
+        kinds = set()
+        vals = []
+        for side in bc.values():
+            for k, v in side.items():
+                kinds.add(k)
+                vals.append(v)
+        if any(float(v) != 0 for v in vals):
+            raise NotImplementedError("only homogeneous boundary conditions are on the hot path")
+        if kinds == {"D"}:
+            return "D"
+        if kinds == {"N"}:
+            return "N"
+    raise ValueError(f"unsupported bc {bc!r}")
+
+
+def FunctionSpace(N, family="Fourier", bc=None, dtype="d", quad=None, domain=None, device=None,
+                  **kw):
+    """shenfun.FunctionSpace (forms/arguments.py:30) for the hot-path spaces."""
+    fam = family.lower()
+    if domain is not None and fam not in ("fourier", "f"):
+        if tuple(float(d) for d in domain) != (-1.0, 1.0):
+            raise NotImplementedError("only the reference domain [-1, 1] is on the hot path")
+    if fam in ("fourier", "f"):
+        if np.dtype(dtype).kind == "c":
+            return C2C(N, device=device)
+        return R2C(N, device=device)
+    kind = _bc_kind(bc)
+    if fam in ("chebyshev", "c"):
+        if quad not in (None, "GC"):
+            raise NotImplementedError("Chebyshev-Gauss (GC) quadrature only")
+        cls = {None: ChebyshevOrthogonal, "D": ChebyshevShenDirichlet, "N": ChebyshevShenNeumann}[kind]
+        return cls(N, quad="GC", device=device)
+    if fam in ("legendre", "l"):
+        if quad not in (None, "LG"):
+            raise NotImplementedError("Legendre-Gauss (LG) quadrature only")
+        cls = {None: LegendreOrthogonal, "D": LegendreShenDirichlet, "N": LegendreShenNeumann}[kind]
+        return cls(N, quad="LG", device=device)
+    raise NotImplementedError(f"family {family!r} is not on the hot path")

@@ -1,0 +1,402 @@
+"""Banded solvers of the hot path on the GPU (shenfun/la.py,
+shenfun/chebyshev/la.py interfaces; compute via the C ABI).
+
+* ``TDMA``, ``PDMA``, ``DiagMA``: ``__call__(b, u=None, axis=0, constraints=())``
+  (la.py:216-249, 291-543); LU on the device once (TDMA_LU/PDMA_LU), solve
+  along any axis in place on ``u``.
+* ``Solver(mats)``: band pattern -> solver class (la.py:17-40).
+* ``SolverGeneric1ND(mats)``: one non-periodic axis, per-mode factors
+  (la.py:1183-1434), mode 0 solved by its own solver (with constraints).
+* ``Helmholtz``: Chebyshev alfa*A + beta*B solver (chebyshev/la.py:131-351),
+  ``__call__(b, u=None, constraints=())`` and ``matvec(v, c)``.
+* ``ADDSolver`` / ``ANNSolver``: Chebyshev Poisson (chebyshev/la.py:11-128).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .matrices import SparseMatrix, TPMatrix
+
+
+def _geom(t):
+    return _lib.arr(t.shape), _lib.arr(t.stride())
+
+
+def _dtype_code(t):
+    if t.dtype == torch.complex128:
+        return _lib.C128
+    if t.dtype == torch.float64:
+        return _lib.F64
+    raise TypeError(f"unsupported dtype {t.dtype} (float64 / complex128 only)")
+
+
+def _check(t, name):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA (HIP) torch tensor")
+    if t.dim() < 1 or t.dim() > 3:
+        raise ValueError(f"{name} must have 1..3 dims")
+
+
+def _prepare(b, u):
+    _check(b, "b")
+    if u is None:
+        return b
+    _check(u, "u")
+    if u.shape != b.shape or u.dtype != b.dtype:
+        raise ValueError("u and b must have the same shape and dtype")
+    if u.data_ptr() != b.data_ptr():
+        u.copy_(b)
+    return u
+
+
+def _sum_mats(mats):
+    if isinstance(mats, (list, tuple)):
+        m = mats[0]
+        for x in mats[1:]:
+            m = m + x
+        return m
+    return mats
+
+
+class _Banded:
+    """Common part of the DIA-data banded solvers (BandedMatrixSolver, la.py:251-288)."""
+
+    offsets = ()
+    lu_fn = None
+    solve_fn = None
+
+    def __init__(self, mats):
+        mat = _sum_mats(mats)
+        assert isinstance(mat, SparseMatrix)
+        assert mat.shape[0] == mat.shape[1]
+        self.mat = mat
+        n = mat.shape[0]
+        data = np.zeros((len(self.offsets), n))
+        for r, off in enumerate(self.offsets):
+            if off in mat:
+                v = mat.diagonal_array(off) * float(mat.scale)
+                if off >= 0:
+                    data[r, off:] = v
+                else:
+                    data[r, :n + off] = v
+        self._data = data  # host DIA rows (offsets sorted), scipy dia layout
+        self._lu = None
+        self._device = None
+
+    def _dev(self, device):
+        if self._lu is None or self._device != device:
+            d = torch.from_numpy(self._data.copy()).to(device)
+            if self.lu_fn is not None:
+                _lib.call(self.lu_fn, _lib.ptr(d), d.shape[1], 1, _lib.stream_ptr())
+            self._lu = d
+            self._device = device
+        return self._lu
+
+    def perform_lu(self, device="cuda"):
+        return self._dev(torch.device(device))
+
+    def apply_constraints(self, b, constraints, axis=0):
+        if len(constraints) > 0:
+            assert len(constraints) == 1
+            assert constraints[0][0] == 0, "Can only fix first row"
+            self._pin_row0()
+            s = [slice(None)] * b.dim()
+            s[axis] = 0
+            b[tuple(s)] = constraints[0][1]
+        return b
+
+    def _pin_row0(self):
+        raise NotImplementedError
+
+    def __call__(self, b, u=None, axis=0, constraints=()):
+        u = _prepare(b, u)
+        self.apply_constraints(u, constraints, axis)
+        data = self._dev(u.device)
+        shp, st = _geom(u)
+        _lib.call(self.solve_fn, _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, axis, _lib.ptr(data),
+                  data.shape[1], 0, 0, _lib.stream_ptr())
+        return u
+
+    solve = __call__
+
+
+class DiagMA(_Banded):
+    offsets = (0,)
+    lu_fn = None
+    solve_fn = "sfa_diagma_solve"
+
+    def _pin_row0(self):
+        self._data[0, 0] = 1.0
+        self._lu = None
+
+
+class TDMA(_Banded):
+    offsets = (-2, 0, 2)
+    lu_fn = "sfa_tdma_lu"
+    solve_fn = "sfa_tdma_solve"
+
+    def _pin_row0(self):
+        # la.py:355-364
+        self._data[1, 0] = 1.0
+        self._data[2, 2] = 0.0
+        self._lu = None
+
+
+class PDMA(_Banded):
+    offsets = (-4, -2, 0, 2, 4)
+    lu_fn = "sfa_pdma_lu"
+    solve_fn = "sfa_pdma_solve"
+
+    def _pin_row0(self):
+        # la.py:459-474
+        self._data[2, 0] = 1.0
+        self._data[3, 2] = 0.0
+        self._data[4, 4] = 0.0
+        self._lu = None
+
+
+class Solve:
+    """General sparse fallback of the reference (SuperLU, la.py:823-844) is not
+    part of the GPU hot path."""
+
+    def __init__(self, mats):
+        raise NotImplementedError("general sparse systems are not on the HIP hot path")
+
+
+def Solver(mats):
+    """Return the solver matching the band pattern of ``mats`` (la.py:17-40)."""
+    mat = _sum_mats(mats)
+    return mat.get_solver()(mat)
+
+
+class SolverGeneric1ND:
+    """Per-mode banded solves along the single non-periodic axis (la.py:1183-1434)."""
+
+    def __init__(self, mats):
+        assert isinstance(mats, (list, tuple)) and all(isinstance(m, TPMatrix) for m in mats)
+        self.mats = list(mats)
+        self.naxes = mats[0].naxes[0]
+        self.space = mats[0].space
+        shape = list(self.space.shape(True))
+        ax = self.naxes
+        n = mats[0].pmat.shape[0]
+        self.n = n
+        mshape = shape[:ax] + shape[ax + 1:]
+        self.nmodes = int(np.prod(mshape)) if mshape else 1
+        keys = set()
+        for m in mats:
+            keys |= set(m.pmat.keys())
+        keys = tuple(sorted(keys))
+        if keys == (0,):
+            self.kind, self.offsets, lu, sol = "diag", (0,), None, "sfa_diagma_solve"
+        elif set(keys) <= {-2, 0, 2}:
+            self.kind, self.offsets, lu, sol = "tdma", (-2, 0, 2), "sfa_tdma_lu", "sfa_tdma_solve"
+        elif set(keys) <= {-4, -2, 0, 2, 4}:
+            self.kind, self.offsets, lu, sol = "pdma", (-4, -2, 0, 2, 4), "sfa_pdma_lu", "sfa_pdma_solve"
+        else:
+            raise NotImplementedError(f"band pattern {keys} not supported")
+        self._lu_fn, self._solve_fn = lu, sol
+        # per-mode DIA data (nmodes, rows, n), C order over the non-axis dims
+        data = np.zeros((self.nmodes, len(self.offsets), n))
+        for m in mats:
+            sc = np.broadcast_to(m.scale, [s if i != ax else 1 for i, s in enumerate(shape)])
+            sc = np.ascontiguousarray(np.moveaxis(sc, ax, -1)).reshape(-1) * float(m.pmat.scale)
+            for r, off in enumerate(self.offsets):
+                if off not in m.pmat:
+                    continue
+                v = m.pmat.diagonal_array(off)
+                if off >= 0:
+                    data[:, r, off:] += sc[:, None] * v[None, :]
+                else:
+                    data[:, r, :n + off] += sc[:, None] * v[None, :]
+        self._data = data
+        self._dev = None
+        # mode 0 (Fourier wavenumber 0 / (0, 0)) has its own solver
+        mat0 = SparseMatrix({off: data[0, r, max(off, 0):n + min(off, 0)]
+                             for r, off in enumerate(self.offsets)}, (n, n))
+        self.sol0 = {"diag": DiagMA, "tdma": TDMA, "pdma": PDMA}[self.kind](mat0)
+
+    def perform_lu(self, device):
+        if self._dev is None or self._dev.device != device:
+            d = torch.from_numpy(self._data).to(device)
+            if self._lu_fn is not None:
+                _lib.call(self._lu_fn, _lib.ptr(d), self.n, self.nmodes, _lib.stream_ptr())
+            self._dev = d
+        return self._dev
+
+    def __call__(self, b, u=None, constraints=(), fast=True):
+        u = _prepare(b, u)
+        data = self.perform_lu(u.device)
+        ax = self.naxes
+        # mode 0 line: index 0 along every non-axis dim
+        s = [0] * u.dim()
+        s[ax] = slice(None)
+        line0 = u[tuple(s)]
+        self.sol0(line0.unsqueeze(0) if line0.dim() == 0 else line0, axis=0,
+                  constraints=constraints)
+        shp, st = _geom(u)
+        _lib.call(self._solve_fn, _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, ax, _lib.ptr(data),
+                  self.n, data.shape[1] * data.shape[2], 1, _lib.stream_ptr())
+        return u
+
+
+def _helm_diags(A, B):
+    """Unscaled diagonals (A0, A2, A4, Bm2, B0, B2) of the Chebyshev SD/SN
+    Galerkin matrices, as LU_Helmholtz_1D reads them (la.pyx:717-733)."""
+    M = A.shape[0]
+
+    def full(mat, k, n):
+        if n <= 0:
+            return np.zeros(1)
+        if k not in mat:
+            return np.zeros(n)
+        return mat.diagonal_array(k)[:n]
+
+    return [full(A, 0, M), full(A, 2, M - 2), full(A, 4, M - 4), full(B, -2, M - 2),
+            full(B, 0, M), full(B, 2, M - 2)]
+
+
+class Helmholtz:
+    """Chebyshev Helmholtz solver alfa*A + beta*B (chebyshev/la.py:131-351).
+
+    ``Helmholtz(A, B)`` with the scales carried by the (TP)matrices, or
+    ``Helmholtz(A, B, alfa, beta)``; A is the stiffness (ASDSDmat/ASNSNmat),
+    B the mass matrix (BSDSDmat/BSNSNmat).
+    """
+
+    def __init__(self, *args, device="cuda"):
+        if len(args) == 1:
+            args = args[0]
+        assert len(args) in (2, 4)
+        Am, Bm = args[:2]
+        byk = {m.get_key(): m for m in (Am, Bm)}
+        self.A = byk.get("ASDSDmat", byk.get("ASNSNmat"))
+        self.B = byk.get("BSDSDmat", byk.get("BSNSNmat"))
+        assert self.A is not None and self.B is not None
+        if len(args) == 2:
+            alfa, beta = self.A.scale, self.B.scale
+        else:
+            alfa, beta = args[2], args[3]
+        Apm = self.A.pmat if isinstance(self.A, TPMatrix) else self.A
+        Bpm = self.B.pmat if isinstance(self.B, TPMatrix) else self.B
+        if isinstance(self.A, TPMatrix):
+            alfa = np.asarray(alfa) * float(Apm.scale)
+            beta = np.asarray(beta) * float(Bpm.scale)
+        space = Apm.testfunction[0]
+        self.neumann = space.boundary_condition() == "Neumann"
+        self.axis = self.A.axis if isinstance(self.A, TPMatrix) else space.axis
+        T = self.A.tensorproductspace if isinstance(self.A, TPMatrix) else space.tensorproductspace
+        self.T = T
+        N = Apm.shape[0] + 2
+        self.N = N
+        if T is not None:
+            shape = list(T.shape(True))
+        else:
+            shape = [N]
+        ab_shape = list(shape)
+        ab_shape[self.axis] = 1
+        alfa = np.broadcast_to(np.atleast_1d(np.asarray(alfa, float)), ab_shape)
+        beta = np.broadcast_to(np.atleast_1d(np.asarray(beta, float)), ab_shape)
+        self.alfa = np.ascontiguousarray(alfa)
+        self.beta = np.ascontiguousarray(beta)
+        dev = torch.device(device)
+        ax = self.axis
+        self._ab = [torch.from_numpy(np.array(np.moveaxis(x, ax, -1), dtype=float).reshape(-1)).to(dev)
+                    for x in (self.alfa, self.beta)]
+        self._diags = [torch.from_numpy(np.ascontiguousarray(d, dtype=float)).to(dev)
+                       for d in _helm_diags(Apm, Bpm)]
+        # matvec tail: A[k][k+2j] = At[k] * w_{k+2j}, w = 1 (SD) or (k+2j)^2 (SN)
+        M = N - 2
+        A2 = _helm_diags(Apm, Bpm)[1]
+        if self.neumann:
+            k = np.arange(M - 2, dtype=float)
+            At = A2 / (k + 2) ** 2
+        else:
+            At = A2
+        self._At = torch.from_numpy(np.ascontiguousarray(At)).to(dev)
+        self.shape = shape
+        self.u0 = torch.zeros(shape, dtype=torch.float64, device=dev)
+        self.u1 = torch.zeros(shape, dtype=torch.float64, device=dev)
+        self.u2 = torch.zeros(shape, dtype=torch.float64, device=dev)
+        self.L = torch.zeros(shape, dtype=torch.float64, device=dev)
+        fshape, fst = _geom(self.u0)
+        A0, A2d, A4, Bm2, B0, B2 = self._diags
+        _lib.call("sfa_helmholtz_lu", int(self.neumann), N, _lib.ptr(A0), _lib.ptr(A2d), _lib.ptr(A4),
+                  _lib.ptr(Bm2), _lib.ptr(B0), _lib.ptr(B2), _lib.ptr(self._ab[0]),
+                  _lib.ptr(self._ab[1]), 1, _lib.ptr(self.u0), _lib.ptr(self.u1), _lib.ptr(self.u2),
+                  _lib.ptr(self.L), self.u0.dim(), fshape, fst, ax, _lib.stream_ptr())
+
+    def __call__(self, b, u=None, constraints=()):
+        _check(b, "b")
+        if u is None:
+            u = b
+        _check(u, "u")
+        if tuple(b.shape) != tuple(self.shape) or u.shape != b.shape:
+            raise ValueError(f"arrays must have shape {tuple(self.shape)}")
+        shp, bst = _geom(b)
+        _, ust = _geom(u)
+        _, fst = _geom(self.u0)
+        _lib.call("sfa_helmholtz_solve", _lib.ptr(b), _lib.ptr(u), _dtype_code(u), u.dim(), shp, bst,
+                  ust, self.axis, int(self.neumann), _lib.ptr(self.u0), _lib.ptr(self.u1),
+                  _lib.ptr(self.u2), _lib.ptr(self.L), fst, _lib.stream_ptr())
+        if constraints != ():
+            assert len(constraints) == 1
+            u[(0,) * u.dim()] = constraints[0][1]
+        return u
+
+    def matvec(self, v, c):
+        _check(v, "v")
+        _check(c, "c")
+        shp, vst = _geom(v)
+        _, cst = _geom(c)
+        A0, _, _, Bm2, B0, B2 = self._diags
+        _lib.call("sfa_helmholtz_matvec", _lib.ptr(v), _lib.ptr(c), _dtype_code(v), v.dim(), shp, vst,
+                  cst, self.axis, int(self.neumann), _lib.ptr(A0), _lib.ptr(self._At), _lib.ptr(Bm2),
+                  _lib.ptr(B0), _lib.ptr(B2), _lib.ptr(self._ab[0]), _lib.ptr(self._ab[1]), 1,
+                  _lib.stream_ptr())
+        return c
+
+
+class ADDSolver:
+    """Chebyshev Dirichlet Poisson A u = b (chebyshev/la.py:11-63)."""
+
+    def __init__(self, mats, device="cuda"):
+        A = _sum_mats(mats)
+        assert A.get_key() == "ASDSDmat"
+        self.mat = A
+        M = A.shape[0]
+        dev = torch.device(device)
+        self._d = torch.from_numpy(A.diagonal_array(0)).to(dev)
+        self._d1 = torch.from_numpy(A.diagonal_array(2) if M > 2 else np.zeros(1)).to(dev)
+        self.scale = float(A.scale)
+
+    def __call__(self, b, u=None, axis=0, constraints=()):
+        _check(b, "b")
+        if u is None:
+            u = b
+        shp, bst = _geom(b)
+        _, ust = _geom(u)
+        _lib.call("sfa_poisson_add_solve", _lib.ptr(b), _lib.ptr(u), _dtype_code(u), u.dim(), shp,
+                  bst, ust, axis, _lib.ptr(self._d), _lib.ptr(self._d1), self.scale, _lib.stream_ptr())
+        return u
+
+
+class ANNSolver:
+    """Chebyshev Neumann Poisson (chebyshev/la.py:66-128): solved as the
+    Neumann Helmholtz system with beta = 0, whose k = 0 gauge fixes u_0; the
+    constraint (default (0, 0)) then sets u_0."""
+
+    def __init__(self, mats, device="cuda"):
+        A = _sum_mats(mats)
+        assert A.get_key() == "ASNSNmat"
+        from .matrices import mass_matrix
+        space = A.testfunction[0]
+        self._H = Helmholtz(A, mass_matrix(space), float(A.scale), 0.0, device=device)
+
+    def __call__(self, b, u=None, axis=0, constraints=((0, 0),)):
+        if b.dim() != 1:
+            raise NotImplementedError("ANNSolver on multi-dimensional arrays: use Helmholtz")
+        u = self._H(b, u)
+        for row, val in constraints:
+            u[row] = val
+        return u

@@ -1,0 +1,305 @@
+"""Galerkin matrices of the hot path in shenfun's diagonal-dict form (host, FP64).
+
+Mirrors matrixbase.SparseMatrix (matrixbase.py:26-531), SpectralMatrix
+(:534-801) and TPMatrix (:1275-1646) for the bases of this package, with the
+closed-form diagonals of
+
+* Chebyshev: BSDSDmat chebyshev/matrices.py:221-249, ASDSDmat :1047-1079,
+  BSNSNmat :377-397, ASNSNmat :1096-1121;
+* Legendre: BSDSDmat legendre/matrices.py:110-134, ASDSDmat :271-290,
+  BSNSNmat :139-160, ASNSNmat :295-312.
+
+These are tiny O(N) host arrays; the device kernels receive them once.
+"""
+from numbers import Number
+
+import numpy as np
+import scipy.sparse
+
+PI = np.pi
+
+
+class SparseMatrix(dict):
+    """Dict offset -> diagonal (array or scalar) of a square banded matrix."""
+
+    def __init__(self, d, shape, scale=1.0):
+        super().__init__(sorted(d.items()))
+        self.shape = tuple(shape)
+        self.scale = scale
+
+    def get_key(self):
+        return self.__class__.__name__
+
+    def sorted_keys(self):
+        return np.array(sorted(self.keys()))
+
+    def diagonal_array(self, key, scaled=False):
+        """Full-length diagonal ``key`` as an array (scalars broadcast)."""
+        n = self.shape[0] - abs(key)
+        v = np.broadcast_to(np.atleast_1d(np.asarray(self[key], dtype=float)), (n,)).copy()
+        if scaled:
+            v *= float(np.asarray(self.scale).ravel()[0]) if np.ndim(self.scale) else self.scale
+        return v
+
+    def diags(self, format="dia", scaled=True):
+        keys = sorted(self.keys())
+        data = [self.diagonal_array(k, scaled=scaled) for k in keys]
+        m = scipy.sparse.diags(data, keys, shape=self.shape, format=format)
+        return m
+
+    def todense(self):
+        return self.diags("csr").toarray()
+
+    def __mul__(self, x):
+        assert isinstance(x, Number) or np.ndim(x) == 0
+        return SparseMatrix({k: self.diagonal_array(k) for k in self}, self.shape,
+                            scale=self.scale * float(x))
+
+    __rmul__ = __mul__
+
+    def __add__(self, other):
+        assert self.shape == other.shape
+        out = {}
+        for m in (self, other):
+            for k in m:
+                v = m.diagonal_array(k) * float(m.scale)
+                out[k] = out[k] + v if k in out else v
+        return SparseMatrix(out, self.shape, scale=1.0)
+
+    def get_solver(self):
+        """Band pattern -> solver class (matrixbase.py:439-474)."""
+        from . import la
+        if float(self.scale) == 0:
+            return la.Solve
+        keys = tuple(self.sorted_keys())
+        if keys == (0,):
+            return la.DiagMA
+        if keys == (-2, 0, 2):
+            return la.TDMA
+        if keys == (-4, -2, 0, 2, 4):
+            return la.PDMA
+        return la.Solve
+
+
+class SpectralMatrix(SparseMatrix):
+    """SparseMatrix with test/trial spaces (matrixbase.py:534-801)."""
+
+    def __init__(self, test, trial, scale=1.0):
+        self.testfunction = test
+        self.trialfunction = trial
+        d = self.assemble()
+        N = test[0].N - 2
+        super().__init__(d, (N, N), scale=scale)
+
+    @property
+    def axis(self):
+        return self.testfunction[0].axis
+
+    @property
+    def tensorproductspace(self):
+        return self.testfunction[0].tensorproductspace
+
+    def assemble(self):  # pragma: no cover - abstract
+        raise NotImplementedError
+
+
+def _cheb_h(N):
+    h = np.full(N, PI / 2)
+    h[0] = PI
+    return h
+
+
+class ChebBSDSDmat(SpectralMatrix):
+    """chebyshev/matrices.py:221-249: {-2: -pi/2, 0: h_k + h_{k+2}, 2: -pi/2}."""
+
+    def assemble(self):
+        N = self.testfunction[0].N
+        h = _cheb_h(N)
+        return {-2: -h[1], 0: h[:-2] + h[2:], 2: -h[1]}
+
+    def get_key(self):
+        return "BSDSDmat"
+
+
+class ChebASDSDmat(SpectralMatrix):
+    """chebyshev/matrices.py:1047-1079: a_kk = -2 pi (k+1)(k+2), a_{k,k+2j} = -4 pi (k+1)."""
+
+    def assemble(self):
+        N = self.testfunction[0].N
+        k = np.arange(N, dtype=float)
+        d = {0: -2 * PI * (k[:N - 2] + 1) * (k[:N - 2] + 2)}
+        for i in range(2, N - 2, 2):
+            d[i] = -4 * PI * (k[:N - 2 - i] + 1)
+        return d
+
+    def get_key(self):
+        return "ASDSDmat"
+
+
+class ChebBSNSNmat(SpectralMatrix):
+    """chebyshev/matrices.py:377-397."""
+
+    def assemble(self):
+        N = self.testfunction[0].N
+        h = _cheb_h(N)
+        k = np.arange(N, dtype=float)
+        al = (k / (k + 2)) ** 2
+        M = N - 2
+        d = {0: h[:-2] + al[:-2] ** 2 * h[2:], 2: -al[:M - 2] * h[2:M]}
+        d[-2] = d[2].copy()
+        return d
+
+    def get_key(self):
+        return "BSNSNmat"
+
+
+class ChebASNSNmat(SpectralMatrix):
+    """chebyshev/matrices.py:1096-1121."""
+
+    def assemble(self):
+        N = self.testfunction[0].N
+        k = np.arange(N - 2, dtype=float)
+        d = {0: -2 * PI * k ** 2 * (k + 1) / (k + 2)}
+        for i in range(2, N - 2, 2):
+            d[i] = -4 * PI * (k[:-i] + i) ** 2 * (k[:-i] + 1) / (k[:-i] + 2) ** 2
+        return d
+
+    def get_key(self):
+        return "ASNSNmat"
+
+
+def _leg_h(N):
+    k = np.arange(N, dtype=float)
+    return 2.0 / (2 * k + 1)
+
+
+class LegBSDSDmat(SpectralMatrix):
+    """legendre/matrices.py:110-134 (unscaled basis)."""
+
+    def assemble(self):
+        h = _leg_h(self.testfunction[0].N)
+        d = {0: h[:-2] + h[2:], -2: -h[2:-2]}
+        d[2] = d[-2].copy()
+        return d
+
+    def get_key(self):
+        return "BSDSDmat"
+
+
+class LegASDSDmat(SpectralMatrix):
+    """legendre/matrices.py:271-290: (phi'_j, phi'_k) = 4k + 6."""
+
+    def assemble(self):
+        k = np.arange(self.testfunction[0].N - 2, dtype=float)
+        return {0: 4 * k + 6}
+
+    def get_key(self):
+        return "ASDSDmat"
+
+
+class LegBSNSNmat(SpectralMatrix):
+    """legendre/matrices.py:139-160."""
+
+    def assemble(self):
+        N = self.testfunction[0].N
+        k = np.arange(N - 2, dtype=float)
+        al = k * (k + 1) / (k + 2) / (k + 3)
+        h = _leg_h(N)
+        d = {0: h[:-2] + al ** 2 * h[2:], 2: -al[:-2] * h[2:-2]}
+        d[-2] = d[2].copy()
+        return d
+
+    def get_key(self):
+        return "BSNSNmat"
+
+
+class LegASNSNmat(SpectralMatrix):
+    """legendre/matrices.py:295-312."""
+
+    def assemble(self):
+        k = np.arange(self.testfunction[0].N - 2, dtype=float)
+        al = k * (k + 1) / (k + 2) / (k + 3)
+        d0 = 2.0 / (2 * k + 1)
+        return {0: d0 * al * (k + 0.5) * ((k + 2) * (k + 3) - k * (k + 1))}
+
+    def get_key(self):
+        return "ASNSNmat"
+
+
+def mass_matrix(space):
+    """(phi_j, phi_k)_w for a composite space."""
+    fam, bc = space.family(), space.bc_kind
+    cls = {("chebyshev", "D"): ChebBSDSDmat, ("chebyshev", "N"): ChebBSNSNmat,
+           ("legendre", "D"): LegBSDSDmat, ("legendre", "N"): LegBSNSNmat}[(fam, bc)]
+    return cls((space, 0), (space, 0))
+
+
+def stiffness_matrix(space):
+    """Chebyshev: (phi''_j, phi_k)_w (ASDSDmat/ASNSNmat, negative definite);
+    Legendre: (phi'_j, phi'_k) (positive), as in the reference's mat dicts."""
+    fam, bc = space.family(), space.bc_kind
+    cls = {("chebyshev", "D"): ChebASDSDmat, ("chebyshev", "N"): ChebASNSNmat,
+           ("legendre", "D"): LegASDSDmat, ("legendre", "N"): LegASNSNmat}[(fam, bc)]
+    if fam == "chebyshev":
+        return cls((space, 0), (space, 2))
+    return cls((space, 1), (space, 1))
+
+
+class TPMatrix:
+    """Tensor-product matrix: one SpectralMatrix along ``naxes`` and Fourier
+    diagonals folded into ``scale`` (TPMatrix.get_simplified,
+    matrixbase.py:1328-1365)."""
+
+    def __init__(self, pmat, space, scale, naxis):
+        self.pmat = pmat
+        self.space = space
+        self.trialspace = space
+        self.scale = np.asarray(scale, dtype=float)
+        self.naxes = [naxis]
+        self.dimensions = space.dimensions
+
+    @property
+    def mats(self):
+        out = [None] * self.dimensions
+        out[self.naxes[0]] = self.pmat
+        return out
+
+    def get_key(self):
+        return self.pmat.get_key()
+
+    @property
+    def testfunction(self):
+        return self.pmat.testfunction
+
+    @property
+    def axis(self):
+        return self.naxes[0]
+
+    @property
+    def tensorproductspace(self):
+        return self.space
+
+
+def helmholtz_tpmatrices(T, alpha=0.0):
+    """The simplified TPMatrix list of inner(v, alpha*u - div(grad(u))) for a
+    tensor-product space T with one Chebyshev/Legendre axis and Fourier
+    elsewhere: [A (stiffness) with its scale, B (mass) with scale
+    alpha + |k|^2] (forms/inner.py:374-493 -> get_simplified_tpmatrices,
+    matrixbase.py:1648-1677)."""
+    ax = T.nonperiodic_axis()
+    S = T.bases[ax]
+    shape = list(T.shape(True))
+    shape[ax] = 1
+    k2 = np.zeros(shape)
+    for i, b in enumerate(T.bases):
+        if i == ax:
+            continue
+        k = b.wavenumbers().astype(float)
+        s = [1] * T.dimensions
+        s[i] = len(k)
+        k2 = k2 + (k ** 2).reshape(s)
+    A = stiffness_matrix(S)
+    B = mass_matrix(S)
+    # Chebyshev: -lap -> -(phi'', phi)_w = -A ; Legendre: (phi', phi') = +A
+    a_scale = -1.0 if S.family() == "chebyshev" else 1.0
+    return [TPMatrix(A, T, np.full(shape, a_scale), ax), TPMatrix(B, T, alpha + k2, ax)]

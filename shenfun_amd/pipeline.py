@@ -1,0 +1,44 @@
+"""Fused forward -> Helmholtz solve -> backward on the GPU.
+
+``HelmholtzSolver3D`` runs (alpha - lap) u = f on a Chebyshev(SD/SN) x C2C x
+R2C TensorProductSpace in one C-ABI call (sfa_helmholtz3d): the sequence
+T.scalar_product -> chebyshev.la.Helmholtz -> T.backward of the reference
+(tensorproductspace.py:146-190, chebyshev/la.py:288-317,
+tensorproductspace.py:46-96), with the Helmholtz LU computed on the fly from
+beta = alpha + ky^2 + kz^2 instead of streamed factor arrays.
+"""
+import torch
+
+from . import _lib
+from .bases import C2C, ChebyshevBase, R2C
+
+
+class HelmholtzSolver3D:
+    def __init__(self, T, alpha, precision="fp64", refine=2):
+        b = T.bases
+        if not (len(b) == 3 and isinstance(b[0], ChebyshevBase) and b[0].is_composite
+                and isinstance(b[1], C2C) and isinstance(b[2], R2C)):
+            raise ValueError("expects Chebyshev(SD/SN) x C2C x R2C")
+        self.T = T
+        self.alpha = float(alpha)
+        self.neumann = b[0].bc_kind == "N"
+        self.precision = {"fp64": 0, "mixed": 1}[precision]
+        self.refine = int(refine)
+        self.shape = T.shape(False)
+        Nx, Ny, Nz = self.shape
+        nbytes = _lib.lib().sfa_helmholtz3d_workspace(Nx, Ny, Nz, self.precision)
+        self.work = torch.empty(int(nbytes), dtype=torch.uint8, device=T.device)
+
+    def __call__(self, f, u=None):
+        if not (f.is_cuda and f.dtype == torch.float64 and f.is_contiguous()
+                and tuple(f.shape) == tuple(self.shape)):
+            raise ValueError(f"f must be a contiguous float64 CUDA tensor of shape {self.shape}")
+        if u is None:
+            u = torch.empty_like(f)
+        if not (u.is_contiguous() and u.shape == f.shape and u.dtype == f.dtype):
+            raise ValueError("u must match f")
+        Nx, Ny, Nz = self.shape
+        _lib.call("sfa_helmholtz3d", _lib.ptr(f), _lib.ptr(u), Nx, Ny, Nz, self.alpha,
+                  int(self.neumann), self.precision, self.refine, _lib.ptr(self.work),
+                  self.work.numel(), _lib.stream_ptr())
+        return u

@@ -1,0 +1,161 @@
+"""Tensor-product spaces on one GPU (tensorproductspace.py:262-469 mirror).
+
+One GPU holds the whole array (no pencils, no transfers): the forward and
+scalar-product transforms run axis by axis from the last axis to the first,
+the backward transform from the first to the last
+(ForwardTransform/ScalarTransform/BackwardTransform,
+tensorproductspace.py:45-259), each axis being one call of the 1D space's
+HIP transform on the strided device array.
+"""
+import numpy as np
+import torch
+
+from .bases import FourierBase, R2C
+
+
+class TensorProductSpace:
+    def __init__(self, comm, bases, axes=None, dtype=None, slab=False, device=None, **kw):
+        """``comm`` is accepted for signature compatibility and ignored: one
+        GPU holds the full array."""
+        self.bases = list(bases)
+        self.dimensions = len(self.bases)
+        if not 1 <= self.dimensions <= 3:
+            raise ValueError("1..3 dimensional spaces are supported")
+        if axes is not None and tuple(axes) != tuple(range(self.dimensions)) and \
+                tuple(axes) != tuple(range(self.dimensions))[::-1]:
+            raise NotImplementedError("custom axis orders are not supported")
+        for i, b in enumerate(self.bases):
+            if isinstance(b, R2C) and i != self.dimensions - 1:
+                raise ValueError("an R2C space must be the last axis")
+            b.axis = i
+            b.dimensions = self.dimensions
+            b.tensorproductspace = self
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        self._work = {}
+
+    # ---------------------------------------------------------------- info
+    def shape(self, forward_output=True):
+        return tuple(b.shape(forward_output) if isinstance(b, FourierBase) else b.N
+                     for b in self.bases)
+
+    def global_shape(self, forward_output=True):
+        return self.shape(forward_output)
+
+    def local_slice(self, forward_output=True):
+        return tuple(slice(0, s) for s in self.shape(forward_output))
+
+    def physical_dtype(self):
+        return torch.float64 if isinstance(self.bases[-1], R2C) or not any(
+            isinstance(b, FourierBase) for b in self.bases) else torch.complex128
+
+    def spectral_dtype(self):
+        return torch.complex128 if any(isinstance(b, FourierBase) for b in self.bases) \
+            else torch.float64
+
+    def nonperiodic_axis(self):
+        ax = [i for i, b in enumerate(self.bases) if not isinstance(b, FourierBase)]
+        if len(ax) != 1:
+            raise ValueError("exactly one non-periodic axis expected")
+        return ax[0]
+
+    def wavenumbers(self):
+        return [b.wavenumbers() if isinstance(b, FourierBase) else np.arange(b.N)
+                for b in self.bases]
+
+    local_wavenumbers = wavenumbers
+
+    def get_testspace(self, kind="G"):
+        return self
+
+    # ------------------------------------------------------------ helpers
+    def _buf(self, key, shape, dtype):
+        k = (key, tuple(shape), dtype)
+        if k not in self._work:
+            self._work[k] = torch.empty(shape, dtype=dtype, device=self.device)
+        return self._work[k]
+
+    def _shape_after_forward(self, axis):
+        """Array shape after transforming axes >= axis."""
+        s = list(self.shape(False))
+        for i in range(axis, self.dimensions):
+            s[i] = self.shape(True)[i]
+        return s
+
+    def _chain(self, inp, output_array, method, order, kind):
+        cur = inp
+        n = len(order)
+        for j, ax in enumerate(order):
+            b = self.bases[ax]
+            last = j == n - 1
+            k = kind[b.family()] if isinstance(kind, dict) else kind
+            if method == "backward":
+                shp = list(cur.shape)
+                shp[ax] = self.shape(False)[ax]
+                dt = torch.float64 if (isinstance(b, R2C)) else cur.dtype
+            else:
+                shp = list(cur.shape)
+                shp[ax] = self.shape(True)[ax]
+                dt = torch.complex128 if isinstance(b, R2C) else cur.dtype
+            if last and output_array is not None:
+                out = output_array
+            else:
+                out = self._buf((method, j % 2), shp, dt)
+                if out.data_ptr() == cur.data_ptr():
+                    out = self._buf((method, 2), shp, dt)
+            getattr(b, method)(cur, out, kind=k)
+            cur = out
+        if output_array is None:
+            return cur.clone()
+        return cur
+
+    # ---------------------------------------------------------- transforms
+    def scalar_product(self, input_array=None, output_array=None, kind=None, **kw):
+        """(tensorproductspace.py:146-190): last axis first."""
+        return self._chain(input_array, output_array, "scalar_product",
+                           list(range(self.dimensions - 1, -1, -1)), kind)
+
+    def forward(self, input_array=None, output_array=None, kind=None, **kw):
+        """(tensorproductspace.py:205-259): last axis first."""
+        return self._chain(input_array, output_array, "forward",
+                           list(range(self.dimensions - 1, -1, -1)), kind)
+
+    def backward(self, input_array=None, output_array=None, kind=None, **kw):
+        """(tensorproductspace.py:46-96): first axis first."""
+        return self._chain(input_array, output_array, "backward", list(range(self.dimensions)), kind)
+
+
+def Array(space, buffer=None, val=None):
+    """Physical-space array of ``space`` (torch tensor on the GPU)."""
+    if isinstance(space, TensorProductSpace):
+        shape, dtype = space.shape(False), space.physical_dtype()
+        dev = space.device
+    else:
+        shape = (space.N,)
+        dtype = torch.float64 if space.dtype.kind == "f" else torch.complex128
+        dev = space.device
+    if buffer is not None:
+        t = torch.as_tensor(np.asarray(buffer), device=dev).to(dtype)
+        return t.reshape(shape).contiguous()
+    t = torch.zeros(shape, dtype=dtype, device=dev)
+    if val is not None:
+        t.fill_(val)
+    return t
+
+
+def Function(space, buffer=None, val=None):
+    """Spectral-space array of ``space`` (torch tensor on the GPU)."""
+    if isinstance(space, TensorProductSpace):
+        shape, dtype = space.shape(True), space.spectral_dtype()
+        dev = space.device
+    else:
+        shape = (space.shape(True) if isinstance(space, FourierBase) else space.N,)
+        dtype = torch.complex128 if isinstance(space, FourierBase) or space.dtype.kind == "c" \
+            else torch.float64
+        dev = space.device
+    if buffer is not None:
+        t = torch.as_tensor(np.asarray(buffer), device=dev).to(dtype)
+        return t.reshape(shape).contiguous()
+    t = torch.zeros(shape, dtype=dtype, device=dev)
+    if val is not None:
+        t.fill_(val)
+    return t

@@ -1,0 +1,123 @@
+"""GPU parity: banded solvers and the Chebyshev Helmholtz solver vs the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import spectral as S
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if isinstance(a, torch.Tensor) else a
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.fixture(scope="module")
+def sfa():
+    import shenfun_amd
+    return shenfun_amd
+
+
+# tests/test_la.py:7-15 of the reference (N = 10 matrices), restricted to the
+# DiagMA / TDMA / PDMA patterns of the hot path
+N10 = 10
+MATS = [
+    {0: np.arange(N10) + 1.0},
+    {-2: 1.0, 0: -2.0, 2: 1.0},
+    {-4: 0.1, -2: 1.0, 0: -2.0, 2: 1.0, 4: 0.1},
+]
+
+
+@pytest.mark.parametrize("di", MATS)
+@pytest.mark.parametrize("cplx", [False, True])
+def test_xdma(sfa, di, cplx):
+    print(f"[xdma {sorted(di)} cplx={cplx}]", flush=True)
+    M = sfa.SparseMatrix(di, (N10, N10))
+    sol = sfa.la.Solver(M)
+    dense = S.dense(di, N10)
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(N10) + (1j * rng.standard_normal(N10) if cplx else 0)
+    u = sol(dev(b))
+    assert rel(u, np.linalg.solve(dense, b)) < TOL
+    bh = rng.standard_normal((N10, N10, 3)) + (1j * rng.standard_normal((N10, N10, 3)) if cplx else 0)
+    for axis in (0, 1):
+        uh = sol(dev(bh), axis=axis)
+        ref = np.moveaxis(np.linalg.solve(dense, np.moveaxis(bh, axis, 0).reshape(N10, -1))
+                          .reshape(np.moveaxis(bh, axis, 0).shape), 0, axis)
+        assert rel(uh, ref) < TOL
+
+
+@pytest.mark.parametrize("n", [6, 7, 64, 255])
+def test_tdma_pdma_vs_oracle(sfa, n):
+    rng = np.random.default_rng(n)
+    d3 = {-2: rng.uniform(0.1, 1, n - 2), 0: rng.uniform(3, 4, n), 2: rng.uniform(0.1, 1, n - 2)}
+    d5 = {-4: rng.uniform(0.1, .3, n - 4), -2: rng.uniform(0.1, 1, n - 2), 0: rng.uniform(4, 5, n),
+          2: rng.uniform(0.1, 1, n - 2), 4: rng.uniform(0.1, .3, n - 4)}
+    b = rng.standard_normal((n, 9)) + 1j * rng.standard_normal((n, 9))
+    u3 = sfa.la.TDMA(sfa.SparseMatrix(d3, (n, n)))(dev(b), axis=0)
+    assert rel(u3, S.tdma_solve(b, S.tdma_lu(S.tdma_dia(d3, n)), 0)) < TOL
+    u5 = sfa.la.PDMA(sfa.SparseMatrix(d5, (n, n)))(dev(b), axis=0)
+    assert rel(u5, S.pdma_solve(b, S.pdma_lu(S.pdma_dia(d5, n)), 0)) < TOL
+
+
+@pytest.mark.parametrize("bc", ["D", "N"])
+@pytest.mark.parametrize("shape,axis", [((20,), 0), ((32, 7, 5), 0), ((6, 40, 5), 1), ((4, 5, 66), 2),
+                                        ((256, 16, 9), 0)])
+def test_helmholtz(sfa, bc, shape, axis):
+    print(f"[helmholtz {bc} {shape} axis={axis}]", flush=True)
+    N = shape[axis]
+    rng = np.random.default_rng(N)
+    mshape = list(shape)
+    mshape[axis] = 1
+    alfa = -np.ones(mshape) * rng.uniform(0.5, 2)
+    beta = rng.uniform(0.5, 50, mshape)
+    if bc == "N":
+        beta.flat[0] = 0.0  # the k = 0 Neumann gauge
+    V = sfa.FunctionSpace(N, "chebyshev", bc=bc)
+    V.axis = axis
+
+    class _T:  # minimal tensor-product stand-in carrying the shape
+        def __init__(self, s):
+            self._s = s
+
+        def shape(self, fo=True):
+            return tuple(self._s)
+
+    A = sfa.stiffness_matrix(V)
+    B = sfa.mass_matrix(V)
+    if len(shape) > 1:
+        V.tensorproductspace = _T(shape)
+    H = sfa.la.Helmholtz(A, B, alfa, beta)
+    b = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    u = H(dev(b), torch.zeros(shape, dtype=torch.complex128, device="cuda"))
+    d0, d1, d2, L = S.helmholtz_lu(bc, N, np.moveaxis(alfa, axis, 0)[0], np.moveaxis(beta, axis, 0)[0])
+    uref = S.helmholtz_solve(np.moveaxis(b, axis, 0), d0, d1, d2, L, neumann=(bc == "N"))
+    uref = np.moveaxis(uref, 0, axis)
+    assert rel(u, uref) < TOL
+    # factors match the reference LU (la.pyx:717-763)
+    for got, ref in zip((H.u0, H.u1, H.u2, H.L), (d0, d1, d2, L)):
+        assert rel(torch.movedim(got, axis, 0), ref) < TOL
+    # matvec of the solution reproduces b on the unknowns (except the gauge row)
+    c = H.matvec(u, torch.zeros_like(u))
+    cref = S.helmholtz_matvec(uref, np.moveaxis(alfa, axis, 0)[0] if len(shape) > 1 else alfa,
+                              np.moveaxis(beta, axis, 0)[0] if len(shape) > 1 else beta, bc, axis)
+    assert rel(c, cref) < TOL
+
+
+@pytest.mark.parametrize("N", [8, 64, 255])
+def test_poisson_add(sfa, N):
+    V = sfa.FunctionSpace(N, "chebyshev", bc="D")
+    A = sfa.stiffness_matrix(V)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal((N, 4)) + 1j * rng.standard_normal((N, 4))
+    u = sfa.la.ADDSolver(A)(dev(b), torch.zeros(N, 4, dtype=torch.complex128, device="cuda"))
+    ref = S.poisson_add_solve(b, N)
+    assert rel(u[:N - 2], ref[:N - 2]) < TOL
+    dense = S.dense(S.cheb_A_SD(N), N - 2)
+    assert rel(u[:N - 2], np.linalg.solve(dense, b[:N - 2])) < 1e-11

@@ -1,0 +1,144 @@
+"""GPU parity: per-axis transforms (C ABI via shenfun_amd) vs the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import spectral as S
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12  # max-norm relative error vs the FP64 oracle
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if isinstance(a, torch.Tensor) else a
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.fixture(scope="module")
+def sfa():
+    import shenfun_amd
+    return shenfun_amd
+
+
+@pytest.mark.parametrize("n", [8, 16, 30, 34, 64, 96, 100, 128, 254, 256, 512, 1000, 4096])
+def test_fft_c2c_1d(sfa, n):
+    print(f"[c2c n={n}]", flush=True)
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    B = sfa.FunctionSpace(n, "F", dtype="D")
+    out = B.scalar_product(dev(x))
+    assert rel(out, np.fft.fft(x) / n) < TOL
+    back = B.backward(dev(np.fft.fft(x)))
+    assert rel(back, np.fft.ifft(np.fft.fft(x)) * n) < TOL
+
+
+@pytest.mark.parametrize("shape,axis", [((16, 12, 10), 0), ((16, 12, 10), 1), ((16, 12, 10), 2),
+                                        ((6, 256, 5), 1), ((64, 64), 0), ((64, 64), 1)])
+def test_fft_c2c_axes(sfa, shape, axis):
+    print(f"[c2c {shape} axis={axis}]", flush=True)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    B = sfa.FunctionSpace(shape[axis], "F", dtype="D")
+    B.axis = axis
+    out = B.scalar_product(dev(x))
+    assert rel(out, np.fft.fft(x, axis=axis) / shape[axis]) < TOL
+    inplace = dev(x)
+    B.backward(inplace, inplace)
+    assert rel(inplace, np.fft.ifft(x, axis=axis) * shape[axis]) < TOL
+
+
+@pytest.mark.parametrize("shape,axis", [((64,), 0), ((10,), 0), ((34,), 0), ((16, 12, 10), 2),
+                                        ((16, 12, 10), 0), ((8, 20, 6), 1), ((64, 256), 1),
+                                        ((4, 4, 4096), 2)])
+def test_fft_r2c_c2r(sfa, shape, axis):
+    print(f"[r2c {shape} axis={axis}]", flush=True)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(shape)
+    n = shape[axis]
+    B = sfa.FunctionSpace(n, "F", dtype="d")
+    B.axis = axis
+    X = B.scalar_product(dev(x))
+    ref = np.fft.rfft(x, axis=axis) / n
+    assert rel(X, ref) < TOL
+    # c2r ignores the imaginary parts of the k = 0 and Nyquist entries
+    Xc = np.fft.rfft(x, axis=axis)
+    s0 = [slice(None)] * x.ndim
+    s0[axis] = 0
+    Xc[tuple(s0)] += 0.25j
+    back = B.backward(dev(Xc))
+    assert rel(back, np.fft.irfft(Xc, n=n, axis=axis) * n) < TOL
+
+
+@pytest.mark.parametrize("bc", [None, "D", "N"])
+@pytest.mark.parametrize("N", [8, 9, 33, 64, 256])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_chebyshev_1d(sfa, bc, N, cplx):
+    print(f"[cheb bc={bc} N={N} cplx={cplx}]", flush=True)
+    rng = np.random.default_rng(N)
+    f = rng.standard_normal(N) + (1j * rng.standard_normal(N) if cplx else 0)
+    V = sfa.FunctionSpace(N, "chebyshev", bc=bc)
+    for kind in ("fast", "vandermonde"):
+        sp = V.scalar_product(dev(f), kind=kind)
+        ref = S.basis_scalar_product("chebyshev", bc, f)
+        assert rel(sp, ref) < TOL, kind
+        c = rng.standard_normal(N) + (1j * rng.standard_normal(N) if cplx else 0)
+        if bc is not None:
+            c[-2:] = 0
+        bw = V.backward(dev(c), kind=kind)
+        assert rel(bw, S.basis_backward("chebyshev", bc, c)) < TOL, kind
+
+
+@pytest.mark.parametrize("bc", [None, "D", "N"])
+@pytest.mark.parametrize("shape,axis", [((64, 33), 0), ((12, 64), 1), ((256, 20, 6), 0),
+                                        ((5, 128, 7), 1), ((6, 5, 256), 2)])
+def test_chebyshev_axes(sfa, bc, shape, axis):
+    print(f"[cheb axes bc={bc} {shape} axis={axis}]", flush=True)
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    V = sfa.FunctionSpace(shape[axis], "chebyshev", bc=bc)
+    V.axis = axis
+    sp = V.scalar_product(dev(f))
+    assert rel(sp, S.basis_scalar_product("chebyshev", bc, f, axis)) < TOL
+    bw = V.backward(dev(f))
+    fc = f.copy()
+    if bc is not None:
+        s = [slice(None)] * f.ndim
+        s[axis] = slice(shape[axis] - 2, None)
+        fc[tuple(s)] = 0
+    assert rel(bw, S.basis_backward("chebyshev", bc, fc, axis)) < TOL
+
+
+@pytest.mark.parametrize("bc", [None, "D", "N"])
+@pytest.mark.parametrize("N", [8, 33, 64, 200, 512])
+def test_legendre(sfa, bc, N):
+    print(f"[leg bc={bc} N={N}]", flush=True)
+    rng = np.random.default_rng(N + 1)
+    f = rng.standard_normal((N, 5)) + 1j * rng.standard_normal((N, 5))
+    V = sfa.FunctionSpace(N, "legendre", bc=bc)
+    sp = V.scalar_product(dev(f))
+    assert rel(sp, S.basis_scalar_product("legendre", bc, f, 0)) < TOL
+    c = f.copy()
+    if bc is not None:
+        c[-2:] = 0
+    bw = V.backward(dev(c))
+    assert rel(bw, S.basis_backward("legendre", bc, c, 0)) < TOL
+
+
+@pytest.mark.parametrize("family", ["chebyshev", "legendre"])
+@pytest.mark.parametrize("bc", ["D", "N"])
+def test_forward_projection_roundtrip(sfa, family, bc):
+    """forward(backward(u)) == u for u in the space (projection with the mass solve)."""
+    print(f"[proj {family} {bc}]", flush=True)
+    N = 40
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))
+    u[-2:] = 0
+    V = sfa.FunctionSpace(N, family, bc=bc)
+    phys = V.backward(dev(u))
+    uh = V.forward(phys)
+    assert rel(uh, u) < 1e-12
