@@ -39,6 +39,7 @@ _SIGS = {
     "sfa_helmholtz_matvec": "vviiaaaiivvvvvvvlv",
     "sfa_poisson_add_solve": "vviiaaaivvdv",
     "sfa_helmholtz3d": "vvllldiiivlv",
+    "sfa_helmholtz3d_stage_times": "vvllldivlivv",
 }
 
 _TYPES = {"v": _c_void_p, "i": _int, "l": _i64, "d": _dbl, "a": _p64}

@@ -1,0 +1,254 @@
+// Wavefront-parallel Chebyshev Helmholtz LU + solve of one parity chain.
+//
+// The reference factorises alfa*A + beta*B row by row (LU_Helmholtz_1D,
+// la.pyx:717-763) and solves with a forward sweep and a back substitution
+// carrying running parity sums (Solve_Helmholtz_1D_ptr, la.pyx:871-912).
+// All band offsets are even, so the even and odd unknowns form two
+// independent chains of K <= M/2 + 1 rows.  Here one chain is handled by 16
+// lanes (E consecutive rows each) with three scans across the 16 lanes:
+//
+//  1. LU: in homogeneous coordinates the row recurrence
+//       d0 = h - g d1'/d0', d1 = s - g d2'/d0', d2 = c - g d2'/d0'
+//     is linear: zeta_m = N_m zeta_{m-1}, N_m = [[h,-g,0],[s,0,-g],[c,0,-g]],
+//     zeta_m proportional to (d0, d1, d2)_m.  Segment products of N are
+//     scanned (normalised by powers of two); each lane then restarts the
+//     exact row recurrence from the ratios of its exclusive prefix.
+//  2. forward elimination y_m = f_m - (g_m/d0_{m-1}) y_{m-1}: affine scan.
+//  3. back substitution in the state (v_m, T_m = sum_{j>=m} v_j):
+//       v_m = y_m/d0 - ((d1-d2)/d0) v_{m+1} - (d2/d0) T_{m+1},  T_m = v_m + T_{m+1}
+//     a 2x2 affine suffix scan.
+#pragma once
+#include "common.h"
+
+namespace sfa {
+
+// Row coefficients of one chain row (global index i), Neumann column scaling
+// and k = 0 gauge of la.pyx:735-749 included.
+struct HelmRow {
+    double h, s, c, g;
+};
+
+struct HelmTab {
+    const double *A0, *A2, *A4, *Bm2, *B0, *B2;  // unscaled diagonals, M entries each
+};
+
+__device__ inline HelmRow helm_row(const HelmTab& t, int i, int M, double al, double be, bool nm,
+                                   bool zb)
+{
+    HelmRow r;
+    if (i >= M) {  // padding row: identity, no coupling
+        r.h = 1.0;
+        r.s = r.c = r.g = 0.0;
+        return r;
+    }
+    double a0 = t.A0[i], b0 = t.B0[i];
+    double a2 = (i < M - 2) ? t.A2[i] : 0.0, b2 = (i < M - 2) ? t.B2[i] : 0.0;
+    double a4 = (i < M - 4) ? t.A4[i] : 0.0;
+    double bm = (i >= 2) ? t.Bm2[i - 2] : 0.0;
+    if (nm) {
+        if (i == 0) {
+            if (zb) { a0 = 1.0 / al; b0 = 0.0; }
+        } else {
+            const double ii = 1.0 / ((double)i * i);
+            a0 *= ii;
+            b0 *= ii;
+        }
+        const double i2 = 1.0 / ((double)(i + 2) * (i + 2));
+        a2 *= i2;
+        b2 *= i2;
+        a4 *= 1.0 / ((double)(i + 4) * (i + 4));
+        if (i - 2 >= 1) bm *= 1.0 / ((double)(i - 2) * (i - 2));
+    }
+    r.h = al * a0 + be * b0;
+    r.s = al * a2 + be * b2;
+    r.c = al * a4;
+    r.g = be * bm;
+    return r;
+}
+
+struct M3 {
+    double a[9];
+};
+
+__device__ inline M3 m3mul(const M3& x, const M3& y)  // x @ y
+{
+    M3 z;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            z.a[3 * i + j] = x.a[3 * i] * y.a[j] + x.a[3 * i + 1] * y.a[3 + j] + x.a[3 * i + 2] * y.a[6 + j];
+    return z;
+}
+
+__device__ inline void m3norm(M3& x)
+{
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) mx = fmax(mx, fabs(x.a[i]));
+    if (mx > 0.0) {
+        int e;
+        frexp(mx, &e);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) x.a[i] = ldexp(x.a[i], -e);
+    }
+}
+
+__device__ inline double shup(double v, int d) { return __shfl_up(v, d, 16); }
+__device__ inline double shdn(double v, int d) { return __shfl_down(v, d, 16); }
+
+// Solve one chain in place.  X: line in LDS (rows i = 2m + p hold f on entry,
+// the solution on exit).  lane in [0, 16).  K = number of rows of parity p.
+template <int E>
+__device__ inline void helm_chain_scan(double2* X, int p, int M, double al, double be, bool nm,
+                                       const HelmTab& tab, int lane)
+{
+    const bool zb = nm && fabs(be) < 1e-8;
+    const int m0 = lane * E;
+    // ---- 1. LU scan
+    M3 seg;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) seg.a[i] = (i % 4 == 0) ? 1.0 : 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const HelmRow r = helm_row(tab, 2 * (m0 + e) + p, M, al, be, nm, zb);
+        M3 Nm;
+        Nm.a[0] = r.h; Nm.a[1] = -r.g; Nm.a[2] = 0.0;
+        Nm.a[3] = r.s; Nm.a[4] = 0.0;  Nm.a[5] = -r.g;
+        Nm.a[6] = r.c; Nm.a[7] = 0.0;  Nm.a[8] = -r.g;
+        seg = m3mul(Nm, seg);
+    }
+    m3norm(seg);
+#pragma unroll
+    for (int k = 1; k < 16; k <<= 1) {
+        M3 o;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) o.a[i] = shup(seg.a[i], k);
+        if (lane >= k) {
+            seg = m3mul(seg, o);
+            m3norm(seg);
+        }
+    }
+    // exclusive prefix applied to zeta_{-1} = (1, 0, 0): first column of lane-1's product
+    double z0 = shup(seg.a[0], 1), z1 = shup(seg.a[3], 1), z2 = shup(seg.a[6], 1);
+    if (lane == 0) { z0 = 1.0; z1 = 0.0; z2 = 0.0; }
+    double r1 = z1 / z0, r2 = z2 / z0;
+    double inv[E], e1[E], e2[E], am[E];
+    double lastinv = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const HelmRow r = helm_row(tab, 2 * (m0 + e) + p, M, al, be, nm, zb);
+        const double d0 = r.h - r.g * r1;
+        const double d1 = r.s - r.g * r2;
+        const double d2 = r.c - r.g * r2;
+        const double iv = 1.0 / d0;
+        am[e] = -r.g;  // times 1/d0_{m-1}, filled below
+        inv[e] = iv;
+        r1 = d1 * iv;
+        r2 = d2 * iv;
+        e1[e] = r1;
+        e2[e] = r2;
+        lastinv = iv;
+    }
+    {
+        double previnv = shup(lastinv, 1);
+        if (lane == 0) previnv = 0.0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            am[e] *= previnv;
+            previnv = inv[e];
+        }
+    }
+    // ---- 2. forward elimination (affine scan), f from LDS
+    double2 f[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = 2 * (m0 + e) + p;
+        f[e] = (i < M) ? X[i] : make_double2(0.0, 0.0);
+    }
+    double A = 1.0;
+    double2 B = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        A = am[e] * A;
+        B = make_double2(am[e] * B.x + f[e].x, am[e] * B.y + f[e].y);
+    }
+#pragma unroll
+    for (int k = 1; k < 16; k <<= 1) {
+        const double oA = shup(A, k), oBx = shup(B.x, k), oBy = shup(B.y, k);
+        if (lane >= k) {
+            B = make_double2(A * oBx + B.x, A * oBy + B.y);
+            A = A * oA;
+        }
+    }
+    double2 y = make_double2(shup(B.x, 1), shup(B.y, 1));
+    if (lane == 0) y = make_double2(0.0, 0.0);
+    double2 yp[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        y = make_double2(f[e].x + am[e] * y.x, f[e].y + am[e] * y.y);
+        yp[e] = make_double2(y.x * inv[e], y.y * inv[e]);
+    }
+    // ---- 3. back substitution (2x2 affine suffix scan)
+    // Phi = [[-(e1-e2), -e2], [-(e1-e2), 1-e2]], b = (yp, yp)
+    double P00 = 1.0, P01 = 0.0, P10 = 0.0, P11 = 1.0;
+    double2 c0 = make_double2(0.0, 0.0), c1 = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int e = E - 1; e >= 0; --e) {
+        const double q = -(e1[e] - e2[e]), w = -e2[e], w1 = 1.0 - e2[e];
+        const double n00 = q * P00 + w * P10, n01 = q * P01 + w * P11;
+        const double n10 = q * P00 + w1 * P10, n11 = q * P01 + w1 * P11;
+        const double2 nc0 = make_double2(q * c0.x + w * c1.x + yp[e].x, q * c0.y + w * c1.y + yp[e].y);
+        const double2 nc1 = make_double2(q * c0.x + w1 * c1.x + yp[e].x, q * c0.y + w1 * c1.y + yp[e].y);
+        P00 = n00; P01 = n01; P10 = n10; P11 = n11;
+        c0 = nc0;
+        c1 = nc1;
+    }
+#pragma unroll
+    for (int k = 1; k < 16; k <<= 1) {
+        const double o00 = shdn(P00, k), o01 = shdn(P01, k), o10 = shdn(P10, k), o11 = shdn(P11, k);
+        const double oc0x = shdn(c0.x, k), oc0y = shdn(c0.y, k), oc1x = shdn(c1.x, k),
+                     oc1y = shdn(c1.y, k);
+        if (lane + k < 16) {
+            // (P, c) o (O, oc) = (P O, P oc + c)
+            const double2 nc0 = make_double2(P00 * oc0x + P01 * oc1x + c0.x, P00 * oc0y + P01 * oc1y + c0.y);
+            const double2 nc1 = make_double2(P10 * oc0x + P11 * oc1x + c1.x, P10 * oc0y + P11 * oc1y + c1.y);
+            const double n00 = P00 * o00 + P01 * o10, n01 = P00 * o01 + P01 * o11;
+            const double n10 = P10 * o00 + P11 * o10, n11 = P10 * o01 + P11 * o11;
+            P00 = n00; P01 = n01; P10 = n10; P11 = n11;
+            c0 = nc0;
+            c1 = nc1;
+        }
+    }
+    double2 sv = make_double2(shdn(c0.x, 1), shdn(c0.y, 1));
+    double2 sT = make_double2(shdn(c1.x, 1), shdn(c1.y, 1));
+    if (lane == 15) {
+        sv = make_double2(0.0, 0.0);
+        sT = make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int e = E - 1; e >= 0; --e) {
+        const double q = -(e1[e] - e2[e]), w = -e2[e], w1 = 1.0 - e2[e];
+        const double2 v = make_double2(q * sv.x + w * sT.x + yp[e].x, q * sv.y + w * sT.y + yp[e].y);
+        const double2 T = make_double2(q * sv.x + w1 * sT.x + yp[e].x, q * sv.y + w1 * sT.y + yp[e].y);
+        sv = v;
+        sT = T;
+        const int i = 2 * (m0 + e) + p;
+        if (i < M) {
+            double2 out = v;
+            if (nm) {
+                if (i == 0) {
+                    // gauge of la.pyx:907-909 on the first LU pivot d0[0]
+                    const double d00 = 1.0 / inv[0];
+                    if ((d00 - 1.0) * (d00 - 1.0) < 1e-16) out = make_double2(0.0, 0.0);
+                } else {
+                    const double s = 1.0 / ((double)i * i);
+                    out = make_double2(out.x * s, out.y * s);
+                }
+            }
+            X[i] = out;
+        }
+    }
+}
+
+}  // namespace sfa

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "fft_device.h"
+#include "helm_scan.h"
 
 namespace sfa {
 
@@ -84,6 +85,7 @@ struct XPassArgs {
     int neumann;
     double alpha;
     int Ny;
+    int E;                 // rows per lane of a 16-lane chain
     const double* K2;      // N-2 stencil entries
     const double* tabs;    // 6*(N-2) diagonals
     const double2* tw;     // FFT table N
@@ -140,105 +142,30 @@ __global__ void __launch_bounds__(256) xpass_kernel(XPassArgs a)
         F[t * ldn + k] = v;
     }
     __syncthreads();
-    // 4. Helmholtz LU on the fly + solve, one thread per (line, parity)
-    if (tid < 2 * Tv) {
-        const int t = tid >> 1, p = tid & 1;
-        const int64_t jz = jz0 + t;
-        const double ky = (jy < a.Ny / 2 || (a.Ny % 2 == 1 && jy == a.Ny / 2)) ? (double)jy
-                                                                             : (double)(jy - a.Ny);
-        const double kz = (double)jz;
-        const double be = a.alpha + ky * ky + kz * kz;
-        const double al = -1.0;
-        const bool nm = a.neumann != 0;
-        const bool zb = nm && fabs(be) < 1e-8;
-        const double* tA0 = a.tabs;
-        const double* tA2 = tA0 + M;
-        const double* tA4 = tA2 + M;
-        const double* tBm2 = tA4 + M;
-        const double* tB0 = tBm2 + M;
-        const double* tB2 = tB0 + M;
-        double2* X = F + t * ldn;  // rhs in, y/d0 then u out
-        double2* E = G + t * ldn;  // (d1/d0, d2/d0) per row
-        double d0p = 0, d1p = 0, d2p = 0;
-        double2 yp = make_double2(0.0, 0.0);
-        for (int i = p; i < M; i += 2) {
-            double a0 = tA0[i], b0 = tB0[i];
-            double a2 = (i < M - 2) ? tA2[i] : 0.0, b2 = (i < M - 2) ? tB2[i] : 0.0;
-            double a4 = (i < M - 4) ? tA4[i] : 0.0;
-            if (nm) {
-                if (i == 0) {
-                    if (zb) { a0 = 1.0 / al; b0 = 0.0; }
-                } else {
-                    a0 /= (double)i * i;
-                    b0 /= (double)i * i;
-                }
-                a2 /= (double)(i + 2) * (i + 2);
-                b2 /= (double)(i + 2) * (i + 2);
-                a4 /= (double)(i + 4) * (i + 4);
+    // 4. Helmholtz LU on the fly + solve: 16 lanes per (line, parity) chain
+    {
+        const int t = tid >> 5, p = (tid >> 4) & 1, lane = tid & 15;
+        if (t < Tv) {
+            const int64_t jz = jz0 + t;
+            const double ky = (jy < a.Ny / 2 || (a.Ny % 2 == 1 && jy == a.Ny / 2)) ? (double)jy
+                                                                                 : (double)(jy - a.Ny);
+            const double kz = (double)jz;
+            const double be = a.alpha + ky * ky + kz * kz;
+            const int M = N - 2;
+            HelmTab tab{a.tabs, a.tabs + M, a.tabs + 2 * M, a.tabs + 3 * M, a.tabs + 4 * M,
+                        a.tabs + 5 * M};
+            double2* X = F + t * ldn;
+            switch (a.E) {
+                case 1: helm_chain_scan<1>(X, p, M, -1.0, be, a.neumann != 0, tab, lane); break;
+                case 2: helm_chain_scan<2>(X, p, M, -1.0, be, a.neumann != 0, tab, lane); break;
+                case 4: helm_chain_scan<4>(X, p, M, -1.0, be, a.neumann != 0, tab, lane); break;
+                case 8: helm_chain_scan<8>(X, p, M, -1.0, be, a.neumann != 0, tab, lane); break;
+                default: helm_chain_scan<16>(X, p, M, -1.0, be, a.neumann != 0, tab, lane); break;
             }
-            double d0v, d1v, d2v;
-            double2 yv = X[i];
-            if (i < 2) {
-                d0v = al * a0 + be * b0;
-                d1v = al * a2 + be * b2;
-                d2v = al * a4;
-            } else {
-                double bm = tBm2[i - 2];
-                if (nm && i - 2 >= 1) bm /= (double)(i - 2) * (i - 2);
-                const double l = be * bm / d0p;
-                d0v = al * a0 + be * b0 - l * d1p;
-                d1v = (i < M - 2) ? al * a2 + be * b2 - l * d2p : 0.0;
-                d2v = (i < M - 4) ? al * a4 - l * d2p : 0.0;
-                yv.x -= l * yp.x;
-                yv.y -= l * yp.y;
+            if (p == 0 && lane == 0) {  // BC slots of a fresh solution are zero
+                X[M] = make_double2(0.0, 0.0);
+                X[M + 1] = make_double2(0.0, 0.0);
             }
-            const double inv = 1.0 / d0v;
-            X[i] = make_double2(yv.x * inv, yv.y * inv);
-            E[i] = make_double2(d1v * inv, d2v * inv);
-            yp = yv;
-            d0p = d0v;
-            d1p = d1v;
-            d2p = d2v;
-        }
-        const int il = (M - 1) - (((M - 1) - p) & 1);
-        double2 n2 = X[il];
-        double2 n4 = make_double2(0.0, 0.0);
-        double2 S = make_double2(0.0, 0.0);
-        if (il - 2 >= 0) {
-            const int i = il - 2;
-            const double2 e = E[i];
-            double2 cur = X[i];
-            cur.x -= e.x * n2.x;
-            cur.y -= e.x * n2.y;
-            X[i] = cur;
-            n4 = n2;
-            n2 = cur;
-        }
-        for (int i = il - 4; i >= 0; i -= 2) {
-            const double2 e = E[i];
-            S.x += n4.x;
-            S.y += n4.y;
-            double2 cur = X[i];
-            cur.x -= e.x * n2.x + e.y * S.x;
-            cur.y -= e.x * n2.y + e.y * S.y;
-            X[i] = cur;
-            n4 = n2;
-            n2 = cur;
-        }
-        if (nm) {
-            // gauge of la.pyx:907-909 uses the first LU diagonal d0[0]
-            double a00 = tA0[0], b00 = tB0[0];
-            if (zb) { a00 = 1.0 / al; b00 = 0.0; }
-            const double d00 = al * a00 + be * b00;
-            if (p == 0 && (d00 - 1.0) * (d00 - 1.0) < 1e-16) X[0] = make_double2(0.0, 0.0);
-            for (int i = (p == 0 ? 2 : 1); i < M; i += 2) {
-                const double s = 1.0 / ((double)i * i);
-                X[i] = make_double2(X[i].x * s, X[i].y * s);
-            }
-        }
-        if (p == 0) {  // BC slots of a fresh solution are zero
-            X[M] = make_double2(0.0, 0.0);
-            X[M + 1] = make_double2(0.0, 0.0);
         }
     }
     __syncthreads();
@@ -312,11 +239,15 @@ int xpass_launch(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann,
     a.d2 = Nzh;
     a.N = Nx;
     a.ldn = Nx + 2;
-    int T = (int)(65536 / (2 * 16 * (int64_t)a.ldn));
-    if (T < 1) T = 1;
-    if (T > 16) T = 16;
+    // 256 threads = 8 lines x 2 parity chains x 16 lanes
+    int T = 8;
     if (T > Nzh) T = Nzh;
     a.T = T;
+    const int K = (Nx - 2 + 1) / 2;  // rows of the longer parity chain
+    int E = 1;
+    while (16 * E < K) E *= 2;
+    SFA_CHECK(E <= 16, SFA_ERR_UNSUPPORTED, "Nx=%d too large for the fused x-pass (max 514)", Nx);
+    a.E = E;
     a.neumann = neumann;
     a.alpha = alpha;
     a.Ny = Ny;
@@ -371,4 +302,51 @@ extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t N
     if ((rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, +1, 1.0, stream))) return rc;
     if ((rc = sfa_fft_c2r(W, u, 3, rshape, cstr, rstr, 2, 1.0, stream))) return rc;
     return SFA_OK;
+}
+
+// Same pipeline as sfa_helmholtz3d, run nrep times with hipEvents recorded on
+// `stream` between the five kernels; stage_ms[5] receives the summed
+// milliseconds of each stage (z R2C, y C2C, x-pass, y C2C inverse, z C2R).
+extern "C" int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t Nx, int64_t Ny,
+                                           int64_t Nz, double alpha, int neumann, void* work,
+                                           int64_t work_bytes, int nrep, float* stage_ms,
+                                           void* stream)
+{
+    SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
+              "need Nx >= 8, Ny >= 1, even Nz >= 2");
+    SFA_CHECK(work_bytes >= sfa_helmholtz3d_workspace(Nx, Ny, Nz, 0), SFA_ERR_ARG,
+              "workspace too small");
+    SFA_CHECK(nrep >= 1 && stage_ms != nullptr, SFA_ERR_ARG, "bad nrep/stage_ms");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t Nzh = Nz / 2 + 1;
+    double2* W = (double2*)work;
+    const int64_t rshape[3] = {Nx, Ny, Nz};
+    const int64_t rstr[3] = {Ny * Nz, Nz, 1};
+    const int64_t cstr[3] = {Ny * Nzh, Nzh, 1};
+    const int64_t cshape[3] = {Nx, Ny, Nzh};
+    hipEvent_t ev[6];
+    for (int i = 0; i < 6; ++i) SFA_HIP(hipEventCreate(&ev[i]));
+    for (int i = 0; i < 5; ++i) stage_ms[i] = 0.f;
+    int rc = SFA_OK;
+    for (int r = 0; r < nrep && rc == SFA_OK; ++r) {
+        hipEventRecord(ev[0], s);
+        rc = sfa_fft_r2c(f, W, 3, rshape, rstr, cstr, 2, 1.0 / (double)Nz, stream);
+        hipEventRecord(ev[1], s);
+        if (!rc) rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, -1, 1.0 / (double)Ny, stream);
+        hipEventRecord(ev[2], s);
+        if (!rc) rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, alpha, neumann, s);
+        hipEventRecord(ev[3], s);
+        if (!rc) rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, +1, 1.0, stream);
+        hipEventRecord(ev[4], s);
+        if (!rc) rc = sfa_fft_c2r(W, u, 3, rshape, cstr, rstr, 2, 1.0, stream);
+        hipEventRecord(ev[5], s);
+        hipEventSynchronize(ev[5]);
+        for (int i = 0; i < 5; ++i) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+            stage_ms[i] += ms;
+        }
+    }
+    for (int i = 0; i < 6; ++i) hipEventDestroy(ev[i]);
+    return rc;
 }

@@ -1,0 +1,335 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Every oracle function is checked against something that fixes it
+independently: the reference's own compiled fastgl (golden fixture), the
+reference's doctest known answers, library routines with the same definition
+(numpy chebvander/legvander/chebval/legval, explicit DFT/DCT sums), dense
+linear algebra on small sizes, closed-form manufactured solutions, and
+brute-force quadrature of the Galerkin matrices.
+"""
+import json
+import os
+
+import numpy as np
+import numpy.polynomial.chebyshev as npc
+import numpy.polynomial.legendre as npl
+import pytest
+
+from oracle import spectral as S
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+EPS = np.finfo(float).eps
+
+
+def kat():
+    with open(os.path.join(GOLD, "kat.json")) as f:
+        return json.load(f)
+
+
+# ----------------------------------------------------------------- quadrature
+def test_leg_nodes_match_reference_fastgl():
+    g = np.load(os.path.join(GOLD, "fastgl_leggauss.npz"))
+    for N in g["Ns"]:
+        x, w = S.leg_points_weights(int(N))
+        xr, wr = g[f"x{N}"], g[f"w{N}"]
+        assert np.max(np.abs(x - xr)) <= 4 * EPS, N
+        assert np.max(np.abs(w - wr) / wr) <= 32 * EPS, N
+
+
+def test_leg_nodes_kat_and_numpy():
+    x, _ = S.leg_points_weights(4)
+    assert np.allclose(x, kat()["leg_backward_L1_4"], atol=1e-8, rtol=0)
+    for N in (7, 64):
+        xn, wn = npl.leggauss(N)
+        x, w = S.leg_points_weights(N)
+        assert np.allclose(x, xn, atol=1e-14) and np.allclose(w, wn, atol=1e-14)
+
+
+def test_cheb_points():
+    for N in (5, 64):
+        x, w = S.cheb_points_weights(N)
+        xn, wn = npc.chebgauss(N)
+        assert np.allclose(x, xn, atol=1e-15) and np.allclose(w, wn, atol=1e-15)
+        assert np.all(np.diff(x) < 0)  # descending (chebyshev/bases.py:213-216)
+
+
+# ----------------------------------------------------------------- transforms
+def test_dct_definitions_bruteforce():
+    """scipy's DCT-II/III with norm=None equal FFTW's REDFT10/REDFT01 sums."""
+    N = 9
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(N)
+    n = np.arange(N)
+    k = n[:, None]
+    redft10 = 2 * np.sum(x[None, :] * np.cos(np.pi * k * (2 * n[None, :] + 1) / (2 * N)), axis=1)
+    assert np.allclose(S._dct(x, 2, 0), redft10, atol=1e-13)
+    redft01 = x[0] + 2 * np.sum(x[None, 1:] * np.cos(np.pi * n[1:][None, :] * (2 * k + 1) / (2 * N)),
+                                axis=1)
+    assert np.allclose(S._dct(x, 3, 0), redft01, atol=1e-13)
+
+
+@pytest.mark.parametrize("N", [8, 33, 64])
+def test_cheb_fast_vs_vandermonde(N):
+    """test_transforms.py:120-153 pattern: fast (DCT) == Vandermonde."""
+    x, w = S.cheb_points_weights(N)
+    f = x ** 2 + np.cos(np.pi * x)
+    V = npc.chebvander(x, N - 1)
+    assert np.allclose(S.cheb_scalar_product(f), (f * w) @ V, atol=1e-14)
+    c = np.random.default_rng(N).standard_normal(N)
+    assert np.allclose(S.cheb_backward(c), npc.chebval(x, c), atol=1e-13)
+
+
+@pytest.mark.parametrize("N", [8, 33, 64])
+def test_leg_recursive_vs_vandermonde(N):
+    """test_transforms.py:538-598 pattern (recursive == Vandermonde)."""
+    x, w = S.leg_points_weights(N)
+    rng = np.random.default_rng(N)
+    f = rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))
+    V = npl.legvander(x, N - 1)
+    assert np.allclose(S.leg_scalar_product(f, 0), V.T @ (w[:, None] * f), atol=1e-13)
+    c = rng.standard_normal((N, 2))
+    assert np.allclose(S.leg_backward(c, 0), npl.legval(x, c, tensor=False).T
+                       if False else V @ c, atol=1e-12)
+
+
+def test_leg_backward_kat():
+    """legendre/dlt.py:87-93: backward of u = L_1 on LG(4) gives the nodes."""
+    c = np.zeros(4)
+    c[1] = 1
+    assert np.allclose(S.leg_backward(c), kat()["leg_backward_L1_4"], atol=1e-8, rtol=0)
+
+
+def test_fourier_vs_dft_sum():
+    N = 12
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal(N)
+    xj = S.fourier_points(N)
+    k = S.r2c_wavenumbers(N)
+    ref = np.array([np.sum(f * np.exp(-1j * kk * xj)) / N for kk in k])
+    assert np.allclose(S.basis_scalar_product("fourier-r2c", None, f), ref, atol=1e-14)
+    kc = S.c2c_wavenumbers(N)
+    fc = f + 1j * rng.standard_normal(N)
+    refc = np.array([np.sum(fc * np.exp(-1j * kk * xj)) / N for kk in kc])
+    assert np.allclose(S.basis_scalar_product("fourier-c2c", None, fc), refc, atol=1e-14)
+    # backward: unnormalised synthesis on the mesh
+    uh = S.basis_scalar_product("fourier-r2c", None, f)
+    assert np.allclose(S.basis_backward("fourier-r2c", None, uh, N=N), f, atol=1e-13)
+    assert S.c2c_wavenumbers(8)[4] == -4 and S.r2c_wavenumbers(8)[4] == 4
+
+
+# ----------------------------------------------------------------- stencils
+@pytest.mark.parametrize("family", ["chebyshev", "legendre"])
+@pytest.mark.parametrize("bc", ["D", "N"])
+def test_stencil_satisfies_bcs(family, bc):
+    N = 12
+    K2 = S.stencil_k2(family, bc, N)
+    mod = npc if family == "chebyshev" else npl
+    for k in range(N - 2):
+        c = np.zeros(N)
+        c[k] = 1
+        c[k + 2] = K2[k]
+        p = c if bc == "D" else mod.chebder(c) if family == "chebyshev" else mod.legder(c)
+        val = mod.chebval if family == "chebyshev" else mod.legval
+        assert abs(val(-1.0, p)) < 1e-12 and abs(val(1.0, p)) < 1e-12, (k, bc)
+
+
+def test_composite_scalar_product_and_to_ortho_are_adjoint():
+    """s = K t and t = K^T u with the same stencil: <K t, u> = <t, K^T u>."""
+    N = 10
+    K2 = S.stencil_k2("chebyshev", "N", N)
+    rng = np.random.default_rng(1)
+    t = rng.standard_normal(N)
+    u = rng.standard_normal(N)
+    u[-2:] = 0
+    s = S.composite_scalar_product(t, K2)
+    assert np.allclose(s[:N - 2] @ u[:N - 2], t @ S.to_ortho(u, K2))
+
+
+# ----------------------------------------------------------------- matrices
+def test_mass_matrix_kats():
+    d = kat()
+    for key, fn, N, tol in (("cheb_BSDSD_10", S.cheb_B_SD, 10, 0), ("cheb_BSDSD_6", S.cheb_B_SD, 6, 0),
+                            ("leg_BSDSD_10", S.leg_B_SD, 10, 1e-7)):
+        B = fn(N)
+        for off, v in d[key].items():
+            got = np.atleast_1d(B[int(off)])
+            want = np.array(v)
+            if len(want) == 1:
+                assert np.all(np.abs(got - want[0]) <= tol), key
+            else:
+                assert np.all(np.abs(got - want) <= tol), key
+
+
+def _quad_matrices(family, bc, N):
+    """Brute-force Galerkin matrices by exact quadrature (check_sanity pattern,
+    matrixbase.py:1680-1734): B_kj = (phi_j, phi_k)_w, A: Chebyshev
+    (phi_j'', phi_k)_w, Legendre (phi_j', phi_k')."""
+    K2 = S.stencil_k2(family, bc, N)
+    M = N - 2
+    P = np.zeros((M, N))
+    for k in range(M):
+        P[k, k] = 1
+        P[k, k + 2] = K2[k]
+    Q = 2 * N + 4
+    if family == "chebyshev":
+        x, w = npc.chebgauss(Q)
+        V = npc.chebvander(x, N - 1)
+        D2 = np.zeros((N, N))
+        D2[:N - 2] = npc.chebder(np.eye(N), 2)
+        phi = V @ P.T
+        ddphi = V @ D2 @ P.T
+        B = phi.T @ (w[:, None] * phi)
+        A = phi.T @ (w[:, None] * ddphi)
+    else:
+        x, w = npl.leggauss(Q)
+        V = npl.legvander(x, N - 1)
+        D1 = np.zeros((N, N))
+        D1[:N - 1] = npl.legder(np.eye(N), 1)
+        phi = V @ P.T
+        dphi = V @ D1 @ P.T
+        B = phi.T @ (w[:, None] * phi)
+        A = dphi.T @ (w[:, None] * dphi)
+    return A, B
+
+
+@pytest.mark.parametrize("family", ["chebyshev", "legendre"])
+@pytest.mark.parametrize("bc", ["D", "N"])
+def test_matrices_vs_quadrature(family, bc):
+    N = 14
+    A, B = _quad_matrices(family, bc, N)
+    fA = {("chebyshev", "D"): S.cheb_A_SD, ("chebyshev", "N"): S.cheb_A_SN,
+          ("legendre", "D"): S.leg_A_SD, ("legendre", "N"): S.leg_A_SN}[(family, bc)]
+    fB = {("chebyshev", "D"): S.cheb_B_SD, ("chebyshev", "N"): S.cheb_B_SN,
+          ("legendre", "D"): S.leg_B_SD, ("legendre", "N"): S.leg_B_SN}[(family, bc)]
+    # Chebyshev weighted (phi_j'', phi_k)_w is not symmetric: rows k = test
+    assert np.allclose(S.dense(fA(N), N - 2), A, atol=1e-10 * np.max(np.abs(A)))
+    assert np.allclose(S.dense(fB(N), N - 2), B, atol=1e-12)
+
+
+# ----------------------------------------------------------------- banded
+def test_tdma_pdma_diag_vs_dense():
+    """test_la.py:7-46 matrices (N = 10) vs dense solves, along axes 0 and 1."""
+    N = 10
+    for d in ({-2: 1.0, 0: -2.0, 2: 1.0}, {-4: 0.1, -2: 1.0, 0: -2.0, 2: 1.0, 4: 0.1}):
+        D = S.dense(d, N)
+        b = np.random.default_rng(0).standard_normal((N, N))
+        if len(d) == 3:
+            lu = S.tdma_lu(S.tdma_dia(d, N))
+            u0 = S.tdma_solve(b, lu, 0)
+            u1 = S.tdma_solve(b, lu, 1)
+        else:
+            lu = S.pdma_lu(S.pdma_dia(d, N))
+            u0 = S.pdma_solve(b, lu, 0)
+            u1 = S.pdma_solve(b, lu, 1)
+        assert np.allclose(u0, np.linalg.solve(D, b))
+        assert np.allclose(u1, np.linalg.solve(D, b.T).T)
+    dg = np.arange(N) + 1.0
+    assert np.allclose(S.diagma_solve(np.ones(N), dg), 1 / dg)
+
+
+@pytest.mark.parametrize("bc", ["D", "N"])
+def test_helmholtz_lu_solve_vs_dense(bc):
+    """LU_Helmholtz + Solve_Helmholtz == dense solve of alfa A + beta B;
+    matvec(solve(b)) == b (test_matrices.py:559-592 pattern)."""
+    N = 24
+    M = N - 2
+    A = S.dense(S.cheb_A_SD(N) if bc == "D" else S.cheb_A_SN(N), M)
+    B = S.dense(S.cheb_B_SD(N) if bc == "D" else S.cheb_B_SN(N), M)
+    alfa = np.array([-1.0, -1.0, 0.5])
+    beta = np.array([2.0, 37.0, 9.0])
+    d0, d1, d2, L = S.helmholtz_lu(bc, N, alfa, beta)
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))
+    u = S.helmholtz_solve(b, d0, d1, d2, L, neumann=(bc == "N"))
+    for m in range(3):
+        H = alfa[m] * A + beta[m] * B
+        assert np.allclose(u[:M, m], np.linalg.solve(H, b[:M, m]), rtol=1e-12, atol=1e-14)
+    c = S.helmholtz_matvec(u, alfa, beta, bc)
+    assert np.allclose(c[:M], b[:M], atol=1e-12)
+
+
+def test_helmholtz_neumann_gauge():
+    """beta = 0 Neumann: u_0 is gauged to 0, the rest solves rows 1.. exactly."""
+    N = 16
+    M = N - 2
+    d0, d1, d2, L = S.helmholtz_lu("N", N, np.array([-1.0]), np.array([0.0]))
+    assert abs(d0[0, 0] - 1.0) < 1e-15
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal((N, 1))
+    u = S.helmholtz_solve(b, d0, d1, d2, L, neumann=True)
+    assert u[0, 0] == 0.0
+    A = S.dense(S.cheb_A_SN(N), M)
+    assert np.allclose((-A @ u[:M, 0])[1:], b[1:M, 0])
+
+
+def test_poisson_add_vs_dense():
+    N = 20
+    A = S.dense(S.cheb_A_SD(N), N - 2)
+    b = np.random.default_rng(3).standard_normal(N)
+    u = S.poisson_add_solve(b, N)
+    assert np.allclose(u[:N - 2], np.linalg.solve(A, b[:N - 2]))
+
+
+# ----------------------------------------------------------------- end to end
+@pytest.mark.parametrize("seed", [0, 1])
+def test_poisson2d_manufactured(seed):
+    """2D Legendre-SD x R2C Poisson: the discrete solution is exact (golden fixture)."""
+    g = np.load(os.path.join(GOLD, f"poisson2d_seed{seed}.npz"))
+    _, _, f, u, uh = S.poisson2d_manufactured(64, 64, seed)
+    assert np.allclose(f, g["f"], atol=1e-13)
+    ug, _, uhg = S.legendre_fourier_2d(g["f"], 0.0)
+    assert np.max(np.abs(uhg - g["u_hat"])) / np.max(np.abs(g["u_hat"])) < 1e-13
+    assert np.max(np.abs(ug - g["u"])) / np.max(np.abs(g["u"])) < 1e-13
+
+
+@pytest.mark.parametrize("bc", ["D", "N"])
+def test_helmholtz3d_manufactured(bc):
+    """(alpha - lap) u = f on Chebyshev x C2C x R2C: pick u in the discrete space,
+    evaluate f = alpha u - u_xx - u_yy - u_zz exactly at the nodes, recover u."""
+    Nx, Ny, Nz = 12, 8, 6
+    alpha = 2.0
+    rng = np.random.default_rng(7)
+    K2 = S.stencil_k2("chebyshev", bc, Nx)
+    uh = rng.standard_normal((Nx, Ny, Nz // 2 + 1)) + 1j * rng.standard_normal((Nx, Ny, Nz // 2 + 1))
+    uh[-2:] = 0
+    uh[:, Ny // 2] = 0  # drop the C2C Nyquist mode (its derivative is not real-symmetric)
+    uh[:, :, -1] = 0    # and the R2C Nyquist plane
+    uh[:, :, 0] = 0     # keep Hermitian symmetry trivially: only kz > 0 planes
+    if bc == "N":
+        uh[0] = 0       # gauge
+    t = S.to_ortho(uh, K2, 0)
+    x, _ = S.cheb_points_weights(Nx)
+    d2t = np.zeros_like(t)
+    d2t[:Nx - 2] = npc.chebder(t, 2, axis=0)
+    ky = S.c2c_wavenumbers(Ny)[None, :, None]
+    kz = S.r2c_wavenumbers(Nz)[None, None, :]
+
+    def synth(c):
+        g = np.fft.irfftn(c, s=(Ny, Nz), axes=(1, 2)) * Ny * Nz
+        return npc.chebval(x, np.moveaxis(g, 0, -1)).transpose(2, 0, 1) if False else \
+            np.einsum("jk,kyz->jyz", npc.chebvander(x, Nx - 1), g)
+
+    u = synth(t)
+    f = synth((alpha + ky ** 2 + kz ** 2) * t - d2t)
+    ug, _, uhg = S.helmholtz3d_cheb_fourier(f, alpha, bc)
+    assert np.max(np.abs(ug - u)) / np.max(np.abs(u)) < 1e-12
+    assert np.max(np.abs(uhg[:Nx - 2] - uh[:Nx - 2])) / np.max(np.abs(uh)) < 1e-12
+
+
+# ----------------------------------------------------------------- planted mistakes
+def test_pins_catch_planted_mistakes(monkeypatch):
+    """A wrong stencil sign or a wrong LU update must fail the pins above."""
+    orig = S.stencil_k2
+    monkeypatch.setattr(S, "stencil_k2", lambda f, b, N: -orig(f, b, N))
+    with pytest.raises(AssertionError):
+        test_stencil_satisfies_bcs("chebyshev", "D")
+    monkeypatch.setattr(S, "stencil_k2", orig)
+    orig_lu = S.helmholtz_lu
+
+    def bad_lu(*a, **k):
+        d0, d1, d2, L = orig_lu(*a, **k)
+        return d0, d1, -d2, L
+
+    monkeypatch.setattr(S, "helmholtz_lu", bad_lu)
+    with pytest.raises(AssertionError):
+        test_helmholtz_lu_solve_vs_dense("D")
