@@ -125,7 +125,8 @@ def main():
     import ctypes
     stage = (ctypes.c_float * 5)()
     _lib.call("sfa_helmholtz3d_stage_times", _lib.ptr(f), _lib.ptr(u), Nx, Ny, Nz, alpha, 0,
-              _lib.ptr(H.work), H.work.numel(), args.steps, stage, _lib.stream_ptr())
+              _lib.ptr(H.plan), _lib.ptr(H.work), H.work.numel(), args.steps, stage,
+              _lib.stream_ptr())
     names = ["z_r2c", "y_c2c", "x_pass", "y_c2c_inv", "z_c2r"]
     avg_ms = {n: stage[i] / args.steps for i, n in enumerate(names)}
     R, C, sbytes = stage_bytes(Nx, Ny, Nz)

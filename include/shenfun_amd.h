@@ -180,18 +180,28 @@ int sfa_poisson_add_solve(const void* b, void* u, int dtype, int ndim, const int
 
 /* (alpha - lap) u = f on Chebyshev(SD or SN) x C2C x R2C, f and u physical
  * (Nx, Ny, Nz) C-contiguous float64: scalar product -> Helmholtz solve
- * (alfa = -1, beta = alpha + ky^2 + kz^2, LU computed on the fly) ->
- * backward.  Equivalent to the reference sequence
- * TensorProductSpace.scalar_product (tensorproductspace.py:146-190) ->
- * chebyshev.la.Helmholtz.__call__ (chebyshev/la.py:288-317) ->
- * TensorProductSpace.backward (tensorproductspace.py:46-96).
- * precision: 0 = FP64 throughout; 1 = mixed (complex64 intermediates, FP64
+ * (alfa = -1, beta = alpha + ky^2 + kz^2) -> backward.  Equivalent to the
+ * reference sequence TensorProductSpace.scalar_product
+ * (tensorproductspace.py:146-190) -> chebyshev.la.Helmholtz.__call__
+ * (chebyshev/la.py:288-317) -> TensorProductSpace.backward
+ * (tensorproductspace.py:46-96).
+ *
+ * Planning (the counterpart of chebyshev.la.Helmholtz.__init__, which
+ * factorises once per solver, chebyshev/la.py:223-276): sfa_helmholtz3d_plan
+ * fills a device buffer of sfa_helmholtz3d_plan_bytes() bytes with the LU
+ * factors of every distinct beta and the line -> entry map, synchronising
+ * `stream`.  plan may be NULL in sfa_helmholtz3d, in which case the factors
+ * are computed on the fly (wavefront scan of the LU recurrence).
+ * precision: 0 = FP64 throughout; 1 = mixed (complex64 transforms, FP64
  * solve and FP64 residual refinement, `refine` sweeps).
  * work: device workspace of at least sfa_helmholtz3d_workspace() bytes. */
+int64_t sfa_helmholtz3d_plan_bytes(int64_t Nx, int64_t Ny, int64_t Nz);
+int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double alpha, int neumann,
+                         void* plan, int64_t plan_bytes, void* stream);
 int64_t sfa_helmholtz3d_workspace(int64_t Nx, int64_t Ny, int64_t Nz, int precision);
 int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
-                    double alpha, int neumann, int precision, int refine, void* work,
-                    int64_t work_bytes, void* stream);
+                    double alpha, int neumann, int precision, int refine, const void* plan,
+                    void* work, int64_t work_bytes, void* stream);
 
 /* Profiling variant of sfa_helmholtz3d (FP64): runs the pipeline nrep times
  * with hipEvents recorded on `stream` between its kernels and adds each
@@ -199,8 +209,8 @@ int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t 
  * inverse, z C2R).  Synchronises the stream.  No reference counterpart: it
  * exists to attribute the pipeline's time to its kernels. */
 int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
-                                double alpha, int neumann, void* work, int64_t work_bytes,
-                                int nrep, float* stage_ms, void* stream);
+                                double alpha, int neumann, const void* plan, void* work,
+                                int64_t work_bytes, int nrep, float* stage_ms, void* stream);
 
 #ifdef __cplusplus
 }

@@ -38,8 +38,9 @@ _SIGS = {
     "sfa_helmholtz_solve": "vviiaaaiivvvvav",
     "sfa_helmholtz_matvec": "vviiaaaiivvvvvvvlv",
     "sfa_poisson_add_solve": "vviiaaaivvdv",
-    "sfa_helmholtz3d": "vvllldiiivlv",
-    "sfa_helmholtz3d_stage_times": "vvllldivlivv",
+    "sfa_helmholtz3d_plan": "llldivlv",
+    "sfa_helmholtz3d": "vvllldiiivvlv",
+    "sfa_helmholtz3d_stage_times": "vvllldivvlivv",
 }
 
 _TYPES = {"v": _c_void_p, "i": _int, "l": _i64, "d": _dbl, "a": _p64}
@@ -65,12 +66,15 @@ def lib():
         L.sfa_version.restype = _int
         L.sfa_helmholtz3d_workspace.restype = _i64
         L.sfa_helmholtz3d_workspace.argtypes = [_i64, _i64, _i64, _int]
+        L.sfa_helmholtz3d_plan_bytes.restype = _i64
+        L.sfa_helmholtz3d_plan_bytes.argtypes = [_i64, _i64, _i64]
         _lib = L
     return _lib
 
 
 def exported_symbols():
-    return sorted(list(_SIGS) + ["sfa_version", "sfa_last_error", "sfa_helmholtz3d_workspace"])
+    return sorted(list(_SIGS) + ["sfa_version", "sfa_last_error", "sfa_helmholtz3d_workspace",
+                                 "sfa_helmholtz3d_plan_bytes"])
 
 
 def arr(vals):

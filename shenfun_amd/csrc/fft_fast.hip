@@ -1,0 +1,431 @@
+// Register-resident power-of-two FFT kernels (FP64) for the hot shapes:
+//   * C2C along a strided axis whose neighbour dimension is contiguous
+//     (the y axis of (Nx, Ny, Nz/2+1)): 16 lines per block, lanes run over
+//     the lines so every global access is a 256-byte row segment;
+//   * R2C / C2R along the contiguous axis (z): rows packed as N/2 complex.
+// Also the v2 fused x-pass (DCT-II -> stencil -> Helmholtz scan solve ->
+// to_ortho -> DCT-III) on 16 flattened (ky, kz) lines per block.
+#include "fft_pow2.h"
+#include "helm_scan.h"
+
+namespace sfa {
+
+constexpr int TL = 16;  // lines per block
+
+// ----------------------------------------------------------------- C2C strided
+template <int N, bool INV>
+__global__ void __launch_bounds__(TL * Pow2<N>::Q)
+    c2c_strided_kernel(const double2* in, double2* out, int64_t in_sa,
+                       int64_t in_s1, int64_t in_s2, int64_t out_sa, int64_t out_s1,
+                       int64_t out_s2, int64_t d2, int64_t nlines, double scale,
+                       const double2* __restrict__ twg)
+{
+    using Pl = Pow2<N>;
+    constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
+    extern __shared__ double2 sm[];
+    double2* tw = sm;
+    double2* ex = sm + N;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int t = tid % TL, j = tid / TL;
+    const int64_t L = (int64_t)blockIdx.x * TL + t;
+    const bool valid = L < nlines;
+    const int64_t i1 = valid ? L / d2 : 0, i2 = valid ? L - i1 * d2 : 0;
+    for (int i = tid; i < N; i += nth) tw[i] = twg[i];
+    double2 a[P];
+    const double2* src = in + i1 * in_s1 + i2 * in_s2;
+#pragma unroll
+    for (int r = 0; r < P; ++r) a[r] = valid ? src[(int64_t)(j + Q * r) * in_sa] : make_double2(0.0, 0.0);
+    __syncthreads();
+    pow2_fft<N, INV>(a, j, ex + t * LS, tw);
+    if (valid) {
+        double2* dst = out + i1 * out_s1 + i2 * out_s2;
+#pragma unroll
+        for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+            for (int k2 = 0; k2 < Q; ++k2)
+                dst[(int64_t)pow2_out_index<N>(j, h, k2) * out_sa] = cscale(a[h * Q + k2], scale);
+    }
+}
+
+// ----------------------------------------------------------------- R2C contiguous
+// rows of 2H reals (stride in_rs), output rows of H+1 complex (stride out_rs).
+template <int H>
+__global__ void __launch_bounds__(TL * Pow2<H>::Q)
+    r2c_contig_kernel(const double* __restrict__ in, double2* __restrict__ out, int64_t in_rs,
+                      int64_t out_rs, int64_t d2, int64_t in_s1, int64_t out_s1, int64_t nrows,
+                      double scale, const double2* __restrict__ twg /* 2H entries */)
+{
+    using Pl = Pow2<H>;
+    constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
+    constexpr int N = 2 * H;
+    extern __shared__ double2 sm[];
+    double2* tw = sm;          // exp(-2 pi i m / N), m < N
+    double2* ex = sm + N;      // TL lines x LS
+    double2* twH = ex + TL * LS;  // exp(-2 pi i m / H), m < H
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int j = tid % Q, t = tid / Q;  // j fastest: contiguous row segments
+    const int64_t R = (int64_t)blockIdx.x * TL + t;
+    const bool valid = R < nrows;
+    const int64_t i1 = valid ? R / d2 : 0, i2 = valid ? R - i1 * d2 : 0;
+    for (int i = tid; i < N; i += nth) {
+        tw[i] = twg[i];
+        if ((i & 1) == 0) twH[i >> 1] = twg[i];
+    }
+    double2 a[P];
+    const double* src = in + i1 * in_s1 + i2 * in_rs;
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        const int m = j + Q * r;
+        a[r] = valid ? *reinterpret_cast<const double2*>(src + 2 * m) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    double2* ln = ex + t * LS;
+    pow2_fft<H, false>(a, j, ln, twH);
+#pragma unroll
+    for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<H>(j, h, k2)] = a[h * Q + k2];
+    __syncthreads();
+    if (valid) {
+        double2* dst = out + i1 * out_s1 + i2 * out_rs;
+        for (int k = j; k <= H; k += Q) {
+            const double2 zk = ln[k == H ? 0 : k];
+            const double2 zc = cconj(ln[k == 0 ? 0 : H - k]);
+            const double2 d = cmul(tw[k], csub(zk, zc));
+            dst[k] = make_double2(0.5 * (zk.x + zc.x + d.y) * scale, 0.5 * (zk.y + zc.y - d.x) * scale);
+        }
+    }
+}
+
+template <int H>
+__global__ void __launch_bounds__(TL * Pow2<H>::Q)
+    c2r_contig_kernel(const double2* __restrict__ in, double* __restrict__ out, int64_t in_rs,
+                      int64_t out_rs, int64_t d2, int64_t in_s1, int64_t out_s1, int64_t nrows,
+                      double scale, const double2* __restrict__ twg /* 2H entries */)
+{
+    using Pl = Pow2<H>;
+    constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
+    constexpr int N = 2 * H;
+    extern __shared__ double2 sm[];
+    double2* tw = sm;
+    double2* ex = sm + N;
+    double2* twH = ex + TL * LS;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int j = tid % Q, t = tid / Q;
+    const int64_t R = (int64_t)blockIdx.x * TL + t;
+    const bool valid = R < nrows;
+    const int64_t i1 = valid ? R / d2 : 0, i2 = valid ? R - i1 * d2 : 0;
+    for (int i = tid; i < N; i += nth) {
+        tw[i] = twg[i];
+        if ((i & 1) == 0) twH[i >> 1] = twg[i];
+    }
+    double2* ln = ex + t * LS;
+    const double2* src = in + i1 * in_s1 + i2 * in_rs;
+    for (int k = j; k <= H; k += Q) {
+        double2 v = valid ? src[k] : make_double2(0.0, 0.0);
+        if (k == 0 || k == H) v.y = 0.0;
+        ln[k] = v;
+    }
+    __syncthreads();
+    double2 a[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        const int k = j + Q * r;
+        const double2 xk = ln[k];
+        const double2 xc = cconj(ln[H - k]);
+        const double2 d = cmul(cconj(tw[k]), csub(xk, xc));
+        a[r] = make_double2(xk.x + xc.x - d.y, xk.y + xc.y + d.x);
+    }
+    __syncthreads();
+    pow2_fft<H, true>(a, j, ln, twH);
+    if (valid) {
+        double* dst = out + i1 * out_s1 + i2 * out_rs;
+#pragma unroll
+        for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+            for (int k2 = 0; k2 < Q; ++k2) {
+                const int m = pow2_out_index<H>(j, h, k2);
+                const double2 z = a[h * Q + k2];
+                *reinterpret_cast<double2*>(dst + 2 * m) = make_double2(z.x * scale, z.y * scale);
+            }
+    }
+}
+
+// ----------------------------------------------------------------- host dispatch
+template <int N, bool INV>
+static int launch_c2c_strided(const Geom& gi, const Geom& go, const void* in, void* out,
+                              double scale, hipStream_t s)
+{
+    using Pl = Pow2<N>;
+    const double2* tw = fft_table(N);
+    SFA_CHECK(tw, SFA_ERR_HIP, "twiddle table allocation failed");
+    const int64_t nl = gi.d1 * gi.d2;
+    const size_t lds = sizeof(double2) * ((size_t)N + (size_t)TL * Pl::LS);
+    auto k = c2c_strided_kernel<N, INV>;
+    if (lds > 64 * 1024)
+        hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, dim3((unsigned)((nl + TL - 1) / TL)), dim3(TL * Pl::Q), lds, s,
+                       (const double2*)in, (double2*)out, gi.sa, gi.s1, gi.s2, go.sa, go.s1, go.s2,
+                       gi.d2, nl, scale, tw);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+// returns 1 if handled (rc in *rc), 0 if the shape needs the generic kernel
+int fast_c2c(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
+             hipStream_t s, int* rc)
+{
+    const int64_t n = gi.n;
+    // strided axis with a contiguous neighbour dim (lines run along s2 == 1)
+    if (!(gi.s2 == 1 && go.s2 == 1 && gi.sa != 1 && go.sa != 1)) return 0;
+    const bool inv = sign > 0;
+#define SFA_C2C(NN)                                                                       \
+    case NN:                                                                              \
+        *rc = inv ? launch_c2c_strided<NN, true>(gi, go, in, out, scale, s)               \
+                  : launch_c2c_strided<NN, false>(gi, go, in, out, scale, s);             \
+        return 1;
+    switch (n) {
+        SFA_C2C(16) SFA_C2C(32) SFA_C2C(64) SFA_C2C(128) SFA_C2C(256) SFA_C2C(512)
+        default: return 0;
+    }
+#undef SFA_C2C
+}
+
+template <int H, bool R2C>
+static int launch_rc(const Geom& gi, const Geom& go, const void* in, void* out, double scale,
+                     hipStream_t s)
+{
+    using Pl = Pow2<H>;
+    const double2* tw = fft_table(2 * H);
+    SFA_CHECK(tw, SFA_ERR_HIP, "twiddle table allocation failed");
+    const int64_t nr = gi.d1 * gi.d2;
+    const size_t lds = sizeof(double2) * ((size_t)2 * H + (size_t)TL * Pl::LS + H);
+    dim3 grid((unsigned)((nr + TL - 1) / TL)), block(TL * Pl::Q);
+    if (R2C) {
+        auto k = r2c_contig_kernel<H>;
+        if (lds > 64 * 1024)
+            hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k, grid, block, lds, s, (const double*)in, (double2*)out, gi.s2, go.s2,
+                           gi.d2, gi.s1, go.s1, nr, scale, tw);
+    } else {
+        auto k = c2r_contig_kernel<H>;
+        if (lds > 64 * 1024)
+            hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k, grid, block, lds, s, (const double2*)in, (double*)out, gi.s2, go.s2,
+                           gi.d2, gi.s1, go.s1, nr, scale, tw);
+    }
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+// real length n along a contiguous axis (sa == 1 on both sides)
+int fast_rc(bool r2c, const Geom& gi, const Geom& go, const void* in, void* out, double scale,
+            hipStream_t s, int* rc)
+{
+    if (!(gi.sa == 1 && go.sa == 1)) return 0;
+    // rows must not overlap the packed complex loads: real rows 16-byte aligned
+    const int64_t H = gi.n / 2;
+    const int64_t real_rs = r2c ? gi.s2 : go.s2;
+    const int64_t real_s1 = r2c ? gi.s1 : go.s1;
+    if ((real_rs & 1) || (real_s1 & 1)) return 0;
+#define SFA_RC(HH)                                                                            \
+    case HH:                                                                                  \
+        *rc = r2c ? launch_rc<HH, true>(gi, go, in, out, scale, s)                            \
+                  : launch_rc<HH, false>(gi, go, in, out, scale, s);                          \
+        return 1;
+    switch (H) {
+        SFA_RC(16) SFA_RC(32) SFA_RC(64) SFA_RC(128) SFA_RC(256)
+        default: return 0;
+    }
+#undef SFA_RC
+}
+
+}  // namespace sfa
+
+namespace sfa {
+
+struct XPass2Args {
+    double2* W;
+    int64_t sa;     // x stride (= Ny * Nzh); lines L = ky * Nzh + kz have stride 1
+    int64_t nl;     // Ny * Nzh
+    int Nzh, Ny;
+    int neumann;
+    double alpha;
+    const double* K2;
+    const int* linemap;  // line -> factor table entry
+    const double* fac;   // per-entry, per-parity row factors (K rows x 4)
+    int K;
+    const double2* tw;   // exp(-2 pi i m/N)
+    const double2* tq;   // exp(-i pi k/(2N))
+};
+
+template <int N>
+__global__ void __launch_bounds__(TL * Pow2<N>::Q) xpass2_kernel(XPass2Args a)
+{
+    using Pl = Pow2<N>;
+    constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
+    constexpr int M = N - 2;
+    constexpr int K = (M + 1) / 2;  // rows of the even chain
+    constexpr int E = (K + 15) / 16;
+    constexpr int NTH = TL * Q;
+    constexpr int ROUNDS = (2 * TL * 16) / NTH;
+    extern __shared__ double2 sm[];
+    double2* tw = sm;
+    double2* ex = sm + N;
+    const int tid = threadIdx.x;
+    const int t = tid % TL, j = tid / TL;
+    const int64_t L = (int64_t)blockIdx.x * TL + t;
+    const bool valid = L < a.nl;
+    for (int i = tid; i < N; i += NTH) tw[i] = a.tw[i];
+    // 1. load with the DCT-II even/odd permutation: v[m] = x[2m] (m < N/2), x[2(N-1-m)+1]
+    double2 v[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        const int m = j + Q * r;
+        const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
+        v[r] = valid ? a.W[(int64_t)x * a.sa + L] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    double2* ln = ex + t * LS;
+    pow2_fft<N, false>(v, j, ln, tw);
+#pragma unroll
+    for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<N>(j, h, k2)] = v[h * Q + k2];
+    __syncthreads();
+    // 2a. DCT-II post-processing X_k = sc (Re(w (V_k + conj V_{N-k})) + i Im(w (V_k - conj V_{N-k})))
+    //     computed for this thread's k (registers), then written back in place
+    const double sc = 3.14159265358979323846 / (2.0 * N);
+#pragma unroll
+    for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) {
+            const int k = pow2_out_index<N>(j, h, k2);
+            const double2 Pv = v[h * Q + k2];
+            const double2 Qv = cconj(ln[k == 0 ? 0 : N - k]);
+            const double2 w = a.tq[k];
+            const double2 s1 = cmul(w, cadd(Pv, Qv));
+            const double2 s2 = cmul(w, csub(Pv, Qv));
+            v[h * Q + k2] = make_double2(s1.x * sc, s2.y * sc);
+        }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<N>(j, h, k2)] = v[h * Q + k2];
+    __syncthreads();
+    // 2b. stencil + Helmholtz solve with the planned factors, 16 lanes per (line, parity)
+#pragma unroll 1
+    for (int rho = 0; rho < ROUNDS; ++rho) {
+        const int c = rho * (NTH / 16) + tid / 16;
+        const int ct = c >> 1, p = c & 1, lane = tid & 15;
+        double2* X = ex + ct * LS;
+        const int64_t Lc = (int64_t)blockIdx.x * TL + ct;
+        const int u = (Lc < a.nl) ? a.linemap[Lc] : 0;
+        const double* fc = a.fac + ((int64_t)2 * u + p) * a.K * 4;
+        double2 f[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = 2 * (lane * E + e) + p;
+            if (i < M) {
+                const double2 x0 = X[i], x2 = X[i + 2];
+                const double kk = a.K2[i];
+                f[e] = make_double2(x0.x + kk * x2.x, x0.y + kk * x2.y);
+            } else {
+                f[e] = make_double2(0.0, 0.0);
+            }
+        }
+        helm_chain_solve_fac<E, 16>(f, X, p, M, a.neumann != 0, fc, a.K, lane);
+        if (p == 0 && lane == 0) {
+            X[M] = make_double2(0.0, 0.0);
+            X[M + 1] = make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+    // 3a. to_ortho t_k = u_k (k < N-2) + K2_{k-2} u_{k-2}, for this thread's k = j + Q r
+    const double2 t0 = ln[0];
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        const int k = j + Q * r;
+        double2 c = (k < N - 2) ? ln[k] : make_double2(0.0, 0.0);
+        if (k >= 2) {
+            const double kk = a.K2[k - 2];
+            const double2 uu = ln[k - 2];
+            c.x += kk * uu.x;
+            c.y += kk * uu.y;
+        }
+        v[r] = c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < P; ++r) ln[j + Q * r] = v[r];
+    __syncthreads();
+    // 3b. DCT-III pre-processing V_k = conj(w_k) ((a_k + b_{N-k}) + i (b_k - a_{N-k}))
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        const int k = j + Q * r;
+        const double2 cn = (k == 0) ? make_double2(0.0, 0.0) : ln[N - k];
+        const double2 z = make_double2(v[r].x + cn.y, v[r].y - cn.x);
+        v[r] = cmul(cconj(a.tq[k]), z);
+    }
+    __syncthreads();
+    pow2_fft<N, true>(v, j, ln, tw);
+    // 4. store 0.5 y + t_0/2 with the inverse permutation
+    if (valid) {
+#pragma unroll
+        for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+            for (int k2 = 0; k2 < Q; ++k2) {
+                const int m = pow2_out_index<N>(j, h, k2);
+                const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
+                const double2 y = v[h * Q + k2];
+                a.W[(int64_t)x * a.sa + L] = make_double2(0.5 * y.x + 0.5 * t0.x, 0.5 * y.y + 0.5 * t0.y);
+            }
+    }
+}
+
+template <int N>
+static int launch_xpass2(XPass2Args& a, hipStream_t s)
+{
+    using Pl = Pow2<N>;
+    const size_t lds = sizeof(double2) * ((size_t)N + (size_t)TL * Pl::LS);
+    auto k = xpass2_kernel<N>;
+    if (lds > 64 * 1024)
+        hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, dim3((unsigned)((a.nl + TL - 1) / TL)), dim3(TL * Pl::Q), lds, s, a);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+// returns 1 if handled
+int fast_xpass(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, const double* K2,
+               const int* linemap, const double* fac, hipStream_t s, int* rc)
+{
+    if (!(Nx == 64 || Nx == 128 || Nx == 256) || linemap == nullptr || fac == nullptr) return 0;
+    XPass2Args a;
+    a.W = W;
+    a.sa = (int64_t)Ny * Nzh;
+    a.nl = (int64_t)Ny * Nzh;
+    a.Nzh = Nzh;
+    a.Ny = Ny;
+    a.neumann = neumann;
+    a.alpha = alpha;
+    a.K2 = K2;
+    a.linemap = linemap;
+    a.fac = fac;
+    a.K = (Nx - 2 + 1) / 2;
+    a.tw = fft_table(Nx);
+    a.tq = quarter_table(Nx);
+    if (!a.tw || !a.tq) {
+        *rc = fail(SFA_ERR_HIP, "table allocation failed");
+        return 1;
+    }
+    switch (Nx) {
+        case 64: *rc = launch_xpass2<64>(a, s); break;
+        case 128: *rc = launch_xpass2<128>(a, s); break;
+        default: *rc = launch_xpass2<256>(a, s); break;
+    }
+    return 1;
+}
+
+}  // namespace sfa

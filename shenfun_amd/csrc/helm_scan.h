@@ -94,14 +94,17 @@ __device__ inline void m3norm(M3& x)
     }
 }
 
-__device__ inline double shup(double v, int d) { return __shfl_up(v, d, 16); }
-__device__ inline double shdn(double v, int d) { return __shfl_down(v, d, 16); }
+template <int W>
+__device__ inline double shup(double v, int d) { return __shfl_up(v, d, W); }
+template <int W>
+__device__ inline double shdn(double v, int d) { return __shfl_down(v, d, W); }
 
-// Solve one chain in place.  X: line in LDS (rows i = 2m + p hold f on entry,
-// the solution on exit).  lane in [0, 16).  K = number of rows of parity p.
-template <int E>
-__device__ inline void helm_chain_scan(double2* X, int p, int M, double al, double be, bool nm,
-                                       const HelmTab& tab, int lane)
+// Solve one chain.  f: right-hand side of this lane's rows i = 2 (lane E + e) + p
+// (zero for i >= M); X: the line (LDS), solution rows written there.
+// W lanes (aligned group) per chain; lane in [0, W).
+template <int E, int W = 16>
+__device__ inline void helm_chain_scan_f(const double2 (&f)[E], double2* X, int p, int M, double al,
+                                         double be, bool nm, const HelmTab& tab, int lane)
 {
     const bool zb = nm && fabs(be) < 1e-8;
     const int m0 = lane * E;
@@ -120,17 +123,17 @@ __device__ inline void helm_chain_scan(double2* X, int p, int M, double al, doub
     }
     m3norm(seg);
 #pragma unroll
-    for (int k = 1; k < 16; k <<= 1) {
+    for (int k = 1; k < W; k <<= 1) {
         M3 o;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) o.a[i] = shup(seg.a[i], k);
+        for (int i = 0; i < 9; ++i) o.a[i] = shup<W>(seg.a[i], k);
         if (lane >= k) {
             seg = m3mul(seg, o);
             m3norm(seg);
         }
     }
     // exclusive prefix applied to zeta_{-1} = (1, 0, 0): first column of lane-1's product
-    double z0 = shup(seg.a[0], 1), z1 = shup(seg.a[3], 1), z2 = shup(seg.a[6], 1);
+    double z0 = shup<W>(seg.a[0], 1), z1 = shup<W>(seg.a[3], 1), z2 = shup<W>(seg.a[6], 1);
     if (lane == 0) { z0 = 1.0; z1 = 0.0; z2 = 0.0; }
     double r1 = z1 / z0, r2 = z2 / z0;
     double inv[E], e1[E], e2[E], am[E];
@@ -151,7 +154,7 @@ __device__ inline void helm_chain_scan(double2* X, int p, int M, double al, doub
         lastinv = iv;
     }
     {
-        double previnv = shup(lastinv, 1);
+        double previnv = shup<W>(lastinv, 1);
         if (lane == 0) previnv = 0.0;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -159,13 +162,7 @@ __device__ inline void helm_chain_scan(double2* X, int p, int M, double al, doub
             previnv = inv[e];
         }
     }
-    // ---- 2. forward elimination (affine scan), f from LDS
-    double2 f[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int i = 2 * (m0 + e) + p;
-        f[e] = (i < M) ? X[i] : make_double2(0.0, 0.0);
-    }
+    // ---- 2. forward elimination (affine scan)
     double A = 1.0;
     double2 B = make_double2(0.0, 0.0);
 #pragma unroll
@@ -174,14 +171,14 @@ __device__ inline void helm_chain_scan(double2* X, int p, int M, double al, doub
         B = make_double2(am[e] * B.x + f[e].x, am[e] * B.y + f[e].y);
     }
 #pragma unroll
-    for (int k = 1; k < 16; k <<= 1) {
-        const double oA = shup(A, k), oBx = shup(B.x, k), oBy = shup(B.y, k);
+    for (int k = 1; k < W; k <<= 1) {
+        const double oA = shup<W>(A, k), oBx = shup<W>(B.x, k), oBy = shup<W>(B.y, k);
         if (lane >= k) {
             B = make_double2(A * oBx + B.x, A * oBy + B.y);
             A = A * oA;
         }
     }
-    double2 y = make_double2(shup(B.x, 1), shup(B.y, 1));
+    double2 y = make_double2(shup<W>(B.x, 1), shup<W>(B.y, 1));
     if (lane == 0) y = make_double2(0.0, 0.0);
     double2 yp[E];
 #pragma unroll
@@ -205,11 +202,11 @@ __device__ inline void helm_chain_scan(double2* X, int p, int M, double al, doub
         c1 = nc1;
     }
 #pragma unroll
-    for (int k = 1; k < 16; k <<= 1) {
-        const double o00 = shdn(P00, k), o01 = shdn(P01, k), o10 = shdn(P10, k), o11 = shdn(P11, k);
-        const double oc0x = shdn(c0.x, k), oc0y = shdn(c0.y, k), oc1x = shdn(c1.x, k),
-                     oc1y = shdn(c1.y, k);
-        if (lane + k < 16) {
+    for (int k = 1; k < W; k <<= 1) {
+        const double o00 = shdn<W>(P00, k), o01 = shdn<W>(P01, k), o10 = shdn<W>(P10, k), o11 = shdn<W>(P11, k);
+        const double oc0x = shdn<W>(c0.x, k), oc0y = shdn<W>(c0.y, k), oc1x = shdn<W>(c1.x, k),
+                     oc1y = shdn<W>(c1.y, k);
+        if (lane + k < W) {
             // (P, c) o (O, oc) = (P O, P oc + c)
             const double2 nc0 = make_double2(P00 * oc0x + P01 * oc1x + c0.x, P00 * oc0y + P01 * oc1y + c0.y);
             const double2 nc1 = make_double2(P10 * oc0x + P11 * oc1x + c1.x, P10 * oc0y + P11 * oc1y + c1.y);
@@ -220,9 +217,9 @@ __device__ inline void helm_chain_scan(double2* X, int p, int M, double al, doub
             c1 = nc1;
         }
     }
-    double2 sv = make_double2(shdn(c0.x, 1), shdn(c0.y, 1));
-    double2 sT = make_double2(shdn(c1.x, 1), shdn(c1.y, 1));
-    if (lane == 15) {
+    double2 sv = make_double2(shdn<W>(c0.x, 1), shdn<W>(c0.y, 1));
+    double2 sT = make_double2(shdn<W>(c1.x, 1), shdn<W>(c1.y, 1));
+    if (lane == W - 1) {
         sv = make_double2(0.0, 0.0);
         sT = make_double2(0.0, 0.0);
     }
@@ -249,6 +246,181 @@ __device__ inline void helm_chain_scan(double2* X, int p, int M, double al, doub
             X[i] = out;
         }
     }
+}
+
+// Variant reading the right-hand side from X (rows i = 2m + p) first.
+template <int E, int W = 16>
+__device__ inline void helm_chain_scan(double2* X, int p, int M, double al, double be, bool nm,
+                                       const HelmTab& tab, int lane)
+{
+    double2 f[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = 2 * (lane * E + e) + p;
+        f[e] = (i < M) ? X[i] : make_double2(0.0, 0.0);
+    }
+    helm_chain_scan_f<E, W>(f, X, p, M, al, be, nm, tab, lane);
+}
+
+}  // namespace sfa
+
+namespace sfa {
+
+// Solve one chain with precomputed per-row factors fc[4m + {0,1,2,3}] =
+// (1/d0, d1/d0, d2/d0, -L_{m-1}) for the chain's K rows (row 0 slot 3 holds the
+// Neumann gauge flag instead).  Forward elimination and back substitution as
+// the affine scans of helm_chain_scan_f; no division on the device.
+template <int E, int W>
+__device__ inline void helm_chain_solve_fac(const double2 (&f)[E], double2* X, int p, int M, bool nm,
+                                            const double* __restrict__ fc, int K, int lane)
+{
+    const int m0 = lane * E;
+    double inv[E], e1[E], e2[E], am[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int m = m0 + e;
+        if (m < K) {
+            const double2 q0 = *reinterpret_cast<const double2*>(fc + 4 * m);
+            const double2 q1 = *reinterpret_cast<const double2*>(fc + 4 * m + 2);
+            inv[e] = q0.x;
+            e1[e] = q0.y;
+            e2[e] = q1.x;
+            am[e] = q1.y;
+        } else {
+            inv[e] = 1.0;
+            e1[e] = e2[e] = am[e] = 0.0;
+        }
+    }
+    const bool gauge = nm && m0 == 0 && am[0] != 0.0;
+    if (m0 == 0) am[0] = 0.0;
+    double A = 1.0;
+    double2 B = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        A = am[e] * A;
+        B = make_double2(am[e] * B.x + f[e].x, am[e] * B.y + f[e].y);
+    }
+#pragma unroll
+    for (int k = 1; k < W; k <<= 1) {
+        const double oA = shup<W>(A, k), oBx = shup<W>(B.x, k), oBy = shup<W>(B.y, k);
+        if (lane >= k) {
+            B = make_double2(A * oBx + B.x, A * oBy + B.y);
+            A = A * oA;
+        }
+    }
+    double2 y = make_double2(shup<W>(B.x, 1), shup<W>(B.y, 1));
+    if (lane == 0) y = make_double2(0.0, 0.0);
+    double2 yp[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        y = make_double2(f[e].x + am[e] * y.x, f[e].y + am[e] * y.y);
+        yp[e] = make_double2(y.x * inv[e], y.y * inv[e]);
+    }
+    double P00 = 1.0, P01 = 0.0, P10 = 0.0, P11 = 1.0;
+    double2 c0 = make_double2(0.0, 0.0), c1 = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int e = E - 1; e >= 0; --e) {
+        const double q = -(e1[e] - e2[e]), w = -e2[e], w1 = 1.0 - e2[e];
+        const double n00 = q * P00 + w * P10, n01 = q * P01 + w * P11;
+        const double n10 = q * P00 + w1 * P10, n11 = q * P01 + w1 * P11;
+        const double2 nc0 = make_double2(q * c0.x + w * c1.x + yp[e].x, q * c0.y + w * c1.y + yp[e].y);
+        const double2 nc1 = make_double2(q * c0.x + w1 * c1.x + yp[e].x, q * c0.y + w1 * c1.y + yp[e].y);
+        P00 = n00; P01 = n01; P10 = n10; P11 = n11;
+        c0 = nc0;
+        c1 = nc1;
+    }
+#pragma unroll
+    for (int k = 1; k < W; k <<= 1) {
+        const double o00 = shdn<W>(P00, k), o01 = shdn<W>(P01, k), o10 = shdn<W>(P10, k), o11 = shdn<W>(P11, k);
+        const double oc0x = shdn<W>(c0.x, k), oc0y = shdn<W>(c0.y, k), oc1x = shdn<W>(c1.x, k),
+                     oc1y = shdn<W>(c1.y, k);
+        if (lane + k < W) {
+            const double2 nc0 = make_double2(P00 * oc0x + P01 * oc1x + c0.x, P00 * oc0y + P01 * oc1y + c0.y);
+            const double2 nc1 = make_double2(P10 * oc0x + P11 * oc1x + c1.x, P10 * oc0y + P11 * oc1y + c1.y);
+            const double n00 = P00 * o00 + P01 * o10, n01 = P00 * o01 + P01 * o11;
+            const double n10 = P10 * o00 + P11 * o10, n11 = P10 * o01 + P11 * o11;
+            P00 = n00; P01 = n01; P10 = n10; P11 = n11;
+            c0 = nc0;
+            c1 = nc1;
+        }
+    }
+    double2 sv = make_double2(shdn<W>(c0.x, 1), shdn<W>(c0.y, 1));
+    double2 sT = make_double2(shdn<W>(c1.x, 1), shdn<W>(c1.y, 1));
+    if (lane == W - 1) {
+        sv = make_double2(0.0, 0.0);
+        sT = make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int e = E - 1; e >= 0; --e) {
+        const double q = -(e1[e] - e2[e]), w = -e2[e], w1 = 1.0 - e2[e];
+        const double2 v = make_double2(q * sv.x + w * sT.x + yp[e].x, q * sv.y + w * sT.y + yp[e].y);
+        const double2 T = make_double2(q * sv.x + w1 * sT.x + yp[e].x, q * sv.y + w1 * sT.y + yp[e].y);
+        sv = v;
+        sT = T;
+        const int i = 2 * (m0 + e) + p;
+        if (i < M) {
+            double2 out = v;
+            if (nm) {
+                if (i == 0) {
+                    if (gauge) out = make_double2(0.0, 0.0);
+                } else {
+                    const double s = 1.0 / ((double)i * i);
+                    out = make_double2(out.x * s, out.y * s);
+                }
+            }
+            X[i] = out;
+        }
+    }
+}
+
+// One thread per (table entry, parity): the exact row recurrence of
+// LU_Helmholtz_1D (la.pyx:751-763) with alfa = -1, beta = alpha + s[u],
+// stored as per-row factors (1/d0, d1/d0, d2/d0, -L) in fac[(2u + p) K + m].
+static __global__ void helm_factor_kernel(HelmTab tab, int M, int K, int neumann, double alpha,
+                                   const double* __restrict__ svals, int64_t nu,
+                                   double* __restrict__ fac)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= 2 * nu) return;
+    const int64_t u = gid >> 1;
+    const int p = (int)(gid & 1);
+    const double al = -1.0, be = alpha + svals[u];
+    const bool nm = neumann != 0;
+    const bool zb = nm && fabs(be) < 1e-8;
+    double* fc = fac + (2 * u + p) * (int64_t)K * 4;
+    double d0p = 1.0, d1p = 0.0, d2p = 0.0, d00 = 0.0;
+    for (int m = 0; m < K; ++m) {
+        const int i = 2 * m + p;
+        if (i >= M) {
+            fc[4 * m] = 1.0;
+            fc[4 * m + 1] = fc[4 * m + 2] = fc[4 * m + 3] = 0.0;
+            continue;
+        }
+        const HelmRow r = helm_row(tab, i, M, al, be, nm, zb);
+        double d0, d1, d2, a;
+        if (m == 0) {
+            d0 = r.h;
+            d1 = r.s;
+            d2 = r.c;
+            a = 0.0;
+            d00 = d0;
+        } else {
+            const double l = r.g / d0p;
+            d0 = r.h - l * d1p;
+            d1 = (i < M - 2) ? r.s - l * d2p : 0.0;
+            d2 = (i < M - 4) ? r.c - l * d2p : 0.0;
+            a = -l;
+        }
+        const double iv = 1.0 / d0;
+        fc[4 * m] = iv;
+        fc[4 * m + 1] = d1 * iv;
+        fc[4 * m + 2] = d2 * iv;
+        fc[4 * m + 3] = a;
+        d0p = d0;
+        d1p = d1;
+        d2p = d2;
+    }
+    if (p == 0) fc[3] = (nm && (d00 - 1.0) * (d00 - 1.0) < 1e-16) ? 1.0 : 0.0;
 }
 
 }  // namespace sfa

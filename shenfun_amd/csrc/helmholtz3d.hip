@@ -11,6 +11,7 @@
 //   z: C2R                         W -> u
 #include <math.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -229,8 +230,92 @@ static const double* cheb_k2(int N, int neumann)
     return d;
 }
 
-int xpass_launch(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, hipStream_t s)
+int fast_xpass(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, const double* K2,
+               const int* linemap, const double* fac, hipStream_t s, int* rc);
+
+// ------------------------------------------------------------------ plan
+// Device plan of the fused solve: the Helmholtz LU factors of every distinct
+// beta = alpha + ky^2 + kz^2 (the counterpart of chebyshev.la.Helmholtz's
+// factor arrays, chebyshev/la.py:270-276, deduplicated), and the map from a
+// (ky, kz) line to its entry.  Layout (bytes): linemap int32[nl] | s double[nu] |
+// fac double[nu][2][K][4].
+struct PlanLayout {
+    int64_t nl, nu, K, off_s, off_fac, bytes;
+};
+
+static int64_t round256(int64_t b) { return (b + 255) / 256 * 256; }
+
+static void plan_shape(int64_t Nx, int64_t Ny, int64_t Nz, PlanLayout* L, std::vector<int>* map,
+                       std::vector<double>* svals)
 {
+    const int64_t Nzh = Nz / 2 + 1;
+    L->nl = Ny * Nzh;
+    L->K = (Nx - 2 + 1) / 2;
+    std::vector<int64_t> sv((size_t)L->nl);
+    for (int64_t j = 0; j < Ny; ++j) {
+        const int64_t ky = (j < Ny / 2 || (Ny % 2 == 1 && j == Ny / 2)) ? j : j - Ny;
+        for (int64_t k = 0; k < Nzh; ++k) sv[(size_t)(j * Nzh + k)] = ky * ky + k * k;
+    }
+    std::vector<int64_t> us(sv);
+    std::sort(us.begin(), us.end());
+    us.erase(std::unique(us.begin(), us.end()), us.end());
+    L->nu = (int64_t)us.size();
+    L->off_s = round256(L->nl * 4);
+    L->off_fac = L->off_s + round256(L->nu * 8);
+    L->bytes = L->off_fac + L->nu * 2 * L->K * 4 * 8;
+    if (map) {
+        map->resize((size_t)L->nl);
+        for (size_t i = 0; i < sv.size(); ++i)
+            (*map)[i] = (int)(std::lower_bound(us.begin(), us.end(), sv[i]) - us.begin());
+    }
+    if (svals) {
+        svals->resize(us.size());
+        for (size_t i = 0; i < us.size(); ++i) (*svals)[i] = (double)us[i];
+    }
+}
+
+struct PlanInfo {
+    int64_t Nx, Ny, Nz, bytes;
+    double alpha;
+    int neumann;
+    int64_t off_fac;
+};
+static std::mutex g_plan_mu;
+static std::map<const void*, PlanInfo> g_plans;
+
+// offset of the factor table inside a registered plan, or -1 if the plan does
+// not match (sizes, alpha, bc) or was not built by sfa_helmholtz3d_plan
+static int64_t plan_fac_offset(const void* plan, int64_t Nx, int64_t Ny, int64_t Nz, double alpha,
+                               int neumann)
+{
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(plan);
+    if (it == g_plans.end()) return -1;
+    const PlanInfo& p = it->second;
+    if (p.Nx == Nx && p.Ny == Ny && p.Nz == Nz && p.alpha == alpha && p.neumann == neumann)
+        return p.off_fac;
+    return -1;
+}
+
+static bool plan_ok(const void* plan, int64_t Nx, int64_t Ny, int64_t Nz, double alpha, int neumann)
+{
+    return plan_fac_offset(plan, Nx, Ny, Nz, alpha, neumann) >= 0;
+}
+
+int xpass_launch(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, hipStream_t s,
+                 const void* plan = nullptr)
+{
+    if (plan != nullptr) {
+        const double* K2 = cheb_k2(Nx, neumann);
+        SFA_CHECK(K2, SFA_ERR_HIP, "table allocation failed");
+        const int64_t off = plan_fac_offset(plan, Nx, Ny, 2 * (Nzh - 1), alpha, neumann);
+        SFA_CHECK(off >= 0, SFA_ERR_ARG, "plan does not match the solve");
+        const char* pb = (const char*)plan;
+        int rc = 0;
+        if (fast_xpass(W, Nx, Ny, Nzh, alpha, neumann, K2, (const int*)pb,
+                       (const double*)(pb + off), s, &rc))
+            return rc;
+    }
     XPassArgs a;
     a.W = W;
     a.sa = (int64_t)Ny * Nzh;
@@ -278,10 +363,54 @@ extern "C" int64_t sfa_helmholtz3d_workspace(int64_t Nx, int64_t Ny, int64_t Nz,
     return Nx * Ny * (Nz / 2 + 1) * (int64_t)sizeof(double2);
 }
 
-extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
-                               double alpha, int neumann, int precision, int refine, void* work,
-                               int64_t work_bytes, void* stream)
+extern "C" int64_t sfa_helmholtz3d_plan_bytes(int64_t Nx, int64_t Ny, int64_t Nz)
 {
+    if (Nx < 8 || Ny < 1 || Nz < 2) return -1;
+    PlanLayout L;
+    plan_shape(Nx, Ny, Nz, &L, nullptr, nullptr);
+    return L.bytes;
+}
+
+extern "C" int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double alpha, int neumann,
+                                    void* plan, int64_t plan_bytes, void* stream)
+{
+    SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
+              "need Nx >= 8, Ny >= 1, even Nz >= 2");
+    SFA_CHECK(plan != nullptr, SFA_ERR_ARG, "plan buffer is NULL");
+    PlanLayout L;
+    std::vector<int> map;
+    std::vector<double> sv;
+    plan_shape(Nx, Ny, Nz, &L, &map, &sv);
+    SFA_CHECK(plan_bytes >= L.bytes, SFA_ERR_ARG, "plan buffer too small (%lld < %lld)",
+              (long long)plan_bytes, (long long)L.bytes);
+    const double* tabs = helm_tables((int)Nx, neumann);
+    SFA_CHECK(tabs, SFA_ERR_HIP, "table allocation failed");
+    hipStream_t s = (hipStream_t)stream;
+    char* pb = (char*)plan;
+    SFA_HIP(hipMemcpyAsync(pb, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    SFA_HIP(hipMemcpyAsync(pb + L.off_s, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    const int M = (int)Nx - 2;
+    HelmTab tab{tabs, tabs + M, tabs + 2 * M, tabs + 3 * M, tabs + 4 * M, tabs + 5 * M};
+    const int64_t nth = 2 * L.nu;
+    hipLaunchKernelGGL(helm_factor_kernel, dim3((unsigned)((nth + 127) / 128)), dim3(128), 0, s, tab, M,
+                       (int)L.K, neumann, alpha, (const double*)(pb + L.off_s), L.nu,
+                       (double*)(pb + L.off_fac));
+    SFA_LAUNCH_CHECK();
+    // the host vectors are copied before they go out of scope
+    SFA_HIP(hipStreamSynchronize(s));
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        g_plans[plan] = PlanInfo{Nx, Ny, Nz, plan_bytes, alpha, neumann, L.off_fac};
+    }
+    return SFA_OK;
+}
+
+extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
+                               double alpha, int neumann, int precision, int refine, const void* plan,
+                               void* work, int64_t work_bytes, void* stream)
+{
+    SFA_CHECK(plan == nullptr || plan_ok(plan, Nx, Ny, Nz, alpha, neumann), SFA_ERR_ARG,
+              "plan was not built by sfa_helmholtz3d_plan for these sizes/alpha/bc");
     (void)refine;
     SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
               "need Nx >= 8, Ny >= 1, even Nz >= 2");
@@ -297,7 +426,7 @@ extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t N
     int rc;
     if ((rc = sfa_fft_r2c(f, W, 3, rshape, rstr, cstr, 2, 1.0 / (double)Nz, stream))) return rc;
     if ((rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, -1, 1.0 / (double)Ny, stream))) return rc;
-    if ((rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, alpha, neumann, (hipStream_t)stream)))
+    if ((rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, alpha, neumann, (hipStream_t)stream, plan)))
         return rc;
     if ((rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, +1, 1.0, stream))) return rc;
     if ((rc = sfa_fft_c2r(W, u, 3, rshape, cstr, rstr, 2, 1.0, stream))) return rc;
@@ -308,10 +437,12 @@ extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t N
 // `stream` between the five kernels; stage_ms[5] receives the summed
 // milliseconds of each stage (z R2C, y C2C, x-pass, y C2C inverse, z C2R).
 extern "C" int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t Nx, int64_t Ny,
-                                           int64_t Nz, double alpha, int neumann, void* work,
-                                           int64_t work_bytes, int nrep, float* stage_ms,
-                                           void* stream)
+                                           int64_t Nz, double alpha, int neumann, const void* plan,
+                                           void* work, int64_t work_bytes, int nrep,
+                                           float* stage_ms, void* stream)
 {
+    SFA_CHECK(plan == nullptr || plan_ok(plan, Nx, Ny, Nz, alpha, neumann), SFA_ERR_ARG,
+              "plan was not built by sfa_helmholtz3d_plan for these sizes/alpha/bc");
     SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
               "need Nx >= 8, Ny >= 1, even Nz >= 2");
     SFA_CHECK(work_bytes >= sfa_helmholtz3d_workspace(Nx, Ny, Nz, 0), SFA_ERR_ARG,
@@ -334,7 +465,7 @@ extern "C" int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t N
         hipEventRecord(ev[1], s);
         if (!rc) rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, -1, 1.0 / (double)Ny, stream);
         hipEventRecord(ev[2], s);
-        if (!rc) rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, alpha, neumann, s);
+        if (!rc) rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, alpha, neumann, s, plan);
         hipEventRecord(ev[3], s);
         if (!rc) rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, +1, 1.0, stream);
         hipEventRecord(ev[4], s);

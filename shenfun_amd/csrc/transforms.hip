@@ -286,6 +286,13 @@ static int launch_line(int op, bool inv, LineArgs& a, hipStream_t stream)
     return SFA_OK;
 }
 
+int fast_c2c(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
+             hipStream_t s, int* rc);
+int fast_rc(bool r2c, const Geom& gi, const Geom& go, const void* in, void* out, double scale,
+            hipStream_t s, int* rc);
+
+static inline bool aligned16(const void* p) { return (((uintptr_t)p) & 15) == 0; }
+
 // Fill LineArgs from geometries (in_geom for the input, out_geom for the output).
 static int setup_line(LineArgs& a, const void* in, void* out, const Geom& gi, const Geom& go,
                       int n, int nfft, int in_cplx, int out_cplx)
@@ -332,6 +339,8 @@ extern "C" int sfa_fft_c2c(const void* in, void* out, int ndim, const int64_t* s
     if ((rc = make_geom(ndim, shape, in_strides, axis, &gi))) return rc;
     if ((rc = make_geom(ndim, shape, out_strides, axis, &go))) return rc;
     SFA_CHECK(sign == 1 || sign == -1, SFA_ERR_ARG, "sign must be +-1");
+    if (aligned16(in) && aligned16(out) && fast_c2c(gi, go, in, out, sign, scale, (hipStream_t)stream, &rc))
+        return rc;
     LineArgs a;
     if ((rc = setup_line(a, in, out, gi, go, (int)gi.n, (int)gi.n, 1, 1))) return rc;
     a.scale = scale;
@@ -348,6 +357,8 @@ extern "C" int sfa_fft_r2c(const double* in, void* out, int ndim, const int64_t*
     if ((rc = make_geom(ndim, real_shape, out_strides, axis, &go))) return rc;
     const int n = (int)gi.n;
     SFA_CHECK(n >= 2 && n % 2 == 0, SFA_ERR_UNSUPPORTED, "R2C length must be even (got %d)", n);
+    if (aligned16(in) && aligned16(out) && fast_rc(true, gi, go, in, out, scale, (hipStream_t)stream, &rc))
+        return rc;
     LineArgs a;
     if ((rc = setup_line(a, in, out, gi, go, n, n / 2, 0, 1))) return rc;
     a.tw2 = fft_table(n);
@@ -366,6 +377,8 @@ extern "C" int sfa_fft_c2r(const void* in, double* out, int ndim, const int64_t*
     if ((rc = make_geom(ndim, real_shape, out_strides, axis, &go))) return rc;
     const int n = (int)gi.n;
     SFA_CHECK(n >= 2 && n % 2 == 0, SFA_ERR_UNSUPPORTED, "C2R length must be even (got %d)", n);
+    if (aligned16(in) && aligned16(out) && fast_rc(false, gi, go, in, out, scale, (hipStream_t)stream, &rc))
+        return rc;
     LineArgs a;
     if ((rc = setup_line(a, in, out, gi, go, n, n / 2, 1, 0))) return rc;
     a.tw2 = fft_table(n);

@@ -28,6 +28,11 @@ class HelmholtzSolver3D:
         Nx, Ny, Nz = self.shape
         nbytes = _lib.lib().sfa_helmholtz3d_workspace(Nx, Ny, Nz, self.precision)
         self.work = torch.empty(int(nbytes), dtype=torch.uint8, device=T.device)
+        # factorise once (chebyshev/la.py:223-276): LU factors per distinct beta
+        pbytes = _lib.lib().sfa_helmholtz3d_plan_bytes(Nx, Ny, Nz)
+        self.plan = torch.empty(int(pbytes), dtype=torch.uint8, device=T.device)
+        _lib.call("sfa_helmholtz3d_plan", Nx, Ny, Nz, self.alpha, int(self.neumann),
+                  _lib.ptr(self.plan), self.plan.numel(), _lib.stream_ptr())
 
     def __call__(self, f, u=None):
         if not (f.is_cuda and f.dtype == torch.float64 and f.is_contiguous()
@@ -39,6 +44,6 @@ class HelmholtzSolver3D:
             raise ValueError("u must match f")
         Nx, Ny, Nz = self.shape
         _lib.call("sfa_helmholtz3d", _lib.ptr(f), _lib.ptr(u), Nx, Ny, Nz, self.alpha,
-                  int(self.neumann), self.precision, self.refine, _lib.ptr(self.work),
-                  self.work.numel(), _lib.stream_ptr())
+                  int(self.neumann), self.precision, self.refine, _lib.ptr(self.plan),
+                  _lib.ptr(self.work), self.work.numel(), _lib.stream_ptr())
         return u
