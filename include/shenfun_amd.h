@@ -188,19 +188,27 @@ int sfa_poisson_add_solve(const void* b, void* u, int dtype, int ndim, const int
  *
  * Planning (the counterpart of chebyshev.la.Helmholtz.__init__, which
  * factorises once per solver, chebyshev/la.py:223-276): sfa_helmholtz3d_plan
- * fills a device buffer of sfa_helmholtz3d_plan_bytes() bytes with the LU
- * factors of every distinct beta and the line -> entry map, synchronising
- * `stream`.  plan may be NULL in sfa_helmholtz3d, in which case the factors
+ * fills a device buffer of sfa_helmholtz3d_plan_bytes() bytes with, for every
+ * distinct beta, the LU recurrence state at 16 segment starts of each parity
+ * chain (the solve regenerates the factors of a segment from its start
+ * state), and the line -> entry map, synchronising `stream`.  plan may be NULL in sfa_helmholtz3d, in which case the factors
  * are computed on the fly (wavefront scan of the LU recurrence).
- * precision: 0 = FP64 throughout; 1 = mixed (complex64 transforms, FP64
- * solve and FP64 residual refinement, `refine` sweeps).
+ * precision: 0 = FP64 throughout.  1 = mixed: after the FP64 R2C of f, the
+ * kz planes [0, fp64_planes) stay complex128 with FP64 transforms and the
+ * planes [fp64_planes, Nz/2] are stored and transformed as complex64; the
+ * Helmholtz solve and the final C2R to u are FP64.  fp64_planes <= 0 picks
+ * 17 (kz < 17 in FP64); the value is clamped to [2, Nz/2 - 1].  The error of
+ * the complex64 part is about 6e-8 times the largest coefficient above the
+ * cut relative to the largest of f, so the cut is a wavenumber, not a
+ * fraction of Nz.  Mixed precision needs a plan and
+ * power-of-two Nx in {64, 128, 256}, Ny in [16, 512], Nz/2 in [16, 256].
  * work: device workspace of at least sfa_helmholtz3d_workspace() bytes. */
 int64_t sfa_helmholtz3d_plan_bytes(int64_t Nx, int64_t Ny, int64_t Nz);
 int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double alpha, int neumann,
                          void* plan, int64_t plan_bytes, void* stream);
 int64_t sfa_helmholtz3d_workspace(int64_t Nx, int64_t Ny, int64_t Nz, int precision);
 int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
-                    double alpha, int neumann, int precision, int refine, const void* plan,
+                    double alpha, int neumann, int precision, int fp64_planes, const void* plan,
                     void* work, int64_t work_bytes, void* stream);
 
 /* Profiling variant of sfa_helmholtz3d (FP64): runs the pipeline nrep times

@@ -50,6 +50,43 @@ __host__ __device__ inline double2 csub(double2 a, double2 b) { return make_doub
 __host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
+// float2 counterparts (complex64 arithmetic of the mixed-precision kernels)
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// complex type traits: C2 in {double2, float2}
+template <typename C2>
+struct CT;
+template <>
+struct CT<double2> {
+    using R = double;
+    __host__ __device__ static inline double2 mk(double x, double y) { return make_double2(x, y); }
+};
+template <>
+struct CT<float2> {
+    using R = float;
+    __host__ __device__ static inline float2 mk(float x, float y) { return make_float2(x, y); }
+};
+template <typename C2>
+__host__ __device__ inline C2 cmk(double x, double y)
+{
+    using R = typename CT<C2>::R;
+    return CT<C2>::mk((R)x, (R)y);
+}
+__host__ __device__ inline double2 to_c64(float2 a) { return make_double2(a.x, a.y); }
+__host__ __device__ inline double2 to_c64(double2 a) { return a; }
+template <typename C2>
+__host__ __device__ inline C2 from_c64(double2 a);
+template <>
+__host__ __device__ inline double2 from_c64<double2>(double2 a) { return a; }
+template <>
+__host__ __device__ inline float2 from_c64<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
+
 // ---------------------------------------------------------------- geometry
 // A batch of 1D lines inside an up-to-3D strided array: the line axis has
 // length n and stride sa; the remaining two dims (d1, s1) (d2, s2) enumerate

@@ -45,57 +45,46 @@ __device__ constexpr double W16S[16] = {
     1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173};
 
 // multiply by exp(s 2 pi i m/16), s = -1 forward, +1 inverse (compile-time m)
-template <int M, bool INV>
-__device__ inline double2 tw16(double2 a)
+template <int M, bool INV, typename C2>
+__device__ inline C2 tw16(C2 a)
 {
+    using R = typename CT<C2>::R;
     constexpr int m = M & 15;
     if constexpr (m == 0) return a;
     else if constexpr (m == 4) {  // -i (fwd) / +i (inv)
-        return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+        return INV ? CT<C2>::mk(-a.y, a.x) : CT<C2>::mk(a.y, -a.x);
     } else if constexpr (m == 8) {
-        return make_double2(-a.x, -a.y);
+        return CT<C2>::mk(-a.x, -a.y);
     } else if constexpr (m == 12) {
-        return INV ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+        return INV ? CT<C2>::mk(a.y, -a.x) : CT<C2>::mk(-a.y, a.x);
     } else {
-        const double c = W16C[m];
-        const double s = INV ? -W16S[m] : W16S[m];
-        return make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
+        const R c = (R)W16C[m];
+        const R s = (R)(INV ? -W16S[m] : W16S[m]);
+        return CT<C2>::mk(a.x * c - a.y * s, a.x * s + a.y * c);
     }
 }
 
-template <bool INV>
-__device__ inline void dft2(double2& a0, double2& a1)
+template <bool INV, typename C2>
+__device__ inline void dft2(C2& a0, C2& a1)
 {
-    double2 t = a0;
+    C2 t = a0;
     a0 = cadd(t, a1);
     a1 = csub(t, a1);
 }
 
-template <bool INV>
-__device__ inline void dft4(double2& a0, double2& a1, double2& a2, double2& a3)
+template <bool INV, typename C2>
+__device__ inline void dft4(C2& a0, C2& a1, C2& a2, C2& a3)
 {
-    double2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
-    double2 t2 = cadd(a1, a3), t3 = tw16<4, INV>(csub(a1, a3));
+    C2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+    C2 t2 = cadd(a1, a3), t3 = tw16<4, INV>(csub(a1, a3));
     a0 = cadd(t0, t2);
     a2 = csub(t0, t2);
     a1 = cadd(t1, t3);
     a3 = csub(t1, t3);
 }
 
-template <int R, bool INV>
-__device__ inline void dft(double2* a);
-
-template <>
-__device__ inline void dft<2, false>(double2* a) { dft2<false>(a[0], a[1]); }
-template <>
-__device__ inline void dft<2, true>(double2* a) { dft2<true>(a[0], a[1]); }
-template <>
-__device__ inline void dft<4, false>(double2* a) { dft4<false>(a[0], a[1], a[2], a[3]); }
-template <>
-__device__ inline void dft<4, true>(double2* a) { dft4<true>(a[0], a[1], a[2], a[3]); }
-
-template <bool INV>
-__device__ inline void dft8(double2* a)
+template <bool INV, typename C2>
+__device__ inline void dft8(C2* a)
 {
     // 8 = 4 (r1) x 2 (r2): r = 2 r1 + r2, q = q1 + 4 q2
     dft4<INV>(a[0], a[2], a[4], a[6]);
@@ -103,30 +92,26 @@ __device__ inline void dft8(double2* a)
     a[3] = tw16<2, INV>(a[3]);
     a[5] = tw16<4, INV>(a[5]);
     a[7] = tw16<6, INV>(a[7]);
-    double2 b[8];
+    C2 b[8];
 #pragma unroll
     for (int q1 = 0; q1 < 4; ++q1) {
-        double2 x0 = a[2 * q1], x1 = a[2 * q1 + 1];
+        C2 x0 = a[2 * q1], x1 = a[2 * q1 + 1];
         b[q1] = cadd(x0, x1);
         b[q1 + 4] = csub(x0, x1);
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) a[i] = b[i];
 }
-template <>
-__device__ inline void dft<8, false>(double2* a) { dft8<false>(a); }
-template <>
-__device__ inline void dft<8, true>(double2* a) { dft8<true>(a); }
 
-template <bool INV>
-__device__ inline void dft16(double2* a)
+template <bool INV, typename C2>
+__device__ inline void dft16(C2* a)
 {
-    // 16 = 4 x 4: r = 4 r1 + r2; first DFT4 over r1 for each r2 -> q1,
-    // twiddle w16^(r2 q1), then DFT4 over r2 for each q1 -> q2; q = q1 + 4 q2
-    double2 c[16];
+    // 16 = 4 x 4: r = 4 r1 + r2; DFT4 over r1 for each r2 -> q1, twiddle
+    // w16^(r2 q1), then DFT4 over r2 for each q1 -> q2; q = q1 + 4 q2
+    C2 c[16];
 #pragma unroll
     for (int r2 = 0; r2 < 4; ++r2) {
-        double2 x0 = a[r2], x1 = a[4 + r2], x2 = a[8 + r2], x3 = a[12 + r2];
+        C2 x0 = a[r2], x1 = a[4 + r2], x2 = a[8 + r2], x3 = a[12 + r2];
         dft4<INV>(x0, x1, x2, x3);
         c[r2 * 4 + 0] = x0;
         c[r2 * 4 + 1] = x1;
@@ -144,7 +129,7 @@ __device__ inline void dft16(double2* a)
     c[3 * 4 + 3] = tw16<9, INV>(c[3 * 4 + 3]);
 #pragma unroll
     for (int q1 = 0; q1 < 4; ++q1) {
-        double2 x0 = c[0 * 4 + q1], x1 = c[1 * 4 + q1], x2 = c[2 * 4 + q1], x3 = c[3 * 4 + q1];
+        C2 x0 = c[0 * 4 + q1], x1 = c[1 * 4 + q1], x2 = c[2 * 4 + q1], x3 = c[3 * 4 + q1];
         dft4<INV>(x0, x1, x2, x3);
         a[q1] = x0;
         a[q1 + 4] = x1;
@@ -152,25 +137,22 @@ __device__ inline void dft16(double2* a)
         a[q1 + 12] = x3;
     }
 }
-template <>
-__device__ inline void dft<16, false>(double2* a) { dft16<false>(a); }
-template <>
-__device__ inline void dft<16, true>(double2* a) { dft16<true>(a); }
 
 // odd radices 3, 5, 7 by the direct formula with exact constants
-template <int R, bool INV>
-__device__ inline void dft_odd(double2* a)
+template <int R, bool INV, typename C2>
+__device__ inline void dft_odd(C2* a)
 {
+    using T = typename CT<C2>::R;
     constexpr double PI2 = 6.283185307179586476925286766559;
-    double2 b[R];
+    C2 b[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        double2 acc = a[0];
+        C2 acc = a[0];
 #pragma unroll
         for (int r = 1; r < R; ++r) {
             const int m = (r * q) % R;
-            const double c = __builtin_cos(PI2 * m / R);
-            const double s = (INV ? 1.0 : -1.0) * __builtin_sin(PI2 * m / R);
+            const T c = (T)__builtin_cos(PI2 * m / R);
+            const T s = (T)((INV ? 1.0 : -1.0) * __builtin_sin(PI2 * m / R));
             acc.x += a[r].x * c - a[r].y * s;
             acc.y += a[r].x * s + a[r].y * c;
         }
@@ -179,18 +161,21 @@ __device__ inline void dft_odd(double2* a)
 #pragma unroll
     for (int q = 0; q < R; ++q) a[q] = b[q];
 }
-template <>
-__device__ inline void dft<3, false>(double2* a) { dft_odd<3, false>(a); }
-template <>
-__device__ inline void dft<3, true>(double2* a) { dft_odd<3, true>(a); }
-template <>
-__device__ inline void dft<5, false>(double2* a) { dft_odd<5, false>(a); }
-template <>
-__device__ inline void dft<5, true>(double2* a) { dft_odd<5, true>(a); }
-template <>
-__device__ inline void dft<7, false>(double2* a) { dft_odd<7, false>(a); }
-template <>
-__device__ inline void dft<7, true>(double2* a) { dft_odd<7, true>(a); }
+
+template <bool INV, typename C2>
+__device__ inline void dft32(C2* a);
+
+// dft<R, INV>(a): in-register DFT of radix R
+template <int R, bool INV, typename C2>
+__device__ inline void dft(C2* a)
+{
+    if constexpr (R == 2) dft2<INV>(a[0], a[1]);
+    else if constexpr (R == 4) dft4<INV>(a[0], a[1], a[2], a[3]);
+    else if constexpr (R == 8) dft8<INV>(a);
+    else if constexpr (R == 16) dft16<INV>(a);
+    else if constexpr (R == 32) dft32<INV>(a);
+    else dft_odd<R, INV>(a);
+}
 
 __device__ inline double2 twiddle(const double2* __restrict__ tw, int idx, bool inv)
 {

@@ -13,37 +13,38 @@ namespace sfa {
 constexpr int TL = 16;  // lines per block
 
 // ----------------------------------------------------------------- C2C strided
-template <int N, bool INV>
+template <int N, bool INV, typename C2 = double2>
 __global__ void __launch_bounds__(TL * Pow2<N>::Q)
-    c2c_strided_kernel(const double2* in, double2* out, int64_t in_sa,
+    c2c_strided_kernel(const C2* in, C2* out, int64_t in_sa,
                        int64_t in_s1, int64_t in_s2, int64_t out_sa, int64_t out_s1,
                        int64_t out_s2, int64_t d2, int64_t nlines, double scale,
                        const double2* __restrict__ twg)
 {
     using Pl = Pow2<N>;
     constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
-    extern __shared__ double2 sm[];
-    double2* tw = sm;
-    double2* ex = sm + N;
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    C2* tw = reinterpret_cast<C2*>(sm_raw);
+    C2* ex = tw + N;
     const int tid = threadIdx.x, nth = blockDim.x;
     const int t = tid % TL, j = tid / TL;
     const int64_t L = (int64_t)blockIdx.x * TL + t;
     const bool valid = L < nlines;
     const int64_t i1 = valid ? L / d2 : 0, i2 = valid ? L - i1 * d2 : 0;
-    for (int i = tid; i < N; i += nth) tw[i] = twg[i];
-    double2 a[P];
-    const double2* src = in + i1 * in_s1 + i2 * in_s2;
+    for (int i = tid; i < N; i += nth) tw[i] = from_c64<C2>(twg[i]);
+    C2 a[P];
+    const C2* src = in + i1 * in_s1 + i2 * in_s2;
 #pragma unroll
-    for (int r = 0; r < P; ++r) a[r] = valid ? src[(int64_t)(j + Q * r) * in_sa] : make_double2(0.0, 0.0);
+    for (int r = 0; r < P; ++r) a[r] = valid ? src[(int64_t)(j + Q * r) * in_sa] : cmk<C2>(0.0, 0.0);
     __syncthreads();
     pow2_fft<N, INV>(a, j, ex + t * LS, tw);
     if (valid) {
-        double2* dst = out + i1 * out_s1 + i2 * out_s2;
+        C2* dst = out + i1 * out_s1 + i2 * out_s2;
+        const typename CT<C2>::R sc = (typename CT<C2>::R)scale;
 #pragma unroll
         for (int h = 0; h < P / Q; ++h)
 #pragma unroll
             for (int k2 = 0; k2 < Q; ++k2)
-                dst[(int64_t)pow2_out_index<N>(j, h, k2) * out_sa] = cscale(a[h * Q + k2], scale);
+                dst[(int64_t)pow2_out_index<N>(j, h, k2) * out_sa] = cscale(a[h * Q + k2], sc);
     }
 }
 
@@ -152,7 +153,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 }
 
 // ----------------------------------------------------------------- host dispatch
-template <int N, bool INV>
+template <int N, bool INV, typename C2 = double2>
 static int launch_c2c_strided(const Geom& gi, const Geom& go, const void* in, void* out,
                               double scale, hipStream_t s)
 {
@@ -160,12 +161,12 @@ static int launch_c2c_strided(const Geom& gi, const Geom& go, const void* in, vo
     const double2* tw = fft_table(N);
     SFA_CHECK(tw, SFA_ERR_HIP, "twiddle table allocation failed");
     const int64_t nl = gi.d1 * gi.d2;
-    const size_t lds = sizeof(double2) * ((size_t)N + (size_t)TL * Pl::LS);
-    auto k = c2c_strided_kernel<N, INV>;
+    const size_t lds = sizeof(C2) * ((size_t)N + (size_t)TL * Pl::LS);
+    auto k = c2c_strided_kernel<N, INV, C2>;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k, dim3((unsigned)((nl + TL - 1) / TL)), dim3(TL * Pl::Q), lds, s,
-                       (const double2*)in, (double2*)out, gi.sa, gi.s1, gi.s2, go.sa, go.s1, go.s2,
+                       (const C2*)in, (C2*)out, gi.sa, gi.s1, gi.s2, go.sa, go.s1, go.s2,
                        gi.d2, nl, scale, tw);
     SFA_LAUNCH_CHECK();
     return SFA_OK;
@@ -189,6 +190,24 @@ int fast_c2c(const Geom& gi, const Geom& go, const void* in, void* out, int sign
         default: return 0;
     }
 #undef SFA_C2C
+}
+
+// complex64 strided C2C (geometry as fast_c2c); returns 1 if handled
+int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
+                 hipStream_t s, int* rc)
+{
+    if (!(gi.s2 == 1 && go.s2 == 1 && gi.sa != 1 && go.sa != 1)) return 0;
+    const bool inv = sign > 0;
+#define SFA_C2CF(NN)                                                                      \
+    case NN:                                                                              \
+        *rc = inv ? launch_c2c_strided<NN, true, float2>(gi, go, in, out, scale, s)       \
+                  : launch_c2c_strided<NN, false, float2>(gi, go, in, out, scale, s);     \
+        return 1;
+    switch (gi.n) {
+        SFA_C2CF(16) SFA_C2CF(32) SFA_C2CF(64) SFA_C2CF(128) SFA_C2CF(256) SFA_C2CF(512)
+        default: return 0;
+    }
+#undef SFA_C2CF
 }
 
 template <int H, bool R2C>
@@ -245,48 +264,61 @@ int fast_rc(bool r2c, const Geom& gi, const Geom& go, const void* in, void* out,
 namespace sfa {
 
 struct XPass2Args {
-    double2* W;
-    int64_t sa;     // x stride (= Ny * Nzh); lines L = ky * Nzh + kz have stride 1
-    int64_t nl;     // Ny * Nzh
+    void* W;         // lines of C2 values; element (x, L) at W[x * sa + L]
+    int64_t sa;      // x stride (= Ny * Kp)
+    int64_t nl;      // Ny * Kp lines
+    int Kp;          // kz planes held by this array (line pitch per ky)
+    int kz0;         // first kz of this array
     int Nzh, Ny;
     int neumann;
     double alpha;
     const double* K2;
-    const int* linemap;  // line -> factor table entry
-    const double* fac;   // per-entry, per-parity row factors (K rows x 4)
-    int K;
+    XPlan pl;
     const double2* tw;   // exp(-2 pi i m/N)
     const double2* tq;   // exp(-i pi k/(2N))
 };
 
-template <int N>
-__global__ void __launch_bounds__(TL * Pow2<N>::Q) xpass2_kernel(XPass2Args a)
+// C2 = double2: FP64 everywhere.  C2 = float2: complex64 storage and DCT/FFT
+// arithmetic, the Helmholtz solve still FP64 (values widened on entry).
+template <int N, typename C2>
+__global__ void __launch_bounds__(TL * Pow2<N>::Q, 2) xpass2_kernel(XPass2Args a)
 {
     using Pl = Pow2<N>;
+    using R = typename CT<C2>::R;
     constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
     constexpr int M = N - 2;
     constexpr int K = (M + 1) / 2;  // rows of the even chain
     constexpr int E = (K + 15) / 16;
     constexpr int NTH = TL * Q;
     constexpr int ROUNDS = (2 * TL * 16) / NTH;
-    extern __shared__ double2 sm[];
-    double2* tw = sm;
-    double2* ex = sm + N;
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    // LDS: twiddles, DCT quarter-wave table, stencil, then the line exchange
+    // areas.  Tables live in LDS so that no global load is hoisted across the
+    // solve (it would hold registers through it).
+    C2* tw = reinterpret_cast<C2*>(sm_raw);
+    C2* tqs = tw + N;
+    double* k2s = reinterpret_cast<double*>(tqs + N);
+    C2* ex = reinterpret_cast<C2*>(k2s + N);
+    C2* Wd = reinterpret_cast<C2*>(a.W);
     const int tid = threadIdx.x;
     const int t = tid % TL, j = tid / TL;
     const int64_t L = (int64_t)blockIdx.x * TL + t;
     const bool valid = L < a.nl;
-    for (int i = tid; i < N; i += NTH) tw[i] = a.tw[i];
+    for (int i = tid; i < N; i += NTH) {
+        tw[i] = from_c64<C2>(a.tw[i]);
+        tqs[i] = from_c64<C2>(a.tq[i]);
+        k2s[i] = (i < N - 2) ? a.K2[i] : 0.0;
+    }
     // 1. load with the DCT-II even/odd permutation: v[m] = x[2m] (m < N/2), x[2(N-1-m)+1]
-    double2 v[P];
+    C2 v[P];
 #pragma unroll
     for (int r = 0; r < P; ++r) {
         const int m = j + Q * r;
         const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
-        v[r] = valid ? a.W[(int64_t)x * a.sa + L] : make_double2(0.0, 0.0);
+        v[r] = valid ? Wd[(int64_t)x * a.sa + L] : cmk<C2>(0.0, 0.0);
     }
     __syncthreads();
-    double2* ln = ex + t * LS;
+    C2* ln = ex + t * LS;
     pow2_fft<N, false>(v, j, ln, tw);
 #pragma unroll
     for (int h = 0; h < P / Q; ++h)
@@ -294,19 +326,18 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q) xpass2_kernel(XPass2Args a)
         for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<N>(j, h, k2)] = v[h * Q + k2];
     __syncthreads();
     // 2a. DCT-II post-processing X_k = sc (Re(w (V_k + conj V_{N-k})) + i Im(w (V_k - conj V_{N-k})))
-    //     computed for this thread's k (registers), then written back in place
-    const double sc = 3.14159265358979323846 / (2.0 * N);
+    const R sc = (R)(3.14159265358979323846 / (2.0 * N));
 #pragma unroll
     for (int h = 0; h < P / Q; ++h)
 #pragma unroll
         for (int k2 = 0; k2 < Q; ++k2) {
             const int k = pow2_out_index<N>(j, h, k2);
-            const double2 Pv = v[h * Q + k2];
-            const double2 Qv = cconj(ln[k == 0 ? 0 : N - k]);
-            const double2 w = a.tq[k];
-            const double2 s1 = cmul(w, cadd(Pv, Qv));
-            const double2 s2 = cmul(w, csub(Pv, Qv));
-            v[h * Q + k2] = make_double2(s1.x * sc, s2.y * sc);
+            const C2 Pv = v[h * Q + k2];
+            const C2 Qv = cconj(ln[k == 0 ? 0 : N - k]);
+            const C2 w = tqs[k];
+            const C2 s1 = cmul(w, cadd(Pv, Qv));
+            const C2 s2 = cmul(w, csub(Pv, Qv));
+            v[h * Q + k2] = CT<C2>::mk(s1.x * sc, s2.y * sc);
         }
     __syncthreads();
 #pragma unroll
@@ -314,47 +345,59 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q) xpass2_kernel(XPass2Args a)
 #pragma unroll
         for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<N>(j, h, k2)] = v[h * Q + k2];
     __syncthreads();
-    // 2b. stencil + Helmholtz solve with the planned factors, 16 lanes per (line, parity)
+    // 2b. stencil (FP64) + Helmholtz solve, 16 lanes per (line, parity) chain.
+    // Group g = tid / 16 takes chains g * ROUNDS + rho: the parity alternates
+    // with rho, so the row operators (functions of the row index) cannot be
+    // hoisted out of the loop and held in registers across it.
+    static_assert(ROUNDS % 2 == 0, "chains per group must pair both parities");
 #pragma unroll 1
     for (int rho = 0; rho < ROUNDS; ++rho) {
-        const int c = rho * (NTH / 16) + tid / 16;
+        const int c = (tid / 16) * ROUNDS + rho;
         const int ct = c >> 1, p = c & 1, lane = tid & 15;
-        double2* X = ex + ct * LS;
+        C2* X = ex + ct * LS;
         const int64_t Lc = (int64_t)blockIdx.x * TL + ct;
-        const int u = (Lc < a.nl) ? a.linemap[Lc] : 0;
-        const double* fc = a.fac + ((int64_t)2 * u + p) * a.K * 4;
+        int u = 0;
+        if (Lc < a.nl) {
+            const int64_t ky = Lc / a.Kp, kz = a.kz0 + (Lc - ky * a.Kp);
+            u = a.pl.linemap[ky * a.Nzh + kz];
+        }
+        const double be = a.alpha + a.pl.svals[u];
+        const double4 b4 = a.pl.bs[((int64_t)2 * u + p) * 16 + lane];
         double2 f[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int i = 2 * (lane * E + e) + p;
             if (i < M) {
-                const double2 x0 = X[i], x2 = X[i + 2];
-                const double kk = a.K2[i];
+                const double2 x0 = to_c64(X[i]), x2 = to_c64(X[i + 2]);
+                const double kk = k2s[i];
                 f[e] = make_double2(x0.x + kk * x2.x, x0.y + kk * x2.y);
             } else {
                 f[e] = make_double2(0.0, 0.0);
             }
         }
-        helm_chain_solve_fac<E, 16>(f, X, p, M, a.neumann != 0, fc, a.K, lane);
-        if (p == 0 && lane == 0) {
-            X[M] = make_double2(0.0, 0.0);
-            X[M + 1] = make_double2(0.0, 0.0);
+        helm_chain_solve_bs<E, 16>(f, X, p, M, be, a.neumann != 0, b4, lane);
+        // boundary slots M, M+1 (u carries no value there).  The odd chain
+        // runs after the even one and its stencil reads X[M+1], so it clears
+        // them once its own reads (same group, earlier in program order) are done.
+        if (p == 1 && lane == 0) {
+            X[M] = cmk<C2>(0.0, 0.0);
+            X[M + 1] = cmk<C2>(0.0, 0.0);
         }
     }
     __syncthreads();
     // 3a. to_ortho t_k = u_k (k < N-2) + K2_{k-2} u_{k-2}, for this thread's k = j + Q r
-    const double2 t0 = ln[0];
+    const C2 t0 = ln[0];
 #pragma unroll
     for (int r = 0; r < P; ++r) {
         const int k = j + Q * r;
-        double2 c = (k < N - 2) ? ln[k] : make_double2(0.0, 0.0);
+        C2 cc = (k < N - 2) ? ln[k] : cmk<C2>(0.0, 0.0);
         if (k >= 2) {
-            const double kk = a.K2[k - 2];
-            const double2 uu = ln[k - 2];
-            c.x += kk * uu.x;
-            c.y += kk * uu.y;
+            const R kk = (R)k2s[k - 2];
+            const C2 uu = ln[k - 2];
+            cc.x += kk * uu.x;
+            cc.y += kk * uu.y;
         }
-        v[r] = c;
+        v[r] = cc;
     }
     __syncthreads();
 #pragma unroll
@@ -364,32 +407,33 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q) xpass2_kernel(XPass2Args a)
 #pragma unroll
     for (int r = 0; r < P; ++r) {
         const int k = j + Q * r;
-        const double2 cn = (k == 0) ? make_double2(0.0, 0.0) : ln[N - k];
-        const double2 z = make_double2(v[r].x + cn.y, v[r].y - cn.x);
-        v[r] = cmul(cconj(a.tq[k]), z);
+        const C2 cn = (k == 0) ? cmk<C2>(0.0, 0.0) : ln[N - k];
+        const C2 z = CT<C2>::mk(v[r].x + cn.y, v[r].y - cn.x);
+        v[r] = cmul(cconj(tqs[k]), z);
     }
     __syncthreads();
     pow2_fft<N, true>(v, j, ln, tw);
     // 4. store 0.5 y + t_0/2 with the inverse permutation
     if (valid) {
+        const R half = (R)0.5;
 #pragma unroll
         for (int h = 0; h < P / Q; ++h)
 #pragma unroll
             for (int k2 = 0; k2 < Q; ++k2) {
                 const int m = pow2_out_index<N>(j, h, k2);
                 const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
-                const double2 y = v[h * Q + k2];
-                a.W[(int64_t)x * a.sa + L] = make_double2(0.5 * y.x + 0.5 * t0.x, 0.5 * y.y + 0.5 * t0.y);
+                const C2 y = v[h * Q + k2];
+                Wd[(int64_t)x * a.sa + L] = CT<C2>::mk(half * y.x + half * t0.x, half * y.y + half * t0.y);
             }
     }
 }
 
-template <int N>
+template <int N, typename C2>
 static int launch_xpass2(XPass2Args& a, hipStream_t s)
 {
     using Pl = Pow2<N>;
-    const size_t lds = sizeof(double2) * ((size_t)N + (size_t)TL * Pl::LS);
-    auto k = xpass2_kernel<N>;
+    const size_t lds = sizeof(C2) * ((size_t)2 * N + (size_t)TL * Pl::LS) + sizeof(double) * N;
+    auto k = xpass2_kernel<N, C2>;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k, dim3((unsigned)((a.nl + TL - 1) / TL)), dim3(TL * Pl::Q), lds, s, a);
@@ -397,35 +441,198 @@ static int launch_xpass2(XPass2Args& a, hipStream_t s)
     return SFA_OK;
 }
 
-// returns 1 if handled
-int fast_xpass(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, const double* K2,
-               const int* linemap, const double* fac, hipStream_t s, int* rc)
+// Fused x-pass on the kz planes [kz0, kz0 + Kp) held (line pitch Kp) in W.
+// f32 selects complex64 storage/arithmetic.  Returns 1 if handled.
+int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha,
+                   int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc)
 {
-    if (!(Nx == 64 || Nx == 128 || Nx == 256) || linemap == nullptr || fac == nullptr) return 0;
+    if (!(Nx == 64 || Nx == 128 || Nx == 256) || pl.linemap == nullptr || pl.bs == nullptr) return 0;
     XPass2Args a;
     a.W = W;
-    a.sa = (int64_t)Ny * Nzh;
-    a.nl = (int64_t)Ny * Nzh;
+    a.sa = (int64_t)Ny * Kp;
+    a.nl = (int64_t)Ny * Kp;
+    a.Kp = Kp;
+    a.kz0 = kz0;
     a.Nzh = Nzh;
     a.Ny = Ny;
     a.neumann = neumann;
     a.alpha = alpha;
     a.K2 = K2;
-    a.linemap = linemap;
-    a.fac = fac;
-    a.K = (Nx - 2 + 1) / 2;
+    a.pl = pl;
     a.tw = fft_table(Nx);
     a.tq = quarter_table(Nx);
     if (!a.tw || !a.tq) {
         *rc = fail(SFA_ERR_HIP, "table allocation failed");
         return 1;
     }
+#define SFA_XP(NN)                                                                     \
+    case NN:                                                                           \
+        *rc = f32 ? launch_xpass2<NN, float2>(a, s) : launch_xpass2<NN, double2>(a, s); \
+        break;
     switch (Nx) {
-        case 64: *rc = launch_xpass2<64>(a, s); break;
-        case 128: *rc = launch_xpass2<128>(a, s); break;
-        default: *rc = launch_xpass2<256>(a, s); break;
+        SFA_XP(64) SFA_XP(128) SFA_XP(256)
     }
+#undef SFA_XP
     return 1;
+}
+
+int fast_xpass(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, const double* K2,
+               const XPlan& pl, hipStream_t s, int* rc)
+{
+    return fast_xpass_sub(W, false, Nx, Ny, Nzh, Nzh, 0, alpha, neumann, K2, pl, s, rc);
+}
+
+}  // namespace sfa
+
+namespace sfa {
+
+// ----------------------------------------------------------------- kz-split z passes
+// R2C of contiguous rows of 2H reals (FP64 arithmetic), output split by kz:
+// k < Kh -> o64 (row pitch Kh, complex128), k >= Kh -> o32 (row pitch H+1-Kh, complex64).
+template <int H>
+__global__ void __launch_bounds__(TL * Pow2<H>::Q)
+    r2c_split_kernel(const double* __restrict__ in, double2* __restrict__ o64,
+                     float2* __restrict__ o32, int64_t nrows, int Kh, double scale,
+                     const double2* __restrict__ twg)
+{
+    using Pl = Pow2<H>;
+    constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
+    constexpr int N = 2 * H;
+    extern __shared__ double2 sm[];
+    double2* tw = sm;
+    double2* ex = sm + N;
+    double2* twH = ex + TL * LS;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int j = tid % Q, t = tid / Q;
+    const int64_t R = (int64_t)blockIdx.x * TL + t;
+    const bool valid = R < nrows;
+    const int Kl = H + 1 - Kh;
+    for (int i = tid; i < N; i += nth) {
+        tw[i] = twg[i];
+        if ((i & 1) == 0) twH[i >> 1] = twg[i];
+    }
+    double2 a[P];
+    const double* src = in + (valid ? R : 0) * N;
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        const int m = j + Q * r;
+        a[r] = valid ? *reinterpret_cast<const double2*>(src + 2 * m) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    double2* ln = ex + t * LS;
+    pow2_fft<H, false>(a, j, ln, twH);
+#pragma unroll
+    for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<H>(j, h, k2)] = a[h * Q + k2];
+    __syncthreads();
+    if (valid) {
+        for (int k = j; k <= H; k += Q) {
+            const double2 zk = ln[k == H ? 0 : k];
+            const double2 zc = cconj(ln[k == 0 ? 0 : H - k]);
+            const double2 d = cmul(tw[k], csub(zk, zc));
+            const double2 X = make_double2(0.5 * (zk.x + zc.x + d.y) * scale, 0.5 * (zk.y + zc.y - d.x) * scale);
+            if (k < Kh) o64[R * Kh + k] = X;
+            else o32[R * Kl + (k - Kh)] = from_c64<float2>(X);
+        }
+    }
+}
+
+// C2R (FP64 arithmetic) of rows merged from o64 (k < Kh) and o32 (k >= Kh).
+template <int H>
+__global__ void __launch_bounds__(TL * Pow2<H>::Q)
+    c2r_merge_kernel(const double2* __restrict__ i64, const float2* __restrict__ i32,
+                     double* __restrict__ out, int64_t nrows, int Kh, double scale,
+                     const double2* __restrict__ twg)
+{
+    using Pl = Pow2<H>;
+    constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
+    constexpr int N = 2 * H;
+    extern __shared__ double2 sm[];
+    double2* tw = sm;
+    double2* ex = sm + N;
+    double2* twH = ex + TL * LS;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int j = tid % Q, t = tid / Q;
+    const int64_t R = (int64_t)blockIdx.x * TL + t;
+    const bool valid = R < nrows;
+    const int Kl = H + 1 - Kh;
+    for (int i = tid; i < N; i += nth) {
+        tw[i] = twg[i];
+        if ((i & 1) == 0) twH[i >> 1] = twg[i];
+    }
+    double2* ln = ex + t * LS;
+    for (int k = j; k <= H; k += Q) {
+        double2 v = make_double2(0.0, 0.0);
+        if (valid) v = (k < Kh) ? i64[R * Kh + k] : to_c64(i32[R * Kl + (k - Kh)]);
+        if (k == 0 || k == H) v.y = 0.0;
+        ln[k] = v;
+    }
+    __syncthreads();
+    double2 a[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        const int k = j + Q * r;
+        const double2 xk = ln[k];
+        const double2 xc = cconj(ln[H - k]);
+        const double2 d = cmul(cconj(tw[k]), csub(xk, xc));
+        a[r] = make_double2(xk.x + xc.x - d.y, xk.y + xc.y + d.x);
+    }
+    __syncthreads();
+    pow2_fft<H, true>(a, j, ln, twH);
+    if (valid) {
+        double* dst = out + R * N;
+#pragma unroll
+        for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+            for (int k2 = 0; k2 < Q; ++k2) {
+                const int m = pow2_out_index<H>(j, h, k2);
+                const double2 z = a[h * Q + k2];
+                *reinterpret_cast<double2*>(dst + 2 * m) = make_double2(z.x * scale, z.y * scale);
+            }
+    }
+}
+
+template <int H, bool SPLIT>
+static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int Kh, double scale,
+                        hipStream_t s)
+{
+    using Pl = Pow2<H>;
+    const double2* tw = fft_table(2 * H);
+    SFA_CHECK(tw, SFA_ERR_HIP, "twiddle table allocation failed");
+    const size_t lds = sizeof(double2) * ((size_t)2 * H + (size_t)TL * Pl::LS + H);
+    dim3 grid((unsigned)((nrows + TL - 1) / TL)), block(TL * Pl::Q);
+    if (SPLIT) {
+        auto k = r2c_split_kernel<H>;
+        if (lds > 64 * 1024)
+            hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k, grid, block, lds, s, (const double*)in, (double2*)o64, (float2*)o32,
+                           nrows, Kh, scale, tw);
+    } else {
+        auto k = c2r_merge_kernel<H>;
+        if (lds > 64 * 1024)
+            hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k, grid, block, lds, s, (const double2*)o64, (const float2*)o32,
+                           (double*)in, nrows, Kh, scale, tw);
+    }
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+// split (r2c=true: in -> o64/o32) or merge (r2c=false: o64/o32 -> in) of contiguous rows
+int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
+             double scale, hipStream_t s, int* rc)
+{
+#define SFA_SP(HH)                                                                            \
+    case HH:                                                                                  \
+        *rc = r2c ? launch_split<HH, true>(in, o64, o32, nrows, Kh, scale, s)                 \
+                  : launch_split<HH, false>(in, o64, o32, nrows, Kh, scale, s);               \
+        return 1;
+    switch (H) {
+        SFA_SP(16) SFA_SP(32) SFA_SP(64) SFA_SP(128) SFA_SP(256)
+        default: return 0;
+    }
+#undef SFA_SP
 }
 
 }  // namespace sfa

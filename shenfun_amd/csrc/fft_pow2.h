@@ -30,29 +30,30 @@ __device__ constexpr double W32C[32] = {
     0.0, 0.19509032201612826785, 0.38268343236508977173, 0.55557023301960222474,
     0.70710678118654752440, 0.83146961230254523708, 0.92387953251128675613, 0.98078528040323044913};
 
-template <int M, bool INV>
-__device__ inline double2 tw32(double2 a)
+template <int M, bool INV, typename C2>
+__device__ inline C2 tw32(C2 a)
 {
+    using R = typename CT<C2>::R;
     constexpr int m = M & 31;
     if constexpr ((m & 1) == 0) {
         return tw16<(m >> 1), INV>(a);
     } else {
-        const double c = W32C[m];
-        const double sfwd = -W32C[(m + 24) & 31];  // sin(-2 pi m/32) = -cos(2 pi (m - 8)/32)
-        const double s = INV ? -sfwd : sfwd;
-        return make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
+        const R c = (R)W32C[m];
+        const R sfwd = (R)(-W32C[(m + 24) & 31]);  // sin(-2 pi m/32) = -cos(2 pi (m - 8)/32)
+        const R s = INV ? -sfwd : sfwd;
+        return CT<C2>::mk(a.x * c - a.y * s, a.x * s + a.y * c);
     }
 }
 
-template <bool INV>
-__device__ inline void dft32(double2* a)
+template <bool INV, typename C2>
+__device__ inline void dft32(C2* a)
 {
     // 32 = 4 (r1) x 8 (r2): r = 8 r1 + r2; DFT-4 over r1 -> q1, twiddle
     // w32^(r2 q1), DFT-8 over r2 -> q2; q = q1 + 4 q2
-    double2 c[32];
+    C2 c[32];
 #pragma unroll
     for (int r2 = 0; r2 < 8; ++r2) {
-        double2 x0 = a[r2], x1 = a[8 + r2], x2 = a[16 + r2], x3 = a[24 + r2];
+        C2 x0 = a[r2], x1 = a[8 + r2], x2 = a[16 + r2], x3 = a[24 + r2];
         dft4<INV>(x0, x1, x2, x3);
         c[r2 * 4 + 0] = x0;
         c[r2 * 4 + 1] = x1;
@@ -67,7 +68,7 @@ __device__ inline void dft32(double2* a)
 #undef SFA_T32
 #pragma unroll
     for (int q1 = 0; q1 < 4; ++q1) {
-        double2 b[8];
+        C2 b[8];
 #pragma unroll
         for (int r2 = 0; r2 < 8; ++r2) b[r2] = c[r2 * 4 + q1];
         dft8<INV>(b);
@@ -75,23 +76,18 @@ __device__ inline void dft32(double2* a)
         for (int q2 = 0; q2 < 8; ++q2) a[q1 + 4 * q2] = b[q2];
     }
 }
-template <>
-__device__ inline void dft<32, false>(double2* a) { dft32<false>(a); }
-template <>
-__device__ inline void dft<32, true>(double2* a) { dft32<true>(a); }
 
 // One line FFT.  ex: this line's LDS exchange area (Pow2<N>::LS complex);
 // tw: LDS table exp(-2 pi i m/N), m < N.  Contains two __syncthreads(): every
 // thread of the block must call it the same number of times.
-template <int N, bool INV>
-__device__ inline void pow2_fft(double2 (&a)[Pow2<N>::P], int j, double2* ex,
-                                const double2* tw)
+template <int N, bool INV, typename C2>
+__device__ inline void pow2_fft(C2 (&a)[Pow2<N>::P], int j, C2* ex, const C2* tw)
 {
     constexpr int P = Pow2<N>::P, Q = Pow2<N>::Q;
     dft<P, INV>(a);
 #pragma unroll
     for (int k1 = 1; k1 < P; ++k1) {
-        double2 w = tw[j * k1];
+        C2 w = tw[j * k1];
         if (INV) w.y = -w.y;
         a[k1] = cmul(a[k1], w);
     }
@@ -100,7 +96,7 @@ __device__ inline void pow2_fft(double2 (&a)[Pow2<N>::P], int j, double2* ex,
     __syncthreads();
 #pragma unroll
     for (int h = 0; h < P / Q; ++h) {
-        double2 b[Q];
+        C2 b[Q];
         const int k1 = h * Q + j;
 #pragma unroll
         for (int jj = 0; jj < Q; ++jj) b[jj] = ex[k1 * (Q + 1) + jj];

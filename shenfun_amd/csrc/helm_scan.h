@@ -266,33 +266,149 @@ __device__ inline void helm_chain_scan(double2* X, int p, int M, double al, doub
 
 namespace sfa {
 
-// Solve one chain with precomputed per-row factors fc[4m + {0,1,2,3}] =
-// (1/d0, d1/d0, d2/d0, -L_{m-1}) for the chain's K rows (row 0 slot 3 holds the
-// Neumann gauge flag instead).  Forward elimination and back substitution as
-// the affine scans of helm_chain_scan_f; no division on the device.
-template <int E, int W>
-__device__ inline void helm_chain_solve_fac(const double2 (&f)[E], double2* X, int p, int M, bool nm,
-                                            const double* __restrict__ fc, int K, int lane)
+// Row i of (alfa A + beta B) for alfa = -1 from the closed-form Chebyshev
+// SD/SN Galerkin diagonals (chebyshev/matrices.py:221-281, 377-427,
+// 1047-1165), with the Neumann 1/j^2 column scaling and k = 0 gauge row of
+// LU_Helmholtz_1D (la.pyx:735-749); the same values as helm_row on the
+// uploaded tables, computed in registers.  Rows i >= M are identity padding.
+__device__ inline HelmRow helm_row_cf(int i, int M, double be, bool nm, bool zb)
 {
-    const int m0 = lane * E;
-    double inv[E], e1[E], e2[E], am[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int m = m0 + e;
-        if (m < K) {
-            const double2 q0 = *reinterpret_cast<const double2*>(fc + 4 * m);
-            const double2 q1 = *reinterpret_cast<const double2*>(fc + 4 * m + 2);
-            inv[e] = q0.x;
-            e1[e] = q0.y;
-            e2[e] = q1.x;
-            am[e] = q1.y;
+    const double pi = 3.14159265358979323846, hp = 0.5 * pi;
+    HelmRow r;
+    if (i >= M) {
+        r.h = 1.0;
+        r.s = r.c = r.g = 0.0;
+        return r;
+    }
+    const double k = (double)i;
+    const double hn0 = (i == 0) ? pi : hp;
+    double a0, a2 = 0.0, a4 = 0.0, b0, b2 = 0.0, bm = 0.0;
+    if (!nm) {
+        a0 = -2.0 * pi * (k + 1.0) * (k + 2.0);
+        if (i < M - 2) {
+            a2 = -4.0 * pi * (k + 1.0);
+            b2 = -hp;
+        }
+        if (i < M - 4) a4 = -4.0 * pi * (k + 1.0);
+        b0 = hn0 + hp;
+        if (i >= 2) bm = -hp;
+    } else {
+        const double q = k / (k + 2.0), al = q * q;
+        const double k2i = 1.0 / ((k + 2.0) * (k + 2.0));
+        if (i == 0 && zb) {
+            a0 = -1.0;
+            b0 = 0.0;
         } else {
-            inv[e] = 1.0;
-            e1[e] = e2[e] = am[e] = 0.0;
+            a0 = -2.0 * pi * k * k * (k + 1.0) / (k + 2.0);
+            b0 = hn0 + al * al * hp;
+            if (i > 0) {
+                const double ii = 1.0 / (k * k);
+                a0 *= ii;
+                b0 *= ii;
+            }
+        }
+        if (i < M - 2) {
+            a2 = -4.0 * pi * (k + 2.0) * (k + 2.0) * (k + 1.0) * k2i * k2i;
+            b2 = -al * hp * k2i;
+        }
+        if (i < M - 4) a4 = -4.0 * pi * (k + 4.0) * (k + 4.0) * (k + 1.0) * k2i / ((k + 4.0) * (k + 4.0));
+        if (i >= 2) {
+            const double qm = (k - 2.0) / k;
+            bm = -(qm * qm) * hp;
+            if (i - 2 >= 1) bm *= 1.0 / ((k - 2.0) * (k - 2.0));
         }
     }
-    const bool gauge = nm && m0 == 0 && am[0] != 0.0;
-    if (m0 == 0) am[0] = 0.0;
+    r.h = -a0 + be * b0;
+    r.s = -a2 + be * b2;
+    r.c = -a4;
+    r.g = be * bm;
+    return r;
+}
+
+// Device view of a fused-solve plan: line -> entry map, s = ky^2 + kz^2 per
+// entry, segment restart states per (entry, parity), and the operator's
+// diagonals.
+struct XPlan {
+    const int* linemap;   // (ky * Nzh + kz) -> entry
+    const double* svals;  // [nu]
+    const double4* bs;    // [nu][2][16]
+};
+
+// One row of the LU recurrence of LU_Helmholtz_1D (la.pyx:751-763) in ratio
+// form.  State in: (r1, r2, pinv) = (d1/d0, d2/d0, 1/d0) of the previous row of
+// the chain (first = first row of the chain).  Out: this row's 1/d0, the new
+// ratios, and the forward-elimination multiplier am = -L.
+__device__ inline void helm_lu_row(const HelmRow& r, int i, int M, bool first, double& r1, double& r2,
+                                   double& pinv, double& iv, double& am)
+{
+    double d0, d1, d2;
+    if (first) {
+        d0 = r.h;
+        d1 = r.s;
+        d2 = r.c;
+        am = 0.0;
+    } else {
+        d0 = r.h - r.g * r1;
+        d1 = (i < M - 2) ? r.s - r.g * r2 : 0.0;
+        d2 = (i < M - 4) ? r.c - r.g * r2 : 0.0;
+        am = -r.g * pinv;
+    }
+    iv = 1.0 / d0;
+    r1 = d1 * iv;
+    r2 = d2 * iv;
+    pinv = iv;
+}
+
+// Segment restart states of the LU recurrence.  The chain's K rows are cut
+// into 16 segments of E rows (one per lane of the solve).  bs[(2u + p) 16 + l]
+// = (d1/d0, d2/d0, 1/d0, 0) of row l E - 1 of chain p (zero for l = 0), for
+// alfa = -1, beta = alpha + s[u]: the sequential recurrence helm_lu_row,
+// sampled at the segment boundaries.  One thread per (entry, parity).
+static __global__ void helm_bstate_kernel(int M, int E, int neumann,
+                                          double alpha, const double* __restrict__ svals,
+                                          int64_t nu, double4* __restrict__ bs)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= 2 * nu) return;
+    const int64_t u = gid >> 1;
+    const int p = (int)(gid & 1);
+    const double be = alpha + svals[u];
+    const bool nm = neumann != 0;
+    const bool zb = nm && fabs(be) < 1e-8;
+    double4* out = bs + (2 * u + p) * 16;
+    out[0] = make_double4(0.0, 0.0, 0.0, 0.0);
+    double r1 = 0.0, r2 = 0.0, pinv = 0.0, iv, am;
+    for (int m = 0; m < 16 * E; ++m) {
+        if (m > 0 && m % E == 0) out[m / E] = make_double4(r1, r2, pinv, 0.0);
+        const int i = 2 * m + p;
+        const HelmRow r = helm_row_cf(i, M, be, nm, zb);
+        helm_lu_row(r, i, M, m == 0, r1, r2, pinv, iv, am);
+    }
+}
+
+// Solve one chain from the segment restart state bs = (d1/d0, d2/d0, 1/d0) of
+// the row before this lane's segment: the lane regenerates its E rows of LU
+// factors with the exact row recurrence, then the affine scans of
+// helm_chain_solve_fac.  X: the line, any storage type C2.
+template <int E, int W, typename C2>
+__device__ inline void helm_chain_solve_bs(const double2 (&f)[E], C2* X, int p, int M, double be,
+                                           bool nm, double4 bs, int lane)
+{
+    const bool zb = nm && fabs(be) < 1e-8;
+    const int m0 = lane * E;
+    double r1 = bs.x, r2 = bs.y, pinv = bs.z;
+    double inv[E], e1[E], e2[E], am[E];
+    bool gauge = false;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = 2 * (m0 + e) + p;
+        const HelmRow r = helm_row_cf(i, M, be, nm, zb);
+        helm_lu_row(r, i, M, i < 2, r1, r2, pinv, inv[e], am[e]);
+        e1[e] = r1;
+        e2[e] = r2;
+        if (nm && i == 0) gauge = (r.h - 1.0) * (r.h - 1.0) < 1e-16;
+    }
+    // forward elimination y_m = f_m + am_m y_{m-1}: affine prefix scan over lanes
     double A = 1.0;
     double2 B = make_double2(0.0, 0.0);
 #pragma unroll
@@ -368,59 +484,9 @@ __device__ inline void helm_chain_solve_fac(const double2 (&f)[E], double2* X, i
                     out = make_double2(out.x * s, out.y * s);
                 }
             }
-            X[i] = out;
+            X[i] = from_c64<C2>(out);
         }
     }
-}
-
-// One thread per (table entry, parity): the exact row recurrence of
-// LU_Helmholtz_1D (la.pyx:751-763) with alfa = -1, beta = alpha + s[u],
-// stored as per-row factors (1/d0, d1/d0, d2/d0, -L) in fac[(2u + p) K + m].
-static __global__ void helm_factor_kernel(HelmTab tab, int M, int K, int neumann, double alpha,
-                                   const double* __restrict__ svals, int64_t nu,
-                                   double* __restrict__ fac)
-{
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= 2 * nu) return;
-    const int64_t u = gid >> 1;
-    const int p = (int)(gid & 1);
-    const double al = -1.0, be = alpha + svals[u];
-    const bool nm = neumann != 0;
-    const bool zb = nm && fabs(be) < 1e-8;
-    double* fc = fac + (2 * u + p) * (int64_t)K * 4;
-    double d0p = 1.0, d1p = 0.0, d2p = 0.0, d00 = 0.0;
-    for (int m = 0; m < K; ++m) {
-        const int i = 2 * m + p;
-        if (i >= M) {
-            fc[4 * m] = 1.0;
-            fc[4 * m + 1] = fc[4 * m + 2] = fc[4 * m + 3] = 0.0;
-            continue;
-        }
-        const HelmRow r = helm_row(tab, i, M, al, be, nm, zb);
-        double d0, d1, d2, a;
-        if (m == 0) {
-            d0 = r.h;
-            d1 = r.s;
-            d2 = r.c;
-            a = 0.0;
-            d00 = d0;
-        } else {
-            const double l = r.g / d0p;
-            d0 = r.h - l * d1p;
-            d1 = (i < M - 2) ? r.s - l * d2p : 0.0;
-            d2 = (i < M - 4) ? r.c - l * d2p : 0.0;
-            a = -l;
-        }
-        const double iv = 1.0 / d0;
-        fc[4 * m] = iv;
-        fc[4 * m + 1] = d1 * iv;
-        fc[4 * m + 2] = d2 * iv;
-        fc[4 * m + 3] = a;
-        d0p = d0;
-        d1p = d1;
-        d2p = d2;
-    }
-    if (p == 0) fc[3] = (nm && (d00 - 1.0) * (d00 - 1.0) < 1e-16) ? 1.0 : 0.0;
 }
 
 }  // namespace sfa

@@ -231,16 +231,26 @@ static const double* cheb_k2(int N, int neumann)
 }
 
 int fast_xpass(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, const double* K2,
-               const int* linemap, const double* fac, hipStream_t s, int* rc);
+               const XPlan& pl, hipStream_t s, int* rc);
+int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha,
+                   int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc);
+int fast_c2c(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
+             hipStream_t s, int* rc);
+int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
+                 hipStream_t s, int* rc);
+int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
+             double scale, hipStream_t s, int* rc);
 
 // ------------------------------------------------------------------ plan
-// Device plan of the fused solve: the Helmholtz LU factors of every distinct
-// beta = alpha + ky^2 + kz^2 (the counterpart of chebyshev.la.Helmholtz's
-// factor arrays, chebyshev/la.py:270-276, deduplicated), and the map from a
-// (ky, kz) line to its entry.  Layout (bytes): linemap int32[nl] | s double[nu] |
-// fac double[nu][2][K][4].
+// Device plan of the fused solve, the counterpart of chebyshev.la.Helmholtz's
+// factor arrays (chebyshev/la.py:270-276) deduplicated over the distinct
+// beta = alpha + ky^2 + kz^2: the map from a (ky, kz) line to its entry, s per
+// entry and, per (entry, parity chain), the LU recurrence state at the 16
+// segment starts of the chain (the solve regenerates each segment's factors
+// from it).  Layout (bytes): linemap int32[nl] | s double[nu] |
+// bs double4[nu][2][16].
 struct PlanLayout {
-    int64_t nl, nu, K, off_s, off_fac, bytes;
+    int64_t nl, nu, K, E, off_s, off_bs, bytes;
 };
 
 static int64_t round256(int64_t b) { return (b + 255) / 256 * 256; }
@@ -251,6 +261,7 @@ static void plan_shape(int64_t Nx, int64_t Ny, int64_t Nz, PlanLayout* L, std::v
     const int64_t Nzh = Nz / 2 + 1;
     L->nl = Ny * Nzh;
     L->K = (Nx - 2 + 1) / 2;
+    L->E = (L->K + 15) / 16;
     std::vector<int64_t> sv((size_t)L->nl);
     for (int64_t j = 0; j < Ny; ++j) {
         const int64_t ky = (j < Ny / 2 || (Ny % 2 == 1 && j == Ny / 2)) ? j : j - Ny;
@@ -261,8 +272,8 @@ static void plan_shape(int64_t Nx, int64_t Ny, int64_t Nz, PlanLayout* L, std::v
     us.erase(std::unique(us.begin(), us.end()), us.end());
     L->nu = (int64_t)us.size();
     L->off_s = round256(L->nl * 4);
-    L->off_fac = L->off_s + round256(L->nu * 8);
-    L->bytes = L->off_fac + L->nu * 2 * L->K * 4 * 8;
+    L->off_bs = L->off_s + round256(L->nu * 8);
+    L->bytes = L->off_bs + L->nu * 2 * 16 * (int64_t)sizeof(double4);
     if (map) {
         map->resize((size_t)L->nl);
         for (size_t i = 0; i < sv.size(); ++i)
@@ -278,28 +289,39 @@ struct PlanInfo {
     int64_t Nx, Ny, Nz, bytes;
     double alpha;
     int neumann;
-    int64_t off_fac;
+    int64_t off_s, off_bs;
 };
 static std::mutex g_plan_mu;
 static std::map<const void*, PlanInfo> g_plans;
 
-// offset of the factor table inside a registered plan, or -1 if the plan does
-// not match (sizes, alpha, bc) or was not built by sfa_helmholtz3d_plan
-static int64_t plan_fac_offset(const void* plan, int64_t Nx, int64_t Ny, int64_t Nz, double alpha,
-                               int neumann)
+// Device view of a registered plan; false if the plan does not match (sizes,
+// alpha, bc) or was not built by sfa_helmholtz3d_plan.
+static bool plan_view(const void* plan, int64_t Nx, int64_t Ny, int64_t Nz, double alpha, int neumann,
+                      XPlan* v)
 {
-    std::lock_guard<std::mutex> lk(g_plan_mu);
-    auto it = g_plans.find(plan);
-    if (it == g_plans.end()) return -1;
-    const PlanInfo& p = it->second;
-    if (p.Nx == Nx && p.Ny == Ny && p.Nz == Nz && p.alpha == alpha && p.neumann == neumann)
-        return p.off_fac;
-    return -1;
+    int64_t off_s, off_bs;
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        auto it = g_plans.find(plan);
+        if (it == g_plans.end()) return false;
+        const PlanInfo& p = it->second;
+        if (!(p.Nx == Nx && p.Ny == Ny && p.Nz == Nz && p.alpha == alpha && p.neumann == neumann))
+            return false;
+        off_s = p.off_s;
+        off_bs = p.off_bs;
+    }
+    if (v) {
+        const char* pb = (const char*)plan;
+        v->linemap = (const int*)pb;
+        v->svals = (const double*)(pb + off_s);
+        v->bs = (const double4*)(pb + off_bs);
+    }
+    return true;
 }
 
 static bool plan_ok(const void* plan, int64_t Nx, int64_t Ny, int64_t Nz, double alpha, int neumann)
 {
-    return plan_fac_offset(plan, Nx, Ny, Nz, alpha, neumann) >= 0;
+    return plan_view(plan, Nx, Ny, Nz, alpha, neumann, nullptr);
 }
 
 int xpass_launch(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, hipStream_t s,
@@ -308,13 +330,11 @@ int xpass_launch(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann,
     if (plan != nullptr) {
         const double* K2 = cheb_k2(Nx, neumann);
         SFA_CHECK(K2, SFA_ERR_HIP, "table allocation failed");
-        const int64_t off = plan_fac_offset(plan, Nx, Ny, 2 * (Nzh - 1), alpha, neumann);
-        SFA_CHECK(off >= 0, SFA_ERR_ARG, "plan does not match the solve");
-        const char* pb = (const char*)plan;
+        XPlan pv;
+        SFA_CHECK(plan_view(plan, Nx, Ny, 2 * (Nzh - 1), alpha, neumann, &pv), SFA_ERR_ARG,
+                  "plan does not match the solve");
         int rc = 0;
-        if (fast_xpass(W, Nx, Ny, Nzh, alpha, neumann, K2, (const int*)pb,
-                       (const double*)(pb + off), s, &rc))
-            return rc;
+        if (fast_xpass(W, Nx, Ny, Nzh, alpha, neumann, K2, pv, s, &rc)) return rc;
     }
     XPassArgs a;
     a.W = W;
@@ -353,6 +373,55 @@ int xpass_launch(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann,
     return SFA_OK;
 }
 
+// Mixed-precision pipeline: kz planes [0, Kh) stay complex128 with FP64
+// transforms, planes [Kh, Nz/2] are stored and transformed as complex64; the
+// first (R2C of the FP64 f) and last (C2R to the FP64 u) transforms and the
+// Helmholtz solve are FP64.
+static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
+                             double alpha, int neumann, int Kh, const void* plan, void* work,
+                             hipStream_t s)
+{
+    const int64_t Nzh = Nz / 2 + 1;
+    const int Kl = (int)Nzh - Kh;
+    XPlan pv;
+    SFA_CHECK(plan_view(plan, Nx, Ny, Nz, alpha, neumann, &pv), SFA_ERR_ARG,
+              "mixed precision needs a matching plan");
+    const double* K2 = cheb_k2((int)Nx, neumann);
+    SFA_CHECK(K2, SFA_ERR_HIP, "table allocation failed");
+    double2* W64 = (double2*)work;
+    float2* W32 = (float2*)(W64 + Nx * Ny * Kh);
+    const int64_t rows = Nx * Ny;
+    int rc = 0;
+    auto geom = [&](int Kp) {
+        Geom g;
+        g.n = Ny;
+        g.sa = Kp;
+        g.d1 = Nx;
+        g.s1 = Ny * Kp;
+        g.d2 = Kp;
+        g.s2 = 1;
+        return g;
+    };
+    const Geom g64 = geom(Kh), g32 = geom(Kl);
+    if (!split_rc(true, (int)(Nz / 2), f, W64, W32, rows, Kh, 1.0 / (double)Nz, s, &rc))
+        return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz=%lld unsupported", (long long)Nz);
+    if (rc) return rc;
+    if (!fast_c2c(g64, g64, W64, W64, -1, 1.0 / (double)Ny, s, &rc) || rc ||
+        !fast_c2c_f32(g32, g32, W32, W32, -1, 1.0 / (double)Ny, s, &rc) || rc)
+        return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Ny=%lld unsupported", (long long)Ny);
+    if (!fast_xpass_sub(W64, false, (int)Nx, (int)Ny, (int)Nzh, Kh, 0, alpha, neumann, K2, pv, s,
+                        &rc) || rc ||
+        !fast_xpass_sub(W32, true, (int)Nx, (int)Ny, (int)Nzh, Kl, Kh, alpha, neumann, K2, pv, s,
+                        &rc) || rc)
+        return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nx=%lld unsupported", (long long)Nx);
+    if (!fast_c2c(g64, g64, W64, W64, +1, 1.0, s, &rc) || rc ||
+        !fast_c2c_f32(g32, g32, W32, W32, +1, 1.0, s, &rc) || rc)
+        return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Ny unsupported");
+    if (!split_rc(false, (int)(Nz / 2), u, W64, W32, rows, Kh, 1.0, s, &rc))
+        return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz unsupported");
+    return rc;
+}
+
 }  // namespace sfa
 
 using namespace sfa;
@@ -383,40 +452,45 @@ extern "C" int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double a
     plan_shape(Nx, Ny, Nz, &L, &map, &sv);
     SFA_CHECK(plan_bytes >= L.bytes, SFA_ERR_ARG, "plan buffer too small (%lld < %lld)",
               (long long)plan_bytes, (long long)L.bytes);
-    const double* tabs = helm_tables((int)Nx, neumann);
-    SFA_CHECK(tabs, SFA_ERR_HIP, "table allocation failed");
     hipStream_t s = (hipStream_t)stream;
     char* pb = (char*)plan;
     SFA_HIP(hipMemcpyAsync(pb, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice, s));
     SFA_HIP(hipMemcpyAsync(pb + L.off_s, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice, s));
     const int M = (int)Nx - 2;
-    HelmTab tab{tabs, tabs + M, tabs + 2 * M, tabs + 3 * M, tabs + 4 * M, tabs + 5 * M};
     const int64_t nth = 2 * L.nu;
-    hipLaunchKernelGGL(helm_factor_kernel, dim3((unsigned)((nth + 127) / 128)), dim3(128), 0, s, tab, M,
-                       (int)L.K, neumann, alpha, (const double*)(pb + L.off_s), L.nu,
-                       (double*)(pb + L.off_fac));
+    hipLaunchKernelGGL(helm_bstate_kernel, dim3((unsigned)((nth + 127) / 128)), dim3(128), 0, s, M,
+                       (int)L.E, neumann, alpha, (const double*)(pb + L.off_s), L.nu,
+                       (double4*)(pb + L.off_bs));
     SFA_LAUNCH_CHECK();
     // the host vectors are copied before they go out of scope
     SFA_HIP(hipStreamSynchronize(s));
     {
         std::lock_guard<std::mutex> lk(g_plan_mu);
-        g_plans[plan] = PlanInfo{Nx, Ny, Nz, plan_bytes, alpha, neumann, L.off_fac};
+        g_plans[plan] = PlanInfo{Nx, Ny, Nz, plan_bytes, alpha, neumann, L.off_s, L.off_bs};
     }
     return SFA_OK;
 }
 
 extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
-                               double alpha, int neumann, int precision, int refine, const void* plan,
-                               void* work, int64_t work_bytes, void* stream)
+                               double alpha, int neumann, int precision, int fp64_planes,
+                               const void* plan, void* work, int64_t work_bytes, void* stream)
 {
     SFA_CHECK(plan == nullptr || plan_ok(plan, Nx, Ny, Nz, alpha, neumann), SFA_ERR_ARG,
               "plan was not built by sfa_helmholtz3d_plan for these sizes/alpha/bc");
-    (void)refine;
     SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
               "need Nx >= 8, Ny >= 1, even Nz >= 2");
-    SFA_CHECK(precision == 0, SFA_ERR_UNSUPPORTED, "precision %d not built yet", precision);
+    SFA_CHECK(precision == 0 || precision == 1, SFA_ERR_ARG, "precision must be 0 or 1");
     SFA_CHECK(work_bytes >= sfa_helmholtz3d_workspace(Nx, Ny, Nz, precision), SFA_ERR_ARG,
               "workspace too small");
+    if (precision == 1) {
+        const int64_t Nzh = Nz / 2 + 1;
+        int Kh = fp64_planes > 0 ? fp64_planes : 17;
+        if (Kh < 2) Kh = 2;
+        if (Kh > Nzh - 2) Kh = (int)Nzh - 2;
+        SFA_CHECK(Nzh >= 4, SFA_ERR_UNSUPPORTED, "mixed precision needs Nz >= 6");
+        return helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann, Kh, plan, work,
+                                 (hipStream_t)stream);
+    }
     const int64_t Nzh = Nz / 2 + 1;
     double2* W = (double2*)work;
     const int64_t rshape[3] = {Nx, Ny, Nz};

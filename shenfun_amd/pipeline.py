@@ -4,8 +4,15 @@
 R2C TensorProductSpace in one C-ABI call (sfa_helmholtz3d): the sequence
 T.scalar_product -> chebyshev.la.Helmholtz -> T.backward of the reference
 (tensorproductspace.py:146-190, chebyshev/la.py:288-317,
-tensorproductspace.py:46-96), with the Helmholtz LU computed on the fly from
-beta = alpha + ky^2 + kz^2 instead of streamed factor arrays.
+tensorproductspace.py:46-96), with the Helmholtz LU factorised once per distinct beta = alpha + ky^2 + kz^2
+at construction (like chebyshev.la.Helmholtz.__init__).
+
+precision='mixed' keeps the lowest ``fp64_planes`` kz planes (the dominant
+part of smooth data) in complex128 with FP64 transforms and stores/transforms
+the rest as complex64; the solve and the first/last (physical) transforms stay
+FP64.  Its error is data dependent (it relies on the decay of the spectrum
+along z) and is measured against the FP64 oracle by the tests and the
+benchmark.
 """
 import torch
 
@@ -14,7 +21,7 @@ from .bases import C2C, ChebyshevBase, R2C
 
 
 class HelmholtzSolver3D:
-    def __init__(self, T, alpha, precision="fp64", refine=2):
+    def __init__(self, T, alpha, precision="fp64", fp64_planes=0):
         b = T.bases
         if not (len(b) == 3 and isinstance(b[0], ChebyshevBase) and b[0].is_composite
                 and isinstance(b[1], C2C) and isinstance(b[2], R2C)):
@@ -23,7 +30,7 @@ class HelmholtzSolver3D:
         self.alpha = float(alpha)
         self.neumann = b[0].bc_kind == "N"
         self.precision = {"fp64": 0, "mixed": 1}[precision]
-        self.refine = int(refine)
+        self.fp64_planes = int(fp64_planes)
         self.shape = T.shape(False)
         Nx, Ny, Nz = self.shape
         nbytes = _lib.lib().sfa_helmholtz3d_workspace(Nx, Ny, Nz, self.precision)
@@ -44,6 +51,6 @@ class HelmholtzSolver3D:
             raise ValueError("u must match f")
         Nx, Ny, Nz = self.shape
         _lib.call("sfa_helmholtz3d", _lib.ptr(f), _lib.ptr(u), Nx, Ny, Nz, self.alpha,
-                  int(self.neumann), self.precision, self.refine, _lib.ptr(self.plan),
+                  int(self.neumann), self.precision, self.fp64_planes, _lib.ptr(self.plan),
                   _lib.ptr(self.work), self.work.numel(), _lib.stream_ptr())
         return u

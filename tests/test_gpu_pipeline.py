@@ -82,3 +82,27 @@ def test_helmholtz3d_fused_repeat_identical(sfa):
     a = H(f).clone()
     b = H(f)
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("bc", ["D", "N"])
+@pytest.mark.parametrize("shape,planes", [((64, 64, 64), 0), ((128, 32, 64), 0), ((256, 64, 64), 0),
+                                          ((64, 16, 32), 12), ((256, 256, 256), 0)])
+def test_helmholtz3d_mixed(sfa, bc, shape, planes):
+    """precision='mixed' (FP64 low kz planes, FP32 transforms above, FP64
+    solve) stays within 1e-10 relative max error of the FP64 oracle on the
+    smooth workload right-hand side."""
+    print(f"[helm3d mixed {bc} {shape} planes={planes}]", flush=True)
+    Nx, Ny, Nz = shape
+    f = S.helmholtz3d_rhs(Nx, Ny, Nz, seed=5)
+    u_ref, _, _ = S.helmholtz3d_cheb_fourier(f, 1.0, bc)
+    T = sfa.TensorProductSpace(None, (sfa.FunctionSpace(Nx, "chebyshev", bc=bc),
+                                      sfa.FunctionSpace(Ny, "F", dtype="D"),
+                                      sfa.FunctionSpace(Nz, "F", dtype="d")))
+    H = sfa.HelmholtzSolver3D(T, 1.0, precision="mixed", fp64_planes=planes)
+    u = H(dev(f))
+    err = rel(u, u_ref)
+    print(f"  max rel err {err:.3e}", flush=True)
+    assert err < 1e-10
+    # all-FP64 solve of the same input agrees with the oracle to round-off
+    u64 = sfa.HelmholtzSolver3D(T, 1.0)(dev(f))
+    assert rel(u64, u_ref) < 1e-12
