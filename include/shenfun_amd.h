@@ -211,14 +211,16 @@ int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t 
                     double alpha, int neumann, int precision, int fp64_planes, const void* plan,
                     void* work, int64_t work_bytes, void* stream);
 
-/* Profiling variant of sfa_helmholtz3d (FP64): runs the pipeline nrep times
- * with hipEvents recorded on `stream` between its kernels and adds each
- * stage's milliseconds into stage_ms[5] (z R2C, y C2C, x-pass, y C2C
- * inverse, z C2R).  Synchronises the stream.  No reference counterpart: it
- * exists to attribute the pipeline's time to its kernels. */
+/* Profiling variant of sfa_helmholtz3d: runs the pipeline (precision and
+ * fp64_planes as there) nrep times with hipEvents recorded on `stream` between
+ * its stages and adds each stage's milliseconds into stage_ms[5] (z R2C, y
+ * C2C, x-pass, y C2C inverse, z C2R; in mixed precision a stage covers its
+ * complex128 and complex64 launches).  Synchronises the stream.  No reference
+ * counterpart: it exists to attribute the pipeline's time to its kernels. */
 int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
-                                double alpha, int neumann, const void* plan, void* work,
-                                int64_t work_bytes, int nrep, float* stage_ms, void* stream);
+                                double alpha, int neumann, int precision, int fp64_planes,
+                                const void* plan, void* work, int64_t work_bytes, int nrep,
+                                float* stage_ms, void* stream);
 
 #ifdef __cplusplus
 }

@@ -40,7 +40,7 @@ _SIGS = {
     "sfa_poisson_add_solve": "vviiaaaivvdv",
     "sfa_helmholtz3d_plan": "llldivlv",
     "sfa_helmholtz3d": "vvllldiiivvlv",
-    "sfa_helmholtz3d_stage_times": "vvllldivvlivv",
+    "sfa_helmholtz3d_stage_times": "vvllldiiivvlivv",
 }
 
 _TYPES = {"v": _c_void_p, "i": _int, "l": _i64, "d": _dbl, "a": _p64}

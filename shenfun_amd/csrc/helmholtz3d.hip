@@ -379,8 +379,11 @@ int xpass_launch(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann,
 // Helmholtz solve are FP64.
 static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                              double alpha, int neumann, int Kh, const void* plan, void* work,
-                             hipStream_t s)
+                             hipStream_t s, hipEvent_t* ev)
 {
+    auto mark = [&](int i) {
+        if (ev) hipEventRecord(ev[i], s);
+    };
     const int64_t Nzh = Nz / 2 + 1;
     const int Kl = (int)Nzh - Kh;
     XPlan pv;
@@ -403,23 +406,69 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
         return g;
     };
     const Geom g64 = geom(Kh), g32 = geom(Kl);
+    mark(0);
     if (!split_rc(true, (int)(Nz / 2), f, W64, W32, rows, Kh, 1.0 / (double)Nz, s, &rc))
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz=%lld unsupported", (long long)Nz);
     if (rc) return rc;
+    mark(1);
     if (!fast_c2c(g64, g64, W64, W64, -1, 1.0 / (double)Ny, s, &rc) || rc ||
         !fast_c2c_f32(g32, g32, W32, W32, -1, 1.0 / (double)Ny, s, &rc) || rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Ny=%lld unsupported", (long long)Ny);
+    mark(2);
     if (!fast_xpass_sub(W64, false, (int)Nx, (int)Ny, (int)Nzh, Kh, 0, alpha, neumann, K2, pv, s,
                         &rc) || rc ||
         !fast_xpass_sub(W32, true, (int)Nx, (int)Ny, (int)Nzh, Kl, Kh, alpha, neumann, K2, pv, s,
                         &rc) || rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nx=%lld unsupported", (long long)Nx);
+    mark(3);
     if (!fast_c2c(g64, g64, W64, W64, +1, 1.0, s, &rc) || rc ||
         !fast_c2c_f32(g32, g32, W32, W32, +1, 1.0, s, &rc) || rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Ny unsupported");
+    mark(4);
     if (!split_rc(false, (int)(Nz / 2), u, W64, W32, rows, Kh, 1.0, s, &rc))
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz unsupported");
+    mark(5);
     return rc;
+}
+
+// FP64 pipeline: z R2C, y C2C, fused x-pass, y C2C inverse, z C2R.  ev: NULL
+// or 6 events recorded around the five stages.
+static int helmholtz3d_fp64(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
+                            double alpha, int neumann, const void* plan, void* work, hipStream_t s,
+                            hipEvent_t* ev)
+{
+    auto mark = [&](int i) {
+        if (ev) hipEventRecord(ev[i], s);
+    };
+    const int64_t Nzh = Nz / 2 + 1;
+    double2* W = (double2*)work;
+    const int64_t rshape[3] = {Nx, Ny, Nz};
+    const int64_t rstr[3] = {Ny * Nz, Nz, 1};
+    const int64_t cstr[3] = {Ny * Nzh, Nzh, 1};
+    const int64_t cshape[3] = {Nx, Ny, Nzh};
+    void* st = (void*)s;
+    int rc;
+    mark(0);
+    if ((rc = sfa_fft_r2c(f, W, 3, rshape, rstr, cstr, 2, 1.0 / (double)Nz, st))) return rc;
+    mark(1);
+    if ((rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, -1, 1.0 / (double)Ny, st))) return rc;
+    mark(2);
+    if ((rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, alpha, neumann, s, plan))) return rc;
+    mark(3);
+    if ((rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, +1, 1.0, st))) return rc;
+    mark(4);
+    if ((rc = sfa_fft_c2r(W, u, 3, rshape, cstr, rstr, 2, 1.0, st))) return rc;
+    mark(5);
+    return SFA_OK;
+}
+
+static int mixed_planes(int64_t Nz, int fp64_planes)
+{
+    const int64_t Nzh = Nz / 2 + 1;
+    int Kh = fp64_planes > 0 ? fp64_planes : 17;
+    if (Kh < 2) Kh = 2;
+    if (Kh > Nzh - 2) Kh = (int)Nzh - 2;
+    return Kh;
 }
 
 }  // namespace sfa
@@ -482,69 +531,43 @@ extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t N
     SFA_CHECK(precision == 0 || precision == 1, SFA_ERR_ARG, "precision must be 0 or 1");
     SFA_CHECK(work_bytes >= sfa_helmholtz3d_workspace(Nx, Ny, Nz, precision), SFA_ERR_ARG,
               "workspace too small");
+    hipStream_t s = (hipStream_t)stream;
     if (precision == 1) {
-        const int64_t Nzh = Nz / 2 + 1;
-        int Kh = fp64_planes > 0 ? fp64_planes : 17;
-        if (Kh < 2) Kh = 2;
-        if (Kh > Nzh - 2) Kh = (int)Nzh - 2;
-        SFA_CHECK(Nzh >= 4, SFA_ERR_UNSUPPORTED, "mixed precision needs Nz >= 6");
-        return helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann, Kh, plan, work,
-                                 (hipStream_t)stream);
+        SFA_CHECK(Nz / 2 + 1 >= 4, SFA_ERR_UNSUPPORTED, "mixed precision needs Nz >= 6");
+        return helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann, mixed_planes(Nz, fp64_planes),
+                                 plan, work, s, nullptr);
     }
-    const int64_t Nzh = Nz / 2 + 1;
-    double2* W = (double2*)work;
-    const int64_t rshape[3] = {Nx, Ny, Nz};
-    const int64_t rstr[3] = {Ny * Nz, Nz, 1};
-    const int64_t cstr[3] = {Ny * Nzh, Nzh, 1};
-    const int64_t cshape[3] = {Nx, Ny, Nzh};
-    int rc;
-    if ((rc = sfa_fft_r2c(f, W, 3, rshape, rstr, cstr, 2, 1.0 / (double)Nz, stream))) return rc;
-    if ((rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, -1, 1.0 / (double)Ny, stream))) return rc;
-    if ((rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, alpha, neumann, (hipStream_t)stream, plan)))
-        return rc;
-    if ((rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, +1, 1.0, stream))) return rc;
-    if ((rc = sfa_fft_c2r(W, u, 3, rshape, cstr, rstr, 2, 1.0, stream))) return rc;
-    return SFA_OK;
+    return helmholtz3d_fp64(f, u, Nx, Ny, Nz, alpha, neumann, plan, work, s, nullptr);
 }
 
 // Same pipeline as sfa_helmholtz3d, run nrep times with hipEvents recorded on
 // `stream` between the five kernels; stage_ms[5] receives the summed
 // milliseconds of each stage (z R2C, y C2C, x-pass, y C2C inverse, z C2R).
 extern "C" int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t Nx, int64_t Ny,
-                                           int64_t Nz, double alpha, int neumann, const void* plan,
-                                           void* work, int64_t work_bytes, int nrep,
-                                           float* stage_ms, void* stream)
+                                           int64_t Nz, double alpha, int neumann, int precision,
+                                           int fp64_planes, const void* plan, void* work,
+                                           int64_t work_bytes, int nrep, float* stage_ms,
+                                           void* stream)
 {
     SFA_CHECK(plan == nullptr || plan_ok(plan, Nx, Ny, Nz, alpha, neumann), SFA_ERR_ARG,
               "plan was not built by sfa_helmholtz3d_plan for these sizes/alpha/bc");
     SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
               "need Nx >= 8, Ny >= 1, even Nz >= 2");
-    SFA_CHECK(work_bytes >= sfa_helmholtz3d_workspace(Nx, Ny, Nz, 0), SFA_ERR_ARG,
+    SFA_CHECK(precision == 0 || precision == 1, SFA_ERR_ARG, "precision must be 0 or 1");
+    SFA_CHECK(precision == 0 || Nz / 2 + 1 >= 4, SFA_ERR_UNSUPPORTED, "mixed precision needs Nz >= 6");
+    SFA_CHECK(work_bytes >= sfa_helmholtz3d_workspace(Nx, Ny, Nz, precision), SFA_ERR_ARG,
               "workspace too small");
     SFA_CHECK(nrep >= 1 && stage_ms != nullptr, SFA_ERR_ARG, "bad nrep/stage_ms");
     hipStream_t s = (hipStream_t)stream;
-    const int64_t Nzh = Nz / 2 + 1;
-    double2* W = (double2*)work;
-    const int64_t rshape[3] = {Nx, Ny, Nz};
-    const int64_t rstr[3] = {Ny * Nz, Nz, 1};
-    const int64_t cstr[3] = {Ny * Nzh, Nzh, 1};
-    const int64_t cshape[3] = {Nx, Ny, Nzh};
     hipEvent_t ev[6];
     for (int i = 0; i < 6; ++i) SFA_HIP(hipEventCreate(&ev[i]));
     for (int i = 0; i < 5; ++i) stage_ms[i] = 0.f;
     int rc = SFA_OK;
     for (int r = 0; r < nrep && rc == SFA_OK; ++r) {
-        hipEventRecord(ev[0], s);
-        rc = sfa_fft_r2c(f, W, 3, rshape, rstr, cstr, 2, 1.0 / (double)Nz, stream);
-        hipEventRecord(ev[1], s);
-        if (!rc) rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, -1, 1.0 / (double)Ny, stream);
-        hipEventRecord(ev[2], s);
-        if (!rc) rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, alpha, neumann, s, plan);
-        hipEventRecord(ev[3], s);
-        if (!rc) rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, +1, 1.0, stream);
-        hipEventRecord(ev[4], s);
-        if (!rc) rc = sfa_fft_c2r(W, u, 3, rshape, cstr, rstr, 2, 1.0, stream);
-        hipEventRecord(ev[5], s);
+        rc = precision == 1 ? helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann,
+                                                mixed_planes(Nz, fp64_planes), plan, work, s, ev)
+                            : helmholtz3d_fp64(f, u, Nx, Ny, Nz, alpha, neumann, plan, work, s, ev);
+        if (rc) break;
         hipEventSynchronize(ev[5]);
         for (int i = 0; i < 5; ++i) {
             float ms = 0.f;
@@ -552,6 +575,7 @@ extern "C" int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t N
             stage_ms[i] += ms;
         }
     }
+    hipStreamSynchronize(s);
     for (int i = 0; i < 6; ++i) hipEventDestroy(ev[i]);
     return rc;
 }

@@ -54,3 +54,21 @@ class HelmholtzSolver3D:
                   int(self.neumann), self.precision, self.fp64_planes, _lib.ptr(self.plan),
                   _lib.ptr(self.work), self.work.numel(), _lib.stream_ptr())
         return u
+
+    def stage_times(self, f, u, nrep):
+        """Milliseconds per stage (z R2C, y C2C, x-pass, y C2C inverse, z C2R)
+        averaged over nrep runs, from HIP events on the launch stream."""
+        import ctypes
+        ms = (ctypes.c_float * 5)()
+        Nx, Ny, Nz = self.shape
+        _lib.call("sfa_helmholtz3d_stage_times", _lib.ptr(f), _lib.ptr(u), Nx, Ny, Nz, self.alpha,
+                  int(self.neumann), self.precision, self.fp64_planes, _lib.ptr(self.plan),
+                  _lib.ptr(self.work), self.work.numel(), int(nrep), ms, _lib.stream_ptr())
+        names = ("z_r2c", "y_c2c", "x_pass", "y_c2c_inv", "z_c2r")
+        return {n: ms[i] / nrep for i, n in enumerate(names)}
+
+    def fp64_plane_count(self):
+        """kz planes kept in complex128 by precision='mixed' (after clamping)."""
+        Nzh = self.shape[2] // 2 + 1
+        k = self.fp64_planes if self.fp64_planes > 0 else 17
+        return max(2, min(k, Nzh - 2))
