@@ -222,6 +222,47 @@ int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t Nx, int64_t 
                                 const void* plan, void* work, int64_t work_bytes, int nrep,
                                 float* stage_ms, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* Batched 2D solves                                                   */
+/* ------------------------------------------------------------------ */
+
+/* nprob independent (alpha_i - lap) u_i = f_i on Legendre(SD: bc 0, SN: bc 1)
+ * x Fourier R2C, sizes Nx_i x Ny_i (both even, Nx in [8, 1024], Ny in
+ * [4, 1024]), f_i and u_i row-major float64 at element offsets f_off[i],
+ * u_off[i] of F (f_elems) and U (u_elems).  One call of sfa_batch2d_solve
+ * replaces, per problem, the reference sequence
+ * TensorProductSpace.scalar_product (tensorproductspace.py:146-190; Legendre
+ * recursive scalar product spectralbase.py:550-558, R2C fourier/bases.py:157-162)
+ * -> SolverGeneric1ND.__call__ (la.py:1390; per-mode TDMA of
+ * A + (alpha + ky^2) B, legendre/matrices.py:110-162, 271-315)
+ * -> TensorProductSpace.backward (tensorproductspace.py:46-96).  For the
+ * Neumann ky = 0 mode with alpha = 0 the solve pins u_0 = 0.
+ *
+ * Planning: sfa_batch2d_plan fills a device buffer of sfa_batch2d_plan_bytes()
+ * with the problem table, a static largest-first schedule of the problems
+ * over nblocks workgroups, and the composite Legendre and real DFT matrices
+ * per distinct Nx / Ny; it synchronises `stream`.  sfa_batch2d_solve runs all
+ * problems in one persistent launch of nblocks workgroups with tile variant
+ * `variant` (0 <= variant < sfa_batch2d_variants(); its block tile BM x BN is
+ * reported in bm[], bn[]).  work: sfa_batch2d_workspace() bytes.  F and U must
+ * not overlap.  No reference counterpart exists for the batching itself: the
+ * reference solves one problem per call. */
+int64_t sfa_batch2d_plan_bytes(int nprob, const int* Nx, const int* Ny, const int* bc, int nblocks);
+int sfa_batch2d_plan(int nprob, const int* Nx, const int* Ny, const double* alpha, const int* bc,
+                     const int64_t* f_off, const int64_t* u_off, int64_t f_elems, int64_t u_elems,
+                     int nblocks, void* plan, int64_t plan_bytes, void* stream);
+int64_t sfa_batch2d_workspace(int nprob, const int* Nx, const int* Ny, int nblocks);
+int sfa_batch2d_variants(int* bm, int* bn, int cap);
+int sfa_batch2d_solve(const double* F, double* U, const void* plan, void* work, int64_t work_bytes,
+                      int variant, void* stream);
+/* Forget a plan's host-side registration (before its buffer is freed). */
+int sfa_batch2d_release(const void* plan);
+
+/* Self-check of the FP64 MFMA 16x16x4 lane layout the batched GEMMs assume:
+ * writes the 64 x 4 accumulator registers of one MFMA of A = i + 100 k and
+ * B = j + 1000 k into out (256 doubles).  No reference counterpart. */
+int sfa_mfma_f64_layout_probe(double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,10 @@ _SIGS = {
     "sfa_helmholtz3d_plan": "llldivlv",
     "sfa_helmholtz3d": "vvllldiiivvlv",
     "sfa_helmholtz3d_stage_times": "vvllldiiivvlivv",
+    "sfa_batch2d_plan": "ivvvvvvllivlv",
+    "sfa_batch2d_solve": "vvvvliv",
+    "sfa_batch2d_release": "v",
+    "sfa_mfma_f64_layout_probe": "vv",
 }
 
 _TYPES = {"v": _c_void_p, "i": _int, "l": _i64, "d": _dbl, "a": _p64}
@@ -68,13 +72,20 @@ def lib():
         L.sfa_helmholtz3d_workspace.argtypes = [_i64, _i64, _i64, _int]
         L.sfa_helmholtz3d_plan_bytes.restype = _i64
         L.sfa_helmholtz3d_plan_bytes.argtypes = [_i64, _i64, _i64]
+        L.sfa_batch2d_plan_bytes.restype = _i64
+        L.sfa_batch2d_plan_bytes.argtypes = [_int, _c_void_p, _c_void_p, _c_void_p, _int]
+        L.sfa_batch2d_workspace.restype = _i64
+        L.sfa_batch2d_workspace.argtypes = [_int, _c_void_p, _c_void_p, _int]
+        L.sfa_batch2d_variants.restype = _int
+        L.sfa_batch2d_variants.argtypes = [_c_void_p, _c_void_p, _int]
         _lib = L
     return _lib
 
 
 def exported_symbols():
     return sorted(list(_SIGS) + ["sfa_version", "sfa_last_error", "sfa_helmholtz3d_workspace",
-                                 "sfa_helmholtz3d_plan_bytes"])
+                                 "sfa_helmholtz3d_plan_bytes", "sfa_batch2d_plan_bytes",
+                                 "sfa_batch2d_workspace", "sfa_batch2d_variants"])
 
 
 def arr(vals):

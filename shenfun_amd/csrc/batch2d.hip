@@ -1,0 +1,646 @@
+// Batched 2D Helmholtz solves (alpha - lap) u = f on Legendre(SD or SN) x
+// Fourier(R2C) in one persistent launch.
+//
+// Per problem (Nx x Ny physical, M = Nx - 2 composite dofs, Nyh = Ny/2 + 1):
+//   T1 = P f                P = Phi^T diag(w): composite Legendre scalar product
+//                           (spectralbase.py:550-558 with the stencil of
+//                           legendre/bases.py:374 / :502, spectralbase.py:1462-1474)
+//   T2 = T1 Df              R2C along y with 1/Ny (fourier/bases.py:157-162),
+//                           stored as complex (re, im) pairs
+//   per ky: (A + (alpha + ky^2) B) u_hat = T2[:, ky]   (la.py:1346-1364: the
+//                           per-mode TDMA of SolverGeneric1ND; A, B closed forms of
+//                           legendre/matrices.py:110-162, 271-315)
+//   U1 = Phi u_hat          composite synthesis (to_ortho + recursive backward,
+//                           spectralbase.py:1616-1634, :484-491)
+//   u = U1 Di               C2R along y (imag of k = 0 and Nyquist ignored)
+// Phi[j, m] = L_m(x_j) + K2_m L_{m+2}(x_j) on the Legendre-Gauss nodes; Df, Di
+// are the real DFT matrices.  All four contractions are FP64 GEMMs on MFMA
+// (v_mfma_f64_16x16x4f64).  One workgroup owns a problem from start to end; the
+// host assigns problems to workgroups largest first (static LPT schedule), so
+// the launch needs no atomics and every workgroup exits after its list.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <queue>
+#include <vector>
+
+#include "common.h"
+
+namespace sfa {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// ----------------------------------------------------------------- problem table
+struct B2Prob {
+    int64_t f_off, u_off;     // element offsets of f and u (row-major Nx x Ny float64)
+    int64_t phi_off, p_off;   // Phi (Nx x M) and P (M x Nx) in the matrix arena (doubles)
+    int64_t df_off, di_off;   // Df (Ny x (Ny+2)) and Di ((Ny+2) x Ny)
+    int32_t Nx, Ny, bc, pad;  // bc: 0 Dirichlet (SD), 1 Neumann (SN)
+    double alpha;
+};
+
+// ----------------------------------------------------------------- MFMA GEMM
+// MFMA f64 16x16x4 operand layout (lane l of the wave): A[i][k] at i = l % 16,
+// k = l / 16; B[k][j] at j = l % 16, k = l / 16; D[i][j] in register r of lane
+// l with i = 4 r + l / 16, j = l % 16 (checked by sfa_mfma_f64_layout_probe).
+__device__ inline v4d mfma16(double a, double b, v4d c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int B2_THREADS = 256;
+constexpr int B2_BK = 16;
+
+// C (m x n, ldc) = A (m x k, lda) B (k x n, ldb), all row-major FP64, by the
+// whole workgroup (4 waves in a 2 x 2 grid over a BM x BN block tile).  sA:
+// BK x (BM + 1) doubles, sB: BK x BN doubles of LDS.  Out-of-range operand
+// elements load as zero; stores are bounds-checked.
+template <int BM, int BN>
+__device__ void block_gemm(const double* __restrict__ A, int64_t lda, const double* __restrict__ B,
+                           int64_t ldb, double* __restrict__ C, int64_t ldc, int m, int n, int k,
+                           double* sA, double* sB)
+{
+    constexpr int SA = BM + 1;
+    constexpr int TI = BM / 32, TJ = BN / 32;
+    constexpr int LA = BM * B2_BK / B2_THREADS, LB = B2_BK * BN / B2_THREADS;
+    static_assert(LA >= 1 && LB >= 1 && TI >= 1 && TJ >= 1, "tile too small");
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wi = wv >> 1, wj = wv & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int m0 = 0; m0 < m; m0 += BM) {
+        for (int n0 = 0; n0 < n; n0 += BN) {
+            v4d acc[TI][TJ];
+#pragma unroll
+            for (int a = 0; a < TI; ++a)
+#pragma unroll
+                for (int b = 0; b < TJ; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+            double ra[LA], rb[LB];
+            auto load = [&](int k0) {
+#pragma unroll
+                for (int r = 0; r < LA; ++r) {
+                    const int e = tid + B2_THREADS * r;
+                    const int i = e / B2_BK, kk = e % B2_BK;
+                    const int gi = m0 + i, gk = k0 + kk;
+                    ra[r] = (gi < m && gk < k) ? A[(int64_t)gi * lda + gk] : 0.0;
+                }
+#pragma unroll
+                for (int r = 0; r < LB; ++r) {
+                    const int e = tid + B2_THREADS * r;
+                    const int kk = e / BN, j = e % BN;
+                    const int gk = k0 + kk, gj = n0 + j;
+                    rb[r] = (gk < k && gj < n) ? B[(int64_t)gk * ldb + gj] : 0.0;
+                }
+            };
+            load(0);
+            for (int k0 = 0; k0 < k; k0 += B2_BK) {
+                __syncthreads();  // the previous step's fragment reads are done
+#pragma unroll
+                for (int r = 0; r < LA; ++r) {
+                    const int e = tid + B2_THREADS * r;
+                    sA[(e % B2_BK) * SA + e / B2_BK] = ra[r];
+                }
+#pragma unroll
+                for (int r = 0; r < LB; ++r) {
+                    const int e = tid + B2_THREADS * r;
+                    sB[(e / BN) * BN + e % BN] = rb[r];
+                }
+                __syncthreads();
+                if (k0 + B2_BK < k) load(k0 + B2_BK);  // prefetch overlaps the MFMAs
+#pragma unroll
+                for (int s = 0; s < B2_BK / 4; ++s) {
+                    const int kk = 4 * s + lk;
+                    double af[TI], bf[TJ];
+#pragma unroll
+                    for (int a = 0; a < TI; ++a) af[a] = sA[kk * SA + wi * (BM / 2) + a * 16 + lr];
+#pragma unroll
+                    for (int b = 0; b < TJ; ++b) bf[b] = sB[kk * BN + wj * (BN / 2) + b * 16 + lr];
+#pragma unroll
+                    for (int a = 0; a < TI; ++a)
+#pragma unroll
+                        for (int b = 0; b < TJ; ++b) acc[a][b] = mfma16(af[a], bf[b], acc[a][b]);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < TI; ++a)
+#pragma unroll
+                for (int b = 0; b < TJ; ++b) {
+                    const int gj = n0 + wj * (BN / 2) + b * 16 + lr;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int gi = m0 + wi * (BM / 2) + a * 16 + 4 * r + lk;
+                        if (gi < m && gj < n) C[(int64_t)gi * ldc + gj] = acc[a][b][r];
+                    }
+                }
+        }
+    }
+    __syncthreads();  // C complete and LDS free before the caller's next stage
+}
+
+// ----------------------------------------------------------------- per-ky solve
+// Legendre composite Galerkin matrices (legendre/matrices.py:110-162, 271-315),
+// h_k = 2/(2k+1):  SD  A_kk = 4k+6, B_kk = h_k + h_{k+2}, B_{k,k+2} = -h_{k+2};
+// SN  a_k = k(k+1)/((k+2)(k+3)), A_kk = h_k a_k (k+1/2)((k+2)(k+3) - k(k+1)),
+//     B_kk = h_k + a_k^2 h_{k+2}, B_{k,k+2} = -a_k h_{k+2}.  B is symmetric.
+__device__ inline void leg_rows(int i, int bc, double& Akk, double& Bkk, double& Bup)
+{
+    const double k = (double)i;
+    const double h0 = 2.0 / (2.0 * k + 1.0), h2 = 2.0 / (2.0 * k + 5.0);
+    if (bc == 0) {
+        Akk = 4.0 * k + 6.0;
+        Bkk = h0 + h2;
+        Bup = -h2;
+    } else {
+        const double a = k * (k + 1.0) / ((k + 2.0) * (k + 3.0));
+        Akk = h0 * a * (k + 0.5) * ((k + 2.0) * (k + 3.0) - k * (k + 1.0));
+        Bkk = h0 + a * a * h2;
+        Bup = -a * h2;
+    }
+}
+
+// Thomas algorithm per (ky, parity chain) on the complex columns of W0 (M rows,
+// row pitch ldw doubles, column ky at 2 ky).  cp: scratch, M * Nyh doubles.
+// Neumann with beta = 0 (ky = 0, alpha = 0): the singular row 0 is replaced by
+// u_0 = 0 (the constraint (0, 0) of the reference's mode-0 solver).
+__device__ void batch_solve(double* __restrict__ W0, int64_t ldw, int M, int Nyh, int bc,
+                            double alpha, double* __restrict__ cp)
+{
+    for (int c = threadIdx.x; c < 2 * Nyh; c += blockDim.x) {
+        const int ky = c >> 1, p = c & 1;
+        const double beta = alpha + (double)ky * (double)ky;
+        double* cpc = cp + (int64_t)ky * M;
+        double cprev = 0.0, rprev_re = 0.0, rprev_im = 0.0, lprev_up = 0.0;
+        int last = -1;
+        for (int i = p; i < M; i += 2) {
+            double Akk, Bkk, Bup;
+            leg_rows(i, bc, Akk, Bkk, Bup);
+            const double d = Akk + beta * Bkk;
+            const double up = (i + 2 < M) ? beta * Bup : 0.0;
+            const double lo = (i >= 2) ? lprev_up : 0.0;  // B symmetric: B_{i,i-2} = B_{i-2,i}
+            double* x = W0 + (int64_t)i * ldw + 2 * ky;
+            double re = x[0], im = x[1];
+            double cc, rr_re, rr_im;
+            if (bc == 1 && i == 0 && fabs(beta) < 1e-12) {
+                cc = 0.0;
+                rr_re = rr_im = 0.0;
+            } else {
+                const double dd = 1.0 / (d - lo * cprev);
+                cc = up * dd;
+                rr_re = (re - lo * rprev_re) * dd;
+                rr_im = (im - lo * rprev_im) * dd;
+            }
+            cpc[i] = cc;
+            x[0] = rr_re;
+            x[1] = rr_im;
+            cprev = cc;
+            rprev_re = rr_re;
+            rprev_im = rr_im;
+            lprev_up = up;
+            last = i;
+        }
+        double xr = 0.0, xi = 0.0;
+        for (int i = last; i >= 0; i -= 2) {
+            double* x = W0 + (int64_t)i * ldw + 2 * ky;
+            xr = x[0] - cpc[i] * xr;
+            xi = x[1] - cpc[i] * xi;
+            x[0] = xr;
+            x[1] = xi;
+        }
+    }
+    __syncthreads();
+}
+
+// ----------------------------------------------------------------- persistent kernel
+struct B2Args {
+    const double* F;
+    double* U;
+    const B2Prob* probs;
+    const int* blk_ptr;  // nblocks + 1
+    const int* order;    // problem ids, grouped by workgroup
+    const double* mats;  // matrix arena
+    double* work;        // per workgroup: wstride doubles
+    int64_t wstride;
+};
+
+template <int BM, int BN>
+__global__ void __launch_bounds__(B2_THREADS) batch2d_kernel(B2Args a)
+{
+    __shared__ double sA[B2_BK * (BM + 1)];
+    __shared__ double sB[B2_BK * BN];
+    const int b = blockIdx.x;
+    double* W0 = a.work + (int64_t)b * a.wstride;
+    for (int q = a.blk_ptr[b]; q < a.blk_ptr[b + 1]; ++q) {
+        const B2Prob pr = a.probs[a.order[q]];
+        const int Nx = pr.Nx, Ny = pr.Ny, M = Nx - 2, Nyh = Ny / 2 + 1, L = Ny + 2;
+        const double* f = a.F + pr.f_off;
+        double* u = a.U + pr.u_off;
+        double* W1 = W0 + (int64_t)M * L;  // Nx x L
+        const double* Phi = a.mats + pr.phi_off;
+        const double* P = a.mats + pr.p_off;
+        const double* Df = a.mats + pr.df_off;
+        const double* Di = a.mats + pr.di_off;
+        // T1 = P f (M x Ny), held in u
+        block_gemm<BM, BN>(P, Nx, f, Ny, u, Ny, M, Ny, Nx, sA, sB);
+        // T2 = T1 Df (M x L) in W0
+        block_gemm<BM, BN>(u, Ny, Df, L, W0, L, M, L, Ny, sA, sB);
+        // per-ky solve in place, W1 as the Thomas scratch
+        batch_solve(W0, L, M, Nyh, pr.bc, pr.alpha, W1);
+        // U1 = Phi u_hat (Nx x L) in W1
+        block_gemm<BM, BN>(Phi, M, W0, L, W1, L, Nx, L, M, sA, sB);
+        // u = U1 Di (Nx x Ny)
+        block_gemm<BM, BN>(W1, L, Di, Ny, u, Ny, Nx, Ny, L, sA, sB);
+    }
+}
+
+// ----------------------------------------------------------------- matrix builders
+// Phi (Nx x M) and P (M x Nx) for Legendre-Gauss nodes x, weights w: one thread
+// per node, the recurrence L_{k+1} = ((2k+1) x L_k - k L_{k-1})/(k+1) in the
+// anm/anp form of transforms.pyx:219-323.
+__global__ void b2_phi_kernel(const double* __restrict__ x, const double* __restrict__ w, int Nx,
+                              int bc, double* __restrict__ Phi, double* __restrict__ P)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= Nx) return;
+    const int M = Nx - 2;
+    const double xj = x[j], wj = w[j];
+    double Lm = 1.0, L0 = xj;
+    double L1 = 1.5 * xj * L0 - 0.5 * Lm;  // L_2
+    for (int m = 0; m < M; ++m) {
+        // Lm = L_m, L0 = L_{m+1}, L1 = L_{m+2}
+        const double km = (double)m;
+        const double K2 = (bc == 0) ? -1.0 : -km * (km + 1.0) / ((km + 2.0) * (km + 3.0));
+        const double v = Lm + K2 * L1;
+        Phi[(int64_t)j * M + m] = v;
+        P[(int64_t)m * Nx + j] = wj * v;
+        const int n = m + 2;  // L_{n+1} = (1/anm_n) x L_n - (anp_n/anm_n) L_{n-1}
+        const double anm = (n + 1.0) / (2.0 * n + 1.0), anp = (double)n / (2.0 * n + 1.0);
+        const double Ln = (1.0 / anm) * xj * L1 - (anp / anm) * L0;
+        Lm = L0;
+        L0 = L1;
+        L1 = Ln;
+    }
+}
+
+// Df (Ny x (Ny+2)): [y][2k] = cos(2 pi k y/Ny)/Ny, [y][2k+1] = -sin(..)/Ny;
+// Di ((Ny+2) x Ny): [2k][y] = c_k cos, [2k+1][y] = -c_k sin, c_k = 1 at k = 0 and
+// Ny/2, else 2.  Angles reduced exactly ((k y) mod Ny) before sincospi.
+__global__ void b2_dft_kernel(int Ny, double* __restrict__ Df, double* __restrict__ Di)
+{
+    const int Nyh = Ny / 2 + 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Ny * Nyh) return;
+    const int y = (int)(t / Nyh), k = (int)(t % Nyh);
+    const int r = (int)(((int64_t)k * y) % Ny);
+    double s, c;
+    sincospi(2.0 * (double)r / (double)Ny, &s, &c);
+    const int L = Ny + 2;
+    Df[(int64_t)y * L + 2 * k] = c / Ny;
+    Df[(int64_t)y * L + 2 * k + 1] = -s / Ny;
+    const double ck = (k == 0 || 2 * k == Ny) ? 1.0 : 2.0;
+    Di[(int64_t)(2 * k) * Ny + y] = ck * c;
+    Di[(int64_t)(2 * k + 1) * Ny + y] = (k == 0 || 2 * k == Ny) ? 0.0 : -ck * s;
+}
+
+// MFMA lane-layout probe: A (16 x 4) = i + 100 k, B (4 x 16) = j + 1000 k;
+// out[l * 4 + r] = D register r of lane l.
+__global__ void mfma_probe_kernel(double* out)
+{
+    const int l = threadIdx.x;
+    const double a = (double)(l % 16) + 100.0 * (l / 16);
+    const double b = (double)(l % 16) + 1000.0 * (l / 16);
+    v4d d = mfma16(a, b, (v4d){0.0, 0.0, 0.0, 0.0});
+    for (int r = 0; r < 4; ++r) out[l * 4 + r] = d[r];
+}
+
+// ----------------------------------------------------------------- host side
+// Legendre-Gauss nodes (ascending) and weights: Newton on the three-term
+// recurrence in long double from the cosine guess, weights 2/((1-x^2) P'^2),
+// symmetrised; cached per N for the process.
+static std::mutex g_lg_mu;
+static std::map<int, std::pair<std::vector<double>, std::vector<double>>> g_lg;
+
+static void lg_nodes(int N, std::vector<double>& xo, std::vector<double>& wo)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_lg_mu);
+        auto it = g_lg.find(N);
+        if (it != g_lg.end()) {
+            xo = it->second.first;
+            wo = it->second.second;
+            return;
+        }
+    }
+    typedef long double ld;
+    const ld pi = 3.141592653589793238462643383279502884L;
+    std::vector<ld> x(N), w(N);
+    auto pdp = [&](ld xv, ld& p, ld& dp) {
+        ld p0 = 1.0L, p1 = xv;
+        for (int k = 1; k < N; ++k) {
+            const ld p2 = ((2 * k + 1) * xv * p1 - k * p0) / (k + 1);
+            p0 = p1;
+            p1 = p2;
+        }
+        p = p1;
+        dp = N * (xv * p1 - p0) / (xv * xv - 1.0L);
+    };
+    for (int i = 0; i < N; ++i) {
+        ld xv = std::cos(pi * ((ld)(i + 1) - 0.25L) / ((ld)N + 0.5L));
+        for (int it = 0; it < 100; ++it) {
+            ld p, dp;
+            pdp(xv, p, dp);
+            const ld dx = p / dp;
+            xv -= dx;
+            if (std::fabs(dx) < 1e-19L) break;
+        }
+        ld p, dp;
+        pdp(xv, p, dp);
+        x[i] = xv;
+        w[i] = 2.0L / ((1.0L - xv * xv) * dp * dp);
+    }
+    // ascending, rounded, symmetrised
+    std::vector<int> o(N);
+    std::iota(o.begin(), o.end(), 0);
+    std::sort(o.begin(), o.end(), [&](int a, int b) { return x[a] < x[b]; });
+    std::vector<double> xs(N), ws(N);
+    for (int i = 0; i < N; ++i) {
+        xs[i] = (double)x[o[i]];
+        ws[i] = (double)w[o[i]];
+    }
+    xo.assign(N, 0.0);
+    wo.assign(N, 0.0);
+    for (int i = 0; i < N; ++i) {
+        xo[i] = 0.5 * (xs[i] - xs[N - 1 - i]);
+        wo[i] = 0.5 * (ws[i] + ws[N - 1 - i]);
+    }
+    std::lock_guard<std::mutex> lk(g_lg_mu);
+    g_lg[N] = std::make_pair(xo, wo);
+}
+
+struct B2Layout {
+    int64_t off_probs, off_blk, off_order, off_nodes, off_mats, bytes;
+    int64_t n_nodes, n_mats;
+    std::map<std::pair<int, int>, int64_t> phi;  // (Nx, bc) -> offset of Phi (P follows)
+    std::map<int, int64_t> dft;                  // Ny -> offset of Df (Di follows)
+    std::map<int, int64_t> nodes;                // Nx -> offset of x (w follows) in the node area
+};
+
+static int64_t r256(int64_t b) { return (b + 255) / 256 * 256; }
+
+static void b2_layout(int nprob, const int* Nx, const int* Ny, const int* bc, int nblocks, B2Layout* L)
+{
+    int64_t mats = 0, nn = 0;
+    L->phi.clear();
+    L->dft.clear();
+    L->nodes.clear();
+    for (int i = 0; i < nprob; ++i) {
+        const auto key = std::make_pair(Nx[i], bc[i] ? 1 : 0);
+        if (!L->phi.count(key)) {
+            L->phi[key] = mats;
+            mats += 2 * (int64_t)Nx[i] * (Nx[i] - 2);
+        }
+        if (!L->nodes.count(Nx[i])) {
+            L->nodes[Nx[i]] = nn;
+            nn += 2 * (int64_t)Nx[i];
+        }
+        if (!L->dft.count(Ny[i])) {
+            L->dft[Ny[i]] = mats;
+            mats += 2 * (int64_t)Ny[i] * (Ny[i] + 2);
+        }
+    }
+    L->n_mats = mats;
+    L->n_nodes = nn;
+    L->off_probs = 0;
+    L->off_blk = r256((int64_t)nprob * sizeof(B2Prob));
+    L->off_order = L->off_blk + r256((int64_t)(nblocks + 1) * 4);
+    L->off_nodes = L->off_order + r256((int64_t)nprob * 4);
+    L->off_mats = L->off_nodes + r256(nn * 8);
+    L->bytes = L->off_mats + r256(mats * 8);
+}
+
+static double b2_cost(int Nx, int Ny)
+{
+    // flops of the four contractions plus a memory and fixed term (units ~ flops)
+    const double x = Nx, y = Ny;
+    return 4.0 * x * y * (x + y) + 64.0 * x * y + 2.0e5;
+}
+
+struct B2Info {
+    int nprob, nblocks;
+    int64_t off_probs, off_blk, off_order, off_mats;
+    int64_t wstride;  // doubles per workgroup
+};
+static std::mutex g_b2_mu;
+static std::map<const void*, B2Info> g_b2;
+
+static int64_t b2_wstride(int nprob, const int* Nx, const int* Ny)
+{
+    int64_t mx = 0;
+    for (int i = 0; i < nprob; ++i) {
+        const int64_t L = Ny[i] + 2;
+        mx = std::max(mx, (int64_t)(Nx[i] - 2) * L + (int64_t)Nx[i] * L);
+    }
+    return (mx + 31) / 32 * 32;
+}
+
+template <int BM, int BN>
+static int launch_b2(const B2Args& a, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL((batch2d_kernel<BM, BN>), dim3((unsigned)nblocks), dim3(B2_THREADS), 0, s, a);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+}  // namespace sfa
+
+using namespace sfa;
+
+static const int kB2Variants[][2] = {{64, 64}, {32, 64}, {64, 128}, {128, 64}};
+
+extern "C" int sfa_batch2d_variants(int* bm, int* bn, int cap)
+{
+    const int n = (int)(sizeof(kB2Variants) / sizeof(kB2Variants[0]));
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (bm) bm[i] = kB2Variants[i][0];
+        if (bn) bn[i] = kB2Variants[i][1];
+    }
+    return n;
+}
+
+static int b2_check(int nprob, const int* Nx, const int* Ny, const int* bc, int nblocks)
+{
+    SFA_CHECK(nprob >= 1 && Nx && Ny && bc, SFA_ERR_ARG, "need nprob >= 1 and size arrays");
+    SFA_CHECK(nblocks >= 1 && nblocks <= 65536, SFA_ERR_ARG, "nblocks must be in [1, 65536]");
+    for (int i = 0; i < nprob; ++i) {
+        SFA_CHECK(Nx[i] >= 8 && Nx[i] <= 1024 && Nx[i] % 2 == 0, SFA_ERR_ARG,
+                  "problem %d: Nx=%d must be even in [8, 1024]", i, Nx[i]);
+        SFA_CHECK(Ny[i] >= 4 && Ny[i] <= 1024 && Ny[i] % 2 == 0, SFA_ERR_ARG,
+                  "problem %d: Ny=%d must be even in [4, 1024]", i, Ny[i]);
+        SFA_CHECK(bc[i] == 0 || bc[i] == 1, SFA_ERR_ARG, "problem %d: bc must be 0 or 1", i);
+    }
+    return SFA_OK;
+}
+
+extern "C" int64_t sfa_batch2d_plan_bytes(int nprob, const int* Nx, const int* Ny, const int* bc,
+                                          int nblocks)
+{
+    if (b2_check(nprob, Nx, Ny, bc, nblocks)) return -1;
+    B2Layout L;
+    b2_layout(nprob, Nx, Ny, bc, nblocks, &L);
+    return L.bytes;
+}
+
+extern "C" int64_t sfa_batch2d_workspace(int nprob, const int* Nx, const int* Ny, int nblocks)
+{
+    if (nprob < 1 || nblocks < 1 || !Nx || !Ny) return -1;
+    return b2_wstride(nprob, Nx, Ny) * 8 * (int64_t)nblocks;
+}
+
+extern "C" int sfa_batch2d_plan(int nprob, const int* Nx, const int* Ny, const double* alpha,
+                                const int* bc, const int64_t* f_off, const int64_t* u_off,
+                                int64_t f_elems, int64_t u_elems, int nblocks, void* plan,
+                                int64_t plan_bytes, void* stream)
+{
+    int rc;
+    if ((rc = b2_check(nprob, Nx, Ny, bc, nblocks))) return rc;
+    SFA_CHECK(alpha && f_off && u_off && plan, SFA_ERR_ARG, "NULL argument");
+    for (int i = 0; i < nprob; ++i) {
+        const int64_t n = (int64_t)Nx[i] * Ny[i];
+        SFA_CHECK(f_off[i] >= 0 && f_off[i] + n <= f_elems, SFA_ERR_ARG,
+                  "problem %d: f range [%lld, %lld) outside the %lld-element input", i,
+                  (long long)f_off[i], (long long)(f_off[i] + n), (long long)f_elems);
+        SFA_CHECK(u_off[i] >= 0 && u_off[i] + n <= u_elems, SFA_ERR_ARG,
+                  "problem %d: u range outside the %lld-element output", i, (long long)u_elems);
+    }
+    B2Layout L;
+    b2_layout(nprob, Nx, Ny, bc, nblocks, &L);
+    SFA_CHECK(plan_bytes >= L.bytes, SFA_ERR_ARG, "plan buffer too small (%lld < %lld)",
+              (long long)plan_bytes, (long long)L.bytes);
+    // problem table
+    std::vector<B2Prob> probs(nprob);
+    for (int i = 0; i < nprob; ++i) {
+        B2Prob& p = probs[i];
+        p.f_off = f_off[i];
+        p.u_off = u_off[i];
+        p.phi_off = L.phi[std::make_pair(Nx[i], bc[i] ? 1 : 0)];
+        p.p_off = p.phi_off + (int64_t)Nx[i] * (Nx[i] - 2);
+        p.df_off = L.dft[Ny[i]];
+        p.di_off = p.df_off + (int64_t)Ny[i] * (Ny[i] + 2);
+        p.Nx = Nx[i];
+        p.Ny = Ny[i];
+        p.bc = bc[i] ? 1 : 0;
+        p.pad = 0;
+        p.alpha = alpha[i];
+    }
+    // static LPT schedule: largest first onto the least loaded workgroup
+    std::vector<int> ids(nprob);
+    std::iota(ids.begin(), ids.end(), 0);
+    std::stable_sort(ids.begin(), ids.end(),
+                     [&](int a, int b) { return b2_cost(Nx[a], Ny[a]) > b2_cost(Nx[b], Ny[b]); });
+    typedef std::pair<double, int> LB;
+    std::priority_queue<LB, std::vector<LB>, std::greater<LB>> heap;
+    for (int b = 0; b < nblocks; ++b) heap.push(LB(0.0, b));
+    std::vector<std::vector<int>> lists(nblocks);
+    for (int id : ids) {
+        LB t = heap.top();
+        heap.pop();
+        lists[t.second].push_back(id);
+        t.first += b2_cost(Nx[id], Ny[id]);
+        heap.push(t);
+    }
+    std::vector<int> blk(nblocks + 1, 0), order;
+    order.reserve(nprob);
+    for (int b = 0; b < nblocks; ++b) {
+        blk[b] = (int)order.size();
+        for (int id : lists[b]) order.push_back(id);
+    }
+    blk[nblocks] = (int)order.size();
+    // nodes and weights per distinct Nx
+    std::vector<double> nodes((size_t)std::max<int64_t>(L.n_nodes, 1));
+    for (auto& kv : L.nodes) {
+        std::vector<double> x, w;
+        lg_nodes(kv.first, x, w);
+        std::copy(x.begin(), x.end(), nodes.begin() + kv.second);
+        std::copy(w.begin(), w.end(), nodes.begin() + kv.second + kv.first);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    char* pb = (char*)plan;
+    SFA_HIP(hipMemcpyAsync(pb + L.off_probs, probs.data(), probs.size() * sizeof(B2Prob),
+                           hipMemcpyHostToDevice, s));
+    SFA_HIP(hipMemcpyAsync(pb + L.off_blk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, s));
+    SFA_HIP(hipMemcpyAsync(pb + L.off_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
+    SFA_HIP(hipMemcpyAsync(pb + L.off_nodes, nodes.data(), (size_t)L.n_nodes * 8,
+                           hipMemcpyHostToDevice, s));
+    const double* dn = (const double*)(pb + L.off_nodes);
+    double* dm = (double*)(pb + L.off_mats);
+    for (auto& kv : L.phi) {
+        const int n = kv.first.first;
+        const int64_t no = L.nodes[n];
+        hipLaunchKernelGGL(b2_phi_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, dn + no,
+                           dn + no + n, n, kv.first.second, dm + kv.second,
+                           dm + kv.second + (int64_t)n * (n - 2));
+        SFA_LAUNCH_CHECK();
+    }
+    for (auto& kv : L.dft) {
+        const int n = kv.first;
+        const int64_t nt = (int64_t)n * (n / 2 + 1);
+        hipLaunchKernelGGL(b2_dft_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, n,
+                           dm + kv.second, dm + kv.second + (int64_t)n * (n + 2));
+        SFA_LAUNCH_CHECK();
+    }
+    SFA_HIP(hipStreamSynchronize(s));  // host vectors are copied before they go out of scope
+    std::lock_guard<std::mutex> lk(g_b2_mu);
+    g_b2[plan] = B2Info{nprob, nblocks, L.off_probs, L.off_blk, L.off_order, L.off_mats,
+                        b2_wstride(nprob, Nx, Ny)};
+    return SFA_OK;
+}
+
+extern "C" int sfa_batch2d_solve(const double* F, double* U, const void* plan, void* work,
+                                 int64_t work_bytes, int variant, void* stream)
+{
+    B2Info pi;
+    {
+        std::lock_guard<std::mutex> lk(g_b2_mu);
+        auto it = g_b2.find(plan);
+        SFA_CHECK(it != g_b2.end(), SFA_ERR_ARG, "plan was not built by sfa_batch2d_plan");
+        pi = it->second;
+    }
+    SFA_CHECK(F && U && work, SFA_ERR_ARG, "NULL argument");
+    SFA_CHECK(work_bytes >= pi.wstride * 8 * (int64_t)pi.nblocks, SFA_ERR_ARG,
+              "workspace too small (%lld < %lld)", (long long)work_bytes,
+              (long long)(pi.wstride * 8 * (int64_t)pi.nblocks));
+    const int nv = sfa_batch2d_variants(nullptr, nullptr, 0);
+    SFA_CHECK(variant >= 0 && variant < nv, SFA_ERR_ARG, "variant must be in [0, %d)", nv);
+    const char* pb = (const char*)plan;
+    B2Args a;
+    a.F = F;
+    a.U = U;
+    a.probs = (const B2Prob*)(pb + pi.off_probs);
+    a.blk_ptr = (const int*)(pb + pi.off_blk);
+    a.order = (const int*)(pb + pi.off_order);
+    a.mats = (const double*)(pb + pi.off_mats);
+    a.work = (double*)work;
+    a.wstride = pi.wstride;
+    hipStream_t s = (hipStream_t)stream;
+    switch (variant) {
+        case 0: return launch_b2<64, 64>(a, pi.nblocks, s);
+        case 1: return launch_b2<32, 64>(a, pi.nblocks, s);
+        case 2: return launch_b2<64, 128>(a, pi.nblocks, s);
+        default: return launch_b2<128, 64>(a, pi.nblocks, s);
+    }
+}
+
+extern "C" int sfa_batch2d_release(const void* plan)
+{
+    std::lock_guard<std::mutex> lk(g_b2_mu);
+    g_b2.erase(plan);
+    return SFA_OK;
+}
+
+extern "C" int sfa_mfma_f64_layout_probe(double* out, void* stream)
+{
+    SFA_CHECK(out != nullptr, SFA_ERR_ARG, "out is NULL");
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, out);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}

@@ -23,3 +23,45 @@ def helmholtz3d_rhs(Nx, Ny, Nz, seed=0):
     g = np.fft.irfftn(c, s=(Ny, Nz), axes=(1, 2)) * (Ny * Nz)
     f = 0.5 * scipy.fft.dct(g, type=3, axis=0) + g[0:1] / 2
     return np.ascontiguousarray(f)
+
+
+def batch2d_problems(n=4096, seed=0):
+    """Sizes and alphas of the batched uneven 2D Legendre x Fourier workload.
+
+    Nx = 2 round(e^U(ln 32, ln 512) / 2), Ny = 2 round(e^U(ln 32, ln 256) / 2),
+    alpha ~ U(0, 100); draw order: the n uniforms of Nx, then of Ny, then alpha.
+    """
+    rng = np.random.default_rng(seed)
+    nx = 2 * np.round(np.exp(rng.uniform(np.log(32), np.log(512), n)) / 2).astype(np.int64)
+    ny = 2 * np.round(np.exp(rng.uniform(np.log(32), np.log(256), n)) / 2).astype(np.int64)
+    alpha = rng.uniform(0.0, 100.0, n)
+    return nx, ny, alpha, rng
+
+
+def batch2d_rhs(Nx, Ny, rng):
+    """Smooth random f on Legendre-Gauss(Nx) x uniform(Ny) nodes: coefficients
+    complex N(0,1) / (1 + kx^2 + ky^2) on L_kx(x) e^{i ky y} (kx < Nx,
+    ky = 0..Ny/2; draw order real then imaginary parts, shape (Nx, Ny/2+1)),
+    synthesised with the Legendre Vandermonde matrix and an FP64 inverse real
+    FFT (which takes the Hermitian part)."""
+    nyh = Ny // 2 + 1
+    c = rng.standard_normal((Nx, nyh)) + 1j * rng.standard_normal((Nx, nyh))
+    kx = np.arange(Nx, dtype=float)[:, None]
+    ky = np.arange(nyh, dtype=float)[None, :]
+    c /= 1 + kx ** 2 + ky ** 2
+    g = _legvander_gauss(Nx) @ c
+    return np.ascontiguousarray(np.fft.irfft(g, n=Ny, axis=1) * Ny)
+
+
+_VANDER = {}
+
+
+def _legvander_gauss(N):
+    """Legendre Vandermonde matrix L_k(x_j) at the N Gauss nodes (cached)."""
+    V = _VANDER.get(N)
+    if V is None:
+        from numpy.polynomial import legendre as npleg
+
+        from .bases import leggauss
+        V = _VANDER[N] = npleg.legvander(leggauss(N)[0], N - 1)
+    return V

@@ -1,0 +1,140 @@
+"""GPU parity: batched uneven 2D Legendre x Fourier Helmholtz solves in one
+persistent launch vs the FP64 oracle (per problem, legendre_fourier_2d)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import spectral as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sfa():
+    import shenfun_amd
+    return shenfun_amd
+
+
+def _rel(a, b):
+    a = a.cpu().numpy() if isinstance(a, torch.Tensor) else a
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_mfma_f64_layout(sfa):
+    """The FP64 MFMA 16x16x4 accumulator layout the batched GEMMs assume:
+    D[i][j] in register r of lane l with i = 4 r + l // 16, j = l % 16."""
+    from shenfun_amd import _lib
+    print("[mfma layout probe]", flush=True)
+    out = torch.zeros(256, dtype=torch.float64, device="cuda")
+    _lib.call("sfa_mfma_f64_layout_probe", _lib.ptr(out), _lib.stream_ptr())
+    got = out.cpu().numpy().reshape(64, 4)
+    i = np.arange(16)[:, None]
+    k = np.arange(4)[None, :]
+    A = (i + 100 * k).astype(float)                       # 16 x 4
+    B = (np.arange(16)[None, :] + 1000 * np.arange(4)[:, None]).astype(float)  # 4 x 16
+    D = A @ B
+    exp = np.zeros((64, 4))
+    for lane in range(64):
+        for r in range(4):
+            exp[lane, r] = D[4 * r + lane // 16, lane % 16]
+    assert np.array_equal(got, exp)
+
+
+PROBLEMS = [(32, 32, 1.0, "D"), (34, 46, 0.0, "D"), (64, 64, 5.3, "N"), (100, 38, 17.0, "D"),
+            (130, 100, 0.5, "N"), (256, 128, 50.0, "D"), (512, 256, 99.0, "D"),
+            (48, 250, 3.0, "N"), (8, 4, 1.0, "D"), (10, 6, 2.0, "N"), (510, 34, 0.25, "N")]
+
+
+@pytest.fixture(scope="module")
+def batch(sfa):
+    from shenfun_amd.workloads import batch2d_rhs
+    rng = np.random.default_rng(7)
+    fs = [batch2d_rhs(nx, ny, rng) for nx, ny, _, _ in PROBLEMS]
+    H = sfa.BatchedHelmholtz2D([p[0] for p in PROBLEMS], [p[1] for p in PROBLEMS],
+                               [p[2] for p in PROBLEMS], [p[3] for p in PROBLEMS])
+    F = H.pack(fs)
+    return H, F, fs
+
+
+def test_batch2d_matches_oracle(sfa, batch):
+    H, F, fs = batch
+    print("[batch2d vs oracle]", flush=True)
+    U = H(F)
+    for (nx, ny, al, bc), f, u in zip(PROBLEMS, fs, H.unpack(U)):
+        u_ref = S.legendre_fourier_2d(f, al, bc)[0]
+        err = _rel(u, u_ref)
+        print(f"  Nx={nx} Ny={ny} alpha={al} bc={bc}: max rel err {err:.2e}", flush=True)
+        assert err < 1e-11
+
+
+def test_batch2d_variants_and_schedules_bitwise(sfa, batch):
+    """Every tile variant and workgroup count sums each output in the same
+    order (K steps ascending, one MFMA chain per element): identical bits."""
+    from shenfun_amd.batched import tile_variants
+    H, F, _ = batch
+    print("[batch2d variants]", flush=True)
+    H.build(H.cus)
+    H.variant = 0
+    ref = H(F).clone()
+    for nb in (7, H.cus, 2 * H.cus):
+        H.build(nb)
+        for v in range(len(tile_variants())):
+            H.variant = v
+            print(f"  nblocks={nb} variant={v}", flush=True)
+            assert torch.equal(H(F), ref)
+    H.build(H.cus)
+    H.variant = 0
+
+
+def test_batch2d_alone_equals_in_batch(sfa):
+    """A problem solved alone gives the same bits as inside a batch of others."""
+    from shenfun_amd.workloads import batch2d_problems, batch2d_rhs
+    nx, ny, al, rng = batch2d_problems(40, seed=3)
+    fs = [batch2d_rhs(int(a), int(b), rng) for a, b in zip(nx, ny)]
+    H = sfa.BatchedHelmholtz2D(nx, ny, al, "D")
+    U = H(H.pack(fs))
+    parts = H.unpack(U)
+    for i in (0, 7, 19, 39):
+        print(f"[alone {i}: {nx[i]}x{ny[i]}]", flush=True)
+        H1 = sfa.BatchedHelmholtz2D(nx[i:i + 1], ny[i:i + 1], al[i:i + 1], "D", nblocks=1)
+        u1 = H1(H1.pack([fs[i]]))
+        assert torch.equal(u1.view(int(nx[i]), int(ny[i])), parts[i])
+
+
+def test_batch2d_graph_replay(sfa, batch):
+    """The launch captured in a HIP graph: every replay equals the eager result."""
+    H, F, _ = batch
+    print("[batch2d graph]", flush=True)
+    U_eager = H(F).clone()
+    U = torch.empty_like(F)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        H(F, U)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        H(F, U)
+    for rep in range(3):
+        U.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(U, U_eager), f"replay {rep}"
+        (F * 2.0).sum().item()  # eager work between replays
+
+
+def test_batch2d_c4_workload_sample(sfa):
+    """First problems of the 4096-problem seeded workload vs the oracle."""
+    from shenfun_amd.workloads import batch2d_problems, batch2d_rhs
+    nx, ny, al, rng = batch2d_problems(4096, seed=0)
+    n = 12
+    fs = [batch2d_rhs(int(a), int(b), rng) for a, b in zip(nx[:n], ny[:n])]
+    H = sfa.BatchedHelmholtz2D(nx[:n], ny[:n], al[:n], "D")
+    U = H(H.pack(fs))
+    for i, u in enumerate(H.unpack(U)):
+        u_ref = S.legendre_fourier_2d(fs[i], al[i], "D")[0]
+        err = _rel(u, u_ref)
+        print(f"[c4 sample {i}: {nx[i]}x{ny[i]} alpha={al[i]:.2f}] err {err:.2e}", flush=True)
+        assert err < 1e-11
