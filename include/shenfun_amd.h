@@ -258,6 +258,27 @@ int sfa_batch2d_solve(const double* F, double* U, const void* plan, void* work, 
 /* Forget a plan's host-side registration (before its buffer is freed). */
 int sfa_batch2d_release(const void* plan);
 
+/* The same batches with the matrices held by the caller (a resident cache
+ * shared by many batches, as a serving loop keeps it): sfa_batch2d_leg_matrices
+ * writes Phi (Nx x (Nx-2)) then P ((Nx-2) x Nx) of the composite Legendre
+ * basis (bc 0 SD, 1 SN) into sfa_batch2d_leg_elems(Nx) doubles at out;
+ * sfa_batch2d_dft_matrices writes Df (Ny x (Ny+2)) then Di ((Ny+2) x Ny) into
+ * sfa_batch2d_dft_elems(Ny) doubles.  sfa_batch2d_plan_ext builds a plan of
+ * sfa_batch2d_plan_ext_bytes() bytes (problem table and schedule only) whose
+ * problem i uses the matrices at mats + phi_off[i] and mats + df_off[i]
+ * (mats_elems doubles in all); mats must stay allocated while the plan is
+ * used.  Synchronises `stream`. */
+int64_t sfa_batch2d_leg_elems(int Nx);
+int64_t sfa_batch2d_dft_elems(int Ny);
+int sfa_batch2d_leg_matrices(int Nx, int bc, double* out, void* stream);
+int sfa_batch2d_dft_matrices(int Ny, double* out, void* stream);
+int64_t sfa_batch2d_plan_ext_bytes(int nprob, int nblocks);
+int sfa_batch2d_plan_ext(int nprob, const int* Nx, const int* Ny, const double* alpha, const int* bc,
+                         const int64_t* f_off, const int64_t* u_off, int64_t f_elems,
+                         int64_t u_elems, const double* mats, const int64_t* phi_off,
+                         const int64_t* df_off, int64_t mats_elems, int nblocks, void* plan,
+                         int64_t plan_bytes, void* stream);
+
 /* Self-check of the FP64 MFMA 16x16x4 lane layout the batched GEMMs assume:
  * writes the 64 x 4 accumulator registers of one MFMA of A = i + 100 k and
  * B = j + 1000 k into out (256 doubles).  No reference counterpart. */

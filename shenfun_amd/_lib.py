@@ -44,6 +44,9 @@ _SIGS = {
     "sfa_batch2d_plan": "ivvvvvvllivlv",
     "sfa_batch2d_solve": "vvvvliv",
     "sfa_batch2d_release": "v",
+    "sfa_batch2d_leg_matrices": "iivv",
+    "sfa_batch2d_dft_matrices": "ivv",
+    "sfa_batch2d_plan_ext": "ivvvvvvllvvvlivlv",
     "sfa_mfma_f64_layout_probe": "vv",
 }
 
@@ -78,6 +81,11 @@ def lib():
         L.sfa_batch2d_workspace.argtypes = [_int, _c_void_p, _c_void_p, _int]
         L.sfa_batch2d_variants.restype = _int
         L.sfa_batch2d_variants.argtypes = [_c_void_p, _c_void_p, _int]
+        for name in ("sfa_batch2d_leg_elems", "sfa_batch2d_dft_elems"):
+            getattr(L, name).restype = _i64
+            getattr(L, name).argtypes = [_int]
+        L.sfa_batch2d_plan_ext_bytes.restype = _i64
+        L.sfa_batch2d_plan_ext_bytes.argtypes = [_int, _int]
         _lib = L
     return _lib
 
@@ -85,7 +93,9 @@ def lib():
 def exported_symbols():
     return sorted(list(_SIGS) + ["sfa_version", "sfa_last_error", "sfa_helmholtz3d_workspace",
                                  "sfa_helmholtz3d_plan_bytes", "sfa_batch2d_plan_bytes",
-                                 "sfa_batch2d_workspace", "sfa_batch2d_variants"])
+                                 "sfa_batch2d_workspace", "sfa_batch2d_variants",
+                                 "sfa_batch2d_leg_elems", "sfa_batch2d_dft_elems",
+                                 "sfa_batch2d_plan_ext_bytes"])
 
 
 def arr(vals):

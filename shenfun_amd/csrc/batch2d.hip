@@ -377,45 +377,87 @@ static void lg_nodes(int N, std::vector<double>& xo, std::vector<double>& wo)
     g_lg[N] = std::make_pair(xo, wo);
 }
 
-struct B2Layout {
-    int64_t off_probs, off_blk, off_order, off_nodes, off_mats, bytes;
-    int64_t n_nodes, n_mats;
-    std::map<std::pair<int, int>, int64_t> phi;  // (Nx, bc) -> offset of Phi (P follows)
-    std::map<int, int64_t> dft;                  // Ny -> offset of Df (Di follows)
-    std::map<int, int64_t> nodes;                // Nx -> offset of x (w follows) in the node area
-};
+// Device copies of the nodes and weights per N (x then w), for the process.
+static std::map<std::pair<int, int>, double*> g_lg_dev;  // (N, device)
+
+static const double* lg_nodes_dev(int N)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_lg_mu);
+        auto it = g_lg_dev.find(std::make_pair(N, dev));
+        if (it != g_lg_dev.end()) return it->second;
+    }
+    std::vector<double> x, w;
+    lg_nodes(N, x, w);
+    x.insert(x.end(), w.begin(), w.end());
+    double* d = nullptr;
+    if (hipMalloc(&d, x.size() * sizeof(double)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, x.data(), x.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_lg_mu);
+    g_lg_dev[std::make_pair(N, dev)] = d;
+    return d;
+}
+
+static int64_t leg_elems(int Nx) { return 2 * (int64_t)Nx * (Nx - 2); }
+static int64_t dft_elems(int Ny) { return 2 * (int64_t)Ny * (Ny + 2); }
+
+static int build_leg(int Nx, int bc, double* out, hipStream_t s)
+{
+    const double* xw = lg_nodes_dev(Nx);
+    SFA_CHECK(xw, SFA_ERR_HIP, "node table allocation failed");
+    hipLaunchKernelGGL(b2_phi_kernel, dim3((unsigned)((Nx + 127) / 128)), dim3(128), 0, s, xw, xw + Nx, Nx,
+                       bc, out, out + (int64_t)Nx * (Nx - 2));
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+static int build_dft(int Ny, double* out, hipStream_t s)
+{
+    const int64_t nt = (int64_t)Ny * (Ny / 2 + 1);
+    hipLaunchKernelGGL(b2_dft_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, Ny, out,
+                       out + (int64_t)Ny * (Ny + 2));
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
 
 static int64_t r256(int64_t b) { return (b + 255) / 256 * 256; }
 
-static void b2_layout(int nprob, const int* Nx, const int* Ny, const int* bc, int nblocks, B2Layout* L)
+// plan layout: problem table | blk_ptr | order [| matrix arena (self-contained plans)]
+struct B2Layout {
+    int64_t off_probs, off_blk, off_order, off_mats, bytes;
+};
+
+static void b2_layout(int nprob, int nblocks, int64_t mats_elems, B2Layout* L)
 {
-    int64_t mats = 0, nn = 0;
-    L->phi.clear();
-    L->dft.clear();
-    L->nodes.clear();
-    for (int i = 0; i < nprob; ++i) {
-        const auto key = std::make_pair(Nx[i], bc[i] ? 1 : 0);
-        if (!L->phi.count(key)) {
-            L->phi[key] = mats;
-            mats += 2 * (int64_t)Nx[i] * (Nx[i] - 2);
-        }
-        if (!L->nodes.count(Nx[i])) {
-            L->nodes[Nx[i]] = nn;
-            nn += 2 * (int64_t)Nx[i];
-        }
-        if (!L->dft.count(Ny[i])) {
-            L->dft[Ny[i]] = mats;
-            mats += 2 * (int64_t)Ny[i] * (Ny[i] + 2);
-        }
-    }
-    L->n_mats = mats;
-    L->n_nodes = nn;
     L->off_probs = 0;
     L->off_blk = r256((int64_t)nprob * sizeof(B2Prob));
     L->off_order = L->off_blk + r256((int64_t)(nblocks + 1) * 4);
-    L->off_nodes = L->off_order + r256((int64_t)nprob * 4);
-    L->off_mats = L->off_nodes + r256(nn * 8);
-    L->bytes = L->off_mats + r256(mats * 8);
+    L->off_mats = L->off_order + r256((int64_t)nprob * 4);
+    L->bytes = L->off_mats + r256(mats_elems * 8);
+}
+
+// distinct matrices of a batch: (Nx, bc) -> Phi offset, Ny -> Df offset
+static int64_t b2_arena(int nprob, const int* Nx, const int* Ny, const int* bc,
+                        std::map<std::pair<int, int>, int64_t>* phi, std::map<int, int64_t>* dft)
+{
+    int64_t mats = 0;
+    for (int i = 0; i < nprob; ++i) {
+        const auto key = std::make_pair(Nx[i], bc[i] ? 1 : 0);
+        if (!phi->count(key)) {
+            (*phi)[key] = mats;
+            mats += leg_elems(Nx[i]);
+        }
+        if (!dft->count(Ny[i])) {
+            (*dft)[Ny[i]] = mats;
+            mats += dft_elems(Ny[i]);
+        }
+    }
+    return mats;
 }
 
 static double b2_cost(int Nx, int Ny)
@@ -427,7 +469,8 @@ static double b2_cost(int Nx, int Ny)
 
 struct B2Info {
     int nprob, nblocks;
-    int64_t off_probs, off_blk, off_order, off_mats;
+    int64_t off_probs, off_blk, off_order;
+    const double* mats;
     int64_t wstride;  // doubles per workgroup
 };
 static std::mutex g_b2_mu;
@@ -441,6 +484,77 @@ static int64_t b2_wstride(int nprob, const int* Nx, const int* Ny)
         mx = std::max(mx, (int64_t)(Nx[i] - 2) * L + (int64_t)Nx[i] * L);
     }
     return (mx + 31) / 32 * 32;
+}
+
+// Problem table and static LPT schedule into the plan; registers the plan.
+static int b2_plan_core(int nprob, const int* Nx, const int* Ny, const double* alpha, const int* bc,
+                        const int64_t* f_off, const int64_t* u_off, int64_t f_elems, int64_t u_elems,
+                        const double* mats, const int64_t* phi_off, const int64_t* df_off,
+                        int64_t mats_elems, int nblocks, void* plan, int64_t plan_bytes,
+                        hipStream_t s)
+{
+    B2Layout L;
+    b2_layout(nprob, nblocks, 0, &L);
+    SFA_CHECK(plan_bytes >= L.bytes, SFA_ERR_ARG, "plan buffer too small (%lld < %lld)",
+              (long long)plan_bytes, (long long)L.bytes);
+    std::vector<B2Prob> probs(nprob);
+    for (int i = 0; i < nprob; ++i) {
+        const int64_t n = (int64_t)Nx[i] * Ny[i];
+        SFA_CHECK(f_off[i] >= 0 && f_off[i] + n <= f_elems, SFA_ERR_ARG,
+                  "problem %d: f range [%lld, %lld) outside the %lld-element input", i,
+                  (long long)f_off[i], (long long)(f_off[i] + n), (long long)f_elems);
+        SFA_CHECK(u_off[i] >= 0 && u_off[i] + n <= u_elems, SFA_ERR_ARG,
+                  "problem %d: u range outside the %lld-element output", i, (long long)u_elems);
+        SFA_CHECK(phi_off[i] >= 0 && phi_off[i] + leg_elems(Nx[i]) <= mats_elems && df_off[i] >= 0 &&
+                      df_off[i] + dft_elems(Ny[i]) <= mats_elems,
+                  SFA_ERR_ARG, "problem %d: matrix offsets outside the %lld-element arena", i,
+                  (long long)mats_elems);
+        B2Prob& p = probs[i];
+        p.f_off = f_off[i];
+        p.u_off = u_off[i];
+        p.phi_off = phi_off[i];
+        p.p_off = phi_off[i] + (int64_t)Nx[i] * (Nx[i] - 2);
+        p.df_off = df_off[i];
+        p.di_off = df_off[i] + (int64_t)Ny[i] * (Ny[i] + 2);
+        p.Nx = Nx[i];
+        p.Ny = Ny[i];
+        p.bc = bc[i] ? 1 : 0;
+        p.pad = 0;
+        p.alpha = alpha[i];
+    }
+    // static LPT schedule: largest first onto the least loaded workgroup
+    std::vector<int> ids(nprob);
+    std::iota(ids.begin(), ids.end(), 0);
+    std::stable_sort(ids.begin(), ids.end(),
+                     [&](int a, int b) { return b2_cost(Nx[a], Ny[a]) > b2_cost(Nx[b], Ny[b]); });
+    typedef std::pair<double, int> LB;
+    std::priority_queue<LB, std::vector<LB>, std::greater<LB>> heap;
+    for (int b = 0; b < nblocks; ++b) heap.push(LB(0.0, b));
+    std::vector<std::vector<int>> lists(nblocks);
+    for (int id : ids) {
+        LB t = heap.top();
+        heap.pop();
+        lists[t.second].push_back(id);
+        t.first += b2_cost(Nx[id], Ny[id]);
+        heap.push(t);
+    }
+    std::vector<int> blk(nblocks + 1, 0), order;
+    order.reserve(nprob);
+    for (int b = 0; b < nblocks; ++b) {
+        blk[b] = (int)order.size();
+        for (int id : lists[b]) order.push_back(id);
+    }
+    blk[nblocks] = (int)order.size();
+    char* pb = (char*)plan;
+    SFA_HIP(hipMemcpyAsync(pb + L.off_probs, probs.data(), probs.size() * sizeof(B2Prob),
+                           hipMemcpyHostToDevice, s));
+    SFA_HIP(hipMemcpyAsync(pb + L.off_blk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, s));
+    SFA_HIP(hipMemcpyAsync(pb + L.off_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
+    SFA_HIP(hipStreamSynchronize(s));  // host vectors are copied before they go out of scope
+    std::lock_guard<std::mutex> lk(g_b2_mu);
+    g_b2[plan] = B2Info{nprob, nblocks, L.off_probs, L.off_blk, L.off_order, mats,
+                        b2_wstride(nprob, Nx, Ny)};
+    return SFA_OK;
 }
 
 template <int BM, int BN>
@@ -485,8 +599,18 @@ extern "C" int64_t sfa_batch2d_plan_bytes(int nprob, const int* Nx, const int* N
                                           int nblocks)
 {
     if (b2_check(nprob, Nx, Ny, bc, nblocks)) return -1;
+    std::map<std::pair<int, int>, int64_t> phi;
+    std::map<int, int64_t> dft;
     B2Layout L;
-    b2_layout(nprob, Nx, Ny, bc, nblocks, &L);
+    b2_layout(nprob, nblocks, b2_arena(nprob, Nx, Ny, bc, &phi, &dft), &L);
+    return L.bytes;
+}
+
+extern "C" int64_t sfa_batch2d_plan_ext_bytes(int nprob, int nblocks)
+{
+    if (nprob < 1 || nblocks < 1) return -1;
+    B2Layout L;
+    b2_layout(nprob, nblocks, 0, &L);
     return L.bytes;
 }
 
@@ -494,6 +618,22 @@ extern "C" int64_t sfa_batch2d_workspace(int nprob, const int* Nx, const int* Ny
 {
     if (nprob < 1 || nblocks < 1 || !Nx || !Ny) return -1;
     return b2_wstride(nprob, Nx, Ny) * 8 * (int64_t)nblocks;
+}
+
+extern "C" int64_t sfa_batch2d_leg_elems(int Nx) { return (Nx >= 8 && Nx % 2 == 0) ? leg_elems(Nx) : -1; }
+extern "C" int64_t sfa_batch2d_dft_elems(int Ny) { return (Ny >= 4 && Ny % 2 == 0) ? dft_elems(Ny) : -1; }
+
+extern "C" int sfa_batch2d_leg_matrices(int Nx, int bc, double* out, void* stream)
+{
+    SFA_CHECK(Nx >= 8 && Nx <= 1024 && Nx % 2 == 0 && (bc == 0 || bc == 1) && out, SFA_ERR_ARG,
+              "need even Nx in [8, 1024], bc 0/1, out");
+    return build_leg(Nx, bc, out, (hipStream_t)stream);
+}
+
+extern "C" int sfa_batch2d_dft_matrices(int Ny, double* out, void* stream)
+{
+    SFA_CHECK(Ny >= 4 && Ny <= 1024 && Ny % 2 == 0 && out, SFA_ERR_ARG, "need even Ny in [4, 1024], out");
+    return build_dft(Ny, out, (hipStream_t)stream);
 }
 
 extern "C" int sfa_batch2d_plan(int nprob, const int* Nx, const int* Ny, const double* alpha,
@@ -504,95 +644,41 @@ extern "C" int sfa_batch2d_plan(int nprob, const int* Nx, const int* Ny, const d
     int rc;
     if ((rc = b2_check(nprob, Nx, Ny, bc, nblocks))) return rc;
     SFA_CHECK(alpha && f_off && u_off && plan, SFA_ERR_ARG, "NULL argument");
-    for (int i = 0; i < nprob; ++i) {
-        const int64_t n = (int64_t)Nx[i] * Ny[i];
-        SFA_CHECK(f_off[i] >= 0 && f_off[i] + n <= f_elems, SFA_ERR_ARG,
-                  "problem %d: f range [%lld, %lld) outside the %lld-element input", i,
-                  (long long)f_off[i], (long long)(f_off[i] + n), (long long)f_elems);
-        SFA_CHECK(u_off[i] >= 0 && u_off[i] + n <= u_elems, SFA_ERR_ARG,
-                  "problem %d: u range outside the %lld-element output", i, (long long)u_elems);
-    }
+    std::map<std::pair<int, int>, int64_t> phi;
+    std::map<int, int64_t> dft;
+    const int64_t mats = b2_arena(nprob, Nx, Ny, bc, &phi, &dft);
     B2Layout L;
-    b2_layout(nprob, Nx, Ny, bc, nblocks, &L);
+    b2_layout(nprob, nblocks, mats, &L);
     SFA_CHECK(plan_bytes >= L.bytes, SFA_ERR_ARG, "plan buffer too small (%lld < %lld)",
               (long long)plan_bytes, (long long)L.bytes);
-    // problem table
-    std::vector<B2Prob> probs(nprob);
-    for (int i = 0; i < nprob; ++i) {
-        B2Prob& p = probs[i];
-        p.f_off = f_off[i];
-        p.u_off = u_off[i];
-        p.phi_off = L.phi[std::make_pair(Nx[i], bc[i] ? 1 : 0)];
-        p.p_off = p.phi_off + (int64_t)Nx[i] * (Nx[i] - 2);
-        p.df_off = L.dft[Ny[i]];
-        p.di_off = p.df_off + (int64_t)Ny[i] * (Ny[i] + 2);
-        p.Nx = Nx[i];
-        p.Ny = Ny[i];
-        p.bc = bc[i] ? 1 : 0;
-        p.pad = 0;
-        p.alpha = alpha[i];
-    }
-    // static LPT schedule: largest first onto the least loaded workgroup
-    std::vector<int> ids(nprob);
-    std::iota(ids.begin(), ids.end(), 0);
-    std::stable_sort(ids.begin(), ids.end(),
-                     [&](int a, int b) { return b2_cost(Nx[a], Ny[a]) > b2_cost(Nx[b], Ny[b]); });
-    typedef std::pair<double, int> LB;
-    std::priority_queue<LB, std::vector<LB>, std::greater<LB>> heap;
-    for (int b = 0; b < nblocks; ++b) heap.push(LB(0.0, b));
-    std::vector<std::vector<int>> lists(nblocks);
-    for (int id : ids) {
-        LB t = heap.top();
-        heap.pop();
-        lists[t.second].push_back(id);
-        t.first += b2_cost(Nx[id], Ny[id]);
-        heap.push(t);
-    }
-    std::vector<int> blk(nblocks + 1, 0), order;
-    order.reserve(nprob);
-    for (int b = 0; b < nblocks; ++b) {
-        blk[b] = (int)order.size();
-        for (int id : lists[b]) order.push_back(id);
-    }
-    blk[nblocks] = (int)order.size();
-    // nodes and weights per distinct Nx
-    std::vector<double> nodes((size_t)std::max<int64_t>(L.n_nodes, 1));
-    for (auto& kv : L.nodes) {
-        std::vector<double> x, w;
-        lg_nodes(kv.first, x, w);
-        std::copy(x.begin(), x.end(), nodes.begin() + kv.second);
-        std::copy(w.begin(), w.end(), nodes.begin() + kv.second + kv.first);
-    }
     hipStream_t s = (hipStream_t)stream;
-    char* pb = (char*)plan;
-    SFA_HIP(hipMemcpyAsync(pb + L.off_probs, probs.data(), probs.size() * sizeof(B2Prob),
-                           hipMemcpyHostToDevice, s));
-    SFA_HIP(hipMemcpyAsync(pb + L.off_blk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, s));
-    SFA_HIP(hipMemcpyAsync(pb + L.off_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
-    SFA_HIP(hipMemcpyAsync(pb + L.off_nodes, nodes.data(), (size_t)L.n_nodes * 8,
-                           hipMemcpyHostToDevice, s));
-    const double* dn = (const double*)(pb + L.off_nodes);
-    double* dm = (double*)(pb + L.off_mats);
-    for (auto& kv : L.phi) {
-        const int n = kv.first.first;
-        const int64_t no = L.nodes[n];
-        hipLaunchKernelGGL(b2_phi_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, dn + no,
-                           dn + no + n, n, kv.first.second, dm + kv.second,
-                           dm + kv.second + (int64_t)n * (n - 2));
-        SFA_LAUNCH_CHECK();
+    double* dm = (double*)((char*)plan + L.off_mats);
+    for (auto& kv : phi)
+        if ((rc = build_leg(kv.first.first, kv.first.second, dm + kv.second, s))) return rc;
+    for (auto& kv : dft)
+        if ((rc = build_dft(kv.first, dm + kv.second, s))) return rc;
+    std::vector<int64_t> po(nprob), dfo(nprob);
+    for (int i = 0; i < nprob; ++i) {
+        po[i] = phi[std::make_pair(Nx[i], bc[i] ? 1 : 0)];
+        dfo[i] = dft[Ny[i]];
     }
-    for (auto& kv : L.dft) {
-        const int n = kv.first;
-        const int64_t nt = (int64_t)n * (n / 2 + 1);
-        hipLaunchKernelGGL(b2_dft_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, n,
-                           dm + kv.second, dm + kv.second + (int64_t)n * (n + 2));
-        SFA_LAUNCH_CHECK();
-    }
-    SFA_HIP(hipStreamSynchronize(s));  // host vectors are copied before they go out of scope
-    std::lock_guard<std::mutex> lk(g_b2_mu);
-    g_b2[plan] = B2Info{nprob, nblocks, L.off_probs, L.off_blk, L.off_order, L.off_mats,
-                        b2_wstride(nprob, Nx, Ny)};
-    return SFA_OK;
+    return b2_plan_core(nprob, Nx, Ny, alpha, bc, f_off, u_off, f_elems, u_elems, dm, po.data(),
+                        dfo.data(), mats, nblocks, plan, plan_bytes, s);
+}
+
+extern "C" int sfa_batch2d_plan_ext(int nprob, const int* Nx, const int* Ny, const double* alpha,
+                                    const int* bc, const int64_t* f_off, const int64_t* u_off,
+                                    int64_t f_elems, int64_t u_elems, const double* mats,
+                                    const int64_t* phi_off, const int64_t* df_off,
+                                    int64_t mats_elems, int nblocks, void* plan,
+                                    int64_t plan_bytes, void* stream)
+{
+    int rc;
+    if ((rc = b2_check(nprob, Nx, Ny, bc, nblocks))) return rc;
+    SFA_CHECK(alpha && f_off && u_off && plan && mats && phi_off && df_off, SFA_ERR_ARG,
+              "NULL argument");
+    return b2_plan_core(nprob, Nx, Ny, alpha, bc, f_off, u_off, f_elems, u_elems, mats, phi_off,
+                        df_off, mats_elems, nblocks, plan, plan_bytes, (hipStream_t)stream);
 }
 
 extern "C" int sfa_batch2d_solve(const double* F, double* U, const void* plan, void* work,
@@ -602,7 +688,7 @@ extern "C" int sfa_batch2d_solve(const double* F, double* U, const void* plan, v
     {
         std::lock_guard<std::mutex> lk(g_b2_mu);
         auto it = g_b2.find(plan);
-        SFA_CHECK(it != g_b2.end(), SFA_ERR_ARG, "plan was not built by sfa_batch2d_plan");
+        SFA_CHECK(it != g_b2.end(), SFA_ERR_ARG, "plan was not built by sfa_batch2d_plan(_ext)");
         pi = it->second;
     }
     SFA_CHECK(F && U && work, SFA_ERR_ARG, "NULL argument");
@@ -618,7 +704,7 @@ extern "C" int sfa_batch2d_solve(const double* F, double* U, const void* plan, v
     a.probs = (const B2Prob*)(pb + pi.off_probs);
     a.blk_ptr = (const int*)(pb + pi.off_blk);
     a.order = (const int*)(pb + pi.off_order);
-    a.mats = (const double*)(pb + pi.off_mats);
+    a.mats = pi.mats;
     a.work = (double*)work;
     a.wstride = pi.wstride;
     hipStream_t s = (hipStream_t)stream;

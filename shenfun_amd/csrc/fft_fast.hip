@@ -281,7 +281,7 @@ struct XPass2Args {
 // C2 = double2: FP64 everywhere.  C2 = float2: complex64 storage and DCT/FFT
 // arithmetic, the Helmholtz solve still FP64 (values widened on entry).
 template <int N, typename C2>
-__global__ void __launch_bounds__(TL * Pow2<N>::Q, 2) xpass2_kernel(XPass2Args a)
+__device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
 {
     using Pl = Pow2<N>;
     using R = typename CT<C2>::R;
@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q, 2) xpass2_kernel(XPass2Args a
     C2* Wd = reinterpret_cast<C2*>(a.W);
     const int tid = threadIdx.x;
     const int t = tid % TL, j = tid / TL;
-    const int64_t L = (int64_t)blockIdx.x * TL + t;
+    const int64_t L = bid * TL + t;
     const bool valid = L < a.nl;
     for (int i = tid; i < N; i += NTH) {
         tw[i] = from_c64<C2>(a.tw[i]);
@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q, 2) xpass2_kernel(XPass2Args a
         const int c = (tid / 16) * ROUNDS + rho;
         const int ct = c >> 1, p = c & 1, lane = tid & 15;
         C2* X = ex + ct * LS;
-        const int64_t Lc = (int64_t)blockIdx.x * TL + ct;
+        const int64_t Lc = bid * TL + ct;
         int u = 0;
         if (Lc < a.nl) {
             const int64_t ky = Lc / a.Kp, kz = a.kz0 + (Lc - ky * a.Kp);
@@ -428,6 +428,31 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q, 2) xpass2_kernel(XPass2Args a
     }
 }
 
+// workgroups per CU the x-pass is compiled for: at N = 512 one block holds
+// 16 lines of 512 complex values in LDS (~156 KB in FP64)
+template <int N>
+constexpr int xpass_minb() { return N >= 512 ? 1 : 2; }
+
+template <int N, typename C2>
+__global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_kernel(XPass2Args a)
+{
+    xpass2_body<N, C2>(a, blockIdx.x);
+}
+
+// Mixed precision: the complex128 planes' blocks [0, nb64) and the complex64
+// planes' blocks after them in one grid, so the few FP64 blocks run beside the
+// FP32 ones instead of in a launch of their own.
+template <int N>
+__global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_dual_kernel(XPass2Args a64, XPass2Args a32,
+                                                                        int64_t nb64)
+{
+    const int64_t b = blockIdx.x;
+    if (b < nb64)
+        xpass2_body<N, double2>(a64, b);
+    else
+        xpass2_body<N, float2>(a32, b - nb64);
+}
+
 template <int N, typename C2>
 static int launch_xpass2(XPass2Args& a, hipStream_t s)
 {
@@ -441,27 +466,48 @@ static int launch_xpass2(XPass2Args& a, hipStream_t s)
     return SFA_OK;
 }
 
+template <int N>
+static int launch_xpass2_dual(XPass2Args& a64, XPass2Args& a32, hipStream_t s)
+{
+    using Pl = Pow2<N>;
+    const size_t lds = sizeof(double2) * ((size_t)2 * N + (size_t)TL * Pl::LS) + sizeof(double) * N;
+    auto k = xpass2_dual_kernel<N>;
+    if (lds > 64 * 1024)
+        hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int64_t nb64 = (a64.nl + TL - 1) / TL, nb32 = (a32.nl + TL - 1) / TL;
+    hipLaunchKernelGGL(k, dim3((unsigned)(nb64 + nb32)), dim3(TL * Pl::Q), lds, s, a64, a32, nb64);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
 // Fused x-pass on the kz planes [kz0, kz0 + Kp) held (line pitch Kp) in W.
 // f32 selects complex64 storage/arithmetic.  Returns 1 if handled.
+static bool xpass_args(void* W, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha, int neumann,
+                       const double* K2, const XPlan& pl, XPass2Args* a)
+{
+    a->W = W;
+    a->sa = (int64_t)Ny * Kp;
+    a->nl = (int64_t)Ny * Kp;
+    a->Kp = Kp;
+    a->kz0 = kz0;
+    a->Nzh = Nzh;
+    a->Ny = Ny;
+    a->neumann = neumann;
+    a->alpha = alpha;
+    a->K2 = K2;
+    a->pl = pl;
+    a->tw = fft_table(Nx);
+    a->tq = quarter_table(Nx);
+    return a->tw && a->tq;
+}
+
 int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha,
                    int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc)
 {
-    if (!(Nx == 64 || Nx == 128 || Nx == 256) || pl.linemap == nullptr || pl.bs == nullptr) return 0;
+    if (!(Nx == 64 || Nx == 128 || Nx == 256 || Nx == 512) || pl.linemap == nullptr || pl.bs == nullptr)
+        return 0;
     XPass2Args a;
-    a.W = W;
-    a.sa = (int64_t)Ny * Kp;
-    a.nl = (int64_t)Ny * Kp;
-    a.Kp = Kp;
-    a.kz0 = kz0;
-    a.Nzh = Nzh;
-    a.Ny = Ny;
-    a.neumann = neumann;
-    a.alpha = alpha;
-    a.K2 = K2;
-    a.pl = pl;
-    a.tw = fft_table(Nx);
-    a.tq = quarter_table(Nx);
-    if (!a.tw || !a.tq) {
+    if (!xpass_args(W, Nx, Ny, Nzh, Kp, kz0, alpha, neumann, K2, pl, &a)) {
         *rc = fail(SFA_ERR_HIP, "table allocation failed");
         return 1;
     }
@@ -470,9 +516,31 @@ int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, 
         *rc = f32 ? launch_xpass2<NN, float2>(a, s) : launch_xpass2<NN, double2>(a, s); \
         break;
     switch (Nx) {
-        SFA_XP(64) SFA_XP(128) SFA_XP(256)
+        SFA_XP(64) SFA_XP(128) SFA_XP(256) SFA_XP(512)
     }
 #undef SFA_XP
+    return 1;
+}
+
+// Mixed precision x-pass: planes [0, Kh) complex128 in W64 (pitch Kh) and
+// [Kh, Nzh) complex64 in W32 (pitch Nzh - Kh), one launch.
+int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, double alpha,
+                     int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc)
+{
+    if (!(Nx == 64 || Nx == 128 || Nx == 256 || Nx == 512) || pl.linemap == nullptr || pl.bs == nullptr)
+        return 0;
+    XPass2Args a64, a32;
+    if (!xpass_args(W64, Nx, Ny, Nzh, Kh, 0, alpha, neumann, K2, pl, &a64) ||
+        !xpass_args(W32, Nx, Ny, Nzh, Nzh - Kh, Kh, alpha, neumann, K2, pl, &a32)) {
+        *rc = fail(SFA_ERR_HIP, "table allocation failed");
+        return 1;
+    }
+    switch (Nx) {
+        case 64: *rc = launch_xpass2_dual<64>(a64, a32, s); break;
+        case 128: *rc = launch_xpass2_dual<128>(a64, a32, s); break;
+        case 256: *rc = launch_xpass2_dual<256>(a64, a32, s); break;
+        default: *rc = launch_xpass2_dual<512>(a64, a32, s); break;
+    }
     return 1;
 }
 

@@ -234,6 +234,8 @@ int fast_xpass(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, c
                const XPlan& pl, hipStream_t s, int* rc);
 int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha,
                    int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc);
+int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, double alpha,
+                     int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc);
 int fast_c2c(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
              hipStream_t s, int* rc);
 int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
@@ -415,10 +417,8 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
         !fast_c2c_f32(g32, g32, W32, W32, -1, 1.0 / (double)Ny, s, &rc) || rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Ny=%lld unsupported", (long long)Ny);
     mark(2);
-    if (!fast_xpass_sub(W64, false, (int)Nx, (int)Ny, (int)Nzh, Kh, 0, alpha, neumann, K2, pv, s,
-                        &rc) || rc ||
-        !fast_xpass_sub(W32, true, (int)Nx, (int)Ny, (int)Nzh, Kl, Kh, alpha, neumann, K2, pv, s,
-                        &rc) || rc)
+    if (!fast_xpass_mixed(W64, W32, (int)Nx, (int)Ny, (int)Nzh, Kh, alpha, neumann, K2, pv, s, &rc) ||
+        rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nx=%lld unsupported", (long long)Nx);
     mark(3);
     if (!fast_c2c(g64, g64, W64, W64, +1, 1.0, s, &rc) || rc ||

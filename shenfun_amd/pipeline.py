@@ -41,7 +41,10 @@ class HelmholtzSolver3D:
         _lib.call("sfa_helmholtz3d_plan", Nx, Ny, Nz, self.alpha, int(self.neumann),
                   _lib.ptr(self.plan), self.plan.numel(), _lib.stream_ptr())
 
-    def __call__(self, f, u=None):
+    def __call__(self, f, u=None, work=None):
+        """u for (alpha - lap) u = f.  work: optional uint8 CUDA tensor of at
+        least ``workspace_bytes`` bytes used instead of the solver's own
+        workspace (lets several solves of one solver run on different streams)."""
         if not (f.is_cuda and f.dtype == torch.float64 and f.is_contiguous()
                 and tuple(f.shape) == tuple(self.shape)):
             raise ValueError(f"f must be a contiguous float64 CUDA tensor of shape {self.shape}")
@@ -50,10 +53,17 @@ class HelmholtzSolver3D:
         if not (u.is_contiguous() and u.shape == f.shape and u.dtype == f.dtype):
             raise ValueError("u must match f")
         Nx, Ny, Nz = self.shape
+        w = self.work if work is None else work
+        if w.numel() < self.work.numel() or not w.is_cuda:
+            raise ValueError(f"work must hold at least {self.work.numel()} bytes on the device")
         _lib.call("sfa_helmholtz3d", _lib.ptr(f), _lib.ptr(u), Nx, Ny, Nz, self.alpha,
                   int(self.neumann), self.precision, self.fp64_planes, _lib.ptr(self.plan),
-                  _lib.ptr(self.work), self.work.numel(), _lib.stream_ptr())
+                  _lib.ptr(w), w.numel(), _lib.stream_ptr())
         return u
+
+    @property
+    def workspace_bytes(self):
+        return self.work.numel()
 
     def stage_times(self, f, u, nrep):
         """Milliseconds per stage (z R2C, y C2C, x-pass, y C2C inverse, z C2R)

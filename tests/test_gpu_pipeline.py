@@ -45,7 +45,7 @@ def test_poisson2d_legendre_manufactured(sfa, seed):
 
 @pytest.mark.parametrize("bc", ["D", "N"])
 @pytest.mark.parametrize("shape", [(16, 12, 10), (32, 24, 16), (64, 64, 64), (128, 24, 20),
-                                   (256, 12, 10), (256, 32, 64)])
+                                   (256, 12, 10), (256, 32, 64), (512, 16, 32)])
 def test_helmholtz3d_dropin_and_fused(sfa, bc, shape):
     """(alpha - lap) u = f on Chebyshev x C2C x R2C: the drop-in chain
     T.scalar_product -> Helmholtz -> T.backward and the fused pipeline agree
@@ -86,7 +86,8 @@ def test_helmholtz3d_fused_repeat_identical(sfa):
 
 @pytest.mark.parametrize("bc", ["D", "N"])
 @pytest.mark.parametrize("shape,planes", [((64, 64, 64), 0), ((128, 32, 64), 0), ((256, 64, 64), 0),
-                                          ((64, 16, 32), 12), ((256, 256, 256), 0)])
+                                          ((64, 16, 32), 12), ((256, 256, 256), 0),
+                                          ((512, 32, 64), 0)])
 def test_helmholtz3d_mixed(sfa, bc, shape, planes):
     """precision='mixed' (FP64 low kz planes, FP32 transforms above, FP64
     solve) stays within 1e-10 relative max error of the FP64 oracle on the

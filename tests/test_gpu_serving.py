@@ -1,0 +1,94 @@
+"""GPU: the continuous-batching request loop.  Every served result matches the
+FP64 oracle, and the same request served alone gives the same result as in
+the mix (stated tolerance: max relative difference <= 1e-13; the batched
+kernels treat each problem independently, so it is normally 0)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import spectral as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a = a.cpu().numpy() if isinstance(a, torch.Tensor) else a
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _oracle(r):
+    f = r.f.cpu().numpy()
+    if r.family == "cheb1d":
+        N, bc, al = r.key
+        fh = S.basis_scalar_product("chebyshev", bc, f)
+        d0, d1, d2, L = S.helmholtz_lu(bc, N, np.array(-1.0), np.array(al))
+        uh = S.helmholtz_solve(fh, d0, d1, d2, L, neumann=(bc == "N"))
+        return S.basis_backward("chebyshev", bc, uh)
+    if r.family == "leg2d":
+        return S.legendre_fourier_2d(f, r.key[3], r.key[2])[0]
+    return S.helmholtz3d_cheb_fourier(f, r.key[2], r.key[1])[0]
+
+
+@pytest.fixture(scope="module")
+def served():
+    from shenfun_amd.serving import Server, request_stream
+    reqs = request_stream(160, rate=1e5, seed=11, classes=(6, 10, 4))
+    srv = Server(slots=4)
+    print("[serve mix]", flush=True)
+    srv.run(reqs, realtime=False)
+    return srv, reqs
+
+
+def test_served_results_match_oracle(served):
+    srv, reqs = served
+    seen = set()
+    for r in reqs:
+        k = (r.family, r.key)
+        if k in seen:
+            continue
+        seen.add(k)
+        err = _rel(r.u, _oracle(r))
+        print(f"[{r.family} {r.key}] err {err:.2e}", flush=True)
+        assert err < 1e-11
+    assert srv.batches["leg2d"] < sum(r.family == "leg2d" for r in reqs)  # 2D requests were batched
+
+
+def test_alone_equals_in_mix(served):
+    from shenfun_amd.serving import Server
+    _, reqs = served
+    picks = {}
+    for r in reqs:
+        picks.setdefault((r.family, r.key), r)
+    worst = 0.0
+    for r in list(picks.values())[:12]:
+        alone = type(r)(r.rid, r.family, r.key, r.f, 0.0)
+        Server(slots=1).run([alone], realtime=False)
+        d = _rel(alone.u, r.u.cpu().numpy())
+        worst = max(worst, d)
+        print(f"[alone {r.family} {r.key}] max rel diff {d:.1e}", flush=True)
+        assert d <= 1e-13
+    print(f"worst alone-vs-mix difference {worst:.1e}")
+
+
+def test_memory_admission_serialises(served):
+    """A memory budget below any batch admits one batch at a time; results
+    are unchanged."""
+    from shenfun_amd.serving import Request, Server
+    _, reqs = served
+    sub = [Request(r.rid, r.family, r.key, r.f, 0.0) for r in reqs[:40]]
+    srv = Server(slots=4, mem_budget=1)
+    srv.run(sub, realtime=False)
+    for a, b in zip(sub, reqs[:40]):
+        assert _rel(a.u, b.u.cpu().numpy()) <= 1e-13
+
+
+def test_realtime_stream_latency(served):
+    """Open-loop Poisson arrivals: every request completes, latencies are
+    positive and p99 >= p50."""
+    from shenfun_amd.serving import Server, request_stream
+    reqs = request_stream(300, rate=5000, seed=12, classes=(6, 10, 4))
+    srv = Server(slots=4)
+    srv.run(reqs, realtime=True)
+    st = Server.stats(reqs)
+    print(st, flush=True)
+    assert st["requests"] == 300 and 0 < st["p50_ms"] <= st["p99_ms"]
