@@ -145,7 +145,7 @@ def batched_leg(sfa, torch, n=4096, seed=0):
                     f"[32,256], alpha U(0,100), seed {seed}",
         "solves_per_s": n / (ms * 1e-3), "ms_per_batch": ms, "launches": 1,
         "tile": tune["best"]["tile"], "nblocks": tune["best"]["nblocks"],
-        "autotune_ms": {f"{r['tile'][0]}x{r['tile'][1]}@{r['nblocks']}": r["ms"] for r in tune["table"]},
+        "autotune_ms": {"x".join(map(str, r["tile"])) + f"@{r['nblocks']}": r["ms"] for r in tune["table"]},
         "roofline": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": 78.6,
                      "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / 1e12 / 78.6,
                      "flops": flops, "compulsory_bytes": H.bytes_moved()},

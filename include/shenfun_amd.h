@@ -243,8 +243,8 @@ int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t Nx, int64_t 
  * over nblocks workgroups, and the composite Legendre and real DFT matrices
  * per distinct Nx / Ny; it synchronises `stream`.  sfa_batch2d_solve runs all
  * problems in one persistent launch of nblocks workgroups with tile variant
- * `variant` (0 <= variant < sfa_batch2d_variants(); its block tile BM x BN is
- * reported in bm[], bn[]).  work: sfa_batch2d_workspace() bytes.  F and U must
+ * `variant` (0 <= variant < sfa_batch2d_variants(); its block tile BM x BN and
+ * K step BK are reported as bm[] = BM, bn[] = 1000 BN + BK).  work: sfa_batch2d_workspace() bytes.  F and U must
  * not overlap.  No reference counterpart exists for the batching itself: the
  * reference solves one problem per call. */
 int64_t sfa_batch2d_plan_bytes(int nprob, const int* Nx, const int* Ny, const int* bc, int nblocks);

@@ -28,13 +28,13 @@ def _bc_code(b):
 
 
 def tile_variants():
-    """[(BM, BN)] block tiles of the batched kernel, indexed by variant id."""
+    """[(BM, BN, BK)] block tiles and K steps of the batched kernel, indexed by variant id."""
     L = _lib.lib()
     n = L.sfa_batch2d_variants(None, None, 0)
     bm = (_lib.ctypes.c_int * n)()
     bn = (_lib.ctypes.c_int * n)()
     L.sfa_batch2d_variants(bm, bn, n)
-    return [(bm[i], bn[i]) for i in range(n)]
+    return [(bm[i], bn[i] // 1000, bn[i] % 1000) for i in range(n)]
 
 
 class BatchedHelmholtz2D:

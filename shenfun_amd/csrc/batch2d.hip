@@ -51,20 +51,21 @@ __device__ inline v4d mfma16(double a, double b, v4d c)
 }
 
 constexpr int B2_THREADS = 256;
-constexpr int B2_BK = 16;
 
 // C (m x n, ldc) = A (m x k, lda) B (k x n, ldb), all row-major FP64, by the
-// whole workgroup (4 waves in a 2 x 2 grid over a BM x BN block tile).  sA:
-// BK x (BM + 1) doubles, sB: BK x BN doubles of LDS.  Out-of-range operand
-// elements load as zero; stores are bounds-checked.
-template <int BM, int BN>
+// whole workgroup (4 waves in a 2 x 2 grid over a BM x BN block tile, K in
+// steps of BK).  sA: BK x (BM + 1) doubles, sB: BK x BN doubles of LDS.
+// Out-of-range operand elements load as zero; stores are bounds-checked.
+// Each output is one MFMA accumulation chain over K in ascending order, so
+// the result does not depend on BM, BN or BK.
+template <int BM, int BN, int BK>
 __device__ void block_gemm(const double* __restrict__ A, int64_t lda, const double* __restrict__ B,
                            int64_t ldb, double* __restrict__ C, int64_t ldc, int m, int n, int k,
                            double* sA, double* sB)
 {
     constexpr int SA = BM + 1;
     constexpr int TI = BM / 32, TJ = BN / 32;
-    constexpr int LA = BM * B2_BK / B2_THREADS, LB = B2_BK * BN / B2_THREADS;
+    constexpr int LA = BM * BK / B2_THREADS, LB = BK * BN / B2_THREADS;
     static_assert(LA >= 1 && LB >= 1 && TI >= 1 && TJ >= 1, "tile too small");
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wi = wv >> 1, wj = wv & 1;
@@ -81,7 +82,7 @@ __device__ void block_gemm(const double* __restrict__ A, int64_t lda, const doub
 #pragma unroll
                 for (int r = 0; r < LA; ++r) {
                     const int e = tid + B2_THREADS * r;
-                    const int i = e / B2_BK, kk = e % B2_BK;
+                    const int i = e / BK, kk = e % BK;
                     const int gi = m0 + i, gk = k0 + kk;
                     ra[r] = (gi < m && gk < k) ? A[(int64_t)gi * lda + gk] : 0.0;
                 }
@@ -94,12 +95,12 @@ __device__ void block_gemm(const double* __restrict__ A, int64_t lda, const doub
                 }
             };
             load(0);
-            for (int k0 = 0; k0 < k; k0 += B2_BK) {
+            for (int k0 = 0; k0 < k; k0 += BK) {
                 __syncthreads();  // the previous step's fragment reads are done
 #pragma unroll
                 for (int r = 0; r < LA; ++r) {
                     const int e = tid + B2_THREADS * r;
-                    sA[(e % B2_BK) * SA + e / B2_BK] = ra[r];
+                    sA[(e % BK) * SA + e / BK] = ra[r];
                 }
 #pragma unroll
                 for (int r = 0; r < LB; ++r) {
@@ -107,9 +108,9 @@ __device__ void block_gemm(const double* __restrict__ A, int64_t lda, const doub
                     sB[(e / BN) * BN + e % BN] = rb[r];
                 }
                 __syncthreads();
-                if (k0 + B2_BK < k) load(k0 + B2_BK);  // prefetch overlaps the MFMAs
+                if (k0 + BK < k) load(k0 + BK);  // prefetch overlaps the MFMAs
 #pragma unroll
-                for (int s = 0; s < B2_BK / 4; ++s) {
+                for (int s = 0; s < BK / 4; ++s) {
                     const int kk = 4 * s + lk;
                     double af[TI], bf[TJ];
 #pragma unroll
@@ -223,11 +224,11 @@ struct B2Args {
     int64_t wstride;
 };
 
-template <int BM, int BN>
+template <int BM, int BN, int BK>
 __global__ void __launch_bounds__(B2_THREADS) batch2d_kernel(B2Args a)
 {
-    __shared__ double sA[B2_BK * (BM + 1)];
-    __shared__ double sB[B2_BK * BN];
+    __shared__ double sA[BK * (BM + 1)];
+    __shared__ double sB[BK * BN];
     const int b = blockIdx.x;
     double* W0 = a.work + (int64_t)b * a.wstride;
     for (int q = a.blk_ptr[b]; q < a.blk_ptr[b + 1]; ++q) {
@@ -241,15 +242,15 @@ __global__ void __launch_bounds__(B2_THREADS) batch2d_kernel(B2Args a)
         const double* Df = a.mats + pr.df_off;
         const double* Di = a.mats + pr.di_off;
         // T1 = P f (M x Ny), held in u
-        block_gemm<BM, BN>(P, Nx, f, Ny, u, Ny, M, Ny, Nx, sA, sB);
+        block_gemm<BM, BN, BK>(P, Nx, f, Ny, u, Ny, M, Ny, Nx, sA, sB);
         // T2 = T1 Df (M x L) in W0
-        block_gemm<BM, BN>(u, Ny, Df, L, W0, L, M, L, Ny, sA, sB);
+        block_gemm<BM, BN, BK>(u, Ny, Df, L, W0, L, M, L, Ny, sA, sB);
         // per-ky solve in place, W1 as the Thomas scratch
         batch_solve(W0, L, M, Nyh, pr.bc, pr.alpha, W1);
         // U1 = Phi u_hat (Nx x L) in W1
-        block_gemm<BM, BN>(Phi, M, W0, L, W1, L, Nx, L, M, sA, sB);
+        block_gemm<BM, BN, BK>(Phi, M, W0, L, W1, L, Nx, L, M, sA, sB);
         // u = U1 Di (Nx x Ny)
-        block_gemm<BM, BN>(W1, L, Di, Ny, u, Ny, Nx, Ny, L, sA, sB);
+        block_gemm<BM, BN, BK>(W1, L, Di, Ny, u, Ny, Nx, Ny, L, sA, sB);
     }
 }
 
@@ -557,10 +558,10 @@ static int b2_plan_core(int nprob, const int* Nx, const int* Ny, const double* a
     return SFA_OK;
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int BK>
 static int launch_b2(const B2Args& a, int nblocks, hipStream_t s)
 {
-    hipLaunchKernelGGL((batch2d_kernel<BM, BN>), dim3((unsigned)nblocks), dim3(B2_THREADS), 0, s, a);
+    hipLaunchKernelGGL((batch2d_kernel<BM, BN, BK>), dim3((unsigned)nblocks), dim3(B2_THREADS), 0, s, a);
     SFA_LAUNCH_CHECK();
     return SFA_OK;
 }
@@ -569,14 +570,16 @@ static int launch_b2(const B2Args& a, int nblocks, hipStream_t s)
 
 using namespace sfa;
 
-static const int kB2Variants[][2] = {{64, 64}, {32, 64}, {64, 128}, {128, 64}};
+// block tiles (BM, BN, BK) of the launch variants
+static const int kB2Variants[][3] = {{64, 64, 16}, {32, 64, 16}, {64, 128, 16}, {128, 64, 16},
+                                     {64, 64, 32}, {64, 128, 32}, {128, 64, 32}};
 
 extern "C" int sfa_batch2d_variants(int* bm, int* bn, int cap)
 {
     const int n = (int)(sizeof(kB2Variants) / sizeof(kB2Variants[0]));
     for (int i = 0; i < n && i < cap; ++i) {
         if (bm) bm[i] = kB2Variants[i][0];
-        if (bn) bn[i] = kB2Variants[i][1];
+        if (bn) bn[i] = kB2Variants[i][1] * 1000 + kB2Variants[i][2];
     }
     return n;
 }
@@ -709,10 +712,13 @@ extern "C" int sfa_batch2d_solve(const double* F, double* U, const void* plan, v
     a.wstride = pi.wstride;
     hipStream_t s = (hipStream_t)stream;
     switch (variant) {
-        case 0: return launch_b2<64, 64>(a, pi.nblocks, s);
-        case 1: return launch_b2<32, 64>(a, pi.nblocks, s);
-        case 2: return launch_b2<64, 128>(a, pi.nblocks, s);
-        default: return launch_b2<128, 64>(a, pi.nblocks, s);
+        case 0: return launch_b2<64, 64, 16>(a, pi.nblocks, s);
+        case 1: return launch_b2<32, 64, 16>(a, pi.nblocks, s);
+        case 2: return launch_b2<64, 128, 16>(a, pi.nblocks, s);
+        case 3: return launch_b2<128, 64, 16>(a, pi.nblocks, s);
+        case 4: return launch_b2<64, 64, 32>(a, pi.nblocks, s);
+        case 5: return launch_b2<64, 128, 32>(a, pi.nblocks, s);
+        default: return launch_b2<128, 64, 32>(a, pi.nblocks, s);
     }
 }
 

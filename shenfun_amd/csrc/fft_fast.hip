@@ -60,18 +60,17 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
-    double2* tw = sm;          // exp(-2 pi i m / N), m < N
-    double2* ex = sm + N;      // TL lines x LS
-    double2* twH = ex + TL * LS;  // exp(-2 pi i m / H), m < H
+    // LDS: TL lines x LS exchange, then exp(-2 pi i m / H), m < H.  The
+    // post-twiddles exp(-2 pi i k / N) are read from the global table twg
+    // (L2-resident): keeping them out of LDS lets four blocks share a CU.
+    double2* ex = sm;
+    double2* twH = ex + TL * LS;
     const int tid = threadIdx.x, nth = blockDim.x;
     const int j = tid % Q, t = tid / Q;  // j fastest: contiguous row segments
     const int64_t R = (int64_t)blockIdx.x * TL + t;
     const bool valid = R < nrows;
     const int64_t i1 = valid ? R / d2 : 0, i2 = valid ? R - i1 * d2 : 0;
-    for (int i = tid; i < N; i += nth) {
-        tw[i] = twg[i];
-        if ((i & 1) == 0) twH[i >> 1] = twg[i];
-    }
+    for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
     double2 a[P];
     const double* src = in + i1 * in_s1 + i2 * in_rs;
 #pragma unroll
@@ -79,6 +78,10 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
         const int m = j + Q * r;
         a[r] = valid ? *reinterpret_cast<const double2*>(src + 2 * m) : make_double2(0.0, 0.0);
     }
+    // post-twiddles of this thread's outputs k = j + Q i, issued early
+    double2 twr[P + 1];
+#pragma unroll
+    for (int i = 0; i <= P; ++i) twr[i] = (j + Q * i <= H) ? twg[j + Q * i] : make_double2(0.0, 0.0);
     __syncthreads();
     double2* ln = ex + t * LS;
     pow2_fft<H, false>(a, j, ln, twH);
@@ -89,10 +92,13 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     __syncthreads();
     if (valid) {
         double2* dst = out + i1 * out_s1 + i2 * out_rs;
-        for (int k = j; k <= H; k += Q) {
+#pragma unroll
+        for (int i = 0; i <= P; ++i) {
+            const int k = j + Q * i;
+            if (k > H) break;
             const double2 zk = ln[k == H ? 0 : k];
             const double2 zc = cconj(ln[k == 0 ? 0 : H - k]);
-            const double2 d = cmul(tw[k], csub(zk, zc));
+            const double2 d = cmul(twr[i], csub(zk, zc));
             dst[k] = make_double2(0.5 * (zk.x + zc.x + d.y) * scale, 0.5 * (zk.y + zc.y - d.x) * scale);
         }
     }
@@ -108,18 +114,14 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
-    double2* tw = sm;
-    double2* ex = sm + N;
+    double2* ex = sm;
     double2* twH = ex + TL * LS;
     const int tid = threadIdx.x, nth = blockDim.x;
     const int j = tid % Q, t = tid / Q;
     const int64_t R = (int64_t)blockIdx.x * TL + t;
     const bool valid = R < nrows;
     const int64_t i1 = valid ? R / d2 : 0, i2 = valid ? R - i1 * d2 : 0;
-    for (int i = tid; i < N; i += nth) {
-        tw[i] = twg[i];
-        if ((i & 1) == 0) twH[i >> 1] = twg[i];
-    }
+    for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
     double2* ln = ex + t * LS;
     const double2* src = in + i1 * in_s1 + i2 * in_rs;
     for (int k = j; k <= H; k += Q) {
@@ -134,7 +136,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
         const int k = j + Q * r;
         const double2 xk = ln[k];
         const double2 xc = cconj(ln[H - k]);
-        const double2 d = cmul(cconj(tw[k]), csub(xk, xc));
+        const double2 d = cmul(cconj(twg[k]), csub(xk, xc));
         a[r] = make_double2(xk.x + xc.x - d.y, xk.y + xc.y + d.x);
     }
     __syncthreads();
@@ -218,7 +220,7 @@ static int launch_rc(const Geom& gi, const Geom& go, const void* in, void* out, 
     const double2* tw = fft_table(2 * H);
     SFA_CHECK(tw, SFA_ERR_HIP, "twiddle table allocation failed");
     const int64_t nr = gi.d1 * gi.d2;
-    const size_t lds = sizeof(double2) * ((size_t)2 * H + (size_t)TL * Pl::LS + H);
+    const size_t lds = sizeof(double2) * ((size_t)TL * Pl::LS + H);
     dim3 grid((unsigned)((nr + TL - 1) / TL)), block(TL * Pl::Q);
     if (R2C) {
         auto k = r2c_contig_kernel<H>;
@@ -291,17 +293,24 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     constexpr int E = (K + 15) / 16;
     constexpr int NTH = TL * Q;
     constexpr int ROUNDS = (2 * TL * 16) / NTH;
+    static_assert(64 % Q == 0, "a line's threads must lie in one wave");
     extern __shared__ __align__(16) unsigned char sm_raw[];
-    // LDS: twiddles, DCT quarter-wave table, stencil, then the line exchange
-    // areas.  Tables live in LDS so that no global load is hoisted across the
-    // solve (it would hold registers through it).
+    // LDS: twiddles, DCT quarter-wave table, stencil, the line exchange areas,
+    // t_0 per line.  Tables live in LDS so that no global load is hoisted
+    // across the solve (it would hold registers through it).
     C2* tw = reinterpret_cast<C2*>(sm_raw);
     C2* tqs = tw + N;
     double* k2s = reinterpret_cast<double*>(tqs + N);
     C2* ex = reinterpret_cast<C2*>(k2s + N);
+    C2* t0s = ex + TL * LS;
     C2* Wd = reinterpret_cast<C2*>(a.W);
     const int tid = threadIdx.x;
+    // mapping A (global loads/stores): lanes run over the TL lines, so each
+    // x row is one contiguous TL-line segment.  Mapping B (everything between
+    // the two FFT exchanges): a line's Q threads are adjacent lanes of one
+    // wave, so those phases synchronise per wave, not per workgroup.
     const int t = tid % TL, j = tid / TL;
+    const int tb = tid / Q, jb = tid % Q;
     const int64_t L = bid * TL + t;
     const bool valid = L < a.nl;
     for (int i = tid; i < N; i += NTH) {
@@ -317,21 +326,24 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
         v[r] = valid ? Wd[(int64_t)x * a.sa + L] : cmk<C2>(0.0, 0.0);
     }
-    __syncthreads();
-    C2* ln = ex + t * LS;
-    pow2_fft<N, false>(v, j, ln, tw);
+    __syncthreads();  // tables
+    pow2_fft_p1<N, false>(v, j, ex + t * LS, tw);
+    __syncthreads();  // exchange, and the switch to mapping B
+    C2* ln = ex + tb * LS;
+    pow2_fft_p2<N, false>(v, jb, ln);
+    wave_sync();
 #pragma unroll
     for (int h = 0; h < P / Q; ++h)
 #pragma unroll
-        for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<N>(j, h, k2)] = v[h * Q + k2];
-    __syncthreads();
+        for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<N>(jb, h, k2)] = v[h * Q + k2];
+    wave_sync();
     // 2a. DCT-II post-processing X_k = sc (Re(w (V_k + conj V_{N-k})) + i Im(w (V_k - conj V_{N-k})))
     const R sc = (R)(3.14159265358979323846 / (2.0 * N));
 #pragma unroll
     for (int h = 0; h < P / Q; ++h)
 #pragma unroll
         for (int k2 = 0; k2 < Q; ++k2) {
-            const int k = pow2_out_index<N>(j, h, k2);
+            const int k = pow2_out_index<N>(jb, h, k2);
             const C2 Pv = v[h * Q + k2];
             const C2 Qv = cconj(ln[k == 0 ? 0 : N - k]);
             const C2 w = tqs[k];
@@ -339,16 +351,17 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
             const C2 s2 = cmul(w, csub(Pv, Qv));
             v[h * Q + k2] = CT<C2>::mk(s1.x * sc, s2.y * sc);
         }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int h = 0; h < P / Q; ++h)
 #pragma unroll
-        for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<N>(j, h, k2)] = v[h * Q + k2];
-    __syncthreads();
+        for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<N>(jb, h, k2)] = v[h * Q + k2];
+    wave_sync();
     // 2b. stencil (FP64) + Helmholtz solve, 16 lanes per (line, parity) chain.
     // Group g = tid / 16 takes chains g * ROUNDS + rho: the parity alternates
     // with rho, so the row operators (functions of the row index) cannot be
-    // hoisted out of the loop and held in registers across it.
+    // hoisted out of the loop and held in registers across it.  Its lines are
+    // mapping-B lines of its own wave.
     static_assert(ROUNDS % 2 == 0, "chains per group must pair both parities");
 #pragma unroll 1
     for (int rho = 0; rho < ROUNDS; ++rho) {
@@ -383,13 +396,14 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
             X[M] = cmk<C2>(0.0, 0.0);
             X[M + 1] = cmk<C2>(0.0, 0.0);
         }
+        wave_sync();
     }
-    __syncthreads();
-    // 3a. to_ortho t_k = u_k (k < N-2) + K2_{k-2} u_{k-2}, for this thread's k = j + Q r
+    // 3a. to_ortho t_k = u_k (k < N-2) + K2_{k-2} u_{k-2}, for this thread's k = jb + Q r
     const C2 t0 = ln[0];
+    if (jb == 0) t0s[tb] = t0;
 #pragma unroll
     for (int r = 0; r < P; ++r) {
-        const int k = j + Q * r;
+        const int k = jb + Q * r;
         C2 cc = (k < N - 2) ? ln[k] : cmk<C2>(0.0, 0.0);
         if (k >= 2) {
             const R kk = (R)k2s[k - 2];
@@ -399,20 +413,23 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         }
         v[r] = cc;
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
-    for (int r = 0; r < P; ++r) ln[j + Q * r] = v[r];
-    __syncthreads();
+    for (int r = 0; r < P; ++r) ln[jb + Q * r] = v[r];
+    wave_sync();
     // 3b. DCT-III pre-processing V_k = conj(w_k) ((a_k + b_{N-k}) + i (b_k - a_{N-k}))
 #pragma unroll
     for (int r = 0; r < P; ++r) {
-        const int k = j + Q * r;
+        const int k = jb + Q * r;
         const C2 cn = (k == 0) ? cmk<C2>(0.0, 0.0) : ln[N - k];
         const C2 z = CT<C2>::mk(v[r].x + cn.y, v[r].y - cn.x);
         v[r] = cmul(cconj(tqs[k]), z);
     }
-    __syncthreads();
-    pow2_fft<N, true>(v, j, ln, tw);
+    wave_sync();
+    pow2_fft_p1<N, true>(v, jb, ln, tw);
+    __syncthreads();  // exchange, and the switch back to mapping A
+    pow2_fft_p2<N, true>(v, j, ex + t * LS);
+    const C2 t0a = t0s[t];
     // 4. store 0.5 y + t_0/2 with the inverse permutation
     if (valid) {
         const R half = (R)0.5;
@@ -423,7 +440,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
                 const int m = pow2_out_index<N>(j, h, k2);
                 const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
                 const C2 y = v[h * Q + k2];
-                Wd[(int64_t)x * a.sa + L] = CT<C2>::mk(half * y.x + half * t0.x, half * y.y + half * t0.y);
+                Wd[(int64_t)x * a.sa + L] = CT<C2>::mk(half * y.x + half * t0a.x, half * y.y + half * t0a.y);
             }
     }
 }
@@ -457,7 +474,7 @@ template <int N, typename C2>
 static int launch_xpass2(XPass2Args& a, hipStream_t s)
 {
     using Pl = Pow2<N>;
-    const size_t lds = sizeof(C2) * ((size_t)2 * N + (size_t)TL * Pl::LS) + sizeof(double) * N;
+    const size_t lds = sizeof(C2) * ((size_t)2 * N + (size_t)TL * Pl::LS + TL) + sizeof(double) * N;
     auto k = xpass2_kernel<N, C2>;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -470,7 +487,7 @@ template <int N>
 static int launch_xpass2_dual(XPass2Args& a64, XPass2Args& a32, hipStream_t s)
 {
     using Pl = Pow2<N>;
-    const size_t lds = sizeof(double2) * ((size_t)2 * N + (size_t)TL * Pl::LS) + sizeof(double) * N;
+    const size_t lds = sizeof(double2) * ((size_t)2 * N + (size_t)TL * Pl::LS + TL) + sizeof(double) * N;
     auto k = xpass2_dual_kernel<N>;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -567,18 +584,14 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
-    double2* tw = sm;
-    double2* ex = sm + N;
+    double2* ex = sm;
     double2* twH = ex + TL * LS;
     const int tid = threadIdx.x, nth = blockDim.x;
     const int j = tid % Q, t = tid / Q;
     const int64_t R = (int64_t)blockIdx.x * TL + t;
     const bool valid = R < nrows;
     const int Kl = H + 1 - Kh;
-    for (int i = tid; i < N; i += nth) {
-        tw[i] = twg[i];
-        if ((i & 1) == 0) twH[i >> 1] = twg[i];
-    }
+    for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
     double2 a[P];
     const double* src = in + (valid ? R : 0) * N;
 #pragma unroll
@@ -586,6 +599,10 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
         const int m = j + Q * r;
         a[r] = valid ? *reinterpret_cast<const double2*>(src + 2 * m) : make_double2(0.0, 0.0);
     }
+    // post-twiddles of this thread's outputs k = j + Q i, issued early
+    double2 twr[P + 1];
+#pragma unroll
+    for (int i = 0; i <= P; ++i) twr[i] = (j + Q * i <= H) ? twg[j + Q * i] : make_double2(0.0, 0.0);
     __syncthreads();
     double2* ln = ex + t * LS;
     pow2_fft<H, false>(a, j, ln, twH);
@@ -595,10 +612,13 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
         for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<H>(j, h, k2)] = a[h * Q + k2];
     __syncthreads();
     if (valid) {
-        for (int k = j; k <= H; k += Q) {
+#pragma unroll
+        for (int i = 0; i <= P; ++i) {
+            const int k = j + Q * i;
+            if (k > H) break;
             const double2 zk = ln[k == H ? 0 : k];
             const double2 zc = cconj(ln[k == 0 ? 0 : H - k]);
-            const double2 d = cmul(tw[k], csub(zk, zc));
+            const double2 d = cmul(twr[i], csub(zk, zc));
             const double2 X = make_double2(0.5 * (zk.x + zc.x + d.y) * scale, 0.5 * (zk.y + zc.y - d.x) * scale);
             if (k < Kh) o64[R * Kh + k] = X;
             else o32[R * Kl + (k - Kh)] = from_c64<float2>(X);
@@ -617,18 +637,14 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
-    double2* tw = sm;
-    double2* ex = sm + N;
+    double2* ex = sm;
     double2* twH = ex + TL * LS;
     const int tid = threadIdx.x, nth = blockDim.x;
     const int j = tid % Q, t = tid / Q;
     const int64_t R = (int64_t)blockIdx.x * TL + t;
     const bool valid = R < nrows;
     const int Kl = H + 1 - Kh;
-    for (int i = tid; i < N; i += nth) {
-        tw[i] = twg[i];
-        if ((i & 1) == 0) twH[i >> 1] = twg[i];
-    }
+    for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
     double2* ln = ex + t * LS;
     for (int k = j; k <= H; k += Q) {
         double2 v = make_double2(0.0, 0.0);
@@ -643,7 +659,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
         const int k = j + Q * r;
         const double2 xk = ln[k];
         const double2 xc = cconj(ln[H - k]);
-        const double2 d = cmul(cconj(tw[k]), csub(xk, xc));
+        const double2 d = cmul(cconj(twg[k]), csub(xk, xc));
         a[r] = make_double2(xk.x + xc.x - d.y, xk.y + xc.y + d.x);
     }
     __syncthreads();
@@ -668,7 +684,7 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
     using Pl = Pow2<H>;
     const double2* tw = fft_table(2 * H);
     SFA_CHECK(tw, SFA_ERR_HIP, "twiddle table allocation failed");
-    const size_t lds = sizeof(double2) * ((size_t)2 * H + (size_t)TL * Pl::LS + H);
+    const size_t lds = sizeof(double2) * ((size_t)TL * Pl::LS + H);
     dim3 grid((unsigned)((nrows + TL - 1) / TL)), block(TL * Pl::Q);
     if (SPLIT) {
         auto k = r2c_split_kernel<H>;

@@ -77,11 +77,11 @@ __device__ inline void dft32(C2* a)
     }
 }
 
-// One line FFT.  ex: this line's LDS exchange area (Pow2<N>::LS complex);
-// tw: LDS table exp(-2 pi i m/N), m < N.  Contains two __syncthreads(): every
-// thread of the block must call it the same number of times.
+// First half of a line FFT: DFT-P over r in registers, twiddle w_N^(j k1),
+// write to this line's LDS exchange area ex (Pow2<N>::LS complex).  tw: LDS
+// table exp(-2 pi i m/N), m < N.
 template <int N, bool INV, typename C2>
-__device__ inline void pow2_fft(C2 (&a)[Pow2<N>::P], int j, C2* ex, const C2* tw)
+__device__ inline void pow2_fft_p1(C2 (&a)[Pow2<N>::P], int j, C2* ex, const C2* tw)
 {
     constexpr int P = Pow2<N>::P, Q = Pow2<N>::Q;
     dft<P, INV>(a);
@@ -93,7 +93,16 @@ __device__ inline void pow2_fft(C2 (&a)[Pow2<N>::P], int j, C2* ex, const C2* tw
     }
 #pragma unroll
     for (int k1 = 0; k1 < P; ++k1) ex[k1 * (Q + 1) + j] = a[k1];
-    __syncthreads();
+}
+
+// Second half: after the exchange is visible, thread j reads the DFT-Q inputs
+// of k1 = h Q + j from ex and transforms them.  The thread j of this half may
+// be a different thread of the block than the one of the first half (the
+// exchange doubles as a change of thread mapping).
+template <int N, bool INV, typename C2>
+__device__ inline void pow2_fft_p2(C2 (&a)[Pow2<N>::P], int j, const C2* ex)
+{
+    constexpr int P = Pow2<N>::P, Q = Pow2<N>::Q;
 #pragma unroll
     for (int h = 0; h < P / Q; ++h) {
         C2 b[Q];
@@ -104,7 +113,27 @@ __device__ inline void pow2_fft(C2 (&a)[Pow2<N>::P], int j, C2* ex, const C2* tw
 #pragma unroll
         for (int k2 = 0; k2 < Q; ++k2) a[h * Q + k2] = b[k2];
     }
+}
+
+// One line FFT.  ex: this line's LDS exchange area (Pow2<N>::LS complex);
+// tw: LDS table exp(-2 pi i m/N), m < N.  Contains two __syncthreads(): every
+// thread of the block must call it the same number of times.
+template <int N, bool INV, typename C2>
+__device__ inline void pow2_fft(C2 (&a)[Pow2<N>::P], int j, C2* ex, const C2* tw)
+{
+    pow2_fft_p1<N, INV>(a, j, ex, tw);
     __syncthreads();
+    pow2_fft_p2<N, INV>(a, j, ex);
+    __syncthreads();
+}
+
+// Order this wave's LDS accesses (lanes of one wave exchanging through LDS
+// need no workgroup barrier).
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // index of a[h*Q + k2] after pow2_fft for thread j

@@ -375,6 +375,18 @@ int xpass_launch(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann,
     return SFA_OK;
 }
 
+// y C2C of both sub-arrays of the mixed workspace (complex128 planes, then
+// complex64 planes; running them on two streams measured slower).
+static int y_pass_mixed(const Geom& g64, const Geom& g32, double2* W64, float2* W32, int sign,
+                        double scale, hipStream_t s)
+{
+    int rc = 0;
+    if (!fast_c2c(g64, g64, W64, W64, sign, scale, s, &rc) || rc ||
+        !fast_c2c_f32(g32, g32, W32, W32, sign, scale, s, &rc) || rc)
+        return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Ny=%lld unsupported", (long long)g64.n);
+    return SFA_OK;
+}
+
 // Mixed-precision pipeline: kz planes [0, Kh) stay complex128 with FP64
 // transforms, planes [Kh, Nz/2] are stored and transformed as complex64; the
 // first (R2C of the FP64 f) and last (C2R to the FP64 u) transforms and the
@@ -413,17 +425,13 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz=%lld unsupported", (long long)Nz);
     if (rc) return rc;
     mark(1);
-    if (!fast_c2c(g64, g64, W64, W64, -1, 1.0 / (double)Ny, s, &rc) || rc ||
-        !fast_c2c_f32(g32, g32, W32, W32, -1, 1.0 / (double)Ny, s, &rc) || rc)
-        return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Ny=%lld unsupported", (long long)Ny);
+    if ((rc = y_pass_mixed(g64, g32, W64, W32, -1, 1.0 / (double)Ny, s))) return rc;
     mark(2);
     if (!fast_xpass_mixed(W64, W32, (int)Nx, (int)Ny, (int)Nzh, Kh, alpha, neumann, K2, pv, s, &rc) ||
         rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nx=%lld unsupported", (long long)Nx);
     mark(3);
-    if (!fast_c2c(g64, g64, W64, W64, +1, 1.0, s, &rc) || rc ||
-        !fast_c2c_f32(g32, g32, W32, W32, +1, 1.0, s, &rc) || rc)
-        return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Ny unsupported");
+    if ((rc = y_pass_mixed(g64, g32, W64, W32, +1, 1.0, s))) return rc;
     mark(4);
     if (!split_rc(false, (int)(Nz / 2), u, W64, W32, rows, Kh, 1.0, s, &rc))
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz unsupported");
