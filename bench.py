@@ -193,6 +193,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 4096-problem batched 2D Legendre x Fourier leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--only", choices=("batched", "served"), default=None,
+                    help="run one leg alone and print its JSON (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -217,6 +219,13 @@ def main():
 
     import shenfun_amd as sfa
     from shenfun_amd.workloads import helmholtz3d_rhs
+
+    if args.only == "batched":
+        print(json.dumps({"batched2d": batched_leg(sfa, torch)}), flush=True)
+        return
+    if args.only == "served":
+        print(json.dumps({"served": serving_leg(torch)}), flush=True)
+        return
 
     N = args.size
     Nx = Ny = Nz = N
