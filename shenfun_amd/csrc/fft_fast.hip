@@ -49,6 +49,16 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q)
 }
 
 // ----------------------------------------------------------------- R2C contiguous
+// Row exchange areas of the z kernels: with Q < 16 lanes per row, a stride of
+// Q (mod 16) complex elements puts consecutive rows of a wave in
+// complementary LDS banks (the plain P(Q+1)+1 stride made them collide).
+template <int H>
+constexpr int zrow_stride()
+{
+    return Pow2<H>::Q >= 16 ? Pow2<H>::LS
+                            : Pow2<H>::LS + ((Pow2<H>::Q - Pow2<H>::LS % 16) + 16) % 16;
+}
+
 // rows of 2H reals (stride in_rs), output rows of H+1 complex (stride out_rs).
 template <int H>
 __global__ void __launch_bounds__(TL * Pow2<H>::Q)
@@ -57,7 +67,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
                       double scale, const double2* __restrict__ twg /* 2H entries */)
 {
     using Pl = Pow2<H>;
-    constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
+    constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
     // LDS: TL lines x LS exchange, then exp(-2 pi i m / H), m < H.  The
@@ -111,7 +121,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
                       double scale, const double2* __restrict__ twg /* 2H entries */)
 {
     using Pl = Pow2<H>;
-    constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
+    constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
     double2* ex = sm;
@@ -124,10 +134,21 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
     double2* ln = ex + t * LS;
     const double2* src = in + i1 * in_s1 + i2 * in_rs;
-    for (int k = j; k <= H; k += Q) {
-        double2 v = valid ? src[k] : make_double2(0.0, 0.0);
-        if (k == 0 || k == H) v.y = 0.0;
-        ln[k] = v;
+    // all of this thread's loads in flight before the LDS stores
+    double2 vin[P + 1];
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        const int k = j + Q * i;
+        vin[i] = (valid && k <= H) ? src[k] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        const int k = j + Q * i;
+        if (k <= H) {
+            double2 v = vin[i];
+            if (k == 0 || k == H) v.y = 0.0;
+            ln[k] = v;
+        }
     }
     __syncthreads();
     double2 a[P];
@@ -220,7 +241,7 @@ static int launch_rc(const Geom& gi, const Geom& go, const void* in, void* out, 
     const double2* tw = fft_table(2 * H);
     SFA_CHECK(tw, SFA_ERR_HIP, "twiddle table allocation failed");
     const int64_t nr = gi.d1 * gi.d2;
-    const size_t lds = sizeof(double2) * ((size_t)TL * Pl::LS + H);
+    const size_t lds = sizeof(double2) * ((size_t)TL * zrow_stride<H>() + H);
     dim3 grid((unsigned)((nr + TL - 1) / TL)), block(TL * Pl::Q);
     if (R2C) {
         auto k = r2c_contig_kernel<H>;
@@ -265,6 +286,20 @@ int fast_rc(bool r2c, const Geom& gi, const Geom& go, const void* in, void* out,
 
 namespace sfa {
 
+// x-pass LDS line layout: a line's natural-index element k lives at
+// k + k / PADW.  The solve's 16 lanes read rows 2E apart; without the pad
+// element every 2E rows they would all hit the same LDS banks.
+template <int N>
+struct XLay {
+    static constexpr int E = ((N - 2 + 1) / 2 + 15) / 16;  // rows per lane of a 16-lane chain
+    static constexpr int PADW = 2 * E;
+    static constexpr int NP = N + N / PADW;                // padded line length
+    static constexpr int LA = (Pow2<N>::LS > NP + 1) ? Pow2<N>::LS : NP + 1;  // line area stride
+    __host__ __device__ static constexpr int pd(int k) { return k + k / PADW; }
+    // dynamic LDS bytes for storage type C2
+    __host__ __device__ static constexpr size_t lds(size_t c2) { return c2 * (2 * N + TL * LA + TL) + 8 * NP; }
+};
+
 struct XPass2Args {
     void* W;         // lines of C2 values; element (x, L) at W[x * sa + L]
     int64_t sa;      // x stride (= Ny * Kp)
@@ -290,7 +325,10 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
     constexpr int M = N - 2;
     constexpr int K = (M + 1) / 2;  // rows of the even chain
-    constexpr int E = (K + 15) / 16;
+    constexpr int E = XLay<N>::E;
+    constexpr int PADW = XLay<N>::PADW, LA = XLay<N>::LA;
+    static_assert(E == (K + 15) / 16, "layout rows per lane");
+    auto pd = [](int k) { return XLay<N>::pd(k); };
     constexpr int NTH = TL * Q;
     constexpr int ROUNDS = (2 * TL * 16) / NTH;
     static_assert(64 % Q == 0, "a line's threads must lie in one wave");
@@ -301,8 +339,8 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     C2* tw = reinterpret_cast<C2*>(sm_raw);
     C2* tqs = tw + N;
     double* k2s = reinterpret_cast<double*>(tqs + N);
-    C2* ex = reinterpret_cast<C2*>(k2s + N);
-    C2* t0s = ex + TL * LS;
+    C2* ex = reinterpret_cast<C2*>(k2s + XLay<N>::NP);
+    C2* t0s = ex + TL * LA;
     C2* Wd = reinterpret_cast<C2*>(a.W);
     const int tid = threadIdx.x;
     // mapping A (global loads/stores): lanes run over the TL lines, so each
@@ -316,7 +354,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     for (int i = tid; i < N; i += NTH) {
         tw[i] = from_c64<C2>(a.tw[i]);
         tqs[i] = from_c64<C2>(a.tq[i]);
-        k2s[i] = (i < N - 2) ? a.K2[i] : 0.0;
+        k2s[pd(i)] = (i < N - 2) ? a.K2[i] : 0.0;
     }
     // 1. load with the DCT-II even/odd permutation: v[m] = x[2m] (m < N/2), x[2(N-1-m)+1]
     C2 v[P];
@@ -327,15 +365,15 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         v[r] = valid ? Wd[(int64_t)x * a.sa + L] : cmk<C2>(0.0, 0.0);
     }
     __syncthreads();  // tables
-    pow2_fft_p1<N, false>(v, j, ex + t * LS, tw);
+    pow2_fft_p1<N, false>(v, j, ex + t * LA, tw);
     __syncthreads();  // exchange, and the switch to mapping B
-    C2* ln = ex + tb * LS;
+    C2* ln = ex + tb * LA;
     pow2_fft_p2<N, false>(v, jb, ln);
     wave_sync();
 #pragma unroll
     for (int h = 0; h < P / Q; ++h)
 #pragma unroll
-        for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<N>(jb, h, k2)] = v[h * Q + k2];
+        for (int k2 = 0; k2 < Q; ++k2) ln[pd(pow2_out_index<N>(jb, h, k2))] = v[h * Q + k2];
     wave_sync();
     // 2a. DCT-II post-processing X_k = sc (Re(w (V_k + conj V_{N-k})) + i Im(w (V_k - conj V_{N-k})))
     const R sc = (R)(3.14159265358979323846 / (2.0 * N));
@@ -345,7 +383,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         for (int k2 = 0; k2 < Q; ++k2) {
             const int k = pow2_out_index<N>(jb, h, k2);
             const C2 Pv = v[h * Q + k2];
-            const C2 Qv = cconj(ln[k == 0 ? 0 : N - k]);
+            const C2 Qv = cconj(ln[pd(k == 0 ? 0 : N - k)]);
             const C2 w = tqs[k];
             const C2 s1 = cmul(w, cadd(Pv, Qv));
             const C2 s2 = cmul(w, csub(Pv, Qv));
@@ -355,7 +393,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
 #pragma unroll
     for (int h = 0; h < P / Q; ++h)
 #pragma unroll
-        for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<N>(jb, h, k2)] = v[h * Q + k2];
+        for (int k2 = 0; k2 < Q; ++k2) ln[pd(pow2_out_index<N>(jb, h, k2))] = v[h * Q + k2];
     wave_sync();
     // 2b. stencil (FP64) + Helmholtz solve, 16 lanes per (line, parity) chain.
     // Group g = tid / 16 takes chains g * ROUNDS + rho: the parity alternates
@@ -367,7 +405,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     for (int rho = 0; rho < ROUNDS; ++rho) {
         const int c = (tid / 16) * ROUNDS + rho;
         const int ct = c >> 1, p = c & 1, lane = tid & 15;
-        C2* X = ex + ct * LS;
+        C2* X = ex + ct * LA;
         const int64_t Lc = bid * TL + ct;
         int u = 0;
         if (Lc < a.nl) {
@@ -381,33 +419,33 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         for (int e = 0; e < E; ++e) {
             const int i = 2 * (lane * E + e) + p;
             if (i < M) {
-                const double2 x0 = to_c64(X[i]), x2 = to_c64(X[i + 2]);
-                const double kk = k2s[i];
+                const double2 x0 = to_c64(X[pd(i)]), x2 = to_c64(X[pd(i + 2)]);
+                const double kk = k2s[pd(i)];
                 f[e] = make_double2(x0.x + kk * x2.x, x0.y + kk * x2.y);
             } else {
                 f[e] = make_double2(0.0, 0.0);
             }
         }
-        helm_chain_solve_bs<E, 16>(f, X, p, M, be, a.neumann != 0, b4, lane);
+        helm_chain_solve_bs<E, 16, C2, PADW>(f, X, p, M, be, a.neumann != 0, b4, lane);
         // boundary slots M, M+1 (u carries no value there).  The odd chain
         // runs after the even one and its stencil reads X[M+1], so it clears
         // them once its own reads (same group, earlier in program order) are done.
         if (p == 1 && lane == 0) {
-            X[M] = cmk<C2>(0.0, 0.0);
-            X[M + 1] = cmk<C2>(0.0, 0.0);
+            X[pd(M)] = cmk<C2>(0.0, 0.0);
+            X[pd(M + 1)] = cmk<C2>(0.0, 0.0);
         }
         wave_sync();
     }
     // 3a. to_ortho t_k = u_k (k < N-2) + K2_{k-2} u_{k-2}, for this thread's k = jb + Q r
-    const C2 t0 = ln[0];
+    const C2 t0 = ln[pd(0)];
     if (jb == 0) t0s[tb] = t0;
 #pragma unroll
     for (int r = 0; r < P; ++r) {
         const int k = jb + Q * r;
-        C2 cc = (k < N - 2) ? ln[k] : cmk<C2>(0.0, 0.0);
+        C2 cc = (k < N - 2) ? ln[pd(k)] : cmk<C2>(0.0, 0.0);
         if (k >= 2) {
-            const R kk = (R)k2s[k - 2];
-            const C2 uu = ln[k - 2];
+            const R kk = (R)k2s[pd(k - 2)];
+            const C2 uu = ln[pd(k - 2)];
             cc.x += kk * uu.x;
             cc.y += kk * uu.y;
         }
@@ -415,20 +453,20 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     }
     wave_sync();
 #pragma unroll
-    for (int r = 0; r < P; ++r) ln[jb + Q * r] = v[r];
+    for (int r = 0; r < P; ++r) ln[pd(jb + Q * r)] = v[r];
     wave_sync();
     // 3b. DCT-III pre-processing V_k = conj(w_k) ((a_k + b_{N-k}) + i (b_k - a_{N-k}))
 #pragma unroll
     for (int r = 0; r < P; ++r) {
         const int k = jb + Q * r;
-        const C2 cn = (k == 0) ? cmk<C2>(0.0, 0.0) : ln[N - k];
+        const C2 cn = (k == 0) ? cmk<C2>(0.0, 0.0) : ln[pd(N - k)];
         const C2 z = CT<C2>::mk(v[r].x + cn.y, v[r].y - cn.x);
         v[r] = cmul(cconj(tqs[k]), z);
     }
     wave_sync();
     pow2_fft_p1<N, true>(v, jb, ln, tw);
     __syncthreads();  // exchange, and the switch back to mapping A
-    pow2_fft_p2<N, true>(v, j, ex + t * LS);
+    pow2_fft_p2<N, true>(v, j, ex + t * LA);
     const C2 t0a = t0s[t];
     // 4. store 0.5 y + t_0/2 with the inverse permutation
     if (valid) {
@@ -474,7 +512,7 @@ template <int N, typename C2>
 static int launch_xpass2(XPass2Args& a, hipStream_t s)
 {
     using Pl = Pow2<N>;
-    const size_t lds = sizeof(C2) * ((size_t)2 * N + (size_t)TL * Pl::LS + TL) + sizeof(double) * N;
+    const size_t lds = XLay<N>::lds(sizeof(C2));
     auto k = xpass2_kernel<N, C2>;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -487,7 +525,7 @@ template <int N>
 static int launch_xpass2_dual(XPass2Args& a64, XPass2Args& a32, hipStream_t s)
 {
     using Pl = Pow2<N>;
-    const size_t lds = sizeof(double2) * ((size_t)2 * N + (size_t)TL * Pl::LS + TL) + sizeof(double) * N;
+    const size_t lds = XLay<N>::lds(sizeof(double2));
     auto k = xpass2_dual_kernel<N>;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -581,7 +619,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
                      const double2* __restrict__ twg)
 {
     using Pl = Pow2<H>;
-    constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
+    constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
     double2* ex = sm;
@@ -634,7 +672,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
                      const double2* __restrict__ twg)
 {
     using Pl = Pow2<H>;
-    constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
+    constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
     double2* ex = sm;
@@ -646,11 +684,22 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     const int Kl = H + 1 - Kh;
     for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
     double2* ln = ex + t * LS;
-    for (int k = j; k <= H; k += Q) {
+    double2 vin[P + 1];
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        const int k = j + Q * i;
         double2 v = make_double2(0.0, 0.0);
-        if (valid) v = (k < Kh) ? i64[R * Kh + k] : to_c64(i32[R * Kl + (k - Kh)]);
-        if (k == 0 || k == H) v.y = 0.0;
-        ln[k] = v;
+        if (valid && k <= H) v = (k < Kh) ? i64[R * Kh + k] : to_c64(i32[R * Kl + (k - Kh)]);
+        vin[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        const int k = j + Q * i;
+        if (k <= H) {
+            double2 v = vin[i];
+            if (k == 0 || k == H) v.y = 0.0;
+            ln[k] = v;
+        }
     }
     __syncthreads();
     double2 a[P];
@@ -684,7 +733,7 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
     using Pl = Pow2<H>;
     const double2* tw = fft_table(2 * H);
     SFA_CHECK(tw, SFA_ERR_HIP, "twiddle table allocation failed");
-    const size_t lds = sizeof(double2) * ((size_t)TL * Pl::LS + H);
+    const size_t lds = sizeof(double2) * ((size_t)TL * zrow_stride<H>() + H);
     dim3 grid((unsigned)((nrows + TL - 1) / TL)), block(TL * Pl::Q);
     if (SPLIT) {
         auto k = r2c_split_kernel<H>;

@@ -388,9 +388,12 @@ static __global__ void helm_bstate_kernel(int M, int E, int neumann,
 
 // Solve one chain from the segment restart state bs = (d1/d0, d2/d0, 1/d0) of
 // the row before this lane's segment: the lane regenerates its E rows of LU
-// factors with the exact row recurrence, then the affine scans of
-// helm_chain_solve_fac.  X: the line, any storage type C2.
-template <int E, int W, typename C2>
+// factors with the exact row recurrence, then forward elimination and back
+// substitution as affine scans over the W lanes.  X: the line, any storage
+// type C2, holding row i at i + i / PADW (one pad element every PADW rows;
+// PADW = 0: none), which keeps the lanes' rows (stride 2E) in distinct LDS
+// banks.
+template <int E, int W, typename C2, int PADW = 0>
 __device__ inline void helm_chain_solve_bs(const double2 (&f)[E], C2* X, int p, int M, double be,
                                            bool nm, double4 bs, int lane)
 {
@@ -484,7 +487,7 @@ __device__ inline void helm_chain_solve_bs(const double2 (&f)[E], C2* X, int p, 
                     out = make_double2(out.x * s, out.y * s);
                 }
             }
-            X[i] = from_c64<C2>(out);
+            X[PADW ? i + i / PADW : i] = from_c64<C2>(out);
         }
     }
 }
