@@ -7,7 +7,10 @@ device memory free up, so work of several requests overlaps on the device:
 * 1D Chebyshev (SD/SN, N) requests of one class (N, bc, alpha) are stacked
   into one (B, N) array: one scalar-product, one Helmholtz-solve and one
   backward launch for the batch (the solve broadcasts the class's LU factors
-  with a zero line stride).  Reference path per request:
+  with a zero line stride).  Lengths with a prime factor above 31 (whose
+  mixed-radix DCT degenerates to an O(N p) DFT) use the cached dense
+  transform matrices instead (the matrix kind of spectralbase.py:484-491,
+  550-558).  Reference path per request:
   chebyshev.bases scalar_product (chebyshev/bases.py:340-353) ->
   chebyshev.la.Helmholtz (chebyshev/la.py:288-317) -> backward (:315-338).
 * 2D Legendre x Fourier requests of any sizes go together into one launch of
@@ -96,8 +99,17 @@ def _bcc(bc):
     return 0 if bc in ("D", 0) else 1
 
 
+def _largest_prime_factor(n):
+    p, f = 2, 1
+    while p * p <= n:
+        while n % p == 0:
+            f, n = p, n // p
+        p += 1
+    return max(f, n)
+
+
 class Server:
-    def __init__(self, slots=4, mem_budget=8 << 30, cache_budget=4 << 30, max_batch_1d=256,
+    def __init__(self, slots=4, mem_budget=8 << 30, cache_budget=16 << 30, max_batch_1d=256,
                  max_batch_2d=1024, device="cuda"):
         self.device = torch.device(device)
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(int(slots))]
@@ -123,8 +135,12 @@ class Server:
             from .la import Helmholtz
             H = Helmholtz(stiffness_matrix(V), mass_matrix(V), -1.0, float(alpha), device=self.device)
             K2 = torch.from_numpy(np.ascontiguousarray(V.stencil_k2())).to(self.device)
+            dense = None
             nb = 8 * (4 * N + N)
-            return (V, H, K2), nb
+            if _largest_prime_factor(N) > 31:
+                dense = (V._analysis_matrix(), V._synthesis_matrix())
+                nb += 16 * N * N
+            return (V, H, K2, dense), nb
         return self.cache.get(("cheb1d",) + key, build)
 
     def _leg_matrices(self, Nx, bc):
@@ -158,7 +174,7 @@ class Server:
 
     def _launch_cheb1d(self, reqs, stream):
         N, bc, alpha = reqs[0].key
-        V, H, K2 = self._cheb1d_factors(reqs[0].key)
+        V, H, K2, dense = self._cheb1d_factors(reqs[0].key)
         B = len(reqs)
         shp = _lib.arr([B, N])
         st = _lib.arr([N, 1])
@@ -168,13 +184,21 @@ class Server:
             fh = torch.empty_like(F)
             U = torch.empty_like(F)
             sp = _lib.stream_ptr(stream)
-            _lib.call("sfa_cheb_scalar_product", _lib.ptr(F), _lib.ptr(fh), _lib.F64, 2, shp, st, st, 1,
-                      _lib.ptr(K2), sp)
+            if dense is None:
+                _lib.call("sfa_cheb_scalar_product", _lib.ptr(F), _lib.ptr(fh), _lib.F64, 2, shp, st, st,
+                          1, _lib.ptr(K2), sp)
+            else:
+                _lib.call("sfa_line_matmul", _lib.ptr(F), _lib.ptr(fh), _lib.F64, 2, shp, st, st, 1, N,
+                          _lib.ptr(dense[0]), sp)
             _lib.call("sfa_helmholtz_solve", _lib.ptr(fh), _lib.ptr(fh), _lib.F64, 2, shp, st, st, 1,
                       int(H.neumann), _lib.ptr(H.u0), _lib.ptr(H.u1), _lib.ptr(H.u2), _lib.ptr(H.L), fst,
                       sp)
-            _lib.call("sfa_cheb_backward", _lib.ptr(fh), _lib.ptr(U), _lib.F64, 2, shp, st, st, 1,
-                      _lib.ptr(K2), sp)
+            if dense is None:
+                _lib.call("sfa_cheb_backward", _lib.ptr(fh), _lib.ptr(U), _lib.F64, 2, shp, st, st, 1,
+                          _lib.ptr(K2), sp)
+            else:
+                _lib.call("sfa_line_matmul", _lib.ptr(fh), _lib.ptr(U), _lib.F64, 2, shp, st, st, 1, N,
+                          _lib.ptr(dense[1]), sp)
         for i, r in enumerate(reqs):
             r.u = U[i]
         return (F, fh, U), [("cheb1d",) + reqs[0].key]
