@@ -279,6 +279,26 @@ int sfa_batch2d_plan_ext(int nprob, const int* Nx, const int* Ny, const double* 
                          const int64_t* df_off, int64_t mats_elems, int nblocks, void* plan,
                          int64_t plan_bytes, void* stream);
 
+/* Tile-parallel mode of the same batched solve, for batches with fewer
+ * problems than CUs (one large problem would otherwise run on one workgroup):
+ * each of the four contractions runs as one launch over all (problem, 32 x 32
+ * output tile) tasks and the per-ky solve as one launch with a workgroup per
+ * problem.  Same problem table and external matrices as sfa_batch2d_plan_ext
+ * (tensorproductspace.py:146-190, la.py:1390, tensorproductspace.py:46-96 per
+ * problem); every output is bitwise equal to sfa_batch2d_solve's.  The plan
+ * takes sfa_batch2d_wide_plan_bytes() bytes, the workspace
+ * sfa_batch2d_wide_workspace() bytes.  sfa_batch2d_wide_plan synchronises
+ * `stream`; release the plan with sfa_batch2d_release. */
+int64_t sfa_batch2d_wide_plan_bytes(int nprob, const int* Nx, const int* Ny);
+int64_t sfa_batch2d_wide_workspace(int nprob, const int* Nx, const int* Ny);
+int sfa_batch2d_wide_plan(int nprob, const int* Nx, const int* Ny, const double* alpha, const int* bc,
+                          const int64_t* f_off, const int64_t* u_off, int64_t f_elems,
+                          int64_t u_elems, const double* mats, const int64_t* phi_off,
+                          const int64_t* df_off, int64_t mats_elems, void* plan, int64_t plan_bytes,
+                          void* stream);
+int sfa_batch2d_wide_solve(const double* F, double* U, const void* plan, void* work,
+                           int64_t work_bytes, void* stream);
+
 /* Self-check of the FP64 MFMA 16x16x4 lane layout the batched GEMMs assume:
  * writes the 64 x 4 accumulator registers of one MFMA of A = i + 100 k and
  * B = j + 1000 k into out (256 doubles).  No reference counterpart. */

@@ -47,6 +47,8 @@ _SIGS = {
     "sfa_batch2d_leg_matrices": "iivv",
     "sfa_batch2d_dft_matrices": "ivv",
     "sfa_batch2d_plan_ext": "ivvvvvvllvvvlivlv",
+    "sfa_batch2d_wide_plan": "ivvvvvvllvvvlvlv",
+    "sfa_batch2d_wide_solve": "vvvvlv",
     "sfa_mfma_f64_layout_probe": "vv",
 }
 
@@ -86,6 +88,9 @@ def lib():
             getattr(L, name).argtypes = [_int]
         L.sfa_batch2d_plan_ext_bytes.restype = _i64
         L.sfa_batch2d_plan_ext_bytes.argtypes = [_int, _int]
+        for name in ("sfa_batch2d_wide_plan_bytes", "sfa_batch2d_wide_workspace"):
+            getattr(L, name).restype = _i64
+            getattr(L, name).argtypes = [_int, _c_void_p, _c_void_p]
         _lib = L
     return _lib
 
@@ -95,7 +100,8 @@ def exported_symbols():
                                  "sfa_helmholtz3d_plan_bytes", "sfa_batch2d_plan_bytes",
                                  "sfa_batch2d_workspace", "sfa_batch2d_variants",
                                  "sfa_batch2d_leg_elems", "sfa_batch2d_dft_elems",
-                                 "sfa_batch2d_plan_ext_bytes"])
+                                 "sfa_batch2d_plan_ext_bytes", "sfa_batch2d_wide_plan_bytes",
+                                 "sfa_batch2d_wide_workspace"])
 
 
 def arr(vals):

@@ -8,6 +8,10 @@ tensorproductspace.py:46-96), all problems in one launch of
 sfa_batch2d_solve.  Inputs and outputs are packed row-major in one float64
 tensor each (``pack``/``unpack``).  ``autotune`` times the kernel's block-tile
 variants and workgroup counts on the actual batch and keeps the fastest.
+
+``mode="wide"`` runs the tile-parallel form (sfa_batch2d_wide_solve: each
+contraction one launch over all (problem, tile) tasks) for batches with fewer
+problems than CUs; its outputs are bitwise equal to the persistent form's.
 """
 import json
 import os
@@ -38,7 +42,11 @@ def tile_variants():
 
 
 class BatchedHelmholtz2D:
-    def __init__(self, Nx, Ny, alpha, bc="D", nblocks=None, variant=0, device="cuda"):
+    def __init__(self, Nx, Ny, alpha, bc="D", nblocks=None, variant=0, device="cuda",
+                 mode="persistent"):
+        if mode not in ("persistent", "wide"):
+            raise ValueError(f"mode must be 'persistent' or 'wide', got {mode!r}")
+        self.mode = mode
         self.Nx = np.ascontiguousarray(np.asarray(Nx, dtype=np.int32).reshape(-1))
         self.Ny = np.ascontiguousarray(np.asarray(Ny, dtype=np.int32).reshape(-1))
         n = self.Nx.size
@@ -59,11 +67,16 @@ class BatchedHelmholtz2D:
         self.cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         self.variant = int(variant)
         self.plan = None
-        self.build(nblocks if nblocks is not None else self.cus)
+        if mode == "wide":
+            self._build_wide()
+        else:
+            self.build(nblocks if nblocks is not None else self.cus)
 
     # ------------------------------------------------------------ plan
     def build(self, nblocks):
         """(Re)build the device plan for a workgroup count (schedule + matrices)."""
+        if self.mode != "persistent":
+            raise ValueError("build(nblocks) applies to the persistent mode")
         L = _lib.lib()
         ptr = lambda a: _lib.ctypes.c_void_p(a.ctypes.data)
         nb = int(nblocks)
@@ -79,6 +92,37 @@ class BatchedHelmholtz2D:
         wbytes = L.sfa_batch2d_workspace(self.nprob, ptr(self.Nx), ptr(self.Ny), nb)
         self.work = torch.empty(int(wbytes), dtype=torch.uint8, device=self.device)
         self.nblocks = nb
+
+    def _build_wide(self):
+        """Matrix arena (one Phi/P block per distinct (Nx, bc), one Df/Di block
+        per distinct Ny) and the tile-parallel plan."""
+        L = _lib.lib()
+        ptr = lambda a: _lib.ctypes.c_void_p(a.ctypes.data)
+        if L.sfa_batch2d_plan_bytes(self.nprob, ptr(self.Nx), ptr(self.Ny), ptr(self.bc), 1) < 0:
+            raise ValueError("invalid problem sizes (even Nx in [8, 1024], even Ny in [4, 1024])")
+        leg, dft, o = {}, {}, 0
+        for nx, b in sorted(set(zip(self.Nx.tolist(), self.bc.tolist()))):
+            leg[(nx, b)] = o
+            o += -(-int(L.sfa_batch2d_leg_elems(nx)) // 32) * 32
+        for ny in sorted(set(self.Ny.tolist())):
+            dft[ny] = o
+            o += -(-int(L.sfa_batch2d_dft_elems(ny)) // 32) * 32
+        self.mats = torch.empty(o, dtype=torch.float64, device=self.device)
+        sp = _lib.stream_ptr()
+        for (nx, b), off in leg.items():
+            _lib.call("sfa_batch2d_leg_matrices", nx, b, _lib.ptr(self.mats[off:]), sp)
+        for ny, off in dft.items():
+            _lib.call("sfa_batch2d_dft_matrices", ny, _lib.ptr(self.mats[off:]), sp)
+        phi = np.array([leg[(int(a), int(b))] for a, b in zip(self.Nx, self.bc)], dtype=np.int64)
+        df = np.array([dft[int(b)] for b in self.Ny], dtype=np.int64)
+        pbytes = L.sfa_batch2d_wide_plan_bytes(self.nprob, ptr(self.Nx), ptr(self.Ny))
+        self.plan = torch.empty(int(pbytes), dtype=torch.uint8, device=self.device)
+        _lib.call("sfa_batch2d_wide_plan", self.nprob, ptr(self.Nx), ptr(self.Ny), ptr(self.alpha),
+                  ptr(self.bc), ptr(self.offsets), ptr(self.offsets), self.total, self.total,
+                  _lib.ptr(self.mats), ptr(phi), ptr(df), o, _lib.ptr(self.plan), self.plan.numel(), sp)
+        wbytes = L.sfa_batch2d_wide_workspace(self.nprob, ptr(self.Nx), ptr(self.Ny))
+        self.work = torch.empty(int(wbytes), dtype=torch.uint8, device=self.device)
+        self.nblocks = None
 
     def __del__(self):
         try:
@@ -115,16 +159,22 @@ class BatchedHelmholtz2D:
             raise ValueError("U must match F")
         if U.data_ptr() == F.data_ptr():
             raise ValueError("U must not alias F")
-        _lib.call("sfa_batch2d_solve", _lib.ptr(F), _lib.ptr(U), _lib.ptr(self.plan),
-                  _lib.ptr(self.work), self.work.numel(), self.variant, _lib.stream_ptr())
+        if self.mode == "wide":
+            _lib.call("sfa_batch2d_wide_solve", _lib.ptr(F), _lib.ptr(U), _lib.ptr(self.plan),
+                      _lib.ptr(self.work), self.work.numel(), _lib.stream_ptr())
+        else:
+            _lib.call("sfa_batch2d_solve", _lib.ptr(F), _lib.ptr(U), _lib.ptr(self.plan),
+                      _lib.ptr(self.work), self.work.numel(), self.variant, _lib.stream_ptr())
         return U
 
     # ------------------------------------------------------------ tuning
     def autotune(self, F, blocks_per_cu=(1, 2), variants=None, reps=3, cache=None):
         """Time every (tile variant, workgroups per CU) pair on this batch and
-        keep the fastest.  Returns {"best": {...}, "table": [...]}.  With
+        keep the fastest (persistent mode only).  Returns {"best": {...}, "table": [...]}.  With
         ``cache`` (a JSON path) a previous result for the same batch signature
         is reused and new results are stored."""
+        if self.mode != "persistent":
+            raise ValueError("autotune applies to the persistent mode")
         key = self.signature()
         if cache and os.path.exists(cache):
             try:

@@ -59,9 +59,9 @@ constexpr int B2_THREADS = 256;
 // Each output is one MFMA accumulation chain over K in ascending order, so
 // the result does not depend on BM, BN or BK.
 template <int BM, int BN, int BK>
-__device__ void block_gemm(const double* __restrict__ A, int64_t lda, const double* __restrict__ B,
-                           int64_t ldb, double* __restrict__ C, int64_t ldc, int m, int n, int k,
-                           double* sA, double* sB)
+__device__ void gemm_tile(const double* __restrict__ A, int64_t lda, const double* __restrict__ B,
+                          int64_t ldb, double* __restrict__ C, int64_t ldc, int m, int n, int k,
+                          int m0, int n0, double* sA, double* sB)
 {
     constexpr int SA = BM + 1;
     constexpr int TI = BM / 32, TJ = BN / 32;
@@ -70,72 +70,84 @@ __device__ void block_gemm(const double* __restrict__ A, int64_t lda, const doub
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wi = wv >> 1, wj = wv & 1;
     const int lr = lane & 15, lk = lane >> 4;
-    for (int m0 = 0; m0 < m; m0 += BM) {
-        for (int n0 = 0; n0 < n; n0 += BN) {
-            v4d acc[TI][TJ];
+    v4d acc[TI][TJ];
+#pragma unroll
+    for (int a = 0; a < TI; ++a)
+#pragma unroll
+        for (int b = 0; b < TJ; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    double ra[LA], rb[LB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < LA; ++r) {
+            const int e = tid + B2_THREADS * r;
+            const int i = e / BK, kk = e % BK;
+            const int gi = m0 + i, gk = k0 + kk;
+            ra[r] = (gi < m && gk < k) ? A[(int64_t)gi * lda + gk] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < LB; ++r) {
+            const int e = tid + B2_THREADS * r;
+            const int kk = e / BN, j = e % BN;
+            const int gk = k0 + kk, gj = n0 + j;
+            rb[r] = (gk < k && gj < n) ? B[(int64_t)gk * ldb + gj] : 0.0;
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < k; k0 += BK) {
+        __syncthreads();  // the previous step's fragment reads are done
+#pragma unroll
+        for (int r = 0; r < LA; ++r) {
+            const int e = tid + B2_THREADS * r;
+            sA[(e % BK) * SA + e / BK] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < LB; ++r) {
+            const int e = tid + B2_THREADS * r;
+            sB[(e / BN) * BN + e % BN] = rb[r];
+        }
+        __syncthreads();
+        if (k0 + BK < k) load(k0 + BK);  // prefetch overlaps the MFMAs
+#pragma unroll
+        for (int s = 0; s < BK / 4; ++s) {
+            const int kk = 4 * s + lk;
+            double af[TI], bf[TJ];
+#pragma unroll
+            for (int a = 0; a < TI; ++a) af[a] = sA[kk * SA + wi * (BM / 2) + a * 16 + lr];
+#pragma unroll
+            for (int b = 0; b < TJ; ++b) bf[b] = sB[kk * BN + wj * (BN / 2) + b * 16 + lr];
 #pragma unroll
             for (int a = 0; a < TI; ++a)
 #pragma unroll
-                for (int b = 0; b < TJ; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
-            double ra[LA], rb[LB];
-            auto load = [&](int k0) {
-#pragma unroll
-                for (int r = 0; r < LA; ++r) {
-                    const int e = tid + B2_THREADS * r;
-                    const int i = e / BK, kk = e % BK;
-                    const int gi = m0 + i, gk = k0 + kk;
-                    ra[r] = (gi < m && gk < k) ? A[(int64_t)gi * lda + gk] : 0.0;
-                }
-#pragma unroll
-                for (int r = 0; r < LB; ++r) {
-                    const int e = tid + B2_THREADS * r;
-                    const int kk = e / BN, j = e % BN;
-                    const int gk = k0 + kk, gj = n0 + j;
-                    rb[r] = (gk < k && gj < n) ? B[(int64_t)gk * ldb + gj] : 0.0;
-                }
-            };
-            load(0);
-            for (int k0 = 0; k0 < k; k0 += BK) {
-                __syncthreads();  // the previous step's fragment reads are done
-#pragma unroll
-                for (int r = 0; r < LA; ++r) {
-                    const int e = tid + B2_THREADS * r;
-                    sA[(e % BK) * SA + e / BK] = ra[r];
-                }
-#pragma unroll
-                for (int r = 0; r < LB; ++r) {
-                    const int e = tid + B2_THREADS * r;
-                    sB[(e / BN) * BN + e % BN] = rb[r];
-                }
-                __syncthreads();
-                if (k0 + BK < k) load(k0 + BK);  // prefetch overlaps the MFMAs
-#pragma unroll
-                for (int s = 0; s < BK / 4; ++s) {
-                    const int kk = 4 * s + lk;
-                    double af[TI], bf[TJ];
-#pragma unroll
-                    for (int a = 0; a < TI; ++a) af[a] = sA[kk * SA + wi * (BM / 2) + a * 16 + lr];
-#pragma unroll
-                    for (int b = 0; b < TJ; ++b) bf[b] = sB[kk * BN + wj * (BN / 2) + b * 16 + lr];
-#pragma unroll
-                    for (int a = 0; a < TI; ++a)
-#pragma unroll
-                        for (int b = 0; b < TJ; ++b) acc[a][b] = mfma16(af[a], bf[b], acc[a][b]);
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < TI; ++a)
-#pragma unroll
-                for (int b = 0; b < TJ; ++b) {
-                    const int gj = n0 + wj * (BN / 2) + b * 16 + lr;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int gi = m0 + wi * (BM / 2) + a * 16 + 4 * r + lk;
-                        if (gi < m && gj < n) C[(int64_t)gi * ldc + gj] = acc[a][b][r];
-                    }
-                }
+                for (int b = 0; b < TJ; ++b) acc[a][b] = mfma16(af[a], bf[b], acc[a][b]);
         }
     }
+#pragma unroll
+    for (int a = 0; a < TI; ++a)
+#pragma unroll
+        for (int b = 0; b < TJ; ++b) {
+            const int gj = n0 + wj * (BN / 2) + b * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wi * (BM / 2) + a * 16 + 4 * r + lk;
+                if (gi < m && gj < n) C[(int64_t)gi * ldc + gj] = acc[a][b][r];
+            }
+        }
+}
+
+// C (m x n, ldc) = A (m x k, lda) B (k x n, ldb), all row-major FP64, by the
+// whole workgroup (4 waves in a 2 x 2 grid over BM x BN block tiles, K in
+// steps of BK).  sA: BK x (BM + 1) doubles, sB: BK x BN doubles of LDS.
+// Out-of-range operand elements load as zero; stores are bounds-checked.
+// Each output is one MFMA accumulation chain over K in ascending steps of BK
+// (four K = 4 MFMAs per step), so the result does not depend on BM, BN, on
+// which workgroup computes the tile, or on the order of the tiles.
+template <int BM, int BN, int BK>
+__device__ void block_gemm(const double* __restrict__ A, int64_t lda, const double* __restrict__ B,
+                           int64_t ldb, double* __restrict__ C, int64_t ldc, int m, int n, int k,
+                           double* sA, double* sB)
+{
+    for (int m0 = 0; m0 < m; m0 += BM)
+        for (int n0 = 0; n0 < n; n0 += BN) gemm_tile<BM, BN, BK>(A, lda, B, ldb, C, ldc, m, n, k, m0, n0, sA, sB);
     __syncthreads();  // C complete and LDS free before the caller's next stage
 }
 
@@ -252,6 +264,59 @@ __global__ void __launch_bounds__(B2_THREADS) batch2d_kernel(B2Args a)
         // u = U1 Di (Nx x Ny)
         block_gemm<BM, BN, BK>(W1, L, Di, Ny, u, Ny, Nx, Ny, L, sA, sB);
     }
+}
+
+// ----------------------------------------------------------------- tile-parallel mode
+// For batches with fewer problems than CUs: each GEMM stage runs as one
+// launch over all (problem, output tile) tasks, the solve as one launch with a
+// workgroup per problem, so a single large problem spreads over the device.
+// Per-problem intermediates live at work + woff[p] (W0: (Nx-2) x (Ny+2),
+// then W1: Nx x (Ny+2)); T1 is held in u as in the persistent kernel.
+struct B2Task {
+    int32_t prob, m0, n0, pad;
+};
+
+struct B2WideArgs {
+    const double* F;
+    double* U;
+    const B2Prob* probs;
+    const B2Task* tasks;   // this stage's tasks
+    const int64_t* woff;   // per problem, in doubles
+    const double* mats;
+    double* work;
+};
+
+constexpr int B2W_BM = 32, B2W_BN = 32, B2W_BK = 16;
+
+template <int STAGE>
+__global__ void __launch_bounds__(B2_THREADS) batch2d_tile_kernel(B2WideArgs a)
+{
+    __shared__ double sA[B2W_BK * (B2W_BM + 1)];
+    __shared__ double sB[B2W_BK * B2W_BN];
+    const B2Task tk = a.tasks[blockIdx.x];
+    const B2Prob pr = a.probs[tk.prob];
+    const int Nx = pr.Nx, Ny = pr.Ny, M = Nx - 2, L = Ny + 2;
+    double* u = a.U + pr.u_off;
+    double* W0 = a.work + a.woff[tk.prob];
+    double* W1 = W0 + (int64_t)M * L;
+    if (STAGE == 0)  // T1 = P f (M x Ny) in u
+        gemm_tile<B2W_BM, B2W_BN, B2W_BK>(a.mats + pr.p_off, Nx, a.F + pr.f_off, Ny, u, Ny, M, Ny, Nx,
+                                          tk.m0, tk.n0, sA, sB);
+    else if (STAGE == 1)  // T2 = T1 Df (M x L) in W0
+        gemm_tile<B2W_BM, B2W_BN, B2W_BK>(u, Ny, a.mats + pr.df_off, L, W0, L, M, L, Ny, tk.m0, tk.n0, sA, sB);
+    else if (STAGE == 2)  // U1 = Phi u_hat (Nx x L) in W1
+        gemm_tile<B2W_BM, B2W_BN, B2W_BK>(a.mats + pr.phi_off, M, W0, L, W1, L, Nx, L, M, tk.m0, tk.n0, sA, sB);
+    else  // u = U1 Di (Nx x Ny)
+        gemm_tile<B2W_BM, B2W_BN, B2W_BK>(W1, L, a.mats + pr.di_off, Ny, u, Ny, Nx, Ny, L, tk.m0, tk.n0, sA, sB);
+}
+
+__global__ void __launch_bounds__(B2_THREADS) batch2d_wide_solve_kernel(B2WideArgs a)
+{
+    const int p = blockIdx.x;
+    const B2Prob pr = a.probs[p];
+    const int Nx = pr.Nx, Ny = pr.Ny, M = Nx - 2, L = Ny + 2;
+    double* W0 = a.work + a.woff[p];
+    batch_solve(W0, L, M, Ny / 2 + 1, pr.bc, pr.alpha, W0 + (int64_t)M * L);
 }
 
 // ----------------------------------------------------------------- matrix builders
@@ -476,6 +541,14 @@ struct B2Info {
 };
 static std::mutex g_b2_mu;
 static std::map<const void*, B2Info> g_b2;
+// tile-parallel plan: problem table | tasks of the four GEMM stages | woff
+struct B2WideInfo {
+    int nprob;
+    int64_t ntask[4], off_task[4];
+    int64_t off_probs, off_woff, work_doubles;
+    const double* mats;
+};
+static std::map<const void*, B2WideInfo> g_b2w;
 
 static int64_t b2_wstride(int nprob, const int* Nx, const int* Ny)
 {
@@ -488,17 +561,13 @@ static int64_t b2_wstride(int nprob, const int* Nx, const int* Ny)
 }
 
 // Problem table and static LPT schedule into the plan; registers the plan.
-static int b2_plan_core(int nprob, const int* Nx, const int* Ny, const double* alpha, const int* bc,
-                        const int64_t* f_off, const int64_t* u_off, int64_t f_elems, int64_t u_elems,
-                        const double* mats, const int64_t* phi_off, const int64_t* df_off,
-                        int64_t mats_elems, int nblocks, void* plan, int64_t plan_bytes,
-                        hipStream_t s)
+// Validated problem table.
+static int b2_table(int nprob, const int* Nx, const int* Ny, const double* alpha, const int* bc,
+                    const int64_t* f_off, const int64_t* u_off, int64_t f_elems, int64_t u_elems,
+                    const int64_t* phi_off, const int64_t* df_off, int64_t mats_elems,
+                    std::vector<B2Prob>& probs)
 {
-    B2Layout L;
-    b2_layout(nprob, nblocks, 0, &L);
-    SFA_CHECK(plan_bytes >= L.bytes, SFA_ERR_ARG, "plan buffer too small (%lld < %lld)",
-              (long long)plan_bytes, (long long)L.bytes);
-    std::vector<B2Prob> probs(nprob);
+    probs.resize(nprob);
     for (int i = 0; i < nprob; ++i) {
         const int64_t n = (int64_t)Nx[i] * Ny[i];
         SFA_CHECK(f_off[i] >= 0 && f_off[i] + n <= f_elems, SFA_ERR_ARG,
@@ -523,6 +592,23 @@ static int b2_plan_core(int nprob, const int* Nx, const int* Ny, const double* a
         p.pad = 0;
         p.alpha = alpha[i];
     }
+    return SFA_OK;
+}
+
+static int b2_plan_core(int nprob, const int* Nx, const int* Ny, const double* alpha, const int* bc,
+                        const int64_t* f_off, const int64_t* u_off, int64_t f_elems, int64_t u_elems,
+                        const double* mats, const int64_t* phi_off, const int64_t* df_off,
+                        int64_t mats_elems, int nblocks, void* plan, int64_t plan_bytes,
+                        hipStream_t s)
+{
+    B2Layout L;
+    b2_layout(nprob, nblocks, 0, &L);
+    SFA_CHECK(plan_bytes >= L.bytes, SFA_ERR_ARG, "plan buffer too small (%lld < %lld)",
+              (long long)plan_bytes, (long long)L.bytes);
+    std::vector<B2Prob> probs;
+    int rc = b2_table(nprob, Nx, Ny, alpha, bc, f_off, u_off, f_elems, u_elems, phi_off, df_off,
+                      mats_elems, probs);
+    if (rc) return rc;
     // static LPT schedule: largest first onto the least loaded workgroup
     std::vector<int> ids(nprob);
     std::iota(ids.begin(), ids.end(), 0);
@@ -553,9 +639,37 @@ static int b2_plan_core(int nprob, const int* Nx, const int* Ny, const double* a
     SFA_HIP(hipMemcpyAsync(pb + L.off_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
     SFA_HIP(hipStreamSynchronize(s));  // host vectors are copied before they go out of scope
     std::lock_guard<std::mutex> lk(g_b2_mu);
+    g_b2w.erase(plan);  // the address may have held a tile-parallel plan
     g_b2[plan] = B2Info{nprob, nblocks, L.off_probs, L.off_blk, L.off_order, mats,
                         b2_wstride(nprob, Nx, Ny)};
     return SFA_OK;
+}
+
+
+static void b2_wide_shape(int nprob, const int* Nx, const int* Ny, int64_t ntask[4], int64_t* work)
+{
+    auto cdiv = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
+    for (int st = 0; st < 4; ++st) ntask[st] = 0;
+    *work = 0;
+    for (int i = 0; i < nprob; ++i) {
+        const int64_t M = Nx[i] - 2, L = Ny[i] + 2, X = Nx[i], Y = Ny[i];
+        ntask[0] += cdiv(M, B2W_BM) * cdiv(Y, B2W_BN);
+        ntask[1] += cdiv(M, B2W_BM) * cdiv(L, B2W_BN);
+        ntask[2] += cdiv(X, B2W_BM) * cdiv(L, B2W_BN);
+        ntask[3] += cdiv(X, B2W_BM) * cdiv(Y, B2W_BN);
+        *work += (M * L + X * L + 31) / 32 * 32;
+    }
+}
+
+static int64_t b2_wide_bytes(int nprob, const int64_t ntask[4], int64_t off_task[4], int64_t* off_woff)
+{
+    int64_t o = r256((int64_t)nprob * sizeof(B2Prob));
+    for (int st = 0; st < 4; ++st) {
+        off_task[st] = o;
+        o += r256(ntask[st] * (int64_t)sizeof(B2Task));
+    }
+    *off_woff = o;
+    return o + r256((int64_t)nprob * 8);
 }
 
 template <int BM, int BN, int BK>
@@ -726,6 +840,111 @@ extern "C" int sfa_batch2d_release(const void* plan)
 {
     std::lock_guard<std::mutex> lk(g_b2_mu);
     g_b2.erase(plan);
+    g_b2w.erase(plan);
+    return SFA_OK;
+}
+
+extern "C" int64_t sfa_batch2d_wide_plan_bytes(int nprob, const int* Nx, const int* Ny)
+{
+    if (nprob < 1 || !Nx || !Ny) return -1;
+    int64_t nt[4], ot[4], ow, work;
+    b2_wide_shape(nprob, Nx, Ny, nt, &work);
+    return b2_wide_bytes(nprob, nt, ot, &ow);
+}
+
+extern "C" int64_t sfa_batch2d_wide_workspace(int nprob, const int* Nx, const int* Ny)
+{
+    if (nprob < 1 || !Nx || !Ny) return -1;
+    int64_t nt[4], work;
+    b2_wide_shape(nprob, Nx, Ny, nt, &work);
+    return work * 8;
+}
+
+extern "C" int sfa_batch2d_wide_plan(int nprob, const int* Nx, const int* Ny, const double* alpha,
+                                     const int* bc, const int64_t* f_off, const int64_t* u_off,
+                                     int64_t f_elems, int64_t u_elems, const double* mats,
+                                     const int64_t* phi_off, const int64_t* df_off,
+                                     int64_t mats_elems, void* plan, int64_t plan_bytes, void* stream)
+{
+    int rc;
+    if ((rc = b2_check(nprob, Nx, Ny, bc, 1))) return rc;
+    SFA_CHECK(alpha && f_off && u_off && plan && mats && phi_off && df_off, SFA_ERR_ARG,
+              "NULL argument");
+    std::vector<B2Prob> probs;
+    if ((rc = b2_table(nprob, Nx, Ny, alpha, bc, f_off, u_off, f_elems, u_elems, phi_off, df_off,
+                       mats_elems, probs)))
+        return rc;
+    B2WideInfo wi;
+    wi.nprob = nprob;
+    b2_wide_shape(nprob, Nx, Ny, wi.ntask, &wi.work_doubles);
+    const int64_t bytes = b2_wide_bytes(nprob, wi.ntask, wi.off_task, &wi.off_woff);
+    SFA_CHECK(plan_bytes >= bytes, SFA_ERR_ARG, "plan buffer too small (%lld < %lld)",
+              (long long)plan_bytes, (long long)bytes);
+    wi.off_probs = 0;
+    wi.mats = mats;
+    std::vector<B2Task> tasks[4];
+    std::vector<int64_t> woff(nprob);
+    int64_t w = 0;
+    for (int i = 0; i < nprob; ++i) {
+        const int M = Nx[i] - 2, L = Ny[i] + 2, X = Nx[i], Y = Ny[i];
+        const int mm[4] = {M, M, X, X}, nn[4] = {Y, L, L, Y};
+        for (int st = 0; st < 4; ++st)
+            for (int m0 = 0; m0 < mm[st]; m0 += B2W_BM)
+                for (int n0 = 0; n0 < nn[st]; n0 += B2W_BN) tasks[st].push_back(B2Task{i, m0, n0, 0});
+        woff[i] = w;
+        w += ((int64_t)M * L + (int64_t)X * L + 31) / 32 * 32;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    char* pb = (char*)plan;
+    SFA_HIP(hipMemcpyAsync(pb, probs.data(), probs.size() * sizeof(B2Prob), hipMemcpyHostToDevice, s));
+    for (int st = 0; st < 4; ++st)
+        SFA_HIP(hipMemcpyAsync(pb + wi.off_task[st], tasks[st].data(), tasks[st].size() * sizeof(B2Task),
+                               hipMemcpyHostToDevice, s));
+    SFA_HIP(hipMemcpyAsync(pb + wi.off_woff, woff.data(), woff.size() * 8, hipMemcpyHostToDevice, s));
+    SFA_HIP(hipStreamSynchronize(s));  // host vectors are copied before they go out of scope
+    std::lock_guard<std::mutex> lk(g_b2_mu);
+    g_b2.erase(plan);  // the address may have held a persistent-mode plan
+    g_b2w[plan] = wi;
+    return SFA_OK;
+}
+
+extern "C" int sfa_batch2d_wide_solve(const double* F, double* U, const void* plan, void* work,
+                                      int64_t work_bytes, void* stream)
+{
+    B2WideInfo wi;
+    {
+        std::lock_guard<std::mutex> lk(g_b2_mu);
+        auto it = g_b2w.find(plan);
+        SFA_CHECK(it != g_b2w.end(), SFA_ERR_ARG, "plan was not built by sfa_batch2d_wide_plan");
+        wi = it->second;
+    }
+    SFA_CHECK(F && U && work, SFA_ERR_ARG, "NULL argument");
+    SFA_CHECK(work_bytes >= wi.work_doubles * 8, SFA_ERR_ARG, "workspace too small (%lld < %lld)",
+              (long long)work_bytes, (long long)(wi.work_doubles * 8));
+    const char* pb = (const char*)plan;
+    B2WideArgs a;
+    a.F = F;
+    a.U = U;
+    a.probs = (const B2Prob*)pb;
+    a.woff = (const int64_t*)(pb + wi.off_woff);
+    a.mats = wi.mats;
+    a.work = (double*)work;
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 blk(B2_THREADS);
+    a.tasks = (const B2Task*)(pb + wi.off_task[0]);
+    hipLaunchKernelGGL(batch2d_tile_kernel<0>, dim3((unsigned)wi.ntask[0]), blk, 0, s, a);
+    SFA_LAUNCH_CHECK();
+    a.tasks = (const B2Task*)(pb + wi.off_task[1]);
+    hipLaunchKernelGGL(batch2d_tile_kernel<1>, dim3((unsigned)wi.ntask[1]), blk, 0, s, a);
+    SFA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(batch2d_wide_solve_kernel, dim3((unsigned)wi.nprob), blk, 0, s, a);
+    SFA_LAUNCH_CHECK();
+    a.tasks = (const B2Task*)(pb + wi.off_task[2]);
+    hipLaunchKernelGGL(batch2d_tile_kernel<2>, dim3((unsigned)wi.ntask[2]), blk, 0, s, a);
+    SFA_LAUNCH_CHECK();
+    a.tasks = (const B2Task*)(pb + wi.off_task[3]);
+    hipLaunchKernelGGL(batch2d_tile_kernel<3>, dim3((unsigned)wi.ntask[3]), blk, 0, s, a);
+    SFA_LAUNCH_CHECK();
     return SFA_OK;
 }
 

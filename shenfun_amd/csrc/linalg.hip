@@ -112,6 +112,85 @@ __global__ void __launch_bounds__(256) line_matmul_kernel(
     (void)t_fast_out;
 }
 
+// Few-lines form of the same product: one wave per MV_R consecutive rows k,
+// lanes striding j (each M row read once, coalesced), MV_NB lines per
+// blockIdx.y group.  Each output is lane-strided partial sums followed by a
+// fixed butterfly, so it depends only on its own line's data.
+constexpr int MV_R = 2, MV_NB = 4;
+
+template <bool CPLX>
+__global__ void __launch_bounds__(256) line_matvec_kernel(
+    const char* __restrict__ in, char* __restrict__ out, int nin, int nout,
+    const double* __restrict__ M, int64_t in_sa, int64_t in_s1, int64_t in_s2, int64_t out_sa,
+    int64_t out_s1, int64_t out_s2, int64_t d2, int64_t nlines)
+{
+    const int lane = threadIdx.x & 63;
+    const int k0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * MV_R;
+    if (k0 >= nout) return;  // whole wave; no workgroup barrier below
+    const int64_t L0 = (int64_t)blockIdx.y * MV_NB;
+    const int nb = (int)(nlines - L0 < MV_NB ? nlines - L0 : MV_NB);
+    int64_t base[MV_NB];
+#pragma unroll
+    for (int b = 0; b < MV_NB; ++b) {
+        const int64_t L = L0 + (b < nb ? b : 0);
+        const int64_t i1 = L / d2, i2 = L - i1 * d2;
+        base[b] = i1 * in_s1 + i2 * in_s2;
+    }
+    double ax[MV_R][MV_NB], ay[MV_R][MV_NB];
+#pragma unroll
+    for (int r = 0; r < MV_R; ++r)
+#pragma unroll
+        for (int b = 0; b < MV_NB; ++b) ax[r][b] = ay[r][b] = 0.0;
+    const double* Mk[MV_R];
+#pragma unroll
+    for (int r = 0; r < MV_R; ++r) Mk[r] = M + (int64_t)(k0 + r < nout ? k0 + r : k0) * nin;
+#pragma unroll 2
+    for (int j = lane; j < nin; j += 64) {
+        double m[MV_R];
+#pragma unroll
+        for (int r = 0; r < MV_R; ++r) m[r] = Mk[r][j];
+#pragma unroll
+        for (int b = 0; b < MV_NB; ++b) {
+            if (b < nb) {
+                const int64_t off = base[b] + (int64_t)j * in_sa;
+                double2 v;
+                if (CPLX) v = reinterpret_cast<const double2*>(in)[off];
+                else v = make_double2(reinterpret_cast<const double*>(in)[off], 0.0);
+#pragma unroll
+                for (int r = 0; r < MV_R; ++r) {
+                    ax[r][b] += m[r] * v.x;
+                    if (CPLX) ay[r][b] += m[r] * v.y;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < MV_R; ++r)
+#pragma unroll
+        for (int b = 0; b < MV_NB; ++b)
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                ax[r][b] += __shfl_xor(ax[r][b], o);
+                if (CPLX) ay[r][b] += __shfl_xor(ay[r][b], o);
+            }
+    if (lane == 0) {
+#pragma unroll
+        for (int b = 0; b < MV_NB; ++b) {
+            if (b >= nb) break;
+            const int64_t L = L0 + b, i1 = L / d2, i2 = L - i1 * d2;
+#pragma unroll
+            for (int r = 0; r < MV_R; ++r) {
+                const int k = k0 + r;
+                if (k < nout) {
+                    const int64_t off = i1 * out_s1 + i2 * out_s2 + (int64_t)k * out_sa;
+                    if (CPLX) reinterpret_cast<double2*>(out)[off] = make_double2(ax[r][b], ay[r][b]);
+                    else reinterpret_cast<double*>(out)[off] = ax[r][b];
+                }
+            }
+        }
+    }
+}
+
 }  // namespace sfa
 
 using namespace sfa;
@@ -146,6 +225,19 @@ extern "C" int sfa_line_matmul(const void* in, void* out, int dtype, int ndim,
     const int tfi = absl(gi.s2) < absl(gi.sa) ? 1 : 0;
     const int tfo = absl(go.s2) < absl(go.sa) ? 1 : 0;
     // lines enumerated in C order with d1 slow: both geometries share d1, d2
+    if (nlines <= 16) {  // few lines: row-parallel matvec form
+        dim3 g((unsigned)((nout + 4 * MV_R - 1) / (4 * MV_R)), (unsigned)((nlines + MV_NB - 1) / MV_NB));
+        if (dtype == SFA_C128)
+            hipLaunchKernelGGL(line_matvec_kernel<true>, g, dim3(256), 0, (hipStream_t)stream,
+                               (const char*)in, (char*)out, (int)gi.n, (int)nout, M, gi.sa, gi.s1,
+                               gi.s2, go.sa, go.s1, go.s2, gi.d2, nlines);
+        else
+            hipLaunchKernelGGL(line_matvec_kernel<false>, g, dim3(256), 0, (hipStream_t)stream,
+                               (const char*)in, (char*)out, (int)gi.n, (int)nout, M, gi.sa, gi.s1,
+                               gi.s2, go.sa, go.s1, go.s2, gi.d2, nlines);
+        SFA_LAUNCH_CHECK();
+        return SFA_OK;
+    }
     dim3 grid((unsigned)((nlines + 15) / 16), (unsigned)((nout + 63) / 64)), block(256);
     if (dtype == SFA_C128)
         hipLaunchKernelGGL(line_matmul_kernel<true>, grid, block, 0, (hipStream_t)stream,

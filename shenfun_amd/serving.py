@@ -203,6 +203,14 @@ class Server:
             r.u = U[i]
         return (F, fh, U), [("cheb1d",) + reqs[0].key]
 
+    def _wide_2d(self, Nx, Ny):
+        """Tile-parallel mode when one problem would dominate the persistent
+        kernel's makespan: its contraction flops exceed twice the per-CU share."""
+        m = Nx.astype(np.float64)
+        y = Ny.astype(np.float64)
+        fl = m * y * (4 * m + 2) + 2 * m * m * (y + 2)
+        return bool(fl.max() * self.cus > 2.0 * fl.sum())
+
     def _launch_leg2d(self, reqs, stream):
         L = _lib.lib()
         n = len(reqs)
@@ -226,19 +234,31 @@ class Server:
         total = int(sizes.sum())
         nb = min(n, self.cus)
         ptr = lambda a: _lib.ctypes.c_void_p(a.ctypes.data)
+        wide = self._wide_2d(Nx, Ny)
         with torch.cuda.stream(stream):
             F = torch.cat([r.f.reshape(-1) for r in reqs])
             U = torch.empty_like(F)
-            plan = torch.empty(int(L.sfa_batch2d_plan_ext_bytes(n, nb)), dtype=torch.uint8,
-                               device=self.device)
-            work = torch.empty(int(L.sfa_batch2d_workspace(n, ptr(Nx), ptr(Ny), nb)), dtype=torch.uint8,
-                               device=self.device)
             sp = _lib.stream_ptr(stream)
-            _lib.call("sfa_batch2d_plan_ext", n, ptr(Nx), ptr(Ny), ptr(al), ptr(bc), ptr(off), ptr(off),
-                      total, total, _lib.ctypes.c_void_p(base), ptr(phi), ptr(dft), (top - base) // 8, nb,
-                      _lib.ptr(plan), plan.numel(), sp)
-            _lib.call("sfa_batch2d_solve", _lib.ptr(F), _lib.ptr(U), _lib.ptr(plan), _lib.ptr(work),
-                      work.numel(), 0, sp)
+            if wide:
+                plan = torch.empty(int(L.sfa_batch2d_wide_plan_bytes(n, ptr(Nx), ptr(Ny))),
+                                   dtype=torch.uint8, device=self.device)
+                work = torch.empty(int(L.sfa_batch2d_wide_workspace(n, ptr(Nx), ptr(Ny))),
+                                   dtype=torch.uint8, device=self.device)
+                _lib.call("sfa_batch2d_wide_plan", n, ptr(Nx), ptr(Ny), ptr(al), ptr(bc), ptr(off),
+                          ptr(off), total, total, _lib.ctypes.c_void_p(base), ptr(phi), ptr(dft),
+                          (top - base) // 8, _lib.ptr(plan), plan.numel(), sp)
+                _lib.call("sfa_batch2d_wide_solve", _lib.ptr(F), _lib.ptr(U), _lib.ptr(plan),
+                          _lib.ptr(work), work.numel(), sp)
+            else:
+                plan = torch.empty(int(L.sfa_batch2d_plan_ext_bytes(n, nb)), dtype=torch.uint8,
+                                   device=self.device)
+                work = torch.empty(int(L.sfa_batch2d_workspace(n, ptr(Nx), ptr(Ny), nb)),
+                                   dtype=torch.uint8, device=self.device)
+                _lib.call("sfa_batch2d_plan_ext", n, ptr(Nx), ptr(Ny), ptr(al), ptr(bc), ptr(off),
+                          ptr(off), total, total, _lib.ctypes.c_void_p(base), ptr(phi), ptr(dft),
+                          (top - base) // 8, nb, _lib.ptr(plan), plan.numel(), sp)
+                _lib.call("sfa_batch2d_solve", _lib.ptr(F), _lib.ptr(U), _lib.ptr(plan),
+                          _lib.ptr(work), work.numel(), 0, sp)
         for i, r in enumerate(reqs):
             r.u = U[int(off[i]):int(off[i] + sizes[i])].view(int(Nx[i]), int(Ny[i]))
         return (F, U, plan, work, mats), keys
@@ -293,6 +313,8 @@ class Server:
                 self.free_slots.append(slot)
                 self.mem_used -= nbytes
                 self.cache.release(keys)
+                if reqs[0].family == "leg2d":  # drop the plan's host-side registry entry
+                    _lib.lib().sfa_batch2d_release(_lib.ptr(keep[2]))
             else:
                 still.append((ev, slot, reqs, nbytes, keys, keep))
         self.inflight = still

@@ -138,3 +138,28 @@ def test_batch2d_c4_workload_sample(sfa):
         err = _rel(u, u_ref)
         print(f"[c4 sample {i}: {nx[i]}x{ny[i]} alpha={al[i]:.2f}] err {err:.2e}", flush=True)
         assert err < 1e-11
+
+
+def test_batch2d_wide_equals_persistent(sfa, batch):
+    """The tile-parallel mode gives the persistent kernel's bits, on the mixed
+    batch and on a lone large problem, and on repeated calls."""
+    from shenfun_amd.workloads import batch2d_rhs
+    H, F, _ = batch
+    print("[batch2d wide: mixed batch]", flush=True)
+    H.build(H.cus)
+    H.variant = 0
+    ref = H(F).clone()
+    W = sfa.BatchedHelmholtz2D(H.Nx, H.Ny, H.alpha, ["D" if b == 0 else "N" for b in H.bc], mode="wide")
+    for rep in range(2):
+        assert torch.equal(W(F), ref), f"call {rep}"
+    for nx, ny, al, bc in [(512, 512, 3.0, "D"), (390, 72, 0.75, "N")]:
+        print(f"[batch2d wide: lone {nx}x{ny} {bc}]", flush=True)
+        f = batch2d_rhs(nx, ny, np.random.default_rng(nx))
+        P1 = sfa.BatchedHelmholtz2D([nx], [ny], [al], bc)
+        W1 = sfa.BatchedHelmholtz2D([nx], [ny], [al], bc, mode="wide")
+        F1 = P1.pack([f])
+        u = W1(F1)
+        assert torch.equal(u, P1(F1))
+        err = _rel(u.view(nx, ny), S.legendre_fourier_2d(f, al, bc)[0])
+        print(f"  max rel err {err:.2e}", flush=True)
+        assert err < 1e-11

@@ -142,3 +142,29 @@ def test_forward_projection_roundtrip(sfa, family, bc):
     phys = V.backward(dev(u))
     uh = V.forward(phys)
     assert rel(uh, u) < 1e-12
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("shape,axis", [((97, 1), 0), ((3, 130), 1), ((130, 4), 0), ((2, 150, 3), 1),
+                                        ((5, 2, 61), 2), ((17, 70), 1), ((64, 40), 1)])
+def test_line_matmul_few_and_many_lines(sfa, shape, axis, cplx):
+    """sfa_line_matmul (dense per-line contraction; the row-parallel form for
+    up to 16 lines, the tiled form above) vs numpy, strided and contiguous
+    lines, nout != nin."""
+    from shenfun_amd import _lib
+    print(f"[line_matmul {shape} axis={axis} cplx={cplx}]", flush=True)
+    rng = np.random.default_rng(sum(shape) + axis)
+    x = rng.standard_normal(shape) + (1j * rng.standard_normal(shape) if cplx else 0)
+    nin = shape[axis]
+    nout = nin - 3
+    M = rng.standard_normal((nout, nin))
+    ref = np.moveaxis(np.tensordot(M, x, axes=([1], [axis])), 0, axis)
+    X = dev(x)
+    oshape = list(shape)
+    oshape[axis] = nout
+    Y = torch.empty(oshape, dtype=X.dtype, device="cuda")
+    Md = dev(M)
+    _lib.call("sfa_line_matmul", _lib.ptr(X), _lib.ptr(Y), _lib.C128 if cplx else _lib.F64, X.dim(),
+              _lib.arr(X.shape), _lib.arr(X.stride()), _lib.arr(Y.stride()), axis, nout, _lib.ptr(Md),
+              _lib.stream_ptr())
+    assert rel(Y, ref) < 1e-13
