@@ -410,7 +410,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         int u = 0;
         if (Lc < a.nl) {
             const int64_t ky = Lc / a.Kp, kz = a.kz0 + (Lc - ky * a.Kp);
-            u = a.pl.linemap[ky * a.Nzh + kz];
+            if (kz < a.Nzh) u = a.pl.linemap[ky * a.Nzh + kz];  // else a pitch-padding line
         }
         const double be = a.alpha + a.pl.svals[u];
         const double4 b4 = a.pl.bs[((int64_t)2 * u + p) * 16 + lane];
@@ -599,10 +599,12 @@ int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, doub
     return 1;
 }
 
-int fast_xpass(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, const double* K2,
-               const XPlan& pl, hipStream_t s, int* rc)
+// W: lines (ky, kz) at pitch Kp >= Nzh per ky; the Kp - Nzh padding lines
+// are transformed too (their values are never read back).
+int fast_xpass(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int neumann,
+               const double* K2, const XPlan& pl, hipStream_t s, int* rc)
 {
-    return fast_xpass_sub(W, false, Nx, Ny, Nzh, Nzh, 0, alpha, neumann, K2, pl, s, rc);
+    return fast_xpass_sub(W, false, Nx, Ny, Nzh, Kp, 0, alpha, neumann, K2, pl, s, rc);
 }
 
 }  // namespace sfa

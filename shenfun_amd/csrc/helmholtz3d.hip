@@ -230,8 +230,8 @@ static const double* cheb_k2(int N, int neumann)
     return d;
 }
 
-int fast_xpass(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, const double* K2,
-               const XPlan& pl, hipStream_t s, int* rc);
+int fast_xpass(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int neumann,
+               const double* K2, const XPlan& pl, hipStream_t s, int* rc);
 int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha,
                    int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc);
 int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, double alpha,
@@ -326,8 +326,9 @@ static bool plan_ok(const void* plan, int64_t Nx, int64_t Ny, int64_t Nz, double
     return plan_view(plan, Nx, Ny, Nz, alpha, neumann, nullptr);
 }
 
-int xpass_launch(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann, hipStream_t s,
-                 const void* plan = nullptr)
+// W: (Nx, Ny, Nzh) complex lines at kz pitch Kp >= Nzh.
+int xpass_launch(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int neumann,
+                 hipStream_t s, const void* plan = nullptr)
 {
     if (plan != nullptr) {
         const double* K2 = cheb_k2(Nx, neumann);
@@ -336,12 +337,12 @@ int xpass_launch(double2* W, int Nx, int Ny, int Nzh, double alpha, int neumann,
         SFA_CHECK(plan_view(plan, Nx, Ny, 2 * (Nzh - 1), alpha, neumann, &pv), SFA_ERR_ARG,
                   "plan does not match the solve");
         int rc = 0;
-        if (fast_xpass(W, Nx, Ny, Nzh, alpha, neumann, K2, pv, s, &rc)) return rc;
+        if (fast_xpass(W, Nx, Ny, Nzh, Kp, alpha, neumann, K2, pv, s, &rc)) return rc;
     }
     XPassArgs a;
     a.W = W;
-    a.sa = (int64_t)Ny * Nzh;
-    a.s1 = Nzh;
+    a.sa = (int64_t)Ny * Kp;
+    a.s1 = Kp;
     a.d1 = Ny;
     a.d2 = Nzh;
     a.N = Nx;
@@ -439,6 +440,15 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
     return rc;
 }
 
+// kz pitch of the FP64 workspace: Nzh rounded up to a multiple of 8 complex
+// (128 bytes) where the fast x-pass runs and the padding is under 1/8 of the
+// planes; otherwise Nzh.
+static int64_t fp64_pitch(int64_t Nx, int64_t Nzh, const void* plan)
+{
+    const bool fast = plan != nullptr && (Nx == 64 || Nx == 128 || Nx == 256 || Nx == 512);
+    return (fast && Nzh >= 64) ? (Nzh + 7) / 8 * 8 : Nzh;
+}
+
 // FP64 pipeline: z R2C, y C2C, fused x-pass, y C2C inverse, z C2R.  ev: NULL
 // or 6 events recorded around the five stages.
 static int helmholtz3d_fp64(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
@@ -449,21 +459,24 @@ static int helmholtz3d_fp64(const double* f, double* u, int64_t Nx, int64_t Ny, 
         if (ev) hipEventRecord(ev[i], s);
     };
     const int64_t Nzh = Nz / 2 + 1;
+    const int64_t Kp = fp64_pitch(Nx, Nzh, plan);
     double2* W = (double2*)work;
     const int64_t rshape[3] = {Nx, Ny, Nz};
     const int64_t rstr[3] = {Ny * Nz, Nz, 1};
-    const int64_t cstr[3] = {Ny * Nzh, Nzh, 1};
-    const int64_t cshape[3] = {Nx, Ny, Nzh};
+    const int64_t cstr[3] = {Ny * Kp, Kp, 1};
+    // the y passes run over the padding lines too, so that each workgroup's
+    // 16-line row segments start on 128-byte boundaries
+    const int64_t yshape[3] = {Nx, Ny, Kp};
     void* st = (void*)s;
     int rc;
     mark(0);
     if ((rc = sfa_fft_r2c(f, W, 3, rshape, rstr, cstr, 2, 1.0 / (double)Nz, st))) return rc;
     mark(1);
-    if ((rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, -1, 1.0 / (double)Ny, st))) return rc;
+    if ((rc = sfa_fft_c2c(W, W, 3, yshape, cstr, cstr, 1, -1, 1.0 / (double)Ny, st))) return rc;
     mark(2);
-    if ((rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, alpha, neumann, s, plan))) return rc;
+    if ((rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, (int)Kp, alpha, neumann, s, plan))) return rc;
     mark(3);
-    if ((rc = sfa_fft_c2c(W, W, 3, cshape, cstr, cstr, 1, +1, 1.0, st))) return rc;
+    if ((rc = sfa_fft_c2c(W, W, 3, yshape, cstr, cstr, 1, +1, 1.0, st))) return rc;
     mark(4);
     if ((rc = sfa_fft_c2r(W, u, 3, rshape, cstr, rstr, 2, 1.0, st))) return rc;
     mark(5);
@@ -473,6 +486,9 @@ static int helmholtz3d_fp64(const double* f, double* u, int64_t Nx, int64_t Ny, 
 static int mixed_planes(int64_t Nz, int fp64_planes)
 {
     const int64_t Nzh = Nz / 2 + 1;
+    // 17: the complex64 planes' pitch Nz/2 + 1 - 17 is then a multiple of 16
+    // (128-byte rows) for every power-of-two Nz >= 32, which the y passes need
+    // for aligned row segments (Kh = 16 measured 1.35x slower y passes at 256^3)
     int Kh = fp64_planes > 0 ? fp64_planes : 17;
     if (Kh < 2) Kh = 2;
     if (Kh > Nzh - 2) Kh = (int)Nzh - 2;
@@ -486,7 +502,9 @@ using namespace sfa;
 extern "C" int64_t sfa_helmholtz3d_workspace(int64_t Nx, int64_t Ny, int64_t Nz, int precision)
 {
     (void)precision;
-    return Nx * Ny * (Nz / 2 + 1) * (int64_t)sizeof(double2);
+    const int64_t Nzh = Nz / 2 + 1;
+    const int64_t Kp = Nzh >= 64 ? (Nzh + 7) / 8 * 8 : Nzh;  // covers fp64_pitch()
+    return Nx * Ny * Kp * (int64_t)sizeof(double2);
 }
 
 extern "C" int64_t sfa_helmholtz3d_plan_bytes(int64_t Nx, int64_t Ny, int64_t Nz)
