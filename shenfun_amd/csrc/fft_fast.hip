@@ -134,13 +134,17 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
     double2* ln = ex + t * LS;
     const double2* src = in + i1 * in_s1 + i2 * in_rs;
-    // all of this thread's loads in flight before the LDS stores
-    double2 vin[P + 1];
+    // all of this thread's loads in flight before the LDS stores, the
+    // pre-twiddles exp(-2 pi i k / N) of its k = j + Q r with them (LDS, not
+    // registers, limits this kernel's occupancy)
+    double2 vin[P + 1], twr[P];
 #pragma unroll
     for (int i = 0; i <= P; ++i) {
         const int k = j + Q * i;
         vin[i] = (valid && k <= H) ? src[k] : make_double2(0.0, 0.0);
     }
+#pragma unroll
+    for (int r = 0; r < P; ++r) twr[r] = twg[j + Q * r];
 #pragma unroll
     for (int i = 0; i <= P; ++i) {
         const int k = j + Q * i;
@@ -157,7 +161,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
         const int k = j + Q * r;
         const double2 xk = ln[k];
         const double2 xc = cconj(ln[H - k]);
-        const double2 d = cmul(cconj(twg[k]), csub(xk, xc));
+        const double2 d = cmul(cconj(twr[r]), csub(xk, xc));
         a[r] = make_double2(xk.x + xc.x - d.y, xk.y + xc.y + d.x);
     }
     __syncthreads();
@@ -686,7 +690,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     const int Kl = H + 1 - Kh;
     for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
     double2* ln = ex + t * LS;
-    double2 vin[P + 1];
+    double2 vin[P + 1], twr[P];
 #pragma unroll
     for (int i = 0; i <= P; ++i) {
         const int k = j + Q * i;
@@ -694,6 +698,8 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
         if (valid && k <= H) v = (k < Kh) ? i64[R * Kh + k] : to_c64(i32[R * Kl + (k - Kh)]);
         vin[i] = v;
     }
+#pragma unroll
+    for (int r = 0; r < P; ++r) twr[r] = twg[j + Q * r];  // pre-twiddles, issued with the loads
 #pragma unroll
     for (int i = 0; i <= P; ++i) {
         const int k = j + Q * i;
@@ -710,7 +716,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
         const int k = j + Q * r;
         const double2 xk = ln[k];
         const double2 xc = cconj(ln[H - k]);
-        const double2 d = cmul(cconj(twg[k]), csub(xk, xc));
+        const double2 d = cmul(cconj(twr[r]), csub(xk, xc));
         a[r] = make_double2(xk.x + xc.x - d.y, xk.y + xc.y + d.x);
     }
     __syncthreads();
