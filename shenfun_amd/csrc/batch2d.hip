@@ -75,6 +75,11 @@ __device__ void gemm_tile(const double* __restrict__ A, int64_t lda, const doubl
     for (int a = 0; a < TI; ++a)
 #pragma unroll
         for (int b = 0; b < TJ; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    bool rin[TI], cin[TJ];  // this wave's 16 x 16 sub-tiles that intersect C
+#pragma unroll
+    for (int a = 0; a < TI; ++a) rin[a] = m0 + wi * (BM / 2) + a * 16 < m;
+#pragma unroll
+    for (int b = 0; b < TJ; ++b) cin[b] = n0 + wj * (BN / 2) + b * 16 < n;
     double ra[LA], rb[LB];
     auto load = [&](int k0) {
 #pragma unroll
@@ -107,18 +112,24 @@ __device__ void gemm_tile(const double* __restrict__ A, int64_t lda, const doubl
         }
         __syncthreads();
         if (k0 + BK < k) load(k0 + BK);  // prefetch overlaps the MFMAs
+        // MFMAs whose operands are all zero padding are skipped: K steps of 4
+        // past k, and 16 x 16 sub-tiles wholly outside C (never stored).  All
+        // conditions are wave-uniform; a skipped step would add exact zeros.
 #pragma unroll
         for (int s = 0; s < BK / 4; ++s) {
-            const int kk = 4 * s + lk;
-            double af[TI], bf[TJ];
+            if (k0 + 4 * s < k) {
+                const int kk = 4 * s + lk;
+                double af[TI], bf[TJ];
 #pragma unroll
-            for (int a = 0; a < TI; ++a) af[a] = sA[kk * SA + wi * (BM / 2) + a * 16 + lr];
+                for (int a = 0; a < TI; ++a) af[a] = sA[kk * SA + wi * (BM / 2) + a * 16 + lr];
 #pragma unroll
-            for (int b = 0; b < TJ; ++b) bf[b] = sB[kk * BN + wj * (BN / 2) + b * 16 + lr];
+                for (int b = 0; b < TJ; ++b) bf[b] = sB[kk * BN + wj * (BN / 2) + b * 16 + lr];
 #pragma unroll
-            for (int a = 0; a < TI; ++a)
+                for (int a = 0; a < TI; ++a)
 #pragma unroll
-                for (int b = 0; b < TJ; ++b) acc[a][b] = mfma16(af[a], bf[b], acc[a][b]);
+                    for (int b = 0; b < TJ; ++b)
+                        if (rin[a] && cin[b]) acc[a][b] = mfma16(af[a], bf[b], acc[a][b]);
+            }
         }
     }
 #pragma unroll
@@ -138,9 +149,9 @@ __device__ void gemm_tile(const double* __restrict__ A, int64_t lda, const doubl
 // whole workgroup (4 waves in a 2 x 2 grid over BM x BN block tiles, K in
 // steps of BK).  sA: BK x (BM + 1) doubles, sB: BK x BN doubles of LDS.
 // Out-of-range operand elements load as zero; stores are bounds-checked.
-// Each output is one MFMA accumulation chain over K in ascending steps of BK
-// (four K = 4 MFMAs per step), so the result does not depend on BM, BN, on
-// which workgroup computes the tile, or on the order of the tiles.
+// Each output is one MFMA accumulation chain over K in ascending steps of 4
+// (up to the last step holding k), so the result does not depend on BM, BN,
+// BK, on which workgroup computes the tile, or on the order of the tiles.
 template <int BM, int BN, int BK>
 __device__ void block_gemm(const double* __restrict__ A, int64_t lda, const double* __restrict__ B,
                            int64_t ldb, double* __restrict__ C, int64_t ldc, int m, int n, int k,
@@ -685,8 +696,9 @@ static int launch_b2(const B2Args& a, int nblocks, hipStream_t s)
 using namespace sfa;
 
 // block tiles (BM, BN, BK) of the launch variants
+// (64, 128, 32) and (128, 64, 32) need more than 256 VGPRs + AGPRs and spill: left out
 static const int kB2Variants[][3] = {{64, 64, 16}, {32, 64, 16}, {64, 128, 16}, {128, 64, 16},
-                                     {64, 64, 32}, {64, 128, 32}, {128, 64, 32}};
+                                     {64, 64, 32}};
 
 extern "C" int sfa_batch2d_variants(int* bm, int* bn, int cap)
 {
@@ -830,9 +842,7 @@ extern "C" int sfa_batch2d_solve(const double* F, double* U, const void* plan, v
         case 1: return launch_b2<32, 64, 16>(a, pi.nblocks, s);
         case 2: return launch_b2<64, 128, 16>(a, pi.nblocks, s);
         case 3: return launch_b2<128, 64, 16>(a, pi.nblocks, s);
-        case 4: return launch_b2<64, 64, 32>(a, pi.nblocks, s);
-        case 5: return launch_b2<64, 128, 32>(a, pi.nblocks, s);
-        default: return launch_b2<128, 64, 32>(a, pi.nblocks, s);
+        default: return launch_b2<64, 64, 32>(a, pi.nblocks, s);
     }
 }
 
