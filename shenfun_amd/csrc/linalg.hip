@@ -1,6 +1,7 @@
 // Dense per-line transforms: Legendre transform matrices and a strided
 // batched matrix-vector product along one axis (FP64, LDS tiled).
 #include "common.h"
+#include "mfma_gemm.h"
 
 namespace sfa {
 
@@ -191,6 +192,20 @@ __global__ void __launch_bounds__(256) line_matvec_kernel(
     }
 }
 
+// Lines along a strided axis whose other dims flatten to unit-stride line
+// indices: the lines are the columns of X (nin x ncol, row pitch ldx doubles;
+// complex lines are two real columns), and the contraction is the FP64 MFMA
+// GEMM Y = M X, one workgroup per 64 x 64 tile of Y.
+__global__ void __launch_bounds__(GEMM_THREADS)
+    line_gemm_kernel(const double* __restrict__ M, const double* __restrict__ X, double* __restrict__ Y,
+                     int nout, int nin, int64_t ncol, int64_t ldx, int64_t ldy)
+{
+    __shared__ double sA[16 * (64 + 1)];
+    __shared__ double sB[16 * 64];
+    gemm_tile<64, 64, 16>(M, nin, X, ldx, Y, ldy, nout, (int)ncol, nin,
+                          (int)blockIdx.y * 64, (int)blockIdx.x * 64, sA, sB);
+}
+
 }  // namespace sfa
 
 using namespace sfa;
@@ -225,6 +240,19 @@ extern "C" int sfa_line_matmul(const void* in, void* out, int dtype, int ndim,
     const int tfi = absl(gi.s2) < absl(gi.sa) ? 1 : 0;
     const int tfo = absl(go.s2) < absl(go.sa) ? 1 : 0;
     // lines enumerated in C order with d1 slow: both geometries share d1, d2
+    auto flat = [&](const Geom& g) {  // line L at offset L: a column of a row-major matrix
+        return g.s2 == 1 && (g.d1 == 1 || g.s1 == g.d2) && g.sa >= nlines;
+    };
+    if (nlines > 16 && flat(gi) && flat(go) && nlines * (dtype == SFA_C128 ? 2 : 1) <= INT32_MAX) {
+        const int w = dtype == SFA_C128 ? 2 : 1;  // doubles per element
+        const int64_t ncol = nlines * w;
+        dim3 g((unsigned)((ncol + 63) / 64), (unsigned)((nout + 63) / 64));
+        hipLaunchKernelGGL(line_gemm_kernel, g, dim3(GEMM_THREADS), 0, (hipStream_t)stream, M,
+                           (const double*)in, (double*)out, (int)nout, (int)gi.n, ncol, gi.sa * w,
+                           go.sa * w);
+        SFA_LAUNCH_CHECK();
+        return SFA_OK;
+    }
     if (nlines <= 16) {  // few lines: row-parallel matvec form
         dim3 g((unsigned)((nout + 4 * MV_R - 1) / (4 * MV_R)), (unsigned)((nlines + MV_NB - 1) / MV_NB));
         if (dtype == SFA_C128)

@@ -26,6 +26,7 @@ memory budget; a batch that does not fit waits.  Latency of a request is the
 host-observed completion of its batch minus its arrival time.
 """
 import collections
+import itertools
 import time
 
 import numpy as np
@@ -121,7 +122,12 @@ class Server:
         self.max_batch_2d = int(max_batch_2d)
         self.cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         self.inflight = []  # (event, slot, requests, bytes, cache keys, keepalive)
+        # pending requests: arrival order (admitted entries dropped lazily from
+        # the front) plus per-class FIFOs, so forming a batch costs O(batch)
         self.pending = collections.deque()
+        self._q1d = {}                    # (N, bc, alpha) -> deque of 1D requests
+        self._q2d = collections.deque()   # 2D requests
+        self._npending = 0
         self.done = []
         self.batches = collections.Counter()
         self._work3d = {}  # slot -> workspace tensor (grown on demand)
@@ -289,18 +295,41 @@ class Server:
         shp = reqs[0].key[0]
         return n + shp[0] * shp[1] * (shp[2] // 2 + 1) * 16
 
+    def _push(self, r):
+        self.pending.append(r)
+        if r.family == "cheb1d":
+            self._q1d.setdefault(r.key, collections.deque()).append(r)
+        elif r.family == "leg2d":
+            self._q2d.append(r)
+        self._npending += 1
+
+    def _queue(self, head):
+        return self._q1d[head.key] if head.family == "cheb1d" else self._q2d
+
     def _next_batch(self):
-        """Oldest pending request and the requests that can join its launch."""
+        """Oldest pending request and the requests that can join its launch
+        (the front of its class FIFO, oldest first)."""
+        while self.pending and self.pending[0].t_admit is not None:
+            self.pending.popleft()
         if not self.pending:
             return None
         head = self.pending[0]
         if head.family == "cheb3d":
             return [head]
-        if head.family == "cheb1d":
-            out = [r for r in self.pending if r.family == "cheb1d" and r.key == head.key]
-            return out[:self.max_batch_1d]
-        out = [r for r in self.pending if r.family == "leg2d"]
-        return out[:self.max_batch_2d]
+        cap = self.max_batch_1d if head.family == "cheb1d" else self.max_batch_2d
+        return list(itertools.islice(self._queue(head), cap))
+
+    def _take(self, batch, t):
+        """Remove an admitted batch (from _next_batch) from the queues."""
+        if batch[0].family != "cheb3d":
+            q = self._queue(batch[0])
+            for _ in batch:
+                q.popleft()
+            if batch[0].family == "cheb1d" and not q:
+                del self._q1d[batch[0].key]
+        self._npending -= len(batch)
+        for r in batch:
+            r.t_admit = t
 
     def _poll(self, now):
         still = []
@@ -320,18 +349,14 @@ class Server:
         self.inflight = still
 
     def _admit(self, now):
-        while self.free_slots and self.pending:
+        while self.free_slots and self._npending:
             batch = self._next_batch()
             nbytes = self._bytes(batch)
             if self.mem_used + nbytes > self.mem_budget and self.inflight:
                 return  # wait for memory
             slot = self.free_slots.pop(0)
             stream = self.streams[slot]
-            ids = set(id(r) for r in batch)
-            self.pending = collections.deque(r for r in self.pending if id(r) not in ids)
-            t = now()
-            for r in batch:
-                r.t_admit = t
+            self._take(batch, now())
             fam = batch[0].family
             if fam == "cheb1d":
                 keep, keys = self._launch_cheb1d(batch, stream)
@@ -352,6 +377,8 @@ class Server:
         realtime: release each request at its arrival time (open loop); else
         all are pending at once.  Returns the served requests."""
         reqs = sorted(requests, key=lambda r: r.arrival)
+        for r in reqs:
+            r.t_admit = r.t_done = r.u = None
         self.done = []
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
@@ -361,11 +388,11 @@ class Server:
         while len(self.done) < n:
             t = now() if realtime else float("inf")
             while i < n and reqs[i].arrival <= t:
-                self.pending.append(reqs[i])
+                self._push(reqs[i])
                 i += 1
             self._poll(now)
             self._admit(now)
-            if not self.inflight and not self.pending and i < n and realtime:
+            if not self.inflight and not self._npending and i < n and realtime:
                 dt = reqs[i].arrival - now()
                 if dt > 2e-4:
                     time.sleep(dt - 1e-4)

@@ -146,11 +146,13 @@ def test_forward_projection_roundtrip(sfa, family, bc):
 
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("shape,axis", [((97, 1), 0), ((3, 130), 1), ((130, 4), 0), ((2, 150, 3), 1),
-                                        ((5, 2, 61), 2), ((17, 70), 1), ((64, 40), 1)])
+                                        ((5, 2, 61), 2), ((17, 70), 1), ((64, 40), 1), ((97, 40), 0),
+                                        ((70, 5, 9), 0), ((128, 300), 0), ((33, 17), 0)])
 def test_line_matmul_few_and_many_lines(sfa, shape, axis, cplx):
-    """sfa_line_matmul (dense per-line contraction; the row-parallel form for
-    up to 16 lines, the tiled form above) vs numpy, strided and contiguous
-    lines, nout != nin."""
+    """sfa_line_matmul (dense per-line contraction: the row-parallel form for
+    up to 16 lines, the MFMA GEMM form for more lines along a strided axis
+    whose other dims flatten, the tiled form otherwise) vs numpy, strided and
+    contiguous lines, nout != nin."""
     from shenfun_amd import _lib
     print(f"[line_matmul {shape} axis={axis} cplx={cplx}]", flush=True)
     rng = np.random.default_rng(sum(shape) + axis)
