@@ -210,6 +210,15 @@ int64_t sfa_helmholtz3d_workspace(int64_t Nx, int64_t Ny, int64_t Nz, int precis
 int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                     double alpha, int neumann, int precision, int fp64_planes, const void* plan,
                     void* work, int64_t work_bytes, void* stream);
+/* nbatch solves of one shape, alpha and bc in one pass of launches: f and u
+ * hold nbatch (Nx, Ny, Nz) arrays back to back, work at least nbatch x
+ * sfa_helmholtz3d_workspace() bytes; each array's result is bitwise equal to
+ * its sfa_helmholtz3d solve.  Same reference sequence as sfa_helmholtz3d per
+ * array; no reference counterpart for the batch itself (shenfun solves one
+ * TensorProductSpace array per call). */
+int sfa_helmholtz3d_many(const double* f, double* u, int64_t nbatch, int64_t Nx, int64_t Ny,
+                         int64_t Nz, double alpha, int neumann, int precision, int fp64_planes,
+                         const void* plan, void* work, int64_t work_bytes, void* stream);
 
 /* Profiling variant of sfa_helmholtz3d: runs the pipeline (precision and
  * fp64_planes as there) nrep times with hipEvents recorded on `stream` between

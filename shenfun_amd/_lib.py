@@ -40,6 +40,7 @@ _SIGS = {
     "sfa_poisson_add_solve": "vviiaaaivvdv",
     "sfa_helmholtz3d_plan": "llldivlv",
     "sfa_helmholtz3d": "vvllldiiivvlv",
+    "sfa_helmholtz3d_many": "vvlllldiiivvlv",
     "sfa_helmholtz3d_stage_times": "vvllldiiivvlivv",
     "sfa_batch2d_plan": "ivvvvvvllivlv",
     "sfa_batch2d_solve": "vvvvliv",

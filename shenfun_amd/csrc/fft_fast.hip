@@ -305,9 +305,11 @@ struct XLay {
 };
 
 struct XPass2Args {
-    void* W;         // lines of C2 values; element (x, L) at W[x * sa + L]
+    void* W;         // lines of C2 values; element (x, L) of array b at W[b * bstride + x * sa + L]
     int64_t sa;      // x stride (= Ny * Kp)
-    int64_t nl;      // Ny * Kp lines
+    int64_t nl;      // lines of all arrays: nb * nlb
+    int64_t nlb;     // lines per array (= Ny * Kp)
+    int64_t bstride; // elements between consecutive arrays (= Nx * sa)
     int Kp;          // kz planes held by this array (line pitch per ky)
     int kz0;         // first kz of this array
     int Nzh, Ny;
@@ -355,6 +357,8 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     const int tb = tid / Q, jb = tid % Q;
     const int64_t L = bid * TL + t;
     const bool valid = L < a.nl;
+    const int64_t Lb = L / a.nlb;
+    const int64_t Lo = Lb * a.bstride + (L - Lb * a.nlb);  // this line's element x = 0
     for (int i = tid; i < N; i += NTH) {
         tw[i] = from_c64<C2>(a.tw[i]);
         tqs[i] = from_c64<C2>(a.tq[i]);
@@ -366,7 +370,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     for (int r = 0; r < P; ++r) {
         const int m = j + Q * r;
         const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
-        v[r] = valid ? Wd[(int64_t)x * a.sa + L] : cmk<C2>(0.0, 0.0);
+        v[r] = valid ? Wd[(int64_t)x * a.sa + Lo] : cmk<C2>(0.0, 0.0);
     }
     __syncthreads();  // tables
     pow2_fft_p1<N, false>(v, j, ex + t * LA, tw);
@@ -413,7 +417,8 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         const int64_t Lc = bid * TL + ct;
         int u = 0;
         if (Lc < a.nl) {
-            const int64_t ky = Lc / a.Kp, kz = a.kz0 + (Lc - ky * a.Kp);
+            const int64_t Lr = Lc % a.nlb;  // line within its array
+            const int64_t ky = Lr / a.Kp, kz = a.kz0 + (Lr - ky * a.Kp);
             if (kz < a.Nzh) u = a.pl.linemap[ky * a.Nzh + kz];  // else a pitch-padding line
         }
         const double be = a.alpha + a.pl.svals[u];
@@ -482,7 +487,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
                 const int m = pow2_out_index<N>(j, h, k2);
                 const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
                 const C2 y = v[h * Q + k2];
-                Wd[(int64_t)x * a.sa + L] = CT<C2>::mk(half * y.x + half * t0a.x, half * y.y + half * t0a.y);
+                Wd[(int64_t)x * a.sa + Lo] = CT<C2>::mk(half * y.x + half * t0a.x, half * y.y + half * t0a.y);
             }
     }
 }
@@ -542,11 +547,13 @@ static int launch_xpass2_dual(XPass2Args& a64, XPass2Args& a32, hipStream_t s)
 // Fused x-pass on the kz planes [kz0, kz0 + Kp) held (line pitch Kp) in W.
 // f32 selects complex64 storage/arithmetic.  Returns 1 if handled.
 static bool xpass_args(void* W, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha, int neumann,
-                       const double* K2, const XPlan& pl, XPass2Args* a)
+                       const double* K2, const XPlan& pl, int64_t nb, XPass2Args* a)
 {
     a->W = W;
     a->sa = (int64_t)Ny * Kp;
-    a->nl = (int64_t)Ny * Kp;
+    a->nlb = (int64_t)Ny * Kp;
+    a->nl = nb * a->nlb;
+    a->bstride = (int64_t)Nx * a->sa;
     a->Kp = Kp;
     a->kz0 = kz0;
     a->Nzh = Nzh;
@@ -561,12 +568,12 @@ static bool xpass_args(void* W, int Nx, int Ny, int Nzh, int Kp, int kz0, double
 }
 
 int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha,
-                   int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc)
+                   int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc)
 {
     if (!(Nx == 64 || Nx == 128 || Nx == 256 || Nx == 512) || pl.linemap == nullptr || pl.bs == nullptr)
         return 0;
     XPass2Args a;
-    if (!xpass_args(W, Nx, Ny, Nzh, Kp, kz0, alpha, neumann, K2, pl, &a)) {
+    if (!xpass_args(W, Nx, Ny, Nzh, Kp, kz0, alpha, neumann, K2, pl, nb, &a)) {
         *rc = fail(SFA_ERR_HIP, "table allocation failed");
         return 1;
     }
@@ -582,15 +589,16 @@ int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, 
 }
 
 // Mixed precision x-pass: planes [0, Kh) complex128 in W64 (pitch Kh) and
-// [Kh, Nzh) complex64 in W32 (pitch Nzh - Kh), one launch.
+// [Kh, Nzh) complex64 in W32 (pitch Nzh - Kh), one launch; nb arrays back to
+// back in each of W64 and W32.
 int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, double alpha,
-                     int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc)
+                     int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc)
 {
     if (!(Nx == 64 || Nx == 128 || Nx == 256 || Nx == 512) || pl.linemap == nullptr || pl.bs == nullptr)
         return 0;
     XPass2Args a64, a32;
-    if (!xpass_args(W64, Nx, Ny, Nzh, Kh, 0, alpha, neumann, K2, pl, &a64) ||
-        !xpass_args(W32, Nx, Ny, Nzh, Nzh - Kh, Kh, alpha, neumann, K2, pl, &a32)) {
+    if (!xpass_args(W64, Nx, Ny, Nzh, Kh, 0, alpha, neumann, K2, pl, nb, &a64) ||
+        !xpass_args(W32, Nx, Ny, Nzh, Nzh - Kh, Kh, alpha, neumann, K2, pl, nb, &a32)) {
         *rc = fail(SFA_ERR_HIP, "table allocation failed");
         return 1;
     }
@@ -603,12 +611,13 @@ int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, doub
     return 1;
 }
 
-// W: lines (ky, kz) at pitch Kp >= Nzh per ky; the Kp - Nzh padding lines
-// are transformed too (their values are never read back).
+// W: nb arrays (Nx, Ny, Kp) back to back, lines (ky, kz) at pitch Kp >= Nzh
+// per ky; the Kp - Nzh padding lines are transformed too (their values are
+// never read back).
 int fast_xpass(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int neumann,
-               const double* K2, const XPlan& pl, hipStream_t s, int* rc)
+               const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc)
 {
-    return fast_xpass_sub(W, false, Nx, Ny, Nzh, Kp, 0, alpha, neumann, K2, pl, s, rc);
+    return fast_xpass_sub(W, false, Nx, Ny, Nzh, Kp, 0, alpha, neumann, K2, pl, nb, s, rc);
 }
 
 }  // namespace sfa

@@ -231,11 +231,11 @@ static const double* cheb_k2(int N, int neumann)
 }
 
 int fast_xpass(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int neumann,
-               const double* K2, const XPlan& pl, hipStream_t s, int* rc);
+               const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc);
 int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha,
-                   int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc);
+                   int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc);
 int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, double alpha,
-                     int neumann, const double* K2, const XPlan& pl, hipStream_t s, int* rc);
+                     int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc);
 int fast_c2c(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
              hipStream_t s, int* rc);
 int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
@@ -326,9 +326,9 @@ static bool plan_ok(const void* plan, int64_t Nx, int64_t Ny, int64_t Nz, double
     return plan_view(plan, Nx, Ny, Nz, alpha, neumann, nullptr);
 }
 
-// W: (Nx, Ny, Nzh) complex lines at kz pitch Kp >= Nzh.
+// W: nb arrays (Nx, Ny, Nzh) of complex lines at kz pitch Kp >= Nzh, back to back.
 int xpass_launch(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int neumann,
-                 hipStream_t s, const void* plan = nullptr)
+                 hipStream_t s, const void* plan, int64_t nb)
 {
     if (plan != nullptr) {
         const double* K2 = cheb_k2(Nx, neumann);
@@ -337,7 +337,15 @@ int xpass_launch(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int 
         SFA_CHECK(plan_view(plan, Nx, Ny, 2 * (Nzh - 1), alpha, neumann, &pv), SFA_ERR_ARG,
                   "plan does not match the solve");
         int rc = 0;
-        if (fast_xpass(W, Nx, Ny, Nzh, Kp, alpha, neumann, K2, pv, s, &rc)) return rc;
+        if (fast_xpass(W, Nx, Ny, Nzh, Kp, alpha, neumann, K2, pv, nb, s, &rc)) return rc;
+    }
+    if (nb > 1) {  // the generic kernel takes one array per launch
+        for (int64_t b = 0; b < nb; ++b) {
+            const int rc = xpass_launch(W + b * (int64_t)Nx * Ny * Kp, Nx, Ny, Nzh, Kp, alpha, neumann,
+                                        s, nullptr, 1);
+            if (rc) return rc;
+        }
+        return SFA_OK;
     }
     XPassArgs a;
     a.W = W;
@@ -392,9 +400,11 @@ static int y_pass_mixed(const Geom& g64, const Geom& g32, double2* W64, float2* 
 // transforms, planes [Kh, Nz/2] are stored and transformed as complex64; the
 // first (R2C of the FP64 f) and last (C2R to the FP64 u) transforms and the
 // Helmholtz solve are FP64.
+// nb arrays (Nx, Ny, Nz) back to back in f and u; the workspace holds the nb
+// arrays' complex128 planes, then their complex64 planes.
 static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                              double alpha, int neumann, int Kh, const void* plan, void* work,
-                             hipStream_t s, hipEvent_t* ev)
+                             hipStream_t s, hipEvent_t* ev, int64_t nb = 1)
 {
     auto mark = [&](int i) {
         if (ev) hipEventRecord(ev[i], s);
@@ -407,14 +417,14 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
     const double* K2 = cheb_k2((int)Nx, neumann);
     SFA_CHECK(K2, SFA_ERR_HIP, "table allocation failed");
     double2* W64 = (double2*)work;
-    float2* W32 = (float2*)(W64 + Nx * Ny * Kh);
-    const int64_t rows = Nx * Ny;
+    float2* W32 = (float2*)(W64 + nb * Nx * Ny * Kh);
+    const int64_t rows = nb * Nx * Ny;
     int rc = 0;
     auto geom = [&](int Kp) {
         Geom g;
         g.n = Ny;
         g.sa = Kp;
-        g.d1 = Nx;
+        g.d1 = nb * Nx;
         g.s1 = Ny * Kp;
         g.d2 = Kp;
         g.s2 = 1;
@@ -428,7 +438,8 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
     mark(1);
     if ((rc = y_pass_mixed(g64, g32, W64, W32, -1, 1.0 / (double)Ny, s))) return rc;
     mark(2);
-    if (!fast_xpass_mixed(W64, W32, (int)Nx, (int)Ny, (int)Nzh, Kh, alpha, neumann, K2, pv, s, &rc) ||
+    if (!fast_xpass_mixed(W64, W32, (int)Nx, (int)Ny, (int)Nzh, Kh, alpha, neumann, K2, pv, nb, s,
+                          &rc) ||
         rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nx=%lld unsupported", (long long)Nx);
     mark(3);
@@ -450,10 +461,12 @@ static int64_t fp64_pitch(int64_t Nx, int64_t Nzh, const void* plan)
 }
 
 // FP64 pipeline: z R2C, y C2C, fused x-pass, y C2C inverse, z C2R.  ev: NULL
-// or 6 events recorded around the five stages.
+// or 6 events recorded around the five stages.  nb arrays (Nx, Ny, Nz) back
+// to back in f, u and (at the padded pitch) the workspace: the z and y passes
+// see one (nb Nx, Ny, .) array, the x-pass offsets each array's lines.
 static int helmholtz3d_fp64(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                             double alpha, int neumann, const void* plan, void* work, hipStream_t s,
-                            hipEvent_t* ev)
+                            hipEvent_t* ev, int64_t nb = 1)
 {
     auto mark = [&](int i) {
         if (ev) hipEventRecord(ev[i], s);
@@ -461,12 +474,12 @@ static int helmholtz3d_fp64(const double* f, double* u, int64_t Nx, int64_t Ny, 
     const int64_t Nzh = Nz / 2 + 1;
     const int64_t Kp = fp64_pitch(Nx, Nzh, plan);
     double2* W = (double2*)work;
-    const int64_t rshape[3] = {Nx, Ny, Nz};
+    const int64_t rshape[3] = {nb * Nx, Ny, Nz};
     const int64_t rstr[3] = {Ny * Nz, Nz, 1};
     const int64_t cstr[3] = {Ny * Kp, Kp, 1};
     // the y passes run over the padding lines too, so that each workgroup's
     // 16-line row segments start on 128-byte boundaries
-    const int64_t yshape[3] = {Nx, Ny, Kp};
+    const int64_t yshape[3] = {nb * Nx, Ny, Kp};
     void* st = (void*)s;
     int rc;
     mark(0);
@@ -474,7 +487,8 @@ static int helmholtz3d_fp64(const double* f, double* u, int64_t Nx, int64_t Ny, 
     mark(1);
     if ((rc = sfa_fft_c2c(W, W, 3, yshape, cstr, cstr, 1, -1, 1.0 / (double)Ny, st))) return rc;
     mark(2);
-    if ((rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, (int)Kp, alpha, neumann, s, plan))) return rc;
+    if ((rc = xpass_launch(W, (int)Nx, (int)Ny, (int)Nzh, (int)Kp, alpha, neumann, s, plan, nb)))
+        return rc;
     mark(3);
     if ((rc = sfa_fft_c2c(W, W, 3, yshape, cstr, cstr, 1, +1, 1.0, st))) return rc;
     mark(4);
@@ -546,24 +560,35 @@ extern "C" int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double a
     return SFA_OK;
 }
 
-extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
-                               double alpha, int neumann, int precision, int fp64_planes,
-                               const void* plan, void* work, int64_t work_bytes, void* stream)
+extern "C" int sfa_helmholtz3d_many(const double* f, double* u, int64_t nbatch, int64_t Nx,
+                                    int64_t Ny, int64_t Nz, double alpha, int neumann,
+                                    int precision, int fp64_planes, const void* plan, void* work,
+                                    int64_t work_bytes, void* stream)
 {
     SFA_CHECK(plan == nullptr || plan_ok(plan, Nx, Ny, Nz, alpha, neumann), SFA_ERR_ARG,
               "plan was not built by sfa_helmholtz3d_plan for these sizes/alpha/bc");
     SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
               "need Nx >= 8, Ny >= 1, even Nz >= 2");
+    SFA_CHECK(nbatch >= 1, SFA_ERR_ARG, "nbatch must be >= 1");
+    SFA_CHECK(f != nullptr && u != nullptr, SFA_ERR_ARG, "NULL f or u");
     SFA_CHECK(precision == 0 || precision == 1, SFA_ERR_ARG, "precision must be 0 or 1");
-    SFA_CHECK(work_bytes >= sfa_helmholtz3d_workspace(Nx, Ny, Nz, precision), SFA_ERR_ARG,
+    SFA_CHECK(work_bytes / nbatch >= sfa_helmholtz3d_workspace(Nx, Ny, Nz, precision), SFA_ERR_ARG,
               "workspace too small");
     hipStream_t s = (hipStream_t)stream;
     if (precision == 1) {
         SFA_CHECK(Nz / 2 + 1 >= 4, SFA_ERR_UNSUPPORTED, "mixed precision needs Nz >= 6");
         return helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann, mixed_planes(Nz, fp64_planes),
-                                 plan, work, s, nullptr);
+                                 plan, work, s, nullptr, nbatch);
     }
-    return helmholtz3d_fp64(f, u, Nx, Ny, Nz, alpha, neumann, plan, work, s, nullptr);
+    return helmholtz3d_fp64(f, u, Nx, Ny, Nz, alpha, neumann, plan, work, s, nullptr, nbatch);
+}
+
+extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
+                               double alpha, int neumann, int precision, int fp64_planes,
+                               const void* plan, void* work, int64_t work_bytes, void* stream)
+{
+    return sfa_helmholtz3d_many(f, u, 1, Nx, Ny, Nz, alpha, neumann, precision, fp64_planes, plan,
+                                work, work_bytes, stream);
 }
 
 // Same pipeline as sfa_helmholtz3d, run nrep times with hipEvents recorded on

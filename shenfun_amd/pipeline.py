@@ -61,6 +61,30 @@ class HelmholtzSolver3D:
                   _lib.ptr(w), w.numel(), _lib.stream_ptr())
         return u
 
+    def solve_many(self, F, U=None, work=None):
+        """Solves of a stack F (B, Nx, Ny, Nz) of right-hand sides in one pass
+        of launches (sfa_helmholtz3d_many); each U[b] is bitwise equal to
+        self(F[b]).  work: at least B * ``workspace_bytes`` bytes (allocated
+        when None)."""
+        if not (F.is_cuda and F.dtype == torch.float64 and F.is_contiguous() and F.dim() == 4
+                and tuple(F.shape[1:]) == tuple(self.shape)):
+            raise ValueError(f"F must be a contiguous float64 CUDA tensor of shape (B, {self.shape})")
+        if U is None:
+            U = torch.empty_like(F)
+        if not (U.is_contiguous() and U.shape == F.shape and U.dtype == F.dtype):
+            raise ValueError("U must match F")
+        B = F.shape[0]
+        need = B * self.work.numel()
+        if work is None:
+            work = self.work if B == 1 else torch.empty(need, dtype=torch.uint8, device=F.device)
+        if work.numel() < need or not work.is_cuda:
+            raise ValueError(f"work must hold at least {need} bytes on the device")
+        Nx, Ny, Nz = self.shape
+        _lib.call("sfa_helmholtz3d_many", _lib.ptr(F), _lib.ptr(U), B, Nx, Ny, Nz, self.alpha,
+                  int(self.neumann), self.precision, self.fp64_planes, _lib.ptr(self.plan),
+                  _lib.ptr(work), work.numel(), _lib.stream_ptr())
+        return U
+
     @property
     def workspace_bytes(self):
         return self.work.numel()

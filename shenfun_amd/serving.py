@@ -15,9 +15,12 @@ device memory free up, so work of several requests overlaps on the device:
   chebyshev.la.Helmholtz (chebyshev/la.py:288-317) -> backward (:315-338).
 * 2D Legendre x Fourier requests of any sizes go together into one launch of
   the batched persistent kernel (sfa_batch2d_plan_ext + sfa_batch2d_solve)
-  over a resident cache of Legendre/DFT matrices.
-* 3D Chebyshev x Fourier^2 requests run one per launch through the fused
-  solver (sfa_helmholtz3d) with a per-slot workspace.
+  over a resident cache of Legendre/DFT matrices, or of its tile-parallel
+  form (sfa_batch2d_wide_*) when one problem dominates the batch.
+* 3D Chebyshev x Fourier^2 requests of one class (shape, bc, alpha) go
+  through the fused solver with a per-slot workspace: alone as they are
+  (sfa_helmholtz3d), or up to max_batch_3d of them stacked into one pass of
+  launches (sfa_helmholtz3d_many) when several wait.
 
 Factorisations (1D LU factors, Legendre/DFT matrices, 3D plans) live in an
 LRU cache keyed by (basis, sizes, bc, alpha) with a byte budget.  Admission
@@ -111,7 +114,7 @@ def _largest_prime_factor(n):
 
 class Server:
     def __init__(self, slots=4, mem_budget=8 << 30, cache_budget=16 << 30, max_batch_1d=256,
-                 max_batch_2d=1024, device="cuda"):
+                 max_batch_2d=1024, max_batch_3d=8, device="cuda"):
         self.device = torch.device(device)
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(int(slots))]
         self.free_slots = list(range(int(slots)))
@@ -120,6 +123,7 @@ class Server:
         self.cache = FactorCache(cache_budget)
         self.max_batch_1d = int(max_batch_1d)
         self.max_batch_2d = int(max_batch_2d)
+        self.max_batch_3d = int(max_batch_3d)
         self.cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         self.inflight = []  # (event, slot, requests, bytes, cache keys, keepalive)
         # pending requests: arrival order (admitted entries dropped lazily from
@@ -127,6 +131,7 @@ class Server:
         self.pending = collections.deque()
         self._q1d = {}                    # (N, bc, alpha) -> deque of 1D requests
         self._q2d = collections.deque()   # 2D requests
+        self._q3d = {}                    # (shape, bc, alpha) -> deque of 3D requests
         self._npending = 0
         self.done = []
         self.batches = collections.Counter()
@@ -269,18 +274,30 @@ class Server:
             r.u = U[int(off[i]):int(off[i] + sizes[i])].view(int(Nx[i]), int(Ny[i]))
         return (F, U, plan, work, mats), keys
 
-    def _launch_cheb3d(self, req, stream, slot):
-        H = self._cheb3d_solver(req.key)
+    def _launch_cheb3d(self, reqs, stream, slot):
+        """One request: the fused solver on its own f.  Several requests of one
+        class: their right-hand sides stacked and solved in one pass of
+        launches (sfa_helmholtz3d_many); the results are bitwise those of the
+        single solves."""
+        H = self._cheb3d_solver(reqs[0].key)
+        B = len(reqs)
         w = self._work3d.get(slot)
-        if w is None or w.numel() < H.workspace_bytes:
+        if w is None or w.numel() < B * H.workspace_bytes:
             with torch.cuda.stream(stream):
-                w = torch.empty(H.workspace_bytes, dtype=torch.uint8, device=self.device)
+                w = torch.empty(B * H.workspace_bytes, dtype=torch.uint8, device=self.device)
             self._work3d[slot] = w
         with torch.cuda.stream(stream):
-            u = torch.empty_like(req.f)
-            H(req.f, u, work=w)
-        req.u = u
-        return (u,), [("cheb3d",) + req.key]
+            if B == 1:
+                F = reqs[0].f
+                U = torch.empty_like(F)
+                H(F, U, work=w)
+                reqs[0].u = U
+            else:
+                F = torch.stack([r.f for r in reqs])
+                U = H.solve_many(F, work=w)
+                for i, r in enumerate(reqs):
+                    r.u = U[i]
+        return (F, U), [("cheb3d",) + reqs[0].key]
 
     # ------------------------------------------------------------ admission
     @staticmethod
@@ -293,7 +310,8 @@ class Server:
             mx = max((r.key[0] * 2) * (r.key[1] + 2) for r in reqs) * 8
             return 2 * n + mx * min(len(reqs), 256)
         shp = reqs[0].key[0]
-        return n + shp[0] * shp[1] * (shp[2] // 2 + 1) * 16
+        ws = shp[0] * shp[1] * (shp[2] // 2 + 8) * 16  # workspace per solve (padded kz pitch)
+        return (n if len(reqs) == 1 else 2 * n) + len(reqs) * ws
 
     def _push(self, r):
         self.pending.append(r)
@@ -301,10 +319,18 @@ class Server:
             self._q1d.setdefault(r.key, collections.deque()).append(r)
         elif r.family == "leg2d":
             self._q2d.append(r)
+        else:
+            self._q3d.setdefault(r.key, collections.deque()).append(r)
         self._npending += 1
 
     def _queue(self, head):
-        return self._q1d[head.key] if head.family == "cheb1d" else self._q2d
+        if head.family == "cheb1d":
+            return self._q1d[head.key]
+        return self._q2d if head.family == "leg2d" else self._q3d[head.key]
+
+    def _cap(self, family):
+        return {"cheb1d": self.max_batch_1d, "leg2d": self.max_batch_2d,
+                "cheb3d": self.max_batch_3d}[family]
 
     def _next_batch(self):
         """Oldest pending request and the requests that can join its launch
@@ -314,19 +340,17 @@ class Server:
         if not self.pending:
             return None
         head = self.pending[0]
-        if head.family == "cheb3d":
-            return [head]
-        cap = self.max_batch_1d if head.family == "cheb1d" else self.max_batch_2d
-        return list(itertools.islice(self._queue(head), cap))
+        return list(itertools.islice(self._queue(head), self._cap(head.family)))
 
     def _take(self, batch, t):
         """Remove an admitted batch (from _next_batch) from the queues."""
-        if batch[0].family != "cheb3d":
-            q = self._queue(batch[0])
-            for _ in batch:
-                q.popleft()
-            if batch[0].family == "cheb1d" and not q:
-                del self._q1d[batch[0].key]
+        q = self._queue(batch[0])
+        for _ in batch:
+            q.popleft()
+        if not q and batch[0].family == "cheb1d":
+            del self._q1d[batch[0].key]
+        elif not q and batch[0].family == "cheb3d":
+            del self._q3d[batch[0].key]
         self._npending -= len(batch)
         for r in batch:
             r.t_admit = t
@@ -363,7 +387,7 @@ class Server:
             elif fam == "leg2d":
                 keep, keys = self._launch_leg2d(batch, stream)
             else:
-                keep, keys = self._launch_cheb3d(batch[0], stream, slot)
+                keep, keys = self._launch_cheb3d(batch, stream, slot)
             self.cache.acquire(keys)
             ev = torch.cuda.Event()
             ev.record(stream)
