@@ -229,50 +229,60 @@ class Server:
         Ny = np.array([r.key[1] for r in reqs], dtype=np.int32)
         bc = np.array([_bcc(r.key[2]) for r in reqs], dtype=np.int32)
         al = np.array([r.key[3] for r in reqs], dtype=np.float64)
-        keys = []
-        mats = []
-        for i in range(n):
-            mats.append(self._leg_matrices(int(Nx[i]), int(bc[i])))
-            mats.append(self._dft_matrices(int(Ny[i])))
-            keys += [("leg", int(Nx[i]), int(bc[i])), ("dft", int(Ny[i]))]
+        # the batch's distinct matrices, looked up once each
+        lk = list(dict.fromkeys(zip(Nx.tolist(), bc.tolist())))
+        dk = list(dict.fromkeys(Ny.tolist()))
+        legm = [self._leg_matrices(a, b) for a, b in lk]
+        dftm = [self._dft_matrices(y) for y in dk]
+        keys = [("leg", a, b) for a, b in lk] + [("dft", y) for y in dk]
+        mats = legm + dftm
         # one arena view: offsets relative to the lowest matrix address
         base = min(m.data_ptr() for m in mats)
         top = max(m.data_ptr() + m.numel() * 8 for m in mats)
-        phi = np.array([(mats[2 * i].data_ptr() - base) // 8 for i in range(n)], dtype=np.int64)
-        dft = np.array([(mats[2 * i + 1].data_ptr() - base) // 8 for i in range(n)], dtype=np.int64)
+        lo = {k: (m.data_ptr() - base) // 8 for k, m in zip(lk, legm)}
+        do = {k: (m.data_ptr() - base) // 8 for k, m in zip(dk, dftm)}
+        phi = np.array([lo[k] for k in zip(Nx.tolist(), bc.tolist())], dtype=np.int64)
+        dft = np.array([do[k] for k in Ny.tolist()], dtype=np.int64)
         sizes = Nx.astype(np.int64) * Ny.astype(np.int64)
         off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
         total = int(sizes.sum())
+        # the requests' f read in place: element offsets from the lowest address
+        fs = [r.f if r.f.is_contiguous() and r.f.dtype == torch.float64
+              else r.f.to(torch.float64).contiguous() for r in reqs]
+        fp = [f.data_ptr() for f in fs]
+        fbase = min(fp)
+        foff = np.array([(a - fbase) // 8 for a in fp], dtype=np.int64)
+        fel = int(max(a + 8 * int(z) for a, z in zip(fp, sizes)) - fbase) // 8
         nb = min(n, self.cus)
         ptr = lambda a: _lib.ctypes.c_void_p(a.ctypes.data)
         wide = self._wide_2d(Nx, Ny)
         with torch.cuda.stream(stream):
-            F = torch.cat([r.f.reshape(-1) for r in reqs])
-            U = torch.empty_like(F)
+            F = _lib.ctypes.c_void_p(fbase)
+            U = torch.empty(total, dtype=torch.float64, device=self.device)
             sp = _lib.stream_ptr(stream)
             if wide:
                 plan = torch.empty(int(L.sfa_batch2d_wide_plan_bytes(n, ptr(Nx), ptr(Ny))),
                                    dtype=torch.uint8, device=self.device)
                 work = torch.empty(int(L.sfa_batch2d_wide_workspace(n, ptr(Nx), ptr(Ny))),
                                    dtype=torch.uint8, device=self.device)
-                _lib.call("sfa_batch2d_wide_plan", n, ptr(Nx), ptr(Ny), ptr(al), ptr(bc), ptr(off),
-                          ptr(off), total, total, _lib.ctypes.c_void_p(base), ptr(phi), ptr(dft),
+                _lib.call("sfa_batch2d_wide_plan", n, ptr(Nx), ptr(Ny), ptr(al), ptr(bc), ptr(foff),
+                          ptr(off), fel, total, _lib.ctypes.c_void_p(base), ptr(phi), ptr(dft),
                           (top - base) // 8, _lib.ptr(plan), plan.numel(), sp)
-                _lib.call("sfa_batch2d_wide_solve", _lib.ptr(F), _lib.ptr(U), _lib.ptr(plan),
+                _lib.call("sfa_batch2d_wide_solve", F, _lib.ptr(U), _lib.ptr(plan),
                           _lib.ptr(work), work.numel(), sp)
             else:
                 plan = torch.empty(int(L.sfa_batch2d_plan_ext_bytes(n, nb)), dtype=torch.uint8,
                                    device=self.device)
                 work = torch.empty(int(L.sfa_batch2d_workspace(n, ptr(Nx), ptr(Ny), nb)),
                                    dtype=torch.uint8, device=self.device)
-                _lib.call("sfa_batch2d_plan_ext", n, ptr(Nx), ptr(Ny), ptr(al), ptr(bc), ptr(off),
-                          ptr(off), total, total, _lib.ctypes.c_void_p(base), ptr(phi), ptr(dft),
+                _lib.call("sfa_batch2d_plan_ext", n, ptr(Nx), ptr(Ny), ptr(al), ptr(bc), ptr(foff),
+                          ptr(off), fel, total, _lib.ctypes.c_void_p(base), ptr(phi), ptr(dft),
                           (top - base) // 8, nb, _lib.ptr(plan), plan.numel(), sp)
-                _lib.call("sfa_batch2d_solve", _lib.ptr(F), _lib.ptr(U), _lib.ptr(plan),
+                _lib.call("sfa_batch2d_solve", F, _lib.ptr(U), _lib.ptr(plan),
                           _lib.ptr(work), work.numel(), 0, sp)
-        for i, r in enumerate(reqs):
-            r.u = U[int(off[i]):int(off[i] + sizes[i])].view(int(Nx[i]), int(Ny[i]))
-        return (F, U, plan, work, mats), keys
+        for r, part in zip(reqs, U.split(sizes.tolist())):
+            r.u = part.view(r.key[0], r.key[1])
+        return (fs, U, plan, work, mats), keys
 
     def _launch_cheb3d(self, reqs, stream, slot):
         """One request: the fused solver on its own f.  Several requests of one
