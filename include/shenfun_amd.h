@@ -81,6 +81,19 @@ int sfa_cheb_backward(const void* in, void* out, int dtype, int ndim,
                       const int64_t* out_strides, int axis, const double* K2,
                       void* stream);
 
+/* Chebyshev-Gauss-Lobatto variants (x_j = cos(pi j/(N-1))): scalar product
+ * (T_k, f)_w = REDFT00(f)_k pi/(2(N-1)) (chebyshev/bases.py:349-353) with the
+ * composite stencil fused as above, and backward u_j = REDFT00(c)_j / 2 + c_0/2
+ * + (-1)^j c_{N-1}/2 (chebyshev/bases.py:331-338) with to_ortho fused.  DCT-I
+ * of length N runs as the complex FFT of the even extension (length 2(N-1)).
+ * Same arguments as sfa_cheb_scalar_product / sfa_cheb_backward; N >= 2. */
+int sfa_cheb_scalar_product_gl(const void* in, void* out, int dtype, int ndim, const int64_t* shape,
+                               const int64_t* in_strides, const int64_t* out_strides, int axis,
+                               const double* K2, void* stream);
+int sfa_cheb_backward_gl(const void* in, void* out, int dtype, int ndim, const int64_t* shape,
+                         const int64_t* in_strides, const int64_t* out_strides, int axis,
+                         const double* K2, void* stream);
+
 /* Build the dense Legendre transform matrix on the device (the plan step of
  * the recursive transforms, spectralbase.py:484-491, 550-558).
  * analysis=1: M[k][j] = w_j (L_k(x_j) + K2[k] L_{k+2}(x_j))  (k < N-2 when

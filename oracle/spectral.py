@@ -8,7 +8,9 @@ the reference's ``axis`` arguments.
 Conventions restated (Appendix of the reference's own docs and code):
 
 * Chebyshev Gauss (GC) nodes x_j = cos(pi(2j+1)/(2N)), descending, w_j = pi/N
-  (chebyshev/bases.py:213-216 via numpy.polynomial.chebyshev.chebgauss).
+  (chebyshev/bases.py:213-216 via numpy.polynomial.chebyshev.chebgauss);
+  Gauss-Lobatto (GL) nodes x_j = cos(pi j/(N-1)), descending, w_j = pi/(N-1)
+  with both ends halved (chebyshev/bases.py:202-207).
 * Legendre Gauss (LG) nodes ascending, Gauss weights
   (legendre/bases.py:121-126, legendre/fastgl/fastgl_wrap.pyx:22-29).
 * Fourier nodes 2 pi j / N, scalar product = FFT / N, backward = unnormalised
@@ -26,13 +28,19 @@ PI = np.pi
 # --------------------------------------------------------------------------
 
 
-def cheb_points_weights(N):
-    """Chebyshev-Gauss points (descending) and weights.
+def cheb_points_weights(N, quad='GC'):
+    """Chebyshev-Gauss (GC) or Gauss-Lobatto (GL) points (descending) and weights.
 
     chebyshev/bases.py:213-216 (``n_cheb.chebgauss``): x_j = cos(pi(2j+1)/(2N)),
-    w_j = pi/N.
+    w_j = pi/N; chebyshev/bases.py:202-207 (GL, ``-chebpts2``): x_j = cos(pi j/(N-1)),
+    w_j = pi/(N-1), halved at both ends.
     """
     j = np.arange(N, dtype=float)
+    if quad == 'GL':
+        w = np.full(N, PI / (N - 1))
+        w[0] /= 2
+        w[-1] /= 2
+        return np.cos(PI * j / (N - 1)), w
     x = np.cos(PI * (2.0 * j + 1.0) / (2.0 * N))
     w = np.full(N, PI / N)
     return x, w
@@ -125,27 +133,38 @@ def _take(a, sl, axis):
 def _dct(a, typ, axis):
     # chebyshev/bases.py:100-115 (DCTWrap): complex data -> DCT of real and
     # imaginary parts separately.  scipy.fft.dct(norm=None) has the same
-    # unnormalised definition as FFTW's REDFT10 (type 2) / REDFT01 (type 3).
+    # unnormalised definition as FFTW's REDFT10 (type 2) / REDFT01 (type 3) /
+    # REDFT00 (type 1).
     if np.iscomplexobj(a):
         return (scipy.fft.dct(a.real, type=typ, axis=axis)
                 + 1j * scipy.fft.dct(a.imag, type=typ, axis=axis))
     return scipy.fft.dct(a, type=typ, axis=axis)
 
 
-def cheb_scalar_product(f, axis=0):
-    """(T_k, f)_w with GC quadrature.
+def cheb_scalar_product(f, axis=0, quad='GC'):
+    """(T_k, f)_w with GC or GL quadrature.
 
-    chebyshev/bases.py:340-347: out = REDFT10(f) * pi/(2N).
+    chebyshev/bases.py:344-347: GC out = REDFT10(f) * pi/(2N);
+    chebyshev/bases.py:349-353: GL out = REDFT00(f) * pi/(2(N-1)).
     """
     N = f.shape[axis]
+    if quad == 'GL':
+        return _dct(f, 1, axis) * (PI / (2 * (N - 1)))
     return _dct(f, 2, axis) * (PI / (2 * N))
 
 
-def cheb_backward(c, axis=0):
-    """u_j = sum_k c_k T_k(x_j) at GC nodes.
+def cheb_backward(c, axis=0, quad='GC'):
+    """u_j = sum_k c_k T_k(x_j) at GC or GL nodes.
 
-    chebyshev/bases.py:326-329: out = 0.5*REDFT01(c) + c_0/2.
+    chebyshev/bases.py:326-329: GC out = 0.5*REDFT01(c) + c_0/2;
+    chebyshev/bases.py:331-338: GL out = 0.5*REDFT00(c) + c_0/2 + (-1)^j c_{N-1}/2.
     """
+    if quad == 'GL':
+        N = c.shape[axis]
+        out = 0.5 * _dct(c, 1, axis)
+        out += c[_take(c, slice(0, 1), axis)] / 2
+        sgn = _bcast(np.where(np.arange(N) % 2 == 0, 0.5, -0.5), c.ndim, axis)
+        return out + sgn * c[_take(c, slice(N - 1, N), axis)]
     out = 0.5 * _dct(c, 3, axis)
     out += c[_take(c, slice(0, 1), axis)] / 2
     return out
@@ -273,11 +292,12 @@ def to_ortho(u, K2, axis=0):
 # --------------------------------------------------------------------------
 
 
-def basis_scalar_product(family, bc, f, axis=0):
+def basis_scalar_product(family, bc, f, axis=0, quad='GC'):
     """SpectralBase.scalar_product (spectralbase.py:200-233) for one axis.
 
     family in {'chebyshev', 'legendre', 'fourier-r2c', 'fourier-c2c'};
-    bc in {None, 'D', 'N'} (None = orthogonal basis).
+    bc in {None, 'D', 'N'} (None = orthogonal basis); quad 'GC' or 'GL'
+    (Chebyshev only).
     """
     N = f.shape[axis]
     if family == 'fourier-r2c':
@@ -285,7 +305,7 @@ def basis_scalar_product(family, bc, f, axis=0):
     if family == 'fourier-c2c':
         return np.fft.fft(f, axis=axis) / N
     if family == 'chebyshev':
-        t = cheb_scalar_product(f, axis)
+        t = cheb_scalar_product(f, axis, quad)
     else:
         t = leg_scalar_product(f, axis)
     if bc is None:
@@ -293,7 +313,7 @@ def basis_scalar_product(family, bc, f, axis=0):
     return composite_scalar_product(t, stencil_k2(family, bc, N), axis)
 
 
-def basis_backward(family, bc, c, axis=0, N=None):
+def basis_backward(family, bc, c, axis=0, N=None, quad='GC'):
     """SpectralBase.backward (spectralbase.py:265-313) for one axis."""
     if family == 'fourier-r2c':
         n = N if N is not None else 2 * (c.shape[axis] - 1)
@@ -306,7 +326,7 @@ def basis_backward(family, bc, c, axis=0, N=None):
     if bc is not None:
         c = to_ortho(c, stencil_k2(family, bc, Nc), axis)
     if family == 'chebyshev':
-        return cheb_backward(c, axis)
+        return cheb_backward(c, axis, quad)
     return leg_backward(c, axis)
 
 
@@ -337,10 +357,13 @@ def tps_backward(c, bases, shape):
 # --------------------------------------------------------------------------
 
 
-def cheb_norm_sq(N):
-    """chebyshev/bases.py:273-284 (l2_norm_sq, GC): pi/2, first entry pi."""
+def cheb_norm_sq(N, quad='GC'):
+    """chebyshev/bases.py:271-277 (l2_norm_sq): pi/2, first entry pi, and with
+    GL the last entry pi (matrices assemble by quadrature, matrixbase.py:635-636)."""
     h = np.full(N, PI / 2)
     h[0] = PI
+    if quad == 'GL':
+        h[-1] = PI
     return h
 
 
@@ -350,9 +373,9 @@ def leg_norm_sq(N):
     return 2.0 / (2 * k + 1)
 
 
-def cheb_B_SD(N):
+def cheb_B_SD(N, quad='GC'):
     """chebyshev/matrices.py:221-249 BSDSDmat: {-2: -h1, 0: h[:-2]+h[2:], 2: -h1}."""
-    h = cheb_norm_sq(N)
+    h = cheb_norm_sq(N, quad)
     return {-2: np.full(N - 4, -h[1]), 0: h[:-2] + h[2:], 2: np.full(N - 4, -h[1])}
 
 
@@ -370,9 +393,9 @@ def cheb_A_SD(N, nbands=None):
     return d
 
 
-def cheb_B_SN(N):
+def cheb_B_SN(N, quad='GC'):
     """chebyshev/matrices.py:377-397 BSNSNmat."""
-    h = cheb_norm_sq(N)
+    h = cheb_norm_sq(N, quad)
     k = np.arange(N, dtype=float)
     alpha = (k / (k + 2)) ** 2
     M = N - 2
@@ -566,7 +589,7 @@ def diagma_solve(u, d, axis=0):
 # --------------------------------------------------------------------------
 
 
-def helmholtz_lu(family_bc, N, alfa, beta):
+def helmholtz_lu(family_bc, N, alfa, beta, quad='GC'):
     """LU_Helmholtz_1D (la.pyx:717-763) vectorised over modes.
 
     family_bc: 'D' (ASDSD/BSDSD) or 'N' (ASNSN/BSNSN) Chebyshev bases of size
@@ -577,9 +600,9 @@ def helmholtz_lu(family_bc, N, alfa, beta):
     """
     neumann = family_bc == 'N'
     if neumann:
-        A, B = cheb_A_SN(N), cheb_B_SN(N)
+        A, B = cheb_A_SN(N), cheb_B_SN(N, quad)
     else:
-        A, B = cheb_A_SD(N, nbands=2), cheb_B_SD(N)
+        A, B = cheb_A_SD(N, nbands=2), cheb_B_SD(N, quad)
     M = N - 2
     alfa = np.asarray(alfa, dtype=float)
     beta = np.asarray(beta, dtype=float)

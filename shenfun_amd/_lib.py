@@ -27,6 +27,8 @@ _SIGS = {
     "sfa_fft_c2r": "vviaaaidv",
     "sfa_cheb_scalar_product": "vviiaaaivv",
     "sfa_cheb_backward": "vviiaaaivv",
+    "sfa_cheb_scalar_product_gl": "vviiaaaivv",
+    "sfa_cheb_backward_gl": "vviiaaaivv",
     "sfa_legendre_matrix": "vvllvivv",
     "sfa_line_matmul": "vviiaaailvv",
     "sfa_tdma_lu": "vllv",

@@ -202,20 +202,32 @@ class _PolyBase(SpectralBase):
 
 
 class ChebyshevBase(_PolyBase):
-    """Chebyshev-Gauss space; fast transforms are the LDS DCT kernels."""
+    """Chebyshev space on Gauss (GC, default) or Gauss-Lobatto (GL) points; fast
+    transforms are the LDS DCT-II/III (GC) or DCT-I (GL) kernels."""
 
     def family(self):
         return "chebyshev"
 
     def points_and_weights(self, N=None):
-        """chebyshev/bases.py:213-216: x_j = cos(pi(2j+1)/(2N)) (descending), w = pi/N."""
+        """chebyshev/bases.py:202-216.  GC: x_j = cos(pi(2j+1)/(2N)), w = pi/N;
+        GL: x_j = cos(pi j/(N-1)), w = pi/(N-1) with both ends halved (both
+        descending)."""
         N = self.N if N is None else N
         j = np.arange(N, dtype=float)
+        if self.quad == "GL":
+            w = np.full(N, PI / (N - 1))
+            w[0] /= 2
+            w[-1] /= 2
+            return np.cos(PI * j / (N - 1)), w
         return np.cos(PI * (2 * j + 1) / (2 * N)), np.full(N, PI / N)
 
     def norm_sq(self):
+        """Quadrature norms (T_k, T_k)_w, chebyshev/bases.py:271-277: pi/2,
+        pi for k = 0 and, with GL, for k = N-1."""
         h = np.full(self.N, PI / 2)
         h[0] = PI
+        if self.quad == "GL":
+            h[-1] = PI
         return h
 
     def _vander(self):
@@ -249,7 +261,8 @@ class ChebyshevBase(_PolyBase):
         if kind in ("fast", "hip"):
             shp, sin = _geom(inp)
             _, sout = _geom(out)
-            _lib.call("sfa_cheb_scalar_product", _lib.ptr(inp), _lib.ptr(out), self._dtype_code(inp),
+            fn = "sfa_cheb_scalar_product_gl" if self.quad == "GL" else "sfa_cheb_scalar_product"
+            _lib.call(fn, _lib.ptr(inp), _lib.ptr(out), self._dtype_code(inp),
                       inp.dim(), shp, sin, sout, self.axis, _lib.ptr(self._K2()), _lib.stream_ptr())
         else:
             if out.data_ptr() == inp.data_ptr():
@@ -260,7 +273,8 @@ class ChebyshevBase(_PolyBase):
         if kind in ("fast", "hip"):
             shp, sin = _geom(inp)
             _, sout = _geom(out)
-            _lib.call("sfa_cheb_backward", _lib.ptr(inp), _lib.ptr(out), self._dtype_code(inp),
+            fn = "sfa_cheb_backward_gl" if self.quad == "GL" else "sfa_cheb_backward"
+            _lib.call(fn, _lib.ptr(inp), _lib.ptr(out), self._dtype_code(inp),
                       inp.dim(), shp, sin, sout, self.axis, _lib.ptr(self._K2()), _lib.stream_ptr())
         else:
             self._matmul(inp, out, self._synthesis_matrix())
@@ -513,10 +527,10 @@ def FunctionSpace(N, family="Fourier", bc=None, dtype="d", quad=None, domain=Non
         return R2C(N, device=device)
     kind = _bc_kind(bc)
     if fam in ("chebyshev", "c"):
-        if quad not in (None, "GC"):
-            raise NotImplementedError("Chebyshev-Gauss (GC) quadrature only")
+        if quad not in (None, "GC", "GL"):
+            raise NotImplementedError("Chebyshev-Gauss (GC) and Gauss-Lobatto (GL) quadratures only")
         cls = {None: ChebyshevOrthogonal, "D": ChebyshevShenDirichlet, "N": ChebyshevShenNeumann}[kind]
-        return cls(N, quad="GC", device=device)
+        return cls(N, quad=quad or "GC", device=device)
     if fam in ("legendre", "l"):
         if quad not in (None, "LG"):
             raise NotImplementedError("Legendre-Gauss (LG) quadrature only")

@@ -13,7 +13,11 @@
 
 namespace sfa {
 
-enum LineOp { OP_C2C = 0, OP_R2C = 1, OP_C2R = 2, OP_DCT2 = 3, OP_DCT3 = 4 };
+// OP_DCT1F / OP_DCT1B: Chebyshev-Gauss-Lobatto scalar product / backward via
+// DCT-I (REDFT00) of length n, computed as the complex FFT of the line's even
+// extension (length 2(n-1)): the sine terms cancel, so the first n outputs
+// are the DCT-I of the real and imaginary parts at once.
+enum LineOp { OP_C2C = 0, OP_R2C = 1, OP_C2R = 2, OP_DCT2 = 3, OP_DCT3 = 4, OP_DCT1F = 5, OP_DCT1B = 6 };
 
 struct LineArgs {
     const char* in;
@@ -75,7 +79,7 @@ __global__ void __launch_bounds__(256) line_kernel(LineArgs a)
     const int ldn = a.ldn;
     double2* A = smem;
     double2* B = smem + (size_t)T * ldn;
-    double* t0s = reinterpret_cast<double*>(smem + (size_t)2 * T * ldn);  // 2*T doubles
+    double* t0s = reinterpret_cast<double*>(smem + (size_t)2 * T * ldn);  // 4*T doubles
     const char* inb = a.in;
     char* outb = a.out;
     const int64_t ibase = i1 * a.in_s1 + t0 * a.in_s2;
@@ -162,6 +166,47 @@ __global__ void __launch_bounds__(256) line_kernel(LineArgs a)
             const double2 z = make_double2(ck.x + cn.y, ck.y - cn.x);
             A[t * ldn + k] = cmul(cconj(a.tw2[k]), z);
         }
+    } else if (OP == OP_DCT1F) {
+        const int m2 = 2 * (n - 1);
+        for (int e = tid; e < Tv * n; e += nth) {
+            int t, i;
+            tile_index(e, Tv, n, a.t_fast_in, t, i);
+            const double2 v = ld_elem(inb, ibase + t * a.in_s2 + i * a.in_sa, a.in_cplx);
+            A[t * ldn + i] = v;
+            if (i > 0 && i < n - 1) A[t * ldn + m2 - i] = v;
+        }
+    } else if (OP == OP_DCT1B) {
+        const int m2 = 2 * (n - 1);
+        for (int e = tid; e < Tv * n; e += nth) {
+            int t, i;
+            tile_index(e, Tv, n, a.t_fast_in, t, i);
+            B[t * ldn + i] = ld_elem(inb, ibase + t * a.in_s2 + i * a.in_sa, a.in_cplx);
+        }
+        __syncthreads();
+        // to_ortho (composite) t_k = u_k (k < n-2) + K2_{k-2} u_{k-2}, into the even extension
+        for (int e = tid; e < Tv * n; e += nth) {
+            const int t = e / n, k = e - t * n;
+            double2 v = B[t * ldn + k];
+            if (a.K2 != nullptr) {
+                if (k >= n - 2) v = make_double2(0.0, 0.0);
+                if (k >= 2) {
+                    const double kk = a.K2[k - 2];
+                    const double2 u = B[t * ldn + k - 2];
+                    v.x += kk * u.x;
+                    v.y += kk * u.y;
+                }
+            }
+            A[t * ldn + k] = v;
+            if (k > 0 && k < n - 1) A[t * ldn + m2 - k] = v;
+            if (k == 0) {
+                t0s[4 * t] = v.x;
+                t0s[4 * t + 1] = v.y;
+            }
+            if (k == n - 1) {
+                t0s[4 * t + 2] = v.x;
+                t0s[4 * t + 3] = v.y;
+            }
+        }
     }
     __syncthreads();
 
@@ -169,7 +214,7 @@ __global__ void __launch_bounds__(256) line_kernel(LineArgs a)
     double2* F;
     if (OP == OP_C2C)
         F = run_fft<INV>(A, B, a.plan, ldn, Tv, a.tw, tid, nth);
-    else if (OP == OP_R2C || OP == OP_DCT2)
+    else if (OP == OP_R2C || OP == OP_DCT2 || OP == OP_DCT1F || OP == OP_DCT1B)
         F = run_fft<false>(A, B, a.plan, ldn, Tv, a.tw, tid, nth);
     else
         F = run_fft<true>(A, B, a.plan, ldn, Tv, a.tw, tid, nth);
@@ -230,6 +275,32 @@ __global__ void __launch_bounds__(256) line_kernel(LineArgs a)
             }
             st_elem(outb, obase + t * a.out_s2 + k * a.out_sa, a.out_cplx, v);
         }
+    } else if (OP == OP_DCT1F) {
+        // s_k = sc Y_k, composite: s_k += K2_k s_{k+2} (k < n-2)
+        for (int e = tid; e < Tv * n; e += nth) {
+            int t, k;
+            tile_index(e, Tv, n, a.t_fast_out, t, k);
+            double2 v = cscale(F[t * ldn + k], sc);
+            if (a.K2 != nullptr && k < n - 2) {
+                const double kk = a.K2[k] * sc;
+                const double2 u = F[t * ldn + k + 2];
+                v.x += kk * u.x;
+                v.y += kk * u.y;
+            }
+            st_elem(outb, obase + t * a.out_s2 + k * a.out_sa, a.out_cplx, v);
+        }
+    } else if (OP == OP_DCT1B) {
+        // u_j = 0.5 Y_j + 0.5 t_0 + 0.5 (-1)^j t_{n-1}
+        for (int e = tid; e < Tv * n; e += nth) {
+            int t, j;
+            tile_index(e, Tv, n, a.t_fast_out, t, j);
+            const double2 y = F[t * ldn + j];
+            const double sg = (j & 1) ? -0.5 : 0.5;
+            const double2 v = make_double2(
+                (0.5 * y.x + 0.5 * t0s[4 * t] + sg * t0s[4 * t + 2]) * sc,
+                (0.5 * y.y + 0.5 * t0s[4 * t + 1] + sg * t0s[4 * t + 3]) * sc);
+            st_elem(outb, obase + t * a.out_s2 + j * a.out_sa, a.out_cplx, v);
+        }
     } else if (OP == OP_DCT3) {
         for (int e = tid; e < Tv * n; e += nth) {
             int t, i;
@@ -257,7 +328,7 @@ static int choose_T(int ldn, int64_t d2)
 static int launch_line(int op, bool inv, LineArgs& a, hipStream_t stream)
 {
     const int ldn = a.ldn;
-    const size_t lds = (size_t)2 * a.T * ldn * sizeof(double2) + (size_t)2 * a.T * sizeof(double);
+    const size_t lds = (size_t)2 * a.T * ldn * sizeof(double2) + (size_t)4 * a.T * sizeof(double);
     SFA_CHECK(lds <= 160 * 1024, SFA_ERR_UNSUPPORTED, "transform length %d too large for LDS", a.n);
     const int64_t ntile = (a.d2 + a.T - 1) / a.T;
     const int64_t nblk = ntile * a.d1;
@@ -280,6 +351,8 @@ static int launch_line(int op, bool inv, LineArgs& a, hipStream_t stream)
         case OP_C2R: SFA_LK(OP_C2R, true); break;
         case OP_DCT2: SFA_LK(OP_DCT2, false); break;
         case OP_DCT3: SFA_LK(OP_DCT3, true); break;
+        case OP_DCT1F: SFA_LK(OP_DCT1F, false); break;
+        case OP_DCT1B: SFA_LK(OP_DCT1B, false); break;
     }
 #undef SFA_LK
     SFA_LAUNCH_CHECK();
@@ -425,4 +498,45 @@ extern "C" int sfa_cheb_backward(const void* in, void* out, int dtype, int ndim,
 {
     return cheb_common(OP_DCT3, in, out, dtype, ndim, shape, in_strides, out_strides, axis, K2, 1.0,
                        stream);
+}
+
+// Gauss-Lobatto: DCT-I of length n through the FFT of the even extension.
+static int cheb_gl_common(int op, const void* in, void* out, int dtype, int ndim,
+                          const int64_t* shape, const int64_t* in_strides,
+                          const int64_t* out_strides, int axis, const double* K2, double scale,
+                          void* stream)
+{
+    Geom gi, go;
+    int rc;
+    SFA_CHECK(dtype == SFA_F64 || dtype == SFA_C128, SFA_ERR_ARG, "bad dtype %d", dtype);
+    if ((rc = make_geom(ndim, shape, in_strides, axis, &gi))) return rc;
+    if ((rc = make_geom(ndim, shape, out_strides, axis, &go))) return rc;
+    const int n = (int)gi.n;
+    SFA_CHECK(n >= 2, SFA_ERR_ARG, "Gauss-Lobatto needs N >= 2");
+    SFA_CHECK(n >= 3 || K2 == nullptr, SFA_ERR_ARG, "composite basis needs N >= 3");
+    LineArgs a;
+    const int c = dtype == SFA_C128;
+    if ((rc = setup_line(a, in, out, gi, go, n, 2 * (n - 1), c, c))) return rc;
+    a.K2 = K2;
+    a.scale = scale;
+    return launch_line(op, false, a, (hipStream_t)stream);
+}
+
+extern "C" int sfa_cheb_scalar_product_gl(const void* in, void* out, int dtype, int ndim,
+                                          const int64_t* shape, const int64_t* in_strides,
+                                          const int64_t* out_strides, int axis, const double* K2,
+                                          void* stream)
+{
+    const double N = (double)shape[axis >= 0 && axis < ndim ? axis : 0];
+    return cheb_gl_common(OP_DCT1F, in, out, dtype, ndim, shape, in_strides, out_strides, axis, K2,
+                          3.14159265358979323846 / (2.0 * (N - 1.0)), stream);
+}
+
+extern "C" int sfa_cheb_backward_gl(const void* in, void* out, int dtype, int ndim,
+                                    const int64_t* shape, const int64_t* in_strides,
+                                    const int64_t* out_strides, int axis, const double* K2,
+                                    void* stream)
+{
+    return cheb_gl_common(OP_DCT1B, in, out, dtype, ndim, shape, in_strides, out_strides, axis, K2,
+                          1.0, stream);
 }

@@ -103,18 +103,23 @@ class SpectralMatrix(SparseMatrix):
         raise NotImplementedError
 
 
-def _cheb_h(N):
+def _cheb_h(N, quad="GC"):
+    """Quadrature norms of T_k (chebyshev/bases.py:271-277; matrices are
+    assembled by quadrature, matrixbase.py:635-636): GL doubles the last."""
     h = np.full(N, PI / 2)
     h[0] = PI
+    if quad == "GL":
+        h[-1] = PI
     return h
 
 
 class ChebBSDSDmat(SpectralMatrix):
-    """chebyshev/matrices.py:221-249: {-2: -pi/2, 0: h_k + h_{k+2}, 2: -pi/2}."""
+    """chebyshev/matrices.py:221-249: {-2: -pi/2, 0: h_k + h_{k+2}, 2: -pi/2},
+    h the quadrature norms of the space (GL: h_{N-1} = pi)."""
 
     def assemble(self):
         N = self.testfunction[0].N
-        h = _cheb_h(N)
+        h = _cheb_h(N, self.testfunction[0].quad)
         return {-2: -h[1], 0: h[:-2] + h[2:], 2: -h[1]}
 
     def get_key(self):
@@ -141,7 +146,7 @@ class ChebBSNSNmat(SpectralMatrix):
 
     def assemble(self):
         N = self.testfunction[0].N
-        h = _cheb_h(N)
+        h = _cheb_h(N, self.testfunction[0].quad)
         k = np.arange(N, dtype=float)
         al = (k / (k + 2)) ** 2
         M = N - 2

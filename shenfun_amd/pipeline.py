@@ -26,6 +26,9 @@ class HelmholtzSolver3D:
         if not (len(b) == 3 and isinstance(b[0], ChebyshevBase) and b[0].is_composite
                 and isinstance(b[1], C2C) and isinstance(b[2], R2C)):
             raise ValueError("expects Chebyshev(SD/SN) x C2C x R2C")
+        if b[0].quad != "GC":
+            raise NotImplementedError("the fused solver is built for Chebyshev-Gauss (GC) points; "
+                                      "use T.scalar_product -> la.Helmholtz -> T.backward for GL")
         self.T = T
         self.alpha = float(alpha)
         self.neumann = b[0].bc_kind == "N"

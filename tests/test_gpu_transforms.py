@@ -170,3 +170,65 @@ def test_line_matmul_few_and_many_lines(sfa, shape, axis, cplx):
               _lib.arr(X.shape), _lib.arr(X.stride()), _lib.arr(Y.stride()), axis, nout, _lib.ptr(Md),
               _lib.stream_ptr())
     assert rel(Y, ref) < 1e-13
+
+
+@pytest.mark.parametrize("bc", [None, "D", "N"])
+@pytest.mark.parametrize("N", [2, 5, 9, 33, 65, 100, 257])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_chebyshev_gl_1d(sfa, bc, N, cplx):
+    """Gauss-Lobatto scalar product / backward (DCT-I through the FFT of the
+    even extension, length 2(N-1); 100 -> 198 = 2 x 9 x 11) vs the oracle,
+    and the dense (Vandermonde) kind on the GL nodes."""
+    if bc is not None and N < 3:
+        pytest.skip("composite basis needs N >= 3")
+    print(f"[cheb GL bc={bc} N={N} cplx={cplx}]", flush=True)
+    rng = np.random.default_rng(N + 7)
+    f = rng.standard_normal((N, 3)) + (1j * rng.standard_normal((N, 3)) if cplx else 0)
+    V = sfa.FunctionSpace(N, "chebyshev", bc=bc, quad="GL")
+    c = rng.standard_normal((N, 3)) + (1j * rng.standard_normal((N, 3)) if cplx else 0)
+    if bc is not None:
+        c[-2:] = 0
+    for kind in ("fast", "vandermonde"):
+        sp = V.scalar_product(dev(f), kind=kind)
+        assert rel(sp, S.basis_scalar_product("chebyshev", bc, f, 0, quad="GL")) < TOL, kind
+        bw = V.backward(dev(c), kind=kind)
+        assert rel(bw, S.basis_backward("chebyshev", bc, c, 0, quad="GL")) < TOL, kind
+
+
+@pytest.mark.parametrize("bc", [None, "D", "N"])
+def test_chebyshev_gl_forward_roundtrip(sfa, bc):
+    """forward(backward(u)) == u with the GL mass matrix (last norm pi)."""
+    print(f"[cheb GL roundtrip {bc}]", flush=True)
+    N = 40
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((N, 4)) + 1j * rng.standard_normal((N, 4))
+    if bc is not None:
+        u[-2:] = 0
+    V = sfa.FunctionSpace(N, "chebyshev", bc=bc, quad="GL")
+    uh = V.forward(V.backward(dev(u)))
+    assert rel(uh, u) < 1e-12
+
+
+@pytest.mark.parametrize("bc", ["D", "N"])
+def test_helmholtz2d_gl_dropin(sfa, bc):
+    """(alpha - lap) u = f on Chebyshev-GL x Fourier: T.scalar_product ->
+    la.Helmholtz (GL mass matrix) -> T.backward vs the oracle chain."""
+    print(f"[helm2d GL {bc}]", flush=True)
+    Nx, Ny, alpha = 36, 16, 2.0
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal((Nx, Ny))
+    T = sfa.TensorProductSpace(None, (sfa.FunctionSpace(Nx, "chebyshev", bc=bc, quad="GL"),
+                                      sfa.FunctionSpace(Ny, "F", dtype="d")))
+    fh = T.scalar_product(dev(f))
+    H = sfa.la.Helmholtz(*sfa.helmholtz_tpmatrices(T, alpha))
+    uh = H(fh, sfa.Function(T))
+    u = T.backward(uh)
+    fh_ref = S.basis_scalar_product("chebyshev", bc, S.basis_scalar_product("fourier-r2c", None, f, 1),
+                                    0, quad="GL")
+    ky = np.fft.rfftfreq(Ny, 1.0 / Ny)
+    d0, d1, d2, L = S.helmholtz_lu(bc, Nx, np.full(ky.shape, -1.0), alpha + ky ** 2, quad="GL")
+    uh_ref = S.helmholtz_solve(fh_ref, d0, d1, d2, L, neumann=(bc == "N"))
+    u_ref = S.basis_backward("fourier-r2c", None,
+                             S.basis_backward("chebyshev", bc, uh_ref, 0, quad="GL"), 1, N=Ny)
+    assert rel(uh, uh_ref) < 1e-12
+    assert rel(u, u_ref) < 1e-12

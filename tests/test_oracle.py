@@ -206,6 +206,80 @@ def test_matrices_vs_quadrature(family, bc):
     assert np.allclose(S.dense(fB(N), N - 2), B, atol=1e-12)
 
 
+# ----------------------------------------------------------------- Gauss-Lobatto
+def _gl_independent(N):
+    """GL nodes from numpy (chebyshev/bases.py:204 uses -chebpts2) and weights
+    solved from the moment conditions sum_j w_j T_k(x_j) = pi delta_k0."""
+    x = -npc.chebpts2(N)
+    V = npc.chebvander(x, N - 1)
+    mom = np.zeros(N)
+    mom[0] = np.pi
+    return x, np.linalg.solve(V.T, mom)
+
+
+@pytest.mark.parametrize("N", [5, 8, 17, 64])
+def test_cheb_gl_points_weights(N):
+    x, w = S.cheb_points_weights(N, "GL")
+    xi, wi = _gl_independent(N)
+    assert np.allclose(x, xi, atol=1e-15) and np.allclose(w, wi, rtol=1e-12)
+    # exact for T_k, k <= 2N - 3
+    for k in range(2 * N - 2):
+        assert abs(np.sum(w * npc.chebval(x, np.eye(2 * N)[k])) - (np.pi if k == 0 else 0.0)) < 1e-12
+
+
+@pytest.mark.parametrize("N", [6, 9, 33])
+def test_cheb_gl_transforms_vs_vandermonde(N):
+    """GL scalar product = V^T W f, backward = V c, and the discrete
+    orthogonality with the doubled last norm: backward(sp(f) / h) = f."""
+    rng = np.random.default_rng(N)
+    x, w = _gl_independent(N)
+    V = npc.chebvander(x, N - 1)
+    f = rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))
+    assert np.allclose(S.cheb_scalar_product(f, 0, "GL"), V.T @ (w[:, None] * f), atol=1e-13)
+    c = rng.standard_normal((3, N))
+    assert np.allclose(S.cheb_backward(c, 1, "GL"), c @ V.T, atol=1e-13)
+    h = S.cheb_norm_sq(N, "GL")
+    assert h[-1] == np.pi and h[0] == np.pi
+    rt = S.cheb_backward(S.cheb_scalar_product(f, 0, "GL") / h[:, None], 0, "GL")
+    assert np.allclose(rt, f, atol=1e-13)
+
+
+@pytest.mark.parametrize("bc", ["D", "N"])
+def test_gl_mass_matrices_vs_quadrature(bc):
+    """B (phi_j, phi_k) by N-point GL quadrature (the reference assembles by
+    quadrature): only the last diagonal entry differs from GC's."""
+    N = 14
+    K2 = S.stencil_k2("chebyshev", bc, N)
+    P = np.zeros((N - 2, N))
+    for k in range(N - 2):
+        P[k, k] = 1
+        P[k, k + 2] = K2[k]
+    x, w = _gl_independent(N)
+    phi = npc.chebvander(x, N - 1) @ P.T
+    B = phi.T @ (w[:, None] * phi)
+    fB = S.cheb_B_SD if bc == "D" else S.cheb_B_SN
+    assert np.allclose(S.dense(fB(N, "GL"), N - 2), B, atol=1e-12)
+    d = S.dense(fB(N, "GL"), N - 2) - S.dense(fB(N), N - 2)
+    assert np.count_nonzero(np.abs(d) > 1e-14) == 1 and abs(d[-1, -1]) > 0.1
+
+
+@pytest.mark.parametrize("bc", ["D", "N"])
+def test_helmholtz_lu_gl_vs_dense(bc):
+    """LU_Helmholtz with the GL mass matrix == dense solve of alfa A + beta B_GL."""
+    N = 20
+    M = N - 2
+    A = S.dense(S.cheb_A_SD(N) if bc == "D" else S.cheb_A_SN(N), M)
+    B = S.dense(S.cheb_B_SD(N, "GL") if bc == "D" else S.cheb_B_SN(N, "GL"), M)
+    alfa = np.array([-1.0, -1.0])
+    beta = np.array([3.0, 41.0])
+    d0, d1, d2, L = S.helmholtz_lu(bc, N, alfa, beta, quad="GL")
+    b = np.random.default_rng(2).standard_normal((N, 2))
+    u = S.helmholtz_solve(b, d0, d1, d2, L, neumann=(bc == "N"))
+    for m in range(2):
+        assert np.allclose(u[:M, m], np.linalg.solve(alfa[m] * A + beta[m] * B, b[:M, m]),
+                           rtol=1e-12, atol=1e-14)
+
+
 # ----------------------------------------------------------------- banded
 def test_tdma_pdma_diag_vs_dense():
     """test_la.py:7-46 matrices (N = 10) vs dense solves, along axes 0 and 1."""
