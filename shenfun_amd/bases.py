@@ -39,7 +39,74 @@ def _check(t, name, dtype=None):
         raise ValueError(f"{name} must have 1..3 dims")
 
 
-class SpectralBase:
+class Transform:
+    """Planned transform of a space (spectralbase.Transform,
+    spectralbase.py:2371-2416; arrays planned by SpectralBase.plan :602-648):
+    called as ``(input_array=None, output_array=None, kind=None)``.
+    ``input_array`` and ``output_array`` are device arrays owned by the space,
+    allocated on first use; a call without ``input_array`` transforms
+    ``input_array``, and a call without ``output_array`` writes into and
+    returns the planned output array (one per input shape and dtype), so a
+    later call overwrites it, as in the reference.  ``forward`` shares the
+    output array of ``scalar_product`` (forward.output_array aliases the
+    scalar product's, spectralbase.py:643)."""
+
+    def __init__(self, space, fn, name):
+        self._space = space
+        self._fn = fn
+        self._name = name
+
+    def __call__(self, input_array=None, output_array=None, kind=None, **kw):
+        sp = self._space
+        if input_array is None:
+            input_array = self.input_array
+        sp._active.append(self._name)
+        try:
+            return self._fn(sp, input_array, output_array, kind, **kw)
+        finally:
+            sp._active.pop()
+
+    @property
+    def input_array(self):
+        shape, dtype = self._space._planned_shape(self._name, "in")
+        return self._space._planned_array(self._name, "in", shape, dtype)
+
+    @property
+    def output_array(self):
+        shape, dtype = self._space._planned_shape(self._name, "out")
+        return self._space._planned_array(self._name, "out", shape, dtype)
+
+    tmp_array = output_array
+
+
+def _planned_key(name):
+    return "scalar_product" if name == "forward" else name
+
+
+class _Planned:
+    """Planner plumbing shared by the 1D spaces and TensorProductSpace."""
+
+    def _init_planner(self):
+        self._active = []
+        self._plans = {}
+        for name in ("scalar_product", "forward", "backward"):
+            setattr(self, name, Transform(self, getattr(type(self), name), name))
+
+    def _planned_array(self, name, which, shape, dtype):
+        k = (_planned_key(name), which, tuple(int(v) for v in shape), dtype)
+        a = self._plans.get(k)
+        if a is None:
+            a = torch.zeros(k[2], dtype=dtype, device=self.device)
+            self._plans[k] = a
+        return a
+
+    def _planned_out(self, shape, dtype):
+        """Output array of the transform being called (no output_array given)."""
+        name = self._active[-1] if self._active else "scalar_product"
+        return self._planned_array(name, "out", shape, dtype)
+
+
+class SpectralBase(_Planned):
     """Common part of the 1D spaces."""
 
     def __init__(self, N, dtype=np.float64, device=None):
@@ -50,6 +117,16 @@ class SpectralBase:
         self.tensorproductspace = None
         self.device = _dev(device)
         self._cache = {}
+        self._init_planner()
+
+    def _planned_shape(self, name, which):
+        """Default planned arrays: the 1D line, physical <-> spectral."""
+        phys = torch.complex128 if self.dtype.kind == "c" else torch.float64
+        spec = torch.complex128 if isinstance(self, R2C) else phys
+        physical = (which == "in") == (name != "backward")
+        if physical:
+            return (self.shape(False),), phys
+        return (self.shape(True),), spec
 
     # ---------------------------------------------------------------- info
     def family(self):
@@ -85,7 +162,10 @@ class SpectralBase:
         shape = list(inp.shape)
         shape[self.axis] = shape_axis
         if output_array is None:
-            return torch.empty(shape, dtype=dtype, device=inp.device)
+            out = self._planned_out(shape, dtype)
+            if out.data_ptr() == inp.data_ptr():  # transforming the planned output array itself
+                out = torch.empty(shape, dtype=dtype, device=inp.device)
+            return out
         _check(output_array, "output_array", dtype)
         if list(output_array.shape) != shape:
             raise ValueError(f"output_array shape {tuple(output_array.shape)} != {tuple(shape)}")

@@ -10,10 +10,10 @@ HIP transform on the strided device array.
 import numpy as np
 import torch
 
-from .bases import FourierBase, R2C
+from .bases import FourierBase, R2C, _Planned
 
 
-class TensorProductSpace:
+class TensorProductSpace(_Planned):
     def __init__(self, comm, bases, axes=None, dtype=None, slab=False, device=None, **kw):
         """``comm`` is accepted for signature compatibility and ignored: one
         GPU holds the full array."""
@@ -32,6 +32,15 @@ class TensorProductSpace:
             b.tensorproductspace = self
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self._work = {}
+        self._init_planner()
+
+    def _planned_shape(self, name, which):
+        """Planned arrays (tensorproductspace.py:45-259): physical shape and
+        dtype on the physical side, spectral on the other."""
+        physical = (which == "in") == (name != "backward")
+        if physical:
+            return self.shape(False), self.physical_dtype()
+        return self.shape(True), self.spectral_dtype()
 
     # ---------------------------------------------------------------- info
     def shape(self, forward_output=True):
@@ -98,14 +107,16 @@ class TensorProductSpace:
                 dt = torch.complex128 if isinstance(b, R2C) else cur.dtype
             if last and output_array is not None:
                 out = output_array
+            elif last:
+                out = self._planned_out(shp, dt)
+                if out.data_ptr() == cur.data_ptr():
+                    out = torch.empty(shp, dtype=dt, device=self.device)
             else:
                 out = self._buf((method, j % 2), shp, dt)
                 if out.data_ptr() == cur.data_ptr():
                     out = self._buf((method, 2), shp, dt)
             getattr(b, method)(cur, out, kind=k)
             cur = out
-        if output_array is None:
-            return cur.clone()
         return cur
 
     # ---------------------------------------------------------- transforms

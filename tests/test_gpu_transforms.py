@@ -232,3 +232,34 @@ def test_helmholtz2d_gl_dropin(sfa, bc):
                              S.basis_backward("chebyshev", bc, uh_ref, 0, quad="GL"), 1, N=Ny)
     assert rel(uh, uh_ref) < 1e-12
     assert rel(u, u_ref) < 1e-12
+
+
+def test_planned_arrays_semantics(sfa):
+    """spectralbase.py:235-313 / :602-648: transforms own planned device
+    arrays; without arguments they transform input_array into output_array
+    and return it; forward shares the scalar product's output array; an
+    explicit output_array receives the result instead."""
+    print("[planned arrays]", flush=True)
+    N = 24
+    V = sfa.FunctionSpace(N, "chebyshev", bc="D")
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal(N)
+    V.forward.input_array[:] = dev(f)
+    out = V.forward()
+    assert out is V.forward.output_array and out is V.scalar_product.output_array
+    ref = S.basis_scalar_product("chebyshev", "D", f)
+    assert rel(V.scalar_product(dev(f)), ref) < TOL
+    assert V.scalar_product(dev(f)) is V.scalar_product.output_array  # same planned array again
+    mine = torch.empty(N, dtype=torch.float64, device="cuda")
+    assert V.scalar_product(dev(f), mine) is mine and rel(mine, ref) < TOL
+    T = sfa.TensorProductSpace(None, (sfa.FunctionSpace(16, "chebyshev", bc="N"),
+                                      sfa.FunctionSpace(12, "F", dtype="d")))
+    g = rng.standard_normal((16, 12))
+    T.forward.input_array[:] = dev(g)
+    gh = T.forward()
+    assert gh is T.forward.output_array and tuple(gh.shape) == (16, 7)
+    gh0 = gh.cpu().numpy()
+    u = T.backward(gh.clone())
+    assert u is T.backward.output_array
+    again = T.forward(u)  # overwrites the planned output array (gh) in place
+    assert again is gh and rel(again, gh0) < 1e-12
