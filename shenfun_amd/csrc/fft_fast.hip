@@ -680,7 +680,11 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 }
 
 // C2R (FP64 arithmetic) of rows merged from o64 (k < Kh) and o32 (k >= Kh).
-template <int H>
+// KSMALL: Kh <= 2Q, so only this thread's first two elements (k = j, j + Q)
+// can lie in the complex128 planes; every other load is a plain complex64
+// load, and all loads are issued before any conversion (a per-element select
+// of a converted complex64 load waited on each load in turn).
+template <int H, bool KSMALL>
 __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     c2r_merge_kernel(const double2* __restrict__ i64, const float2* __restrict__ i32,
                      double* __restrict__ out, int64_t nrows, int Kh, double scale,
@@ -700,12 +704,39 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
     double2* ln = ex + t * LS;
     double2 vin[P + 1], twr[P];
+    if (KSMALL) {
+        // every load unconditional from a clamped in-range address (row 0 for
+        // rows past the end, k clamped into its sub-array), validity applied
+        // afterwards by selects: no branch separates the loads
+        float2 v32[P + 1];
+        double2 lo[2];
+        const int64_t Rc = valid ? R : 0;
+        const float2* r32 = i32 + Rc * Kl;  // element k >= Kh at r32[k - Kh]
 #pragma unroll
-    for (int i = 0; i <= P; ++i) {
-        const int k = j + Q * i;
-        double2 v = make_double2(0.0, 0.0);
-        if (valid && k <= H) v = (k < Kh) ? i64[R * Kh + k] : to_c64(i32[R * Kl + (k - Kh)]);
-        vin[i] = v;
+        for (int i = 0; i <= P; ++i) {
+            const int k = j + Q * i;
+            const int kc = k < Kh ? Kh : (k > H ? H : k);
+            v32[i] = r32[kc - Kh];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int k = j + Q * i;
+            lo[i] = i64[Rc * Kh + (k < Kh ? k : 0)];
+        }
+#pragma unroll
+        for (int i = 0; i <= P; ++i) {
+            const int k = j + Q * i;
+            const double2 v = (i < 2 && k < Kh) ? lo[i < 2 ? i : 0] : to_c64(v32[i]);
+            vin[i] = (valid && k <= H) ? v : make_double2(0.0, 0.0);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i <= P; ++i) {
+            const int k = j + Q * i;
+            double2 v = make_double2(0.0, 0.0);
+            if (valid && k <= H) v = (k < Kh) ? i64[R * Kh + k] : to_c64(i32[R * Kl + (k - Kh)]);
+            vin[i] = v;
+        }
     }
 #pragma unroll
     for (int r = 0; r < P; ++r) twr[r] = twg[j + Q * r];  // pre-twiddles, issued with the loads
@@ -759,7 +790,7 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double*)in, (double2*)o64, (float2*)o32,
                            nrows, Kh, scale, tw);
     } else {
-        auto k = c2r_merge_kernel<H>;
+        auto k = Kh <= 2 * Pl::Q ? c2r_merge_kernel<H, true> : c2r_merge_kernel<H, false>;
         if (lds > 64 * 1024)
             hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double2*)o64, (const float2*)o32,
