@@ -680,10 +680,10 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 }
 
 // C2R (FP64 arithmetic) of rows merged from o64 (k < Kh) and o32 (k >= Kh).
-// KSMALL: Kh <= 2Q, so only this thread's first two elements (k = j, j + Q)
-// can lie in the complex128 planes; every other load is a plain complex64
-// load, and all loads are issued before any conversion (a per-element select
-// of a converted complex64 load waited on each load in turn).
+// KSMALL: Kh <= 3Q, so only this thread's first three elements (k = j + Q i,
+// i < 3) can lie in the complex128 planes; every other load is a plain
+// complex64 load, and all loads are issued before any conversion (a
+// per-element select of a converted complex64 load waited on each load in turn).
 template <int H, bool KSMALL>
 __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     c2r_merge_kernel(const double2* __restrict__ i64, const float2* __restrict__ i32,
@@ -708,8 +708,9 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
         // every load unconditional from a clamped in-range address (row 0 for
         // rows past the end, k clamped into its sub-array), validity applied
         // afterwards by selects: no branch separates the loads
+        constexpr int NLO = 3;
         float2 v32[P + 1];
-        double2 lo[2];
+        double2 lo[NLO];
         const int64_t Rc = valid ? R : 0;
         const float2* r32 = i32 + Rc * Kl;  // element k >= Kh at r32[k - Kh]
 #pragma unroll
@@ -719,14 +720,14 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
             v32[i] = r32[kc - Kh];
         }
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < NLO; ++i) {
             const int k = j + Q * i;
             lo[i] = i64[Rc * Kh + (k < Kh ? k : 0)];
         }
 #pragma unroll
         for (int i = 0; i <= P; ++i) {
             const int k = j + Q * i;
-            const double2 v = (i < 2 && k < Kh) ? lo[i < 2 ? i : 0] : to_c64(v32[i]);
+            const double2 v = (i < NLO && k < Kh) ? lo[i < NLO ? i : 0] : to_c64(v32[i]);
             vin[i] = (valid && k <= H) ? v : make_double2(0.0, 0.0);
         }
     } else {
@@ -790,7 +791,7 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double*)in, (double2*)o64, (float2*)o32,
                            nrows, Kh, scale, tw);
     } else {
-        auto k = Kh <= 2 * Pl::Q ? c2r_merge_kernel<H, true> : c2r_merge_kernel<H, false>;
+        auto k = Kh <= 3 * Pl::Q ? c2r_merge_kernel<H, true> : c2r_merge_kernel<H, false>;
         if (lds > 64 * 1024)
             hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double2*)o64, (const float2*)o32,
