@@ -16,6 +16,6 @@ from .matrices import (SparseMatrix, TPMatrix, helmholtz_tpmatrices, mass_matrix
                        stiffness_matrix)
 from .batched import BatchedHelmholtz2D  # noqa: F401
 from .pipeline import HelmholtzSolver3D  # noqa: F401
-from .tensorproductspace import Array, Function, TensorProductSpace  # noqa: F401
+from .tensorproductspace import Array, Function, TensorProductSpace, project  # noqa: F401
 
 __version__ = "0.1.0"

@@ -119,6 +119,16 @@ class TensorProductSpace(_Planned):
             cur = out
         return cur
 
+    def mesh(self):
+        """Broadcastable quadrature meshes, one per axis (numpy, host)."""
+        out = []
+        for i, b in enumerate(self.bases):
+            x = np.asarray(b.mesh(), dtype=float)
+            s = [1] * self.dimensions
+            s[i] = x.size
+            out.append(x.reshape(s))
+        return out
+
     # ---------------------------------------------------------- transforms
     def scalar_product(self, input_array=None, output_array=None, kind=None, **kw):
         """(tensorproductspace.py:146-190): last axis first."""
@@ -170,3 +180,30 @@ def Function(space, buffer=None, val=None):
     if val is not None:
         t.fill_(val)
     return t
+
+
+def project(uh, T, output_array=None, fill=True, uh_space=None):
+    """Galerkin projection onto ``T`` (forms/project.py:16-172 for the
+    hot-path cases): find u in T with (u - uh, v)_w = 0 for all v in T.
+
+    * ``uh`` a physical array on T's quadrature mesh: u = T.forward(uh)
+      (scalar product, then the inverse mass matrix);
+    * ``uh`` coefficients of another space ``uh_space`` (a Function of it):
+      u = T.forward(uh_space.backward(uh)) -- the meshes must agree;
+    * ``uh`` a Python callable f(x0, x1, ...) of the mesh coordinates:
+      evaluated on T's mesh, then projected.
+
+    Returns ``output_array`` (a new Function of T when None); with ``fill``
+    it is zeroed first, as the reference does."""
+    if output_array is None:
+        output_array = Function(T)
+    elif fill:
+        output_array.zero_()
+    if callable(uh) and not isinstance(uh, torch.Tensor):
+        xs = T.mesh() if isinstance(T, TensorProductSpace) else [np.asarray(T.mesh(), dtype=float)]
+        shape = T.shape(False) if isinstance(T, TensorProductSpace) else (T.N,)
+        vals = np.broadcast_to(np.asarray(uh(*xs)), shape)
+        uh = Array(T, buffer=np.ascontiguousarray(vals))
+    elif uh_space is not None:
+        uh = uh_space.backward(uh).clone()
+    return T.forward(uh, output_array)

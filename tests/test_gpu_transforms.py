@@ -263,3 +263,31 @@ def test_planned_arrays_semantics(sfa):
     assert u is T.backward.output_array
     again = T.forward(u)  # overwrites the planned output array (gh) in place
     assert again is gh and rel(again, gh0) < 1e-12
+
+
+def test_project(sfa):
+    """project (forms/project.py:16-172): a physical array, a callable on the
+    mesh and a Function of another space all project onto T; projecting a
+    member of T returns its coefficients."""
+    print("[project]", flush=True)
+    Nx, Ny = 24, 16
+    T = sfa.TensorProductSpace(None, (sfa.FunctionSpace(Nx, "chebyshev", bc="D"),
+                                      sfa.FunctionSpace(Ny, "F", dtype="d")))
+    rng = np.random.default_rng(9)
+    uh = sfa.Function(T)
+    uh[:Nx - 2] = dev(rng.standard_normal((Nx - 2, Ny // 2 + 1)) + 1j * rng.standard_normal((Nx - 2, Ny // 2 + 1)))
+    uh[:, 0] = uh[:, 0].real + 0j
+    uh[:, -1] = uh[:, -1].real + 0j
+    u = T.backward(uh).clone()
+    assert rel(sfa.project(u, T), uh.cpu().numpy()) < 1e-12
+    # a callable in the space: (1 - x^2) cos(y) = phi_0-like member of SD x R2C
+    f = lambda x, y: (1 - x ** 2) * np.cos(y)
+    pf = sfa.project(f, T)
+    assert rel(T.backward(pf), f(*T.mesh())) < 1e-12
+    # a Function of the orthogonal space with the same mesh
+    C = sfa.TensorProductSpace(None, (sfa.FunctionSpace(Nx, "chebyshev"),
+                                      sfa.FunctionSpace(Ny, "F", dtype="d")))
+    ch = C.forward(u).clone()
+    out = sfa.Function(T)
+    assert sfa.project(ch, T, output_array=out, uh_space=C) is out
+    assert rel(out, uh.cpu().numpy()) < 1e-12
