@@ -318,6 +318,90 @@ __global__ void helmholtz_solve_kernel(const T* __restrict__ b, T* __restrict__ 
     }
 }
 
+// Long chains, few lines: one workgroup per (line, parity) chain.  The
+// chain's right-hand side and factors are staged in LDS with coalesced loads
+// (1/d0 formed in parallel there), one thread runs the two sweeps from LDS,
+// and the workgroup stores the result; the thread-per-chain kernel above
+// waits on global memory once per row, which costs milliseconds at N = 4096
+// when there are too few lines to hide it.  Same recurrences
+// (la.pyx:871-912), d0 applied as a multiplication by its reciprocal.
+constexpr int HELM_LDS_MIN_M = 512;
+constexpr int64_t HELM_LDS_MAX_LINES = 2048;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    helmholtz_solve_lds_kernel(const T* __restrict__ b, T* __restrict__ u, LineMap gb, LineMap gu,
+                               LineMap gf, int M, int neumann, const double* __restrict__ d0,
+                               const double* __restrict__ d1, const double* __restrict__ d2,
+                               const double* __restrict__ Lf)
+{
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    const int64_t nl = gu.nlines;
+    const int p = (int)(blockIdx.x / nl);
+    const int64_t L = blockIdx.x - (int64_t)p * nl;
+    const int K = (M - p + 1) / 2;  // rows i = p + 2 m, m < K
+    T* y = reinterpret_cast<T*>(sm_raw);
+    double* Ls = reinterpret_cast<double*>(y + K);
+    double* r0 = Ls + K;
+    double* e1 = r0 + K;
+    double* e2 = e1 + K;
+    const T* bl = b + gb.off(L);
+    T* ul = u + gu.off(L);
+    const int64_t of = gf.off(L);
+    const int64_t sb = gb.sa, su = gu.sa, sf = gf.sa;
+    for (int m = threadIdx.x; m < K; m += blockDim.x) {
+        const int64_t i = p + 2 * m;
+        y[m] = bl[i * sb];
+        Ls[m] = m > 0 ? Lf[of + (i - 2) * sf] : 0.0;
+        r0[m] = 1.0 / d0[of + i * sf];
+        e1[m] = d1[of + i * sf];
+        e2[m] = d2[of + i * sf];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T yp = y[0];
+#pragma unroll 8
+        for (int m = 1; m < K; ++m) {
+            yp = sub_mul(y[m], Ls[m], yp);
+            y[m] = yp;
+        }
+        T n2 = mul(y[K - 1], r0[K - 1]);
+        y[K - 1] = n2;
+        T n4 = zero<T>();
+        if (K >= 2) {
+            const T cur = mul(sub_mul(y[K - 2], e1[K - 2], n2), r0[K - 2]);
+            y[K - 2] = cur;
+            n4 = n2;
+            n2 = cur;
+        }
+        T S = zero<T>();
+#pragma unroll 8
+        for (int m = K - 3; m >= 0; --m) {
+            T cur = sub_mul(y[m], e1[m], n2);
+            S = add(S, n4);
+            cur = sub_mul(cur, e2[m], S);
+            cur = mul(cur, r0[m]);
+            y[m] = cur;
+            n4 = n2;
+            n2 = cur;
+        }
+    }
+    __syncthreads();
+    for (int m = threadIdx.x; m < K; m += blockDim.x) {
+        const int i = p + 2 * m;
+        T v = y[m];
+        if (neumann) {
+            if (i == 0) {
+                const double d00 = d0[of];
+                if ((d00 - 1.0) * (d00 - 1.0) < 1e-16) v = zero<T>();
+            } else {
+                v = divd(v, (double)i * i);
+            }
+        }
+        ul[(int64_t)i * su] = v;
+    }
+}
+
 template <typename T>
 __global__ void helmholtz_matvec_kernel(const T* __restrict__ v, T* __restrict__ c, LineMap gv,
                                         LineMap gc, int M, int neumann,
@@ -520,6 +604,26 @@ extern "C" int sfa_helmholtz_solve(const void* b, void* u, int dtype, int ndim, 
     SFA_CHECK(dtype == SFA_F64 || dtype == SFA_C128, SFA_ERR_ARG, "bad dtype %d", dtype);
     SFA_CHECK(N >= 8, SFA_ERR_ARG, "Helmholtz solver needs N >= 8 (got %lld)", (long long)N);
     const int M = (int)N - 2;
+    const size_t elem = dtype == SFA_C128 ? sizeof(double2) : sizeof(double);
+    const size_t lds = (size_t)((M + 1) / 2) * (elem + 4 * sizeof(double));
+    if (M >= HELM_LDS_MIN_M && gu.nlines <= HELM_LDS_MAX_LINES && lds <= 160 * 1024) {
+        dim3 grid((unsigned)(2 * gu.nlines));
+        if (dtype == SFA_C128) {
+            auto k = helmholtz_solve_lds_kernel<double2>;
+            if (lds > 64 * 1024)
+                hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(k, grid, dim3(256), lds, (hipStream_t)stream, (const double2*)b,
+                               (double2*)u, gb, gu, gf, M, neumann, d0, d1, d2, L);
+        } else {
+            auto k = helmholtz_solve_lds_kernel<double>;
+            if (lds > 64 * 1024)
+                hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(k, grid, dim3(256), lds, (hipStream_t)stream, (const double*)b,
+                               (double*)u, gb, gu, gf, M, neumann, d0, d1, d2, L);
+        }
+        SFA_LAUNCH_CHECK();
+        return SFA_OK;
+    }
     dim3 grid = grid_for(2 * gu.nlines, 256);
     if (dtype == SFA_C128)
         hipLaunchKernelGGL(helmholtz_solve_kernel<double2>, grid, dim3(256), 0, (hipStream_t)stream,

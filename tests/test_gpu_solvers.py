@@ -68,8 +68,10 @@ def test_tdma_pdma_vs_oracle(sfa, n):
 
 @pytest.mark.parametrize("bc", ["D", "N"])
 @pytest.mark.parametrize("shape,axis", [((20,), 0), ((32, 7, 5), 0), ((6, 40, 5), 1), ((4, 5, 66), 2),
-                                        ((256, 16, 9), 0)])
+                                        ((256, 16, 9), 0), ((1024,), 0), ((4096, 3), 0), ((2, 2050), 1)])
 def test_helmholtz(sfa, bc, shape, axis):
+    """Chebyshev Helmholtz LU/solve/matvec vs the oracle; N >= 514 with few
+    lines takes the LDS-staged long-chain solve."""
     print(f"[helmholtz {bc} {shape} axis={axis}]", flush=True)
     N = shape[axis]
     rng = np.random.default_rng(N)
