@@ -376,6 +376,19 @@ struct XPlan {
 // form.  State in: (r1, r2, pinv) = (d1/d0, d2/d0, 1/d0) of the previous row of
 // the chain (first = first row of the chain).  Out: this row's 1/d0, the new
 // ratios, and the forward-elimination multiplier am = -L.
+// 1/d for the LU rows: hardware reciprocal refined by two Newton steps
+// (5 instructions against the IEEE division's scale/fixup sequence; within an
+// ulp for the normal-range pivots here).  The plan builder and the x-pass both
+// regenerate rows through helm_lu_row, so their states agree.
+__device__ inline double helm_rcp(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 __device__ inline void helm_lu_row(const HelmRow& r, int i, int M, bool first, double& r1, double& r2,
                                    double& pinv, double& iv, double& am)
 {
@@ -391,7 +404,7 @@ __device__ inline void helm_lu_row(const HelmRow& r, int i, int M, bool first, d
         d2 = (i < M - 4) ? r.c - r.g * r2 : 0.0;
         am = -r.g * pinv;
     }
-    iv = 1.0 / d0;
+    iv = helm_rcp(d0);
     r1 = d1 * iv;
     r2 = d2 * iv;
     pinv = iv;
