@@ -372,6 +372,24 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
         v[r] = valid ? Wd[(int64_t)x * a.sa + Lo] : cmk<C2>(0.0, 0.0);
     }
+    // plan entries of this group's solve lines (the group's chains are lines
+    // ct0 .. ct0 + ROUNDS/2 - 1, each with both parities), looked up here so the
+    // line-map latency overlaps the loads and FFT; before the solve only the
+    // independent svals / restart-state loads remain
+    static_assert(ROUNDS % 2 == 0, "chains per group must pair both parities");
+    constexpr int NLR = ROUNDS / 2;
+    int ug[NLR];
+#pragma unroll
+    for (int q = 0; q < NLR; ++q) {
+        const int64_t Lc = bid * TL + (tid / 16) * NLR + q;
+        int uq = 0;
+        if (Lc < a.nl) {
+            const int64_t Lr = Lc % a.nlb;  // line within its array
+            const int64_t ky = Lr / a.Kp, kz = a.kz0 + (Lr - ky * a.Kp);
+            if (kz < a.Nzh) uq = a.pl.linemap[ky * a.Nzh + kz];  // else a pitch-padding line
+        }
+        ug[q] = uq;
+    }
     __syncthreads();  // tables
     pow2_fft_p1<N, false>(v, j, ex + t * LA, tw);
     __syncthreads();  // exchange, and the switch to mapping B
@@ -408,19 +426,15 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     // with rho, so the row operators (functions of the row index) cannot be
     // hoisted out of the loop and held in registers across it.  Its lines are
     // mapping-B lines of its own wave.
-    static_assert(ROUNDS % 2 == 0, "chains per group must pair both parities");
 #pragma unroll 1
     for (int rho = 0; rho < ROUNDS; ++rho) {
         const int c = (tid / 16) * ROUNDS + rho;
         const int ct = c >> 1, p = c & 1, lane = tid & 15;
         C2* X = ex + ct * LA;
-        const int64_t Lc = bid * TL + ct;
-        int u = 0;
-        if (Lc < a.nl) {
-            const int64_t Lr = Lc % a.nlb;  // line within its array
-            const int64_t ky = Lr / a.Kp, kz = a.kz0 + (Lr - ky * a.Kp);
-            if (kz < a.Nzh) u = a.pl.linemap[ky * a.Nzh + kz];  // else a pitch-padding line
-        }
+        int u = ug[0];
+#pragma unroll
+        for (int q = 1; q < NLR; ++q)
+            if ((rho >> 1) == q) u = ug[q];
         const double be = a.alpha + a.pl.svals[u];
         const double4 b4 = a.pl.bs[((int64_t)2 * u + p) * 16 + lane];
         double2 f[E];
