@@ -144,6 +144,41 @@ int sfa_diagma_solve(void* u, int dtype, int ndim, const int64_t* shape,
                      const int64_t* strides, int axis, const double* d, int64_t n,
                      int64_t data_line_stride, int64_t skip_lines, void* stream);
 
+/* The other band patterns of SparseMatrix.get_solver (matrixbase.py:439-474),
+ * DIA rows in offset order, same layout and line conventions as above.
+ * LU in place, nbatch systems of (rows, n):
+ *   TDMA_O   (-1, 0, 1), 3 rows, n >= 1: replaces TDMA_O_LU la.pyx:254-263
+ *   FDMA     (-2, 0, 2, 4), 4 rows, n >= 4: replaces FDMA_LU la.pyx:199-211
+ *   HeptaDMA (-4, -2, 0, 2, 4, 6, 8), 7 rows, n >= 8: replaces HeptaDMA_LU
+ *            la.pyx:265-303
+ * TwoDMA (0, 2) and ThreeDMA (0, 2, 4) are upper triangular: no LU. */
+int sfa_tdma_o_lu(double* data, int64_t n, int64_t nbatch, void* stream);
+int sfa_fdma_lu(double* data, int64_t n, int64_t nbatch, void* stream);
+int sfa_heptadma_lu(double* data, int64_t n, int64_t nbatch, void* stream);
+
+/* Solves in place along `axis` (arguments as sfa_tdma_solve).  Replace
+ * TDMA_O_inner_solve_ptr la.pyx:538-549 / TDMA_O_Solve :93-107,
+ * TwoDMA_inner_solve_ptr :555-563 / TwoDMA_Solve :45, ThreeDMA_inner_solve_ptr
+ * :569-580 / ThreeDMA_Solve :29 (its +4 band is read from DIA row 2, where the
+ * reference reads row 1), FDMA_inner_solve_ptr :596-611 / FDMA_Solve :165,
+ * HeptaDMA_inner_solve_ptr :466-490 / HeptaDMA_Solve :181.  Minimum n: TDMA_O 1,
+ * TwoDMA 2, ThreeDMA 4, FDMA 4, HeptaDMA 8. */
+int sfa_tdma_o_solve(void* u, int dtype, int ndim, const int64_t* shape,
+                     const int64_t* strides, int axis, const double* data, int64_t n,
+                     int64_t data_line_stride, int64_t skip_lines, void* stream);
+int sfa_twodma_solve(void* u, int dtype, int ndim, const int64_t* shape,
+                     const int64_t* strides, int axis, const double* data, int64_t n,
+                     int64_t data_line_stride, int64_t skip_lines, void* stream);
+int sfa_threedma_solve(void* u, int dtype, int ndim, const int64_t* shape,
+                       const int64_t* strides, int axis, const double* data, int64_t n,
+                       int64_t data_line_stride, int64_t skip_lines, void* stream);
+int sfa_fdma_solve(void* u, int dtype, int ndim, const int64_t* shape,
+                   const int64_t* strides, int axis, const double* data, int64_t n,
+                   int64_t data_line_stride, int64_t skip_lines, void* stream);
+int sfa_heptadma_solve(void* u, int dtype, int ndim, const int64_t* shape,
+                       const int64_t* strides, int axis, const double* data, int64_t n,
+                       int64_t data_line_stride, int64_t skip_lines, void* stream);
+
 /* Chebyshev Helmholtz LU (alfa*A + beta*B), N basis size (M = N-2 unknowns).
  * A0 (M), A2 (M-2), A4 (M-4): stiffness diagonals 0, +2, +4; Bm2 (M-2), B0 (M),
  * B2 (M-2): mass diagonals (unscaled; the Neumann 1/j^2 column scaling and

@@ -584,6 +584,178 @@ def diagma_solve(u, d, axis=0):
     return np.moveaxis(um, 0, axis)
 
 
+def band_dia(d, n, offsets):
+    """SparseMatrix dict -> scipy-style DIA rows (len(offsets), n), rows in
+    ``offsets`` order; offset o >= 0 fills columns o.., o < 0 columns ..n+o
+    (la.py:256-258, the ``mat.diags('dia').data`` layout)."""
+    data = np.zeros((len(offsets), n))
+    for r, o in enumerate(offsets):
+        if o not in d:
+            continue
+        if o >= 0:
+            data[r, o:] = np.broadcast_to(d[o], (n - o,))
+        else:
+            data[r, :n + o] = np.broadcast_to(d[o], (n + o,))
+    return data
+
+
+def tdma_o_lu(data):
+    """la.pyx:254-263 TDMA_O_LU (offsets -1, 0, 1)."""
+    data = np.array(data, dtype=float, copy=True)
+    n = data.shape[-1]
+    ld = data[0, :n - 1]
+    d = data[1, :]
+    ud = data[2, 1:]
+    for i in range(1, n):
+        ld[i - 1] = ld[i - 1] / d[i - 1]
+        d[i] = d[i] - ld[i - 1] * ud[i - 1]
+    return data
+
+
+def tdma_o_solve(u, data, axis=0):
+    """la.pyx:538-549 TDMA_O_inner_solve_ptr along ``axis``."""
+    um = np.moveaxis(np.array(u, copy=True), axis, 0)
+    n = data.shape[-1]
+    ld = data[0]
+    d = data[1]
+    ud = data[2, 1:]
+    for i in range(1, n):
+        um[i] -= ld[i - 1] * um[i - 1]
+    um[n - 1] = um[n - 1] / d[n - 1]
+    for i in range(n - 2, -1, -1):
+        um[i] = (um[i] - ud[i] * um[i + 1]) / d[i]
+    return np.moveaxis(um, 0, axis)
+
+
+def twodma_solve(u, data, axis=0):
+    """la.pyx:555-563 TwoDMA_inner_solve_ptr (offsets 0, 2; no LU)."""
+    um = np.moveaxis(np.array(u, copy=True), axis, 0)
+    n = data.shape[-1]
+    d = data[0]
+    u1 = data[1, 2:]
+    um[n - 1] = um[n - 1] / d[n - 1]
+    um[n - 2] = um[n - 2] / d[n - 2]
+    for i in range(n - 3, -1, -1):
+        um[i] = (um[i] - u1[i] * um[i + 2]) / d[i]
+    return np.moveaxis(um, 0, axis)
+
+
+def threedma_solve(u, data, axis=0):
+    """la.pyx:569-580 ThreeDMA_inner_solve_ptr (offsets 0, 2, 4; no LU).
+
+    Deviation, deliberate: the reference takes the +4 band from DIA row 1
+    (``u2 = &data[m1+4]``, la.pyx:574; ``data[1, 4:]``, la.py:688), i.e. the +2
+    band shifted by two rows, so its result is the solution of the matrix only
+    when those two bands coincide.  This restatement reads row 2, the +4 band,
+    and so solves the matrix the class documents (la.py:661-668).
+    """
+    um = np.moveaxis(np.array(u, copy=True), axis, 0)
+    n = data.shape[-1]
+    d = data[0]
+    u1 = data[1, 2:]
+    u2 = data[2, 4:]
+    um[n - 1] = um[n - 1] / d[n - 1]
+    um[n - 2] = um[n - 2] / d[n - 2]
+    um[n - 3] = (um[n - 3] - u1[n - 3] * um[n - 1]) / d[n - 3]
+    um[n - 4] = (um[n - 4] - u1[n - 4] * um[n - 2]) / d[n - 4]
+    for i in range(n - 5, -1, -1):
+        um[i] = (um[i] - u1[i] * um[i + 2] - u2[i] * um[i + 4]) / d[i]
+    return np.moveaxis(um, 0, axis)
+
+
+def fdma_lu(data):
+    """la.pyx:199-211 FDMA_LU (offsets -2, 0, 2, 4)."""
+    data = np.array(data, dtype=float, copy=True)
+    n = data.shape[-1]
+    ld = data[0, :n - 2]
+    d = data[1, :]
+    u1 = data[2, 2:]
+    u2 = data[3, 4:]
+    for i in range(2, n):
+        ld[i - 2] = ld[i - 2] / d[i - 2]
+        d[i] = d[i] - ld[i - 2] * u1[i - 2]
+        if i < n - 2:
+            u1[i] = u1[i] - ld[i - 2] * u2[i - 2]
+    return data
+
+
+def fdma_solve(u, data, axis=0):
+    """la.pyx:596-611 FDMA_inner_solve_ptr along ``axis``."""
+    um = np.moveaxis(np.array(u, copy=True), axis, 0)
+    n = data.shape[-1]
+    ld = data[0]
+    d = data[1]
+    u1 = data[2, 2:]
+    u2 = data[3, 4:]
+    for i in range(2, n):
+        um[i] -= ld[i - 2] * um[i - 2]
+    um[n - 1] = um[n - 1] / d[n - 1]
+    um[n - 2] = um[n - 2] / d[n - 2]
+    um[n - 3] = (um[n - 3] - u1[n - 3] * um[n - 1]) / d[n - 3]
+    um[n - 4] = (um[n - 4] - u1[n - 4] * um[n - 2]) / d[n - 4]
+    for i in range(n - 5, -1, -1):
+        um[i] = (um[i] - u1[i] * um[i + 2] - u2[i] * um[i + 4]) / d[i]
+    return np.moveaxis(um, 0, axis)
+
+
+def heptadma_lu(data):
+    """la.pyx:265-303 HeptaDMA_LU (offsets -4, -2, 0, 2, 4, 6, 8; k = 2)."""
+    data = np.array(data, dtype=float, copy=True)
+    n = data.shape[-1]
+    a = data[0, :n - 4]
+    b = data[1, :n - 2]
+    d = data[2, :]
+    e = data[3, 2:]
+    f = data[4, 4:]
+    g = data[5, 6:]
+    h = data[6, 8:]
+    k = 2
+    for i in range(n - 2 * k):
+        lam = b[i] / d[i]
+        d[i + k] -= lam * e[i]
+        e[i + k] -= lam * f[i]
+        if i < n - 6:
+            f[i + k] -= lam * g[i]
+        if i < n - 8:
+            g[i + k] -= lam * h[i]
+        b[i] = lam
+        lam = a[i] / d[i]
+        b[i + k] -= lam * e[i]
+        d[i + 2 * k] -= lam * f[i]
+        if i < n - 6:
+            e[i + 2 * k] -= lam * g[i]
+        if i < n - 8:
+            f[i + 2 * k] -= lam * h[i]
+        a[i] = lam
+    for i in (n - 4, n - 3):
+        lam = b[i] / d[i]
+        d[i + k] -= lam * e[i]
+        b[i] = lam
+    return data
+
+
+def heptadma_solve(u, data, axis=0):
+    """la.pyx:466-490 HeptaDMA_inner_solve_ptr along ``axis``; the reference's
+    eight end rows are the general row with the terms beyond n - 1 dropped."""
+    um = np.moveaxis(np.array(u, copy=True), axis, 0)
+    n = data.shape[-1]
+    a = data[0]
+    b = data[1]
+    d = data[2]
+    up = [data[3, 2:], data[4, 4:], data[5, 6:], data[6, 8:]]
+    um[2] -= b[0] * um[0]
+    um[3] -= b[1] * um[1]
+    for k in range(4, n):
+        um[k] -= b[k - 2] * um[k - 2] + a[k - 4] * um[k - 4]
+    for k in range(n - 1, -1, -1):
+        acc = um[k]
+        for j, c in enumerate(up):
+            if k + 2 * (j + 1) < n:
+                acc = acc - c[k] * um[k + 2 * (j + 1)]
+        um[k] = acc / d[k]
+    return np.moveaxis(um, 0, axis)
+
+
 # --------------------------------------------------------------------------
 # Chebyshev Helmholtz (chebyshev/la.py:131-351, la.pyx:708-973)
 # --------------------------------------------------------------------------

@@ -36,6 +36,14 @@ _SIGS = {
     "sfa_tdma_solve": "viiaaivlllv",
     "sfa_pdma_solve": "viiaaivlllv",
     "sfa_diagma_solve": "viiaaivlllv",
+    "sfa_tdma_o_lu": "vllv",
+    "sfa_fdma_lu": "vllv",
+    "sfa_heptadma_lu": "vllv",
+    "sfa_tdma_o_solve": "viiaaivlllv",
+    "sfa_twodma_solve": "viiaaivlllv",
+    "sfa_threedma_solve": "viiaaivlllv",
+    "sfa_fdma_solve": "viiaaivlllv",
+    "sfa_heptadma_solve": "viiaaivlllv",
     "sfa_helmholtz_lu": "ilvvvvvvvvlvvvviaaiv",
     "sfa_helmholtz_solve": "vviiaaaiivvvvav",
     "sfa_helmholtz_matvec": "vviiaaaiivvvvvvvlv",

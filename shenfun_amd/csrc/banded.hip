@@ -181,6 +181,135 @@ __global__ void diag_solve_kernel(T* __restrict__ u, LineMap g, const double* __
     for (int i = p; i < n; i += 2) v[i * g.sa] = divd(v[i * g.sa], d[i]);
 }
 
+// ----------------------------------------------------------------- other band patterns
+// TwoDMA (offsets 0, 2), ThreeDMA (0, 2, 4), FDMA (-2, 0, 2, 4) and HeptaDMA
+// (-4, -2, 0, 2, 4, 6, 8) with S = 2 (two parity chains per line), and the
+// odd-offset TDMA_O (-1, 0, 1) with S = 1 (one chain).  DIA rows in offset
+// order: NL lower bands (row NL - l holds offset -S l by column, after the LU
+// the unit-L multipliers), the diagonal (row NL), NU upper bands (row NL + k
+// holds offset S k by column).  Forward elimination subtracts the summed lower
+// terms, back substitution subtracts the upper terms one by one, as la.pyx's
+// *_inner_solve_ptr loops do; terms beyond either end of the line are absent.
+template <typename T, int NL, int NU, int S>
+__global__ void band_solve_kernel(T* __restrict__ u, LineMap g, const double* __restrict__ data,
+                                  int n, int64_t dls, int64_t skip)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nl = g.nlines;
+    if (gid >= S * nl) return;
+    const int p = (int)(gid / nl);
+    const int64_t L = gid - (int64_t)p * nl;
+    if (L < skip || p >= n) return;
+    T* v = u + g.off(L);
+    const int64_t sa = g.sa;
+    const double* rows = data + L * dls;
+    const double* d = rows + (int64_t)NL * n;
+    if constexpr (NL > 0) {
+        T w[NL];  // w[l] = u[i - S (l + 1)]
+#pragma unroll
+        for (int l = 0; l < NL; ++l) w[l] = zero<T>();
+        w[0] = v[p * sa];
+        for (int i = p + S; i < n; i += S) {
+            T s = mul(w[0], rows[(int64_t)(NL - 1) * n + i - S]);
+#pragma unroll
+            for (int l = 1; l < NL; ++l) {
+                const int c = i - S * (l + 1);
+                if (c >= 0) s = add(s, mul(w[l], rows[(int64_t)(NL - 1 - l) * n + c]));
+            }
+            const T cur = add(v[i * sa], mul(s, -1.0));
+            v[i * sa] = cur;
+#pragma unroll
+            for (int l = NL - 1; l > 0; --l) w[l] = w[l - 1];
+            w[0] = cur;
+        }
+    }
+    const int il = (S == 2) ? last_of_parity(n, p) : n - 1;
+    T nx[NU];  // nx[k] = u[i + S (k + 1)]
+#pragma unroll
+    for (int k = 0; k < NU; ++k) nx[k] = zero<T>();
+    for (int i = il; i >= 0; i -= S) {
+        T acc = v[i * sa];
+#pragma unroll
+        for (int k = 1; k <= NU; ++k) {
+            const int c = i + S * k;
+            if (c < n) acc = sub_mul(acc, rows[(int64_t)(NL + k) * n + c], nx[k - 1]);
+        }
+        const T cur = divd(acc, d[i]);
+        v[i * sa] = cur;
+#pragma unroll
+        for (int k = NU - 1; k > 0; --k) nx[k] = nx[k - 1];
+        nx[0] = cur;
+    }
+}
+
+// la.pyx:254-263 TDMA_O_LU, one thread per system
+__global__ void tdma_o_lu_kernel(double* __restrict__ data, int n, int64_t nbatch)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbatch) return;
+    double* ld = data + b * 3 * n;
+    double* d = ld + n;
+    double* ud = ld + 2 * n + 1;
+    for (int i = 1; i < n; ++i) {
+        ld[i - 1] = ld[i - 1] / d[i - 1];
+        d[i] = d[i] - ld[i - 1] * ud[i - 1];
+    }
+}
+
+// la.pyx:199-211 FDMA_LU, one thread per (system, parity)
+__global__ void fdma_lu_kernel(double* __restrict__ data, int n, int64_t nbatch)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= 2 * nbatch) return;
+    const int p = (int)(gid / nbatch);
+    const int64_t b = gid - (int64_t)p * nbatch;
+    double* ld = data + b * 4 * n;
+    double* d = ld + n;
+    double* u1 = ld + 2 * n + 2;
+    double* u2 = ld + 3 * n + 4;
+    for (int i = p + 2; i < n; i += 2) {
+        ld[i - 2] = ld[i - 2] / d[i - 2];
+        d[i] = d[i] - ld[i - 2] * u1[i - 2];
+        if (i < n - 2) u1[i] = u1[i] - ld[i - 2] * u2[i - 2];
+    }
+}
+
+// la.pyx:265-303 HeptaDMA_LU (band spacing 2), one thread per (system, parity)
+__global__ void heptadma_lu_kernel(double* __restrict__ data, int n, int64_t nbatch)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= 2 * nbatch) return;
+    const int p = (int)(gid / nbatch);
+    const int64_t bb = gid - (int64_t)p * nbatch;
+    double* a = data + bb * 7 * n;
+    double* b = a + n;
+    double* d = a + 2 * n;
+    double* e = a + 3 * n + 2;
+    double* f = a + 4 * n + 4;
+    double* g = a + 5 * n + 6;
+    double* h = a + 6 * n + 8;
+    for (int i = p; i < n - 4; i += 2) {
+        double lam = b[i] / d[i];
+        d[i + 2] -= lam * e[i];
+        e[i + 2] -= lam * f[i];
+        if (i < n - 6) f[i + 2] -= lam * g[i];
+        if (i < n - 8) g[i + 2] -= lam * h[i];
+        b[i] = lam;
+        lam = a[i] / d[i];
+        b[i + 2] -= lam * e[i];
+        d[i + 4] -= lam * f[i];
+        if (i < n - 6) e[i + 4] -= lam * g[i];
+        if (i < n - 8) f[i + 4] -= lam * h[i];
+        a[i] = lam;
+    }
+    for (int i = n - 4; i <= n - 3; ++i) {
+        if ((i & 1) != p) continue;
+        const double lam = b[i] / d[i];
+        d[i + 2] -= lam * e[i];
+        b[i] = lam;
+    }
+}
+
 // ----------------------------------------------------------------- Helmholtz
 struct HelmDiags {
     const double *A0, *A2, *A4, *Bm2, *B0, *B2;
@@ -556,6 +685,77 @@ extern "C" int sfa_tdma_lu(double* data, int64_t n, int64_t nbatch, void* stream
                        data, (int)n, nbatch);
     SFA_LAUNCH_CHECK();
     return SFA_OK;
+}
+
+template <int NL, int NU, int S>
+static int band_solve(const char* name, int64_t nmin, void* u, int dtype, int ndim,
+                      const int64_t* shape, const int64_t* strides, int axis, const double* data,
+                      int64_t n, int64_t dls, int64_t skip, void* stream)
+{
+    LineMap g;
+    int64_t len;
+    int rc = line_map(ndim, shape, strides, axis, &g, &len);
+    if (rc) return rc;
+    SFA_CHECK(dtype == SFA_F64 || dtype == SFA_C128, SFA_ERR_ARG, "%s: bad dtype %d", name, dtype);
+    SFA_CHECK(n >= nmin && n <= len, SFA_ERR_ARG, "%s: system size %lld must be in [%lld, %lld]",
+              name, (long long)n, (long long)nmin, (long long)len);
+    SFA_CHECK(n <= (1 << 26), SFA_ERR_ARG, "%s: system size %lld too large", name, (long long)n);
+    SFA_CHECK(data != nullptr && dls >= 0 && skip >= 0, SFA_ERR_ARG,
+              "%s: null factors or negative stride/skip", name);
+    if (g.nlines == 0) return SFA_OK;
+    const int bs = 256;
+    const dim3 grid = grid_for(S * g.nlines, bs);
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == SFA_C128)
+        hipLaunchKernelGGL((band_solve_kernel<double2, NL, NU, S>), grid, dim3(bs), 0, s,
+                           (double2*)u, g, data, (int)n, dls, skip);
+    else
+        hipLaunchKernelGGL((band_solve_kernel<double, NL, NU, S>), grid, dim3(bs), 0, s,
+                           (double*)u, g, data, (int)n, dls, skip);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+#define SFA_BAND_SOLVE(fn, NL, NU, S, NMIN)                                                      \
+    extern "C" int fn(void* u, int dtype, int ndim, const int64_t* shape,                        \
+                      const int64_t* strides, int axis, const double* data, int64_t n,           \
+                      int64_t data_line_stride, int64_t skip_lines, void* stream)                \
+    {                                                                                            \
+        return band_solve<NL, NU, S>(#fn, NMIN, u, dtype, ndim, shape, strides, axis, data, n,   \
+                                     data_line_stride, skip_lines, stream);                      \
+    }
+SFA_BAND_SOLVE(sfa_tdma_o_solve, 1, 1, 1, 1)
+SFA_BAND_SOLVE(sfa_twodma_solve, 0, 1, 2, 2)
+SFA_BAND_SOLVE(sfa_threedma_solve, 0, 2, 2, 4)
+SFA_BAND_SOLVE(sfa_fdma_solve, 1, 2, 2, 4)
+SFA_BAND_SOLVE(sfa_heptadma_solve, 2, 4, 2, 8)
+#undef SFA_BAND_SOLVE
+
+static int band_lu(const char* name, void (*kern)(double*, int, int64_t), int chains, int64_t nmin,
+                   double* data, int64_t n, int64_t nbatch, void* stream)
+{
+    SFA_CHECK(data != nullptr && n >= nmin && n <= (1 << 26) && nbatch >= 1, SFA_ERR_ARG,
+              "bad %s size n=%lld nbatch=%lld (n >= %lld)", name, (long long)n,
+              (long long)nbatch, (long long)nmin);
+    hipLaunchKernelGGL(kern, grid_for(chains * nbatch, 256), dim3(256), 0, (hipStream_t)stream,
+                       data, (int)n, nbatch);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+extern "C" int sfa_tdma_o_lu(double* data, int64_t n, int64_t nbatch, void* stream)
+{
+    return band_lu("TDMA_O", tdma_o_lu_kernel, 1, 1, data, n, nbatch, stream);
+}
+
+extern "C" int sfa_fdma_lu(double* data, int64_t n, int64_t nbatch, void* stream)
+{
+    return band_lu("FDMA", fdma_lu_kernel, 2, 4, data, n, nbatch, stream);
+}
+
+extern "C" int sfa_heptadma_lu(double* data, int64_t n, int64_t nbatch, void* stream)
+{
+    return band_lu("HeptaDMA", heptadma_lu_kernel, 2, 8, data, n, nbatch, stream);
 }
 
 extern "C" int sfa_pdma_lu(double* data, int64_t n, int64_t nbatch, void* stream)

@@ -1,9 +1,10 @@
 """Banded solvers of the hot path on the GPU (shenfun/la.py,
 shenfun/chebyshev/la.py interfaces; compute via the C ABI).
 
-* ``TDMA``, ``PDMA``, ``DiagMA``: ``__call__(b, u=None, axis=0, constraints=())``
-  (la.py:216-249, 291-543); LU on the device once (TDMA_LU/PDMA_LU), solve
-  along any axis in place on ``u``.
+* ``DiagMA``, ``TwoDMA``, ``TDMA``, ``TDMA_O``, ``ThreeDMA``, ``FDMA``, ``PDMA``,
+  ``HeptaDMA``: ``__call__(b, u=None, axis=0, constraints=())``
+  (la.py:216-249, 291-820); LU on the device once, solve along any axis in
+  place on ``u``.
 * ``Solver(mats)``: band pattern -> solver class (la.py:17-40).
 * ``SolverGeneric1ND(mats)``: one non-periodic axis, per-mode factors
   (la.py:1183-1434), mode 0 solved by its own solver (with constraints).
@@ -155,6 +156,80 @@ class PDMA(_Banded):
         self._lu = None
 
 
+class TDMA_O(_Banded):
+    """Tridiagonal solver, offsets -1, 0, 1 (la.py:396-447): one chain per line."""
+    offsets = (-1, 0, 1)
+    lu_fn = "sfa_tdma_o_lu"
+    solve_fn = "sfa_tdma_o_solve"
+
+    def _pin_row0(self):
+        # row 0 replaced by the unit row (SparseMatrixSolver.apply_constraints, la.py:77-120)
+        self._data[1, 0] = 1.0
+        self._data[2, 1] = 0.0
+        self._lu = None
+
+
+class TwoDMA(_Banded):
+    """Upper 2-diagonal solver, offsets 0, 2 (la.py:615-659); no LU."""
+    offsets = (0, 2)
+    lu_fn = None
+    solve_fn = "sfa_twodma_solve"
+
+    def _pin_row0(self):
+        # la.py:628-637
+        self._data[0, 0] = 1.0
+        self._data[1, 2] = 0.0
+        self._lu = None
+
+
+class ThreeDMA(_Banded):
+    """Upper 3-diagonal solver, offsets 0, 2, 4 (la.py:661-709); no LU.  The +4
+    band enters the back substitution from its own DIA row (the reference reads
+    the +2 row there, la.py:688)."""
+    offsets = (0, 2, 4)
+    lu_fn = None
+    solve_fn = "sfa_threedma_solve"
+
+    def _pin_row0(self):
+        # la.py:674-684
+        self._data[0, 0] = 1.0
+        self._data[1, 2] = 0.0
+        self._data[2, 4] = 0.0
+        self._lu = None
+
+
+class FDMA(_Banded):
+    """4-diagonal solver, offsets -2, 0, 2, 4 (la.py:545-613)."""
+    offsets = (-2, 0, 2, 4)
+    lu_fn = "sfa_fdma_lu"
+    solve_fn = "sfa_fdma_solve"
+
+    def _pin_row0(self):
+        # la.py:575-586
+        self._data[1, 0] = 1.0
+        self._data[2, 2] = 0.0
+        self._data[3, 4] = 0.0
+        self._lu = None
+
+
+class HeptaDMA(_Banded):
+    """Heptadiagonal solver, offsets -4, -2, 0, 2, 4, 6, 8 (la.py:711-820)."""
+    offsets = (-4, -2, 0, 2, 4, 6, 8)
+    lu_fn = "sfa_heptadma_lu"
+    solve_fn = "sfa_heptadma_solve"
+
+    def _pin_row0(self):
+        # la.py:724-741
+        for r, off in ((2, 0), (3, 2), (4, 4), (5, 6), (6, 8)):
+            self._data[r, off] = 1.0 if off == 0 else 0.0
+        self._lu = None
+
+
+# band patterns of SparseMatrix.get_solver (matrixbase.py:439-474), narrowest
+# first: SolverGeneric1ND takes the first whose offsets cover every mode's keys
+_PATTERNS = (DiagMA, TwoDMA, TDMA_O, TDMA, ThreeDMA, FDMA, PDMA, HeptaDMA)
+
+
 class Solve:
     """General sparse fallback of the reference (SuperLU, la.py:823-844) is not
     part of the GPU hot path."""
@@ -187,15 +262,14 @@ class SolverGeneric1ND:
         for m in mats:
             keys |= set(m.pmat.keys())
         keys = tuple(sorted(keys))
-        if keys == (0,):
-            self.kind, self.offsets, lu, sol = "diag", (0,), None, "sfa_diagma_solve"
-        elif set(keys) <= {-2, 0, 2}:
-            self.kind, self.offsets, lu, sol = "tdma", (-2, 0, 2), "sfa_tdma_lu", "sfa_tdma_solve"
-        elif set(keys) <= {-4, -2, 0, 2, 4}:
-            self.kind, self.offsets, lu, sol = "pdma", (-4, -2, 0, 2, 4), "sfa_pdma_lu", "sfa_pdma_solve"
+        for cls in _PATTERNS:
+            if set(keys) <= set(cls.offsets):
+                break
         else:
             raise NotImplementedError(f"band pattern {keys} not supported")
-        self._lu_fn, self._solve_fn = lu, sol
+        self.kind = cls.__name__.lower().replace("diagma", "diag")
+        self.offsets = cls.offsets
+        self._lu_fn, self._solve_fn = cls.lu_fn, cls.solve_fn
         # per-mode DIA data (nmodes, rows, n), C order over the non-axis dims
         data = np.zeros((self.nmodes, len(self.offsets), n))
         for m in mats:
@@ -214,7 +288,7 @@ class SolverGeneric1ND:
         # mode 0 (Fourier wavenumber 0 / (0, 0)) has its own solver
         mat0 = SparseMatrix({off: data[0, r, max(off, 0):n + min(off, 0)]
                              for r, off in enumerate(self.offsets)}, (n, n))
-        self.sol0 = {"diag": DiagMA, "tdma": TDMA, "pdma": PDMA}[self.kind](mat0)
+        self.sol0 = cls(mat0)
 
     def perform_lu(self, device):
         if self._dev is None or self._dev.device != device:

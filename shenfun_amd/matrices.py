@@ -72,13 +72,25 @@ class SparseMatrix(dict):
         if float(self.scale) == 0:
             return la.Solve
         keys = tuple(self.sorted_keys())
-        if keys == (0,):
-            return la.DiagMA
-        if keys == (-2, 0, 2):
-            return la.TDMA
-        if keys == (-4, -2, 0, 2, 4):
-            return la.PDMA
+        if keys == (-1, 0, 1):
+            return la.TDMA_O if self.issymmetric else la.Solve
+        for cls in (la.DiagMA, la.TwoDMA, la.TDMA, la.ThreeDMA, la.FDMA, la.PDMA, la.HeptaDMA):
+            if keys == cls.offsets:
+                return cls
         return la.Solve
+
+    @property
+    def issymmetric(self):
+        """Symmetric band pattern with equal mirrored diagonals (matrixbase.py:493-503)."""
+        if np.sum(np.array(list(self.keys()))) != 0:
+            return False
+        for key in self:
+            if key <= 0:
+                continue
+            if -key not in self or not np.all(
+                    abs(self.diagonal_array(key) - self.diagonal_array(-key)) < 1e-16):
+                return False
+        return True
 
 
 class SpectralMatrix(SparseMatrix):

@@ -43,6 +43,30 @@ def test_error_reporting_without_gpu_work():
     assert rc == -1
 
 
+def test_band_pattern_solver_selection_and_size_checks():
+    """SparseMatrix.get_solver (matrixbase.py:439-474) and the host-side
+    minimum sizes of the band solvers (no device work)."""
+    import shenfun_amd as sfa
+    from shenfun_amd import _lib, la
+    cases = {(0,): la.DiagMA, (0, 2): la.TwoDMA, (-2, 0, 2): la.TDMA, (0, 2, 4): la.ThreeDMA,
+             (-2, 0, 2, 4): la.FDMA, (-4, -2, 0, 2, 4): la.PDMA,
+             (-4, -2, 0, 2, 4, 6, 8): la.HeptaDMA}
+    for keys, cls in cases.items():
+        assert sfa.SparseMatrix({k: 1.0 + (k == 0) for k in keys}, (12, 12)).get_solver() is cls
+    assert sfa.SparseMatrix({-1: 1.0, 0: -2.0, 1: 1.0}, (9, 9)).get_solver() is la.TDMA_O
+    assert sfa.SparseMatrix({-1: 2.0, 0: -2.0, 1: 1.0}, (9, 9)).get_solver() is la.Solve
+    assert sfa.SparseMatrix({-2: 1.0, 0: 2.0, 4: 1.0}, (9, 9)).get_solver() is la.Solve
+    L = _lib.lib()
+    assert L.sfa_heptadma_lu(None, 8, 1, None) == -1  # null data
+    for fn, nmin in ((L.sfa_fdma_lu, 4), (L.sfa_heptadma_lu, 8), (L.sfa_tdma_o_lu, 1)):
+        assert fn(None, nmin - 1, 1, None) == -1
+    for name, nmin in (("sfa_twodma_solve", 2), ("sfa_threedma_solve", 4), ("sfa_fdma_solve", 4),
+                       ("sfa_heptadma_solve", 8), ("sfa_tdma_o_solve", 1)):
+        rc = getattr(L, name)(None, _lib.F64, 1, _lib.arr([16]), _lib.arr([1]), 0, None, nmin - 1,
+                              0, 0, None)
+        assert rc == -1 and name.encode() in L.sfa_last_error()
+
+
 def test_matrices_match_oracle_closed_forms():
     from oracle import spectral as S
     import shenfun_amd as sfa

@@ -24,14 +24,93 @@ def sfa():
     return shenfun_amd
 
 
-# tests/test_la.py:7-15 of the reference (N = 10 matrices), restricted to the
-# DiagMA / TDMA / PDMA patterns of the hot path
+# tests/test_la.py:8-15 of the reference (N = 10 matrices: DiagMA, TwoDMA,
+# TDMA_O, TDMA, FDMA, PDMA), plus ThreeDMA and HeptaDMA systems
 N10 = 10
 MATS = [
     {0: np.arange(N10) + 1.0},
+    {0: -2.0, 2: 1.0},
+    {-1: 1.0, 0: -2.0, 1: 1.0},
     {-2: 1.0, 0: -2.0, 2: 1.0},
+    {-2: 1.0, 0: -2.0, 2: 1.0, 4: 0.1},
     {-4: 0.1, -2: 1.0, 0: -2.0, 2: 1.0, 4: 0.1},
+    {0: -2.0, 2: 1.0, 4: 0.3},
+    {-4: 0.1, -2: 0.5, 0: -4.0, 2: 1.0, 4: 0.3, 6: -0.2, 8: 0.1},
 ]
+
+# pattern -> (offsets, oracle LU, oracle solve, C-ABI LU, C-ABI solve)
+BANDS = {
+    "twodma": ((0, 2), None, "twodma_solve", None, "sfa_twodma_solve"),
+    "tdma_o": ((-1, 0, 1), "tdma_o_lu", "tdma_o_solve", "sfa_tdma_o_lu", "sfa_tdma_o_solve"),
+    "threedma": ((0, 2, 4), None, "threedma_solve", None, "sfa_threedma_solve"),
+    "fdma": ((-2, 0, 2, 4), "fdma_lu", "fdma_solve", "sfa_fdma_lu", "sfa_fdma_solve"),
+    "heptadma": ((-4, -2, 0, 2, 4, 6, 8), "heptadma_lu", "heptadma_solve", "sfa_heptadma_lu",
+                 "sfa_heptadma_solve"),
+}
+
+
+def _rand_bands(rng, offs, n):
+    d = {o: rng.uniform(0.2, 1.0, n - abs(o)) * (1 if o % 4 else -1) for o in offs}
+    d[0] = rng.uniform(4.0, 5.0, n)
+    return d
+
+
+@pytest.mark.parametrize("kind", sorted(BANDS))
+@pytest.mark.parametrize("n", [8, 9, 64, 255])
+def test_band_patterns_per_line_factors(sfa, kind, n):
+    """Per-line factors (data_line_stride) and skip_lines through the C ABI, LU
+    on the device for all lines at once, against the oracle per line; the
+    space axis is strided (axis 1 of (lines, n, 3))."""
+    from shenfun_amd import _lib
+    print(f"[band {kind} n={n}]", flush=True)
+    offs, olu, osol, clu, csol = BANDS[kind]
+    rng = np.random.default_rng(n + len(offs))
+    nl = 5
+    mats = [_rand_bands(rng, offs, n) for _ in range(nl * 3)]
+    data = np.stack([S.band_dia(d, n, offs) for d in mats])  # (lines, rows, n)
+    dd = dev(data)
+    if clu is not None:
+        _lib.call(clu, _lib.ptr(dd), n, nl * 3, _lib.stream_ptr())
+    b = rng.standard_normal((nl, n, 3)) + 1j * rng.standard_normal((nl, n, 3))
+    u = dev(b)
+    _lib.call(csol, _lib.ptr(u), _lib.C128, 3, _lib.arr(u.shape), _lib.arr(u.stride()), 1,
+              _lib.ptr(dd), n, data.shape[1] * n, 1, _lib.stream_ptr())
+    got = u.cpu().numpy()
+    assert np.array_equal(got[0, :, 0], b[0, :, 0])  # line 0 skipped
+    for i in range(nl):
+        for j in range(3):
+            L = i * 3 + j
+            if L == 0:
+                continue
+            ld = getattr(S, olu)(data[L]) if olu else data[L]
+            if olu:
+                assert rel(dd[L], ld) < 1e-13
+            ref = getattr(S, osol)(b[i, :, j], ld, 0)
+            assert rel(got[i, :, j], ref) < TOL
+            assert rel(got[i, :, j], np.linalg.solve(S.dense(mats[L], n), b[i, :, j])) < 1e-11
+
+
+@pytest.mark.parametrize("kind", sorted(BANDS))
+def test_band_patterns_constraint_row0(sfa, kind):
+    """apply_constraints((0, val)): row 0 becomes the unit row and b[0] = val
+    (la.py:575-586, 628-637, 674-684, 724-741)."""
+    offs = BANDS[kind][0]
+    n = 14
+    rng = np.random.default_rng(7)
+    d = _rand_bands(rng, offs, n)
+    if kind == "tdma_o":
+        d[1] = d[-1]  # get_solver takes TDMA_O for symmetric (-1, 0, 1) only
+    M = sfa.SparseMatrix(d, (n, n))
+    sol = sfa.la.Solver(M)
+    assert type(sol).__name__.lower() == kind
+    b = rng.standard_normal((n, 4))
+    u = sol(dev(b), axis=0, constraints=((0, 0.25),))
+    D = S.dense(d, n)
+    D[0, :] = 0.0
+    D[0, 0] = 1.0
+    bb = b.copy()
+    bb[0] = 0.25
+    assert rel(u, np.linalg.solve(D, bb)) < 1e-11
 
 
 @pytest.mark.parametrize("di", MATS)

@@ -301,6 +301,69 @@ def test_tdma_pdma_diag_vs_dense():
     assert np.allclose(S.diagma_solve(np.ones(N), dg), 1 / dg)
 
 
+# reference tests/test_la.py:8-15 (N = 10) for the patterns beyond TDMA/PDMA,
+# plus random diagonally dominant systems of every pattern at odd and even n
+BANDS = {
+    "twodma": ((0, 2), None, "twodma_solve"),
+    "tdma_o": ((-1, 0, 1), "tdma_o_lu", "tdma_o_solve"),
+    "threedma": ((0, 2, 4), None, "threedma_solve"),
+    "fdma": ((-2, 0, 2, 4), "fdma_lu", "fdma_solve"),
+    "heptadma": ((-4, -2, 0, 2, 4, 6, 8), "heptadma_lu", "heptadma_solve"),
+}
+
+
+def _band_oracle(kind, d, n, b, axis):
+    offs, lu, sol = BANDS[kind]
+    data = S.band_dia(d, n, offs)
+    if lu is not None:
+        data = getattr(S, lu)(data)
+    return getattr(S, sol)(b, data, axis)
+
+
+@pytest.mark.parametrize("kind,d", [
+    ("twodma", {0: -2.0, 2: 1.0}),
+    ("tdma_o", {-1: 1.0, 0: -2.0, 1: 1.0}),
+    ("fdma", {-2: 1.0, 0: -2.0, 2: 1.0, 4: 0.1}),
+])
+def test_band_patterns_reference_matrices(kind, d):
+    """test_la.py:19-46: the solver's u == the sparse direct solve, along both axes."""
+    N = 10
+    D = S.dense(d, N)
+    u = _band_oracle(kind, d, N, np.ones(N), 0)
+    assert np.allclose(u, np.linalg.solve(D, np.ones(N)), rtol=1e-13, atol=1e-13)
+    bh = np.ones((N, N))
+    assert np.allclose(_band_oracle(kind, d, N, bh, 1)[0], u, rtol=1e-13, atol=1e-13)
+    assert np.allclose(_band_oracle(kind, d, N, bh, 0)[:, 0], u, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("kind", sorted(BANDS))
+@pytest.mark.parametrize("n", [8, 9, 33])
+def test_band_patterns_vs_dense(kind, n):
+    """Every band value enters: random bands, complex right-hand sides, both axes."""
+    rng = np.random.default_rng(n)
+    offs = BANDS[kind][0]
+    d = {o: rng.uniform(0.2, 1.0, n - abs(o)) * (1 if o % 4 else -1) for o in offs}
+    d[0] = rng.uniform(4.0, 5.0, n)
+    D = S.dense(d, n)
+    b = rng.standard_normal((n, 4)) + 1j * rng.standard_normal((n, 4))
+    ref = np.linalg.solve(D, b)
+    assert np.allclose(_band_oracle(kind, d, n, b, 0), ref, rtol=1e-12, atol=1e-13)
+    assert np.allclose(_band_oracle(kind, d, n, b.T.copy(), 1), ref.T, rtol=1e-12, atol=1e-13)
+
+
+def test_threedma_reads_its_own_plus4_band():
+    """The +4 band differs from the +2 band: a solve that took the +2 row for
+    it (as la.pyx:574 does) would miss the dense solution."""
+    n = 12
+    d = {0: np.full(n, 4.0), 2: np.full(n - 2, 1.0), 4: np.full(n - 4, -0.5)}
+    b = np.arange(1.0, n + 1)
+    u = _band_oracle("threedma", d, n, b, 0)
+    assert np.allclose(S.dense(d, n) @ u, b, rtol=1e-13, atol=1e-13)
+    wrong = S.band_dia(d, n, (0, 2, 4))
+    wrong[2, 4:] = wrong[1, 4:]
+    assert not np.allclose(S.threedma_solve(b, wrong, 0), u)
+
+
 @pytest.mark.parametrize("bc", ["D", "N"])
 def test_helmholtz_lu_solve_vs_dense(bc):
     """LU_Helmholtz + Solve_Helmholtz == dense solve of alfa A + beta B;
