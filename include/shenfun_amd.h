@@ -94,6 +94,22 @@ int sfa_cheb_backward_gl(const void* in, void* out, int dtype, int ndim, const i
                          const int64_t* in_strides, const int64_t* out_strides, int axis,
                          const double* K2, void* stream);
 
+/* Padding / truncation along `axis` for dealiasing (padded spaces evaluate
+ * their transform on M = floor(padding_factor N) points).  mode 0 pads (in:
+ * the N coefficients, out: the M slots), replaces _padding_backward; mode 1
+ * truncates (in: M, out: N), replaces _truncation_forward; mode 2 copies with
+ * entries [z0, z1) zeroed (dealias_direct, same length).  kind 0: Chebyshev /
+ * Legendre, the first N - nbc entries kept and the nbc boundary slots moved to
+ * the end (spectralbase.py:1362-1389); kind 1: R2C, in/out complex, N the real
+ * length, Nyquist halved on padding and doubled on truncation, imaginary part
+ * dropped (fourier/bases.py:409-435); kind 2: C2C, wavenumbers 0..N/2 and the
+ * last N/2 kept, Nyquist split between +-N/2 (fourier/bases.py:570-598).
+ * Shapes must agree off the axis; in and out must not alias. */
+int sfa_pad_axis(const void* in, void* out, int dtype, int ndim, const int64_t* in_shape,
+                 const int64_t* in_strides, const int64_t* out_shape, const int64_t* out_strides,
+                 int axis, int mode, int kind, int64_t N, int64_t nbc, int64_t z0, int64_t z1,
+                 void* stream);
+
 /* Build the dense Legendre transform matrix on the device (the plan step of
  * the recursive transforms, spectralbase.py:484-491, 550-558).
  * analysis=1: M[k][j] = w_j (L_k(x_j) + K2[k] L_{k+2}(x_j))  (k < N-2 when

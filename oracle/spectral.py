@@ -353,6 +353,105 @@ def tps_backward(c, bases, shape):
 
 
 # --------------------------------------------------------------------------
+# Dealiasing: padded transforms (3/2 rule) and direct 2/3-rule truncation
+# --------------------------------------------------------------------------
+
+
+def _nbc(bc):
+    return 0 if bc is None else 2
+
+
+def pad_backward(c, family, bc, N, M, axis=0):
+    """_padding_backward: N coefficients -> M slots along ``axis``.
+
+    Polynomial (spectralbase.py:1368-1389): dofs 0..N-nbc-1 kept, the nbc
+    boundary slots moved to the end.  R2C (fourier/bases.py:419-429): the
+    N/2+1 entries kept, even N: the Nyquist entry real and halved.  C2C
+    (fourier/bases.py:580-594): wavenumbers 0..N/2 and the last N/2 kept,
+    even N: both copies of the Nyquist entry halved.  ``N`` is the space's
+    size (the real length for R2C), ``M`` the padded physical length.
+    """
+    cm = np.moveaxis(np.asarray(c), axis, 0)
+    if family == 'fourier-r2c':
+        out = np.zeros((M // 2 + 1,) + cm.shape[1:], dtype=cm.dtype)
+        out[:N // 2 + 1] = cm
+        if N % 2 == 0:
+            out[N // 2] = out[N // 2].real * 0.5
+    elif family == 'fourier-c2c':
+        out = np.zeros((M,) + cm.shape[1:], dtype=cm.dtype)
+        h = N // 2
+        out[:h + 1] = cm[:h + 1]
+        if h > 0:
+            out[M - h:] = cm[N - h:]
+        if N % 2 == 0:
+            out[h] *= 0.5
+            out[M - h] *= 0.5
+    else:
+        nb = _nbc(bc)
+        out = np.zeros((M,) + cm.shape[1:], dtype=cm.dtype)
+        out[:N - nb] = cm[:N - nb]
+        if nb:
+            out[M - nb:] = cm[N - nb:]
+    return np.moveaxis(out, 0, axis)
+
+
+def truncate_forward(c, family, bc, N, axis=0):
+    """_truncation_forward: M coefficients -> the N of the space.
+
+    Polynomial (spectralbase.py:1362-1366): dofs 0..N-nbc-1, boundary slots 0.
+    R2C (fourier/bases.py:409-418): first N/2+1 entries, even N: Nyquist
+    real and doubled.  C2C (fourier/bases.py:570-578): wavenumbers 0..N/2 plus
+    the last N/2 added (the two Nyquist halves meet at N/2 for even N).
+    """
+    cm = np.moveaxis(np.asarray(c), axis, 0)
+    M = cm.shape[0]
+    if family == 'fourier-r2c':
+        out = cm[:N // 2 + 1].copy()
+        if N % 2 == 0:
+            out[N // 2] = out[N // 2].real * 2
+    elif family == 'fourier-c2c':
+        h = N // 2
+        out = np.zeros((N,) + cm.shape[1:], dtype=cm.dtype)
+        out[:h + 1] = cm[:h + 1]
+        if h > 0:
+            out[N - h:] += cm[M - h:]
+    else:
+        nb = _nbc(bc)
+        out = np.zeros((N,) + cm.shape[1:], dtype=cm.dtype)
+        out[:N - nb] = cm[:N - nb]
+    return np.moveaxis(out, 0, axis)
+
+
+def dealias_direct_zero(c, family, N, axis=0):
+    """dealias_direct: the top third of the wavenumbers zeroed before backward.
+
+    Polynomial: k >= 2N//3 (spectralbase.py:1374-1376); R2C: k >= N//3
+    (fourier/bases.py:430-433); C2C: slice(N//3, -(N//3)+1)
+    (fourier/bases.py:595-598).
+    """
+    cm = np.moveaxis(np.array(c, copy=True), axis, 0)
+    if family == 'fourier-r2c':
+        cm[N // 3:] = 0
+    elif family == 'fourier-c2c':
+        cm[slice(N // 3, -(N // 3) + 1)] = 0
+    else:
+        cm[2 * N // 3:] = 0
+    return np.moveaxis(cm, 0, axis)
+
+
+def padded_scalar_product(family, bc, f, N, axis=0):
+    """Scalar product of a padded space: on the M = f.shape[axis] points,
+    then truncated (spectralbase.py:225-229)."""
+    return truncate_forward(basis_scalar_product(family, bc, f, axis), family, bc, N, axis)
+
+
+def padded_backward(family, bc, c, N, M, axis=0):
+    """Backward of a padded space: padded coefficients synthesised on M
+    points (spectralbase.py:291-310)."""
+    return basis_backward(family, bc, pad_backward(c, family, bc, N, M, axis), axis, N=M)
+
+
+# --------------------------------------------------------------------------
 # Galerkin matrices (closed forms).  Returned as dict offset -> diagonal.
 # --------------------------------------------------------------------------
 

@@ -109,7 +109,7 @@ class _Planned:
 class SpectralBase(_Planned):
     """Common part of the 1D spaces."""
 
-    def __init__(self, N, dtype=np.float64, device=None):
+    def __init__(self, N, dtype=np.float64, device=None, padding_factor=1, dealias_direct=False):
         self.N = int(N)
         self.dtype = np.dtype(dtype)
         self.axis = 0
@@ -117,6 +117,15 @@ class SpectralBase(_Planned):
         self.tensorproductspace = None
         self.device = _dev(device)
         self._cache = {}
+        # spectralbase.py:51-54: a padded space keeps N coefficients and
+        # evaluates on floor(padding_factor N) points
+        self.padding_factor = 1
+        if padding_factor != 1:
+            self.padding_factor = np.floor(self.N * padding_factor) / self.N if self.N > 0 else 1
+        self.dealias_direct = bool(dealias_direct)
+        if self.dealias_direct and self.padding_factor != 1:
+            raise ValueError("dealias_direct cannot be combined with padding_factor != 1")
+        self._twin_space = None
         self._init_planner()
 
     def _planned_shape(self, name, which):
@@ -142,10 +151,70 @@ class SpectralBase(_Planned):
         return slice(0, self.dim())
 
     def shape(self, forward_output=True):
-        return self.N
+        """Spectral (N) or physical (padded: floor(padding_factor N)) length
+        along the axis (spectralbase.py:860-871)."""
+        return self.N if forward_output else self._phys_len()
 
     def mesh(self):
         return self.points_and_weights()[0]
+
+    # ------------------------------------------------------------ dealiasing
+    @property
+    def is_padded(self):
+        return self.padding_factor != 1
+
+    def _phys_len(self):
+        if self.padding_factor != 1:
+            return int(np.floor(self.padding_factor * self.N))
+        return self.N
+
+    def get_dealiased(self, padding_factor=1.5, dealias_direct=False, **kw):
+        """Space otherwise as self for dealiasing (spectralbase.py:1111-1140):
+        padded by ``padding_factor`` or, with ``dealias_direct``, the top third
+        of the wavenumbers zeroed before backward."""
+        if padding_factor == 1 and not dealias_direct:
+            return self
+        return self._clone(self.N, padding_factor, dealias_direct)
+
+    def _clone(self, N, padding_factor=1, dealias_direct=False):
+        raise NotImplementedError
+
+    def _twin(self):
+        """Unpadded space of the padded length: runs this space's transforms
+        on the floor(padding_factor N) points."""
+        if self._twin_space is None:
+            self._twin_space = self._clone(self._phys_len())
+        t = self._twin_space
+        t.axis = self.axis
+        t.dimensions = self.dimensions
+        return t
+
+    def _tmp(self, key, shape, dtype):
+        k = ("tmp", key, tuple(int(v) for v in shape), dtype)
+        if k not in self._cache:
+            self._cache[k] = torch.empty(k[2], dtype=dtype, device=self.device)
+        return self._cache[k]
+
+    _pad_kind = 0  # sfa_pad_axis family: 0 polynomial, 1 R2C, 2 C2C
+
+    def _pad(self, inp, out, mode, nbc=0, z0=0, z1=0):
+        """sfa_pad_axis: mode 0 pad, 1 truncate, 2 copy with [z0, z1) zeroed."""
+        code = _lib.C128 if inp.dtype == torch.complex128 else _lib.F64
+        ishp, ist = _geom(inp)
+        oshp, ost = _geom(out)
+        _lib.call("sfa_pad_axis", _lib.ptr(inp), _lib.ptr(out), code, inp.dim(), ishp, ist, oshp,
+                  ost, self.axis, mode, self._pad_kind, self.N, nbc, z0, z1, _lib.stream_ptr())
+        return out
+
+    def _direct_range(self):
+        """Wavenumbers zeroed by dealias_direct (spectralbase.py:1374-1376)."""
+        return 2 * self.N // 3, self.N
+
+    def _dealias_input(self, inp, nbc=0):
+        """dealias_direct: backward of a copy with the top third zeroed (the
+        reference zeroes its planned input array in place)."""
+        z0, z1 = self._direct_range()
+        return self._pad(inp, self._tmp("dd", inp.shape, inp.dtype), 2, nbc, z0, max(z0, z1))
 
     @property
     def is_composite(self):
@@ -236,11 +305,24 @@ class _PolyBase(SpectralBase):
         raise NotImplementedError
 
     # ------------------------------------------------------------ transforms
+    def _clone(self, N, padding_factor=1, dealias_direct=False):
+        return type(self)(N, quad=self.quad, dtype=self.dtype, device=self.device,
+                          padding_factor=padding_factor, dealias_direct=dealias_direct)
+
     def scalar_product(self, input_array=None, output_array=None, kind=None):
         kind = self._kind(kind)
         inp = self._tensor(input_array)
         _check(inp, "input_array")
+        if inp.shape[self.axis] != self._phys_len():
+            raise ValueError("input_array has the wrong length along axis")
         out = self._out(inp, output_array, self.N, inp.dtype)
+        if self.is_padded:
+            # padded-length scalar product, truncated to the N coefficients
+            # (spectralbase.py:225-229, _truncation_forward :1362-1366)
+            tmp = self._tmp("sp", inp.shape, inp.dtype)
+            self._twin()._scalar_product(inp, tmp, kind)
+            self._pad(tmp, out, 1, self.N - self.dim())
+            return out
         self._scalar_product(inp, out, kind)
         return out
 
@@ -250,9 +332,21 @@ class _PolyBase(SpectralBase):
         _check(inp, "input_array")
         if inp.shape[self.axis] != self.N:
             raise ValueError("input_array has the wrong length along axis")
-        out = self._out(inp, output_array, self.N, inp.dtype)
+        out = self._out(inp, output_array, self._phys_len(), inp.dtype)
         if out.data_ptr() == inp.data_ptr():
             raise ValueError("backward cannot run in place")
+        nbc = self.N - self.dim()
+        if self.is_padded:
+            # zero-padded coefficients, boundary slots moved to the end
+            # (_padding_backward, spectralbase.py:1368-1389), then the
+            # padded-length synthesis
+            shp = list(inp.shape)
+            shp[self.axis] = self._phys_len()
+            tmp = self._pad(inp, self._tmp("bw", shp, inp.dtype), 0, nbc)
+            self._twin()._backward(tmp, out, kind)
+            return out
+        if self.dealias_direct:
+            inp = self._dealias_input(inp, nbc)
         self._backward(inp, out, kind)
         return out
 
@@ -292,7 +386,7 @@ class ChebyshevBase(_PolyBase):
         """chebyshev/bases.py:202-216.  GC: x_j = cos(pi(2j+1)/(2N)), w = pi/N;
         GL: x_j = cos(pi j/(N-1)), w = pi/(N-1) with both ends halved (both
         descending)."""
-        N = self.N if N is None else N
+        N = self._phys_len() if N is None else N
         j = np.arange(N, dtype=float)
         if self.quad == "GL":
             w = np.full(N, PI / (N - 1))
@@ -311,14 +405,14 @@ class ChebyshevBase(_PolyBase):
         return h
 
     def _vander(self):
-        x, _ = self.points_and_weights()
+        x, _ = self.points_and_weights(self.N)
         j = np.arange(self.N)
         return np.cos(np.outer(np.arccos(x), j))  # T_k(x_j) = cos(k theta_j)
 
     def _analysis_matrix(self):
         def build():
             V = self._vander()
-            _, w = self.points_and_weights()
+            _, w = self.points_and_weights(self.N)
             M = (V * w[:, None]).T.copy()  # M[k, j] = w_j T_k(x_j)
             if self.is_composite:
                 K2 = self.stencil_k2()
@@ -370,7 +464,7 @@ class LegendreBase(_PolyBase):
 
     def points_and_weights(self, N=None):
         """Legendre-Gauss nodes (ascending) and weights (legendre/bases.py:121-126)."""
-        N = self.N if N is None else N
+        N = self._phys_len() if N is None else N
         key = ("pw", N)
         if key not in self._cache:
             self._cache[key] = leggauss(N)
@@ -381,7 +475,7 @@ class LegendreBase(_PolyBase):
         return 2.0 / (2 * k + 1)
 
     def _xw(self):
-        x, w = self.points_and_weights()
+        x, w = self.points_and_weights(self.N)
         return (self._dev_array("x", lambda: x), self._dev_array("w", lambda: w))
 
     def _analysis_matrix(self):
@@ -456,16 +550,24 @@ class FourierBase(SpectralBase):
         return "Periodic"
 
     def points_and_weights(self, N=None):
-        N = self.N if N is None else N
+        N = self._phys_len() if N is None else N
         return np.arange(N, dtype=float) * 2 * PI / N, np.array([1.0 / N])
+
+    def _clone(self, N, padding_factor=1, dealias_direct=False):
+        return type(self)(N, device=self.device, padding_factor=padding_factor,
+                          dealias_direct=dealias_direct)
 
     def forward(self, input_array=None, output_array=None, kind=None):
         return self.scalar_product(input_array, output_array, kind)
 
 
 class R2C(FourierBase):
-    def __init__(self, N, device=None):
-        super().__init__(N, np.float64, device)
+    _pad_kind = 1
+
+    def __init__(self, N, device=None, padding_factor=1, dealias_direct=False):
+        super().__init__(N, np.float64, device, padding_factor, dealias_direct)
+        if self._phys_len() % 2:
+            raise NotImplementedError(f"R2C on {self._phys_len()} points: even lengths only")
 
     def short_name(self):
         return "R2C"
@@ -474,15 +576,27 @@ class R2C(FourierBase):
         return self.N // 2 + 1
 
     def shape(self, forward_output=True):
-        return self.N // 2 + 1 if forward_output else self.N
+        return self.N // 2 + 1 if forward_output else self._phys_len()
 
     def wavenumbers(self):
         return np.fft.rfftfreq(self.N, 1.0 / self.N).astype(int)  # fourier/bases.py:334-342
 
+    def _direct_range(self):
+        return self.N // 3, self.N // 2 + 1  # fourier/bases.py:430-433
+
     def scalar_product(self, input_array=None, output_array=None, kind=None):
         inp = self._tensor(input_array)
         _check(inp, "input_array", torch.float64)
+        if inp.shape[self.axis] != self._phys_len():
+            raise ValueError("input_array has the wrong length along axis")
         out = self._out(inp, output_array, self.N // 2 + 1, torch.complex128)
+        if self.is_padded:
+            # FFT on the padded points (normalised by 1/M), truncated with the
+            # Nyquist rule (fourier/bases.py:409-418)
+            shp = list(inp.shape)
+            shp[self.axis] = self._phys_len() // 2 + 1
+            tmp = self._twin().scalar_product(inp, self._tmp("sp", shp, torch.complex128))
+            return self._pad(tmp, out, 1)
         shp, sin = _geom(inp)
         _, sout = _geom(out)
         _lib.call("sfa_fft_r2c", _lib.ptr(inp), _lib.ptr(out), inp.dim(), shp, sin, sout, self.axis,
@@ -492,7 +606,16 @@ class R2C(FourierBase):
     def backward(self, input_array=None, output_array=None, kind=None):
         inp = self._tensor(input_array)
         _check(inp, "input_array", torch.complex128)
-        out = self._out(inp, output_array, self.N, torch.float64)
+        if inp.shape[self.axis] != self.N // 2 + 1:
+            raise ValueError("input_array has the wrong length along axis")
+        out = self._out(inp, output_array, self._phys_len(), torch.float64)
+        if self.is_padded:
+            shp = list(inp.shape)
+            shp[self.axis] = self._phys_len() // 2 + 1
+            tmp = self._pad(inp, self._tmp("bw", shp, torch.complex128), 0)
+            return self._twin().backward(tmp, out)
+        if self.dealias_direct:
+            inp = self._dealias_input(inp)
         shp = list(inp.shape)
         shp[self.axis] = self.N
         _, sin = _geom(inp)
@@ -503,8 +626,10 @@ class R2C(FourierBase):
 
 
 class C2C(FourierBase):
-    def __init__(self, N, device=None):
-        super().__init__(N, np.complex128, device)
+    _pad_kind = 2
+
+    def __init__(self, N, device=None, padding_factor=1, dealias_direct=False):
+        super().__init__(N, np.complex128, device, padding_factor, dealias_direct)
 
     def short_name(self):
         return "C2C"
@@ -522,11 +647,36 @@ class C2C(FourierBase):
                   sign, scale, _lib.stream_ptr())
         return out
 
+    def _direct_range(self):
+        z = slice(self.N // 3, -(self.N // 3) + 1).indices(self.N)  # fourier/bases.py:595-598
+        return z[0], z[1]
+
     def scalar_product(self, input_array=None, output_array=None, kind=None):
-        return self._c2c(input_array, output_array, -1, 1.0 / self.N)
+        if not self.is_padded:
+            return self._c2c(input_array, output_array, -1, 1.0 / self.N)
+        # padded-length FFT (1/M), wavenumbers folded back (fourier/bases.py:570-578)
+        inp = self._tensor(input_array)
+        _check(inp, "input_array", torch.complex128)
+        if inp.shape[self.axis] != self._phys_len():
+            raise ValueError("input_array has the wrong length along axis")
+        out = self._out(inp, output_array, self.N, torch.complex128)
+        tmp = self._twin().scalar_product(inp, self._tmp("sp", inp.shape, torch.complex128))
+        return self._pad(tmp, out, 1)
 
     def backward(self, input_array=None, output_array=None, kind=None):
-        return self._c2c(input_array, output_array, +1, 1.0)
+        if not (self.is_padded or self.dealias_direct):
+            return self._c2c(input_array, output_array, +1, 1.0)
+        inp = self._tensor(input_array)
+        _check(inp, "input_array", torch.complex128)
+        if inp.shape[self.axis] != self.N:
+            raise ValueError("input_array has the wrong length along axis")
+        if self.dealias_direct:
+            return self._c2c(self._dealias_input(inp), output_array, +1, 1.0)
+        out = self._out(inp, output_array, self._phys_len(), torch.complex128)
+        shp = list(inp.shape)
+        shp[self.axis] = self._phys_len()
+        tmp = self._pad(inp, self._tmp("bw", shp, torch.complex128), 0)
+        return self._twin().backward(tmp, out)
 
 
 # ---------------------------------------------------------------- factories
@@ -535,9 +685,12 @@ def _poly(cls_base, bc_kind, name):
 
 
 class _Init:
-    def __init__(self, N, quad=None, bc=None, dtype=np.float64, device=None, **kw):
-        SpectralBase.__init__(self, N, dtype, device)
+    def __init__(self, N, quad=None, bc=None, dtype=np.float64, device=None, padding_factor=1,
+                 dealias_direct=False, **kw):
         self.quad = quad
+        SpectralBase.__init__(self, N, dtype, device, padding_factor, dealias_direct)
+        if self.is_padded and quad == "GL":
+            raise NotImplementedError("padding is supported on Gauss (GC / LG) points")
 
 
 class ChebyshevOrthogonal(_Init, ChebyshevBase):
@@ -601,19 +754,21 @@ def FunctionSpace(N, family="Fourier", bc=None, dtype="d", quad=None, domain=Non
     if domain is not None and fam not in ("fourier", "f"):
         if tuple(float(d) for d in domain) != (-1.0, 1.0):
             raise NotImplementedError("only the reference domain [-1, 1] is on the hot path")
+    pad = dict(padding_factor=kw.get("padding_factor", 1),
+               dealias_direct=kw.get("dealias_direct", False))
     if fam in ("fourier", "f"):
         if np.dtype(dtype).kind == "c":
-            return C2C(N, device=device)
-        return R2C(N, device=device)
+            return C2C(N, device=device, **pad)
+        return R2C(N, device=device, **pad)
     kind = _bc_kind(bc)
     if fam in ("chebyshev", "c"):
         if quad not in (None, "GC", "GL"):
             raise NotImplementedError("Chebyshev-Gauss (GC) and Gauss-Lobatto (GL) quadratures only")
         cls = {None: ChebyshevOrthogonal, "D": ChebyshevShenDirichlet, "N": ChebyshevShenNeumann}[kind]
-        return cls(N, quad=quad or "GC", device=device)
+        return cls(N, quad=quad or "GC", device=device, **pad)
     if fam in ("legendre", "l"):
         if quad not in (None, "LG"):
             raise NotImplementedError("Legendre-Gauss (LG) quadrature only")
         cls = {None: LegendreOrthogonal, "D": LegendreShenDirichlet, "N": LegendreShenNeumann}[kind]
-        return cls(N, quad="LG", device=device)
+        return cls(N, quad="LG", device=device, **pad)
     raise NotImplementedError(f"family {family!r} is not on the hot path")

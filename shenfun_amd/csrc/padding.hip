@@ -1,0 +1,167 @@
+// Padding / truncation along one axis of a strided 1-3D array, the spectral
+// half of the 3/2-rule dealiasing (SpectralBase._padding_backward /
+// _truncation_forward, spectralbase.py:1362-1389; fourier/bases.py:409-435,
+// 570-598).  A padded space evaluates its transform on M = floor(pf N) points:
+// backward pads the N coefficients into M slots, forward truncates the M
+// coefficients of the padded-length transform back to N.  dealias_direct
+// (no padding) zeroes the top third of the wavenumbers instead.
+//
+// One thread per output element; when the axis is contiguous consecutive
+// threads walk the axis, otherwise they walk lines, so every access of a
+// wave is a contiguous or a line-to-line coalesced segment.
+#include <algorithm>
+
+#include "common.h"
+
+namespace sfa {
+
+struct PadMap {
+    int64_t sa_in, s1_in, s2_in;
+    int64_t sa_out, s1_out, s2_out;
+    int64_t d2, nlines;
+};
+
+enum { PAD_MODE_PAD = 0, PAD_MODE_TRUNCATE = 1, PAD_MODE_ZERO = 2 };
+enum { PAD_KIND_POLY = 0, PAD_KIND_R2C = 1, PAD_KIND_C2C = 2 };
+
+template <typename T>
+__device__ inline T pz();
+template <>
+__device__ inline double pz<double>() { return 0.0; }
+template <>
+__device__ inline double2 pz<double2>() { return make_double2(0.0, 0.0); }
+__device__ inline double pscale(double v, double s) { return v * s; }
+__device__ inline double2 pscale(double2 v, double s) { return make_double2(v.x * s, v.y * s); }
+__device__ inline double padd(double a, double b) { return a + b; }
+__device__ inline double2 padd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+// the real part times s (the Hermitian-symmetric Nyquist coefficient)
+__device__ inline double preal(double v, double s) { return v * s; }
+__device__ inline double2 preal(double2 v, double s) { return make_double2(v.x * s, 0.0); }
+
+// n_in / n_out: lengths along the axis; N: the unpadded size of the space
+// (the real length for R2C); nbc: boundary slots at the end of a composite
+// polynomial space; [z0, z1): the zeroed range of PAD_MODE_ZERO.
+template <typename T, bool AXIS_FAST>
+__global__ void pad_axis_kernel(const T* __restrict__ in, T* __restrict__ out, PadMap g, int n_in,
+                                int n_out, int mode, int kind, int N, int nbc, int z0, int z1)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= g.nlines * n_out) return;
+    int k;
+    int64_t L;
+    if (AXIS_FAST) {
+        L = gid / n_out;
+        k = (int)(gid - L * n_out);
+    } else {
+        k = (int)(gid / g.nlines);
+        L = gid - (int64_t)k * g.nlines;
+    }
+    const int64_t i1 = L / g.d2, i2 = L - i1 * g.d2;
+    const T* src = in + i1 * g.s1_in + i2 * g.s2_in;
+    const int64_t sa = g.sa_in;
+    const bool even = (N & 1) == 0;
+    const int h = N / 2;
+    T v = pz<T>();
+    if (mode == PAD_MODE_PAD) {
+        if (kind == PAD_KIND_POLY) {
+            if (k < n_in - nbc) v = src[k * sa];
+            else if (k >= n_out - nbc) v = src[(int64_t)(k - n_out + n_in) * sa];
+        } else if (kind == PAD_KIND_R2C) {
+            if (k < n_in) {
+                v = src[k * sa];
+                if (even && k == h) v = preal(v, 0.5);
+            }
+        } else {
+            if (k <= h) v = src[k * sa];
+            else if (k >= n_out - h) v = src[(int64_t)(k - n_out + N) * sa];
+            if (even && (k == h || k == n_out - h)) v = pscale(v, 0.5);
+        }
+    } else if (mode == PAD_MODE_TRUNCATE) {
+        if (kind == PAD_KIND_POLY) {
+            if (k < n_out - nbc) v = src[k * sa];
+        } else if (kind == PAD_KIND_R2C) {
+            v = src[k * sa];
+            if (even && k == h) v = preal(v, 2.0);
+        } else {
+            if (k <= h) v = src[k * sa];
+            if (k >= N - h) v = padd(v, src[(int64_t)(k - N + n_in) * sa]);
+        }
+    } else {
+        if (k < z0 || k >= z1) v = src[k * sa];
+    }
+    out[i1 * g.s1_out + i2 * g.s2_out + (int64_t)k * g.sa_out] = v;
+}
+
+}  // namespace sfa
+
+using namespace sfa;
+
+extern "C" int sfa_pad_axis(const void* in, void* out, int dtype, int ndim, const int64_t* in_shape,
+                            const int64_t* in_strides, const int64_t* out_shape,
+                            const int64_t* out_strides, int axis, int mode, int kind, int64_t N,
+                            int64_t nbc, int64_t z0, int64_t z1, void* stream)
+{
+    Geom gi, go;
+    int rc = make_geom(ndim, in_shape, in_strides, axis, &gi);
+    if (rc) return rc;
+    rc = make_geom(ndim, out_shape, out_strides, axis, &go);
+    if (rc) return rc;
+    SFA_CHECK(in != nullptr && out != nullptr && in != out, SFA_ERR_ARG,
+              "pad_axis: null or aliased arrays");
+    SFA_CHECK(dtype == SFA_F64 || dtype == SFA_C128, SFA_ERR_ARG, "pad_axis: bad dtype %d", dtype);
+    SFA_CHECK(mode >= 0 && mode <= 2 && kind >= 0 && kind <= 2, SFA_ERR_ARG,
+              "pad_axis: bad mode %d / kind %d", mode, kind);
+    SFA_CHECK(kind == PAD_KIND_POLY || dtype == SFA_C128, SFA_ERR_ARG,
+              "pad_axis: Fourier coefficients are complex128");
+    for (int d = 0; d < ndim; ++d)
+        SFA_CHECK(d == axis || in_shape[d] == out_shape[d], SFA_ERR_ARG,
+                  "pad_axis: shapes differ off the axis (dim %d)", d);
+    const int64_t ni = gi.n, no = go.n;
+    SFA_CHECK(N >= 1 && ni <= (1 << 28) && no <= (1 << 28), SFA_ERR_ARG, "pad_axis: bad sizes");
+    // lengths each mode and family reads and writes
+    const int64_t nspec = (kind == PAD_KIND_R2C) ? N / 2 + 1 : N;
+    if (mode == PAD_MODE_PAD) {
+        SFA_CHECK(ni == nspec && no >= ni, SFA_ERR_ARG,
+                  "pad_axis: pad needs in length %lld (got %lld) <= out length %lld",
+                  (long long)nspec, (long long)ni, (long long)no);
+        SFA_CHECK(kind != PAD_KIND_C2C || no >= N + (N % 2 == 0 ? 1 : 0), SFA_ERR_ARG,
+                  "pad_axis: C2C padding must grow the line");
+    } else if (mode == PAD_MODE_TRUNCATE) {
+        SFA_CHECK(no == nspec && ni >= no, SFA_ERR_ARG,
+                  "pad_axis: truncate needs out length %lld (got %lld) <= in length %lld",
+                  (long long)nspec, (long long)no, (long long)ni);
+    } else {
+        SFA_CHECK(ni == no && z0 >= 0 && z1 >= z0, SFA_ERR_ARG, "pad_axis: bad zero range");
+    }
+    SFA_CHECK(kind != PAD_KIND_POLY || (nbc >= 0 && nbc <= (mode == PAD_MODE_PAD ? ni : no)),
+              SFA_ERR_ARG, "pad_axis: bad boundary slot count %lld", (long long)nbc);
+    PadMap g;
+    g.sa_in = gi.sa;
+    g.s1_in = gi.s1;
+    g.s2_in = gi.s2;
+    g.sa_out = go.sa;
+    g.s1_out = go.s1;
+    g.s2_out = go.s2;
+    g.d2 = go.d2;
+    g.nlines = go.d1 * go.d2;
+    const int64_t total = g.nlines * no;
+    if (total == 0) return SFA_OK;
+    const int bs = 256;
+    const dim3 grid((unsigned)((total + bs - 1) / bs));
+    hipStream_t s = (hipStream_t)stream;
+    const bool fast = go.sa == 1;
+    const int zz0 = (int)std::min<int64_t>(z0, no), zz1 = (int)std::min<int64_t>(z1, no);
+#define SFA_PAD_LAUNCH(T, F)                                                                     \
+    hipLaunchKernelGGL((pad_axis_kernel<T, F>), grid, dim3(bs), 0, s, (const T*)in, (T*)out, g, \
+                       (int)ni, (int)no, mode, kind, (int)N, (int)nbc, zz0, zz1)
+    if (dtype == SFA_C128) {
+        if (fast) SFA_PAD_LAUNCH(double2, true);
+        else SFA_PAD_LAUNCH(double2, false);
+    } else {
+        if (fast) SFA_PAD_LAUNCH(double, true);
+        else SFA_PAD_LAUNCH(double, false);
+    }
+#undef SFA_PAD_LAUNCH
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}

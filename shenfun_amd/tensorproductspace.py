@@ -32,6 +32,7 @@ class TensorProductSpace(_Planned):
             b.tensorproductspace = self
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self._work = {}
+        self._padded = {}
         self._init_planner()
 
     def _planned_shape(self, name, which):
@@ -44,8 +45,7 @@ class TensorProductSpace(_Planned):
 
     # ---------------------------------------------------------------- info
     def shape(self, forward_output=True):
-        return tuple(b.shape(forward_output) if isinstance(b, FourierBase) else b.N
-                     for b in self.bases)
+        return tuple(b.shape(forward_output) for b in self.bases)
 
     def global_shape(self, forward_output=True):
         return self.shape(forward_output)
@@ -75,6 +75,25 @@ class TensorProductSpace(_Planned):
 
     def get_testspace(self, kind="G"):
         return self
+
+    def get_dealiased(self, padding_factor=1.5, dealias_direct=False):
+        """Space otherwise as self for dealiasing (tensorproductspace.py:543-578):
+        every basis padded by ``padding_factor`` (a number or one per axis), or
+        the top third of each axis' wavenumbers zeroed with ``dealias_direct``."""
+        if np.ndim(padding_factor) == 0:
+            if padding_factor == 1 and not dealias_direct:
+                return self
+            padding_factor = (padding_factor,) * self.dimensions
+        padding_factor = tuple(padding_factor)
+        if len(padding_factor) != self.dimensions:
+            raise ValueError("one padding factor per axis")
+        key = (padding_factor, bool(dealias_direct))
+        if key not in self._padded:
+            bases = [b.get_dealiased(padding_factor=pf, dealias_direct=dealias_direct)
+                     for b, pf in zip(self.bases, padding_factor)]
+            bases = [b._clone(b.N) if b is o else b for b, o in zip(bases, self.bases)]
+            self._padded[key] = TensorProductSpace(None, bases, device=self.device)
+        return self._padded[key]
 
     # ------------------------------------------------------------ helpers
     def _buf(self, key, shape, dtype):
@@ -151,12 +170,11 @@ def Array(space, buffer=None, val=None):
         shape, dtype = space.shape(False), space.physical_dtype()
         dev = space.device
     else:
-        shape = (space.N,)
+        shape = (space.shape(False),)
         dtype = torch.float64 if space.dtype.kind == "f" else torch.complex128
         dev = space.device
     if buffer is not None:
-        t = torch.as_tensor(np.asarray(buffer), device=dev).to(dtype)
-        return t.reshape(shape).contiguous()
+        return _from_buffer(buffer, shape, dtype, dev)
     t = torch.zeros(shape, dtype=dtype, device=dev)
     if val is not None:
         t.fill_(val)
@@ -174,12 +192,22 @@ def Function(space, buffer=None, val=None):
             else torch.float64
         dev = space.device
     if buffer is not None:
-        t = torch.as_tensor(np.asarray(buffer), device=dev).to(dtype)
-        return t.reshape(shape).contiguous()
+        return _from_buffer(buffer, shape, dtype, dev)
     t = torch.zeros(shape, dtype=dtype, device=dev)
     if val is not None:
         t.fill_(val)
     return t
+
+
+def _from_buffer(buffer, shape, dtype, dev):
+    if isinstance(buffer, torch.Tensor):
+        t = buffer.to(device=dev, dtype=dtype)
+    else:
+        a = np.asarray(buffer)
+        if not a.flags.writeable:
+            a = a.copy()  # read-only views (broadcast meshes) are copied
+        t = torch.as_tensor(a, device=dev).to(dtype)
+    return t.reshape(shape).contiguous()
 
 
 def project(uh, T, output_array=None, fill=True, uh_space=None):

@@ -1,0 +1,139 @@
+"""GPU parity: padded (3/2-rule) and dealias_direct transforms vs the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import spectral as S
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if isinstance(a, torch.Tensor) else a
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.fixture(scope="module")
+def sfa():
+    import shenfun_amd
+    return shenfun_amd
+
+
+# (oracle family, shenfun family, bc, dtype)
+SPACES = [
+    ("chebyshev", "C", None, "d"), ("chebyshev", "C", "D", "d"), ("chebyshev", "C", "N", "d"),
+    ("legendre", "L", None, "d"), ("legendre", "L", "D", "d"),
+    ("fourier-r2c", "F", None, "d"), ("fourier-c2c", "F", None, "D"),
+]
+
+
+def _coeffs(fam, N, shape_other, rng):
+    if fam == "fourier-r2c":
+        c = rng.standard_normal((N // 2 + 1,) + shape_other) + 1j * rng.standard_normal(
+            (N // 2 + 1,) + shape_other)
+        c[0] = c[0].real
+        c[N // 2] = c[N // 2].real
+        return c
+    c = rng.standard_normal((N,) + shape_other) + 1j * rng.standard_normal((N,) + shape_other)
+    return c
+
+
+@pytest.mark.parametrize("ofam,fam,bc,dt", SPACES)
+@pytest.mark.parametrize("N,axis", [(16, 0), (24, 1), (20, 2)])
+def test_padded_1d_transforms(sfa, ofam, fam, bc, dt, N, axis):
+    """backward / scalar_product / forward of a padded space along every axis
+    of a 3D array vs the oracle (pad / truncate restatements + M-point
+    transforms)."""
+    print(f"[padded {ofam} {bc} N={N} axis={axis}]", flush=True)
+    if ofam == "fourier-r2c" and axis != 2:
+        pytest.skip("R2C is the last axis")
+    V = sfa.FunctionSpace(N, fam, bc=bc, dtype=dt)
+    P = V.get_dealiased(1.5)
+    assert P.is_padded and P.shape(False) == 3 * N // 2 and P.shape(True) == V.shape(True)
+    P.axis = axis
+    P.dimensions = 3
+    M = 3 * N // 2
+    rng = np.random.default_rng(N + axis)
+    other = (3, 5)
+    c = np.moveaxis(_coeffs(ofam, N, other, rng), 0, axis)
+    if ofam in ("chebyshev", "legendre"):
+        c = c.real.copy()
+        if bc is not None:
+            sl = [slice(None)] * 3
+            sl[axis] = slice(N - 2, N)
+            c[tuple(sl)] = 0
+    u = P.backward(dev(c))
+    ref_u = S.padded_backward(ofam, bc, c, N, M, axis)
+    assert u.shape[axis] == M and rel(u, ref_u) < TOL
+    s = P.scalar_product(dev(ref_u))
+    ref_s = S.padded_scalar_product(ofam, bc, ref_u, N, axis)
+    assert rel(s, ref_s) < TOL
+    # forward of the padded backward recovers the coefficients
+    f = P.forward(dev(ref_u))
+    assert rel(f, c) < 1e-11
+
+
+@pytest.mark.parametrize("ofam,fam,bc,dt", SPACES)
+@pytest.mark.parametrize("N", [12, 17])
+def test_dealias_direct_backward(sfa, ofam, fam, bc, dt, N):
+    print(f"[dealias_direct {ofam} {bc} N={N}]", flush=True)
+    if ofam == "fourier-r2c" and N % 2:
+        pytest.skip("R2C lengths are even")
+    V = sfa.FunctionSpace(N, fam, bc=bc, dtype=dt).get_dealiased(1, dealias_direct=True)
+    rng = np.random.default_rng(N)
+    c = _coeffs(ofam, N, (4,), rng)
+    if ofam in ("chebyshev", "legendre"):
+        c = c.real.copy()
+        if bc is not None:
+            c[N - 2:] = 0
+    cd = dev(c)
+    u = V.backward(cd)
+    ref = S.basis_backward(ofam, bc, S.dealias_direct_zero(c, ofam, N), 0, N=N)
+    assert rel(u, ref) < TOL
+    assert np.array_equal(cd.cpu().numpy(), c)  # the caller's coefficients are left as they were
+
+
+def test_tensor_product_dealiased_product(sfa):
+    """TensorProductSpace.get_dealiased: backward on the padded grid, a
+    pointwise product, forward back: matches the oracle chain, and for the
+    Fourier axes the 3/2 rule's exact truncated convolution property holds."""
+    print("[tps dealiased]", flush=True)
+    N = (12, 16, 20)
+    SD = sfa.FunctionSpace(N[0], "C", bc="D")
+    K1 = sfa.FunctionSpace(N[1], "F", dtype="D")
+    K2 = sfa.FunctionSpace(N[2], "F", dtype="d")
+    T = sfa.TensorProductSpace(None, (SD, K1, K2))
+    Tp = T.get_dealiased(1.5)
+    assert Tp is T.get_dealiased((1.5, 1.5, 1.5))
+    assert Tp.shape(False) == (18, 24, 30) and Tp.shape(True) == T.shape(True)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal((N[0], N[1], N[2]))
+    bases = [("chebyshev", "D"), ("fourier-c2c", None), ("fourier-r2c", None)]
+    a_hat = S.tps_scalar_product(f, bases)
+    a_hat[N[0] - 2:] = 0
+    a_hat[:, N[1] // 2] = 0
+    a_hat[:, :, N[2] // 2] = 0
+    u = Tp.backward(dev(a_hat))
+    ref = a_hat
+    Ms = Tp.shape(False)
+    for ax, (fam, bc) in enumerate(bases):
+        ref = S.padded_backward(fam, bc, ref, N[ax], Ms[ax], ax)
+    assert rel(u, ref) < TOL
+    w = Tp.forward(u * u)
+    wr = ref * ref
+    for ax in range(2, -1, -1):
+        fam, bc = bases[ax]
+        wr = S.padded_scalar_product(fam, bc, wr, N[ax], ax)
+        if fam == "chebyshev":
+            B = S.dense(S.cheb_B_SD(N[ax]), N[ax] - 2)
+            wm = np.moveaxis(wr, ax, 0)
+            sol = np.zeros_like(wm)
+            sol[:N[ax] - 2] = np.linalg.solve(B, wm[:N[ax] - 2].reshape(N[ax] - 2, -1)).reshape(
+                wm[:N[ax] - 2].shape)
+            wr = np.moveaxis(sol, 0, ax)
+    assert rel(w, wr) < 1e-11

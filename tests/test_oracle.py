@@ -470,3 +470,127 @@ def test_pins_catch_planted_mistakes(monkeypatch):
     monkeypatch.setattr(S, "helmholtz_lu", bad_lu)
     with pytest.raises(AssertionError):
         test_helmholtz_lu_solve_vs_dense("D")
+
+
+# ----------------------------------------------------------------- dealiasing
+PAD_FAMS = [("chebyshev", None), ("chebyshev", "D"), ("chebyshev", "N"), ("legendre", None),
+            ("legendre", "D"), ("fourier-r2c", None), ("fourier-c2c", None)]
+
+
+def _spec(fam, N, rng, nyq_real=True):
+    if fam == "fourier-r2c":
+        c = rng.standard_normal(N // 2 + 1) + 1j * rng.standard_normal(N // 2 + 1)
+        c[0] = c[0].real
+        if N % 2 == 0 and nyq_real:
+            c[N // 2] = c[N // 2].real
+        return c
+    if fam == "fourier-c2c":
+        return rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    return rng.standard_normal(N)
+
+
+@pytest.mark.parametrize("fam,bc", PAD_FAMS)
+@pytest.mark.parametrize("N", [12, 16, 15])
+def test_padded_backward_is_the_series_on_the_fine_grid(fam, bc, N):
+    """Padded backward = the N-term expansion evaluated at the M padded points
+    (chebvander / legvander / explicit exponential sums with the symmetric
+    Nyquist term), independent of the pad slices."""
+    if fam == "fourier-r2c" and N % 2:
+        pytest.skip("R2C lengths are even")
+    M = int(np.floor(1.5 * N))
+    if fam == "fourier-r2c" and M % 2:
+        pytest.skip("R2C lengths are even")
+    rng = np.random.default_rng(N)
+    c = _spec(fam, N, rng)
+    if bc is not None:
+        c[N - 2:] = 0
+    u = S.padded_backward(fam, bc, c, N, M)
+    if fam in ("chebyshev", "legendre"):
+        t = S.to_ortho(c, S.stencil_k2(fam, bc, N)) if bc else c
+        if fam == "chebyshev":
+            x, _ = S.cheb_points_weights(M)
+            ref = np.polynomial.chebyshev.chebvander(x, N - 1) @ t
+        else:
+            x, _ = S.leg_points_weights(M)
+            ref = np.polynomial.legendre.legvander(x, N - 1) @ t
+    else:
+        x = S.fourier_points(M)
+        if fam == "fourier-r2c":
+            k = np.arange(N // 2 + 1)
+            w = np.where((k == 0) | ((N % 2 == 0) & (k == N // 2)), 1.0, 2.0)
+            ref = (np.exp(1j * np.outer(x, k)) @ (w * c)).real
+        else:
+            k = np.fft.fftfreq(N, 1.0 / N).astype(int)
+            E = np.exp(1j * np.outer(x, k))
+            if N % 2 == 0:  # Nyquist: the symmetric cos(N x / 2) interpolant
+                E[:, N // 2] = np.cos(N // 2 * x)
+            ref = E @ c
+    assert np.allclose(u, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("fam,bc", PAD_FAMS)
+@pytest.mark.parametrize("N", [12, 16])
+def test_padded_scalar_product_round_trip(fam, bc, N):
+    """Padded scalar product of a padded backward recovers the coefficients
+    times the mass matrix (quadrature on M > N points is exact there)."""
+    M = int(np.floor(1.5 * N))
+    rng = np.random.default_rng(N + 1)
+    c = _spec(fam, N, rng)
+    if bc is not None:
+        c[N - 2:] = 0
+    s = S.padded_scalar_product(fam, bc, S.padded_backward(fam, bc, c, N, M), N)
+    if fam.startswith("fourier"):
+        assert np.allclose(s, c, rtol=1e-12, atol=1e-13)
+        return
+    if bc is None:
+        h = S.cheb_norm_sq(N) if fam == "chebyshev" else S.leg_norm_sq(N)
+        assert np.allclose(s, h * c, rtol=1e-12, atol=1e-13)
+        return
+    Bf = {("chebyshev", "D"): S.cheb_B_SD, ("chebyshev", "N"): S.cheb_B_SN,
+          ("legendre", "D"): S.leg_B_SD}[(fam, bc)]
+    B = S.dense(Bf(N), N - 2)
+    assert np.allclose(s[:N - 2], B @ c[:N - 2], rtol=1e-12, atol=1e-12)
+    assert np.all(s[N - 2:] == 0)
+
+
+@pytest.mark.parametrize("fam", ["fourier-r2c", "fourier-c2c"])
+@pytest.mark.parametrize("N", [12, 16, 18])
+def test_three_halves_rule_gives_the_exact_convolution(fam, N):
+    """3/2 rule: the padded product's padded forward is the exact truncated
+    convolution of the two series (aliasing-free), by direct double sums."""
+    M = 3 * N // 2
+    rng = np.random.default_rng(3 * N)
+    a = _spec(fam, N, rng)
+    b = _spec(fam, N, rng)
+    if fam == "fourier-r2c":
+        a[N // 2] = b[N // 2] = 0  # band |k| < N/2
+        full = lambda c: {**{k: c[k] for k in range(N // 2)},
+                          **{-k: np.conj(c[k]) for k in range(1, N // 2)}}
+        ks = range(N // 2)
+    else:
+        kk = np.fft.fftfreq(N, 1.0 / N).astype(int)
+        a[N // 2] = b[N // 2] = 0
+        full = lambda c: {int(k): c[i] for i, k in enumerate(kk)}
+        ks = [int(k) for k in kk if abs(k) < N // 2]
+    w = S.padded_scalar_product(fam, None, S.padded_backward(fam, None, a, N, M)
+                                * S.padded_backward(fam, None, b, N, M), N)
+    A, B = full(a), full(b)
+    for k in ks:
+        ex = sum(v * B[k - p] for p, v in A.items() if (k - p) in B)
+        idx = k if k >= 0 else N + k
+        assert abs(w[idx] - ex) < 1e-12 * max(1.0, abs(ex))
+
+
+@pytest.mark.parametrize("N", [12, 13, 16, 17, 5])
+def test_dealias_direct_keeps_the_lower_two_thirds(N):
+    """Fourier: exactly the wavenumbers |k| < N//3 survive (R2C and C2C);
+    polynomial: exactly k < 2N//3."""
+    c = np.ones(N, dtype=complex)
+    kept = S.dealias_direct_zero(c, "fourier-c2c", N) != 0
+    k = np.fft.fftfreq(N, 1.0 / N).astype(int)
+    if N // 3 >= 2:
+        assert np.array_equal(kept, np.abs(k) < N // 3)
+    r = S.dealias_direct_zero(np.ones(N // 2 + 1, dtype=complex), "fourier-r2c", N) != 0
+    assert np.array_equal(r, np.arange(N // 2 + 1) < N // 3)
+    p = S.dealias_direct_zero(np.ones(N), "chebyshev", N) != 0
+    assert np.array_equal(p, np.arange(N) < 2 * N // 3)
