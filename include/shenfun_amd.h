@@ -110,6 +110,23 @@ int sfa_pad_axis(const void* in, void* out, int dtype, int ndim, const int64_t* 
                  int axis, int mode, int kind, int64_t N, int64_t nbc, int64_t z0, int64_t z1,
                  void* stream);
 
+/* Nonhomogeneous boundary values of a composite space: for k < nrows,
+ * dst(.., k, ..) += sum_m scales[m*scale_stride + line] * sum_j W[m][k][j] *
+ * src(.., n_src - nbc + j, ..), the nbc boundary slots at the end of each src
+ * line; W device (nmat, nrows, nbc) row-major, scales device (nmat,
+ * scale_stride) or NULL (all 1), lines in C order over the non-axis dims.
+ * Replaces BoundaryValues._add_to_orthogonal (tensorproductspace.py:2070-2089,
+ * the lift added by backward: W = the boundary basis on the physical points),
+ * _add_mass_rhs (:2091-2106, W = -boundary mass matrix) and the bc-matrix
+ * right-hand-side corrections of Helmholtz.__call__ (chebyshev/la.py:307-311)
+ * and SolverGeneric1ND (la.py:1212-1229).  src == dst is allowed when
+ * nrows <= n_src - nbc (rows below the slots read). */
+int sfa_boundary_update(const void* src, void* dst, int dtype, int ndim, const int64_t* src_shape,
+                        const int64_t* src_strides, const int64_t* dst_shape,
+                        const int64_t* dst_strides, int axis, int64_t nbc, int64_t nrows,
+                        int64_t nmat, const double* W, const double* scales, int64_t scale_stride,
+                        void* stream);
+
 /* Build the dense Legendre transform matrix on the device (the plan step of
  * the recursive transforms, spectralbase.py:484-491, 550-558).
  * analysis=1: M[k][j] = w_j (L_k(x_j) + K2[k] L_{k+2}(x_j))  (k < N-2 when

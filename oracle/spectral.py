@@ -452,6 +452,96 @@ def padded_backward(family, bc, c, N, M, axis=0):
 
 
 # --------------------------------------------------------------------------
+# Nonhomogeneous boundary values (Dirichlet or Neumann at both ends)
+# --------------------------------------------------------------------------
+
+
+def _bnd_value(family, j, deriv, side):
+    """P_j^(deriv)(+-1) in closed form: T_j(+-1) = (+-1)^j, T_j'(+-1) =
+    (+-1)^(j+1) j^2; L_j(+-1) = (+-1)^j, L_j'(+-1) = (+-1)^(j+1) j(j+1)/2
+    (chebyshev/bases.py, legendre/bases.py ``bnd_values``)."""
+    sgn = -1.0 if side == 'L' else 1.0
+    if deriv == 0:
+        return sgn ** j
+    d = j * j if family == 'chebyshev' else j * (j + 1) / 2
+    return sgn ** (j + 1) * d
+
+
+def bc_basis(family, kind):
+    """get_bc_basis (utilities/findbasis.py:68-140) for kind 'D' or 'N' at both
+    ends: the 2 x 2 matrix of the boundary functionals on P_first,
+    P_first+1, first the lowest index that makes it invertible; rows of the
+    result are the boundary functions' orthogonal coefficients (left, right)."""
+    deriv = 0 if kind == 'D' else 1
+    for first in range(deriv + 1):
+        A = np.array([[_bnd_value(family, first + m, deriv, side) for m in range(2)]
+                      for side in ('L', 'R')])
+        if abs(np.linalg.det(A)) > 1e-12:
+            break
+    X = np.linalg.solve(A, np.eye(2)).T
+    out = np.zeros((2, first + 2))
+    out[:, first:] = X
+    return out
+
+
+def _poly_cls(family):
+    return np.polynomial.Chebyshev if family == 'chebyshev' else np.polynomial.Legendre
+
+
+def bc_functions(family, kind, x, deriv=0):
+    """phi_bc_j^(deriv)(x), shape (len(x), 2)."""
+    P = _poly_cls(family)
+    C = bc_basis(family, kind)
+    return np.stack([P(C[j]).deriv(deriv)(x) if deriv else P(C[j])(x) for j in range(2)], axis=1)
+
+
+def bc_matrices(family, kind, N):
+    """(A, B), each (N-2, 2): B[k, j] = (phi_bc_j, phi_k)_w, A[k, j] =
+    (phi_bc_j'', phi_k)_w for Chebyshev, -(phi_bc_j'', phi_k) for Legendre
+    (the units of its stiffness matrix (phi', phi')).  Quadrature on N + 6
+    points: exact, and independent of the space's own rule."""
+    Q = N + 6
+    if family == 'chebyshev':
+        x, w = cheb_points_weights(Q)
+    else:
+        x, w = leg_points_weights(Q)
+    P = _poly_cls(family)
+    K2 = stencil_k2(family, kind, N)
+    Phi = np.stack([P.basis(k)(x) + K2[k] * P.basis(k + 2)(x) for k in range(N - 2)], axis=1)
+    B = (Phi * w[:, None]).T @ bc_functions(family, kind, x)
+    A = (Phi * w[:, None]).T @ bc_functions(family, kind, x, 2)
+    return (A if family == 'chebyshev' else -A), B
+
+
+def backward_bc(family, kind, c, x, axis=0):
+    """Backward of a composite space with nonhomogeneous boundary values:
+    the homogeneous synthesis plus the boundary functions weighted by the
+    coefficients' boundary slots N-2, N-1 (to_ortho + _add_to_orthogonal,
+    spectralbase.py:1616-1634, tensorproductspace.py:2070-2089), on points x."""
+    cm = np.moveaxis(np.asarray(c), axis, 0)
+    N = cm.shape[0]
+    u = np.moveaxis(basis_backward(family, kind, c, axis), axis, 0)
+    lift = bc_functions(family, kind, x)  # (len(x), 2)
+    u = u + np.tensordot(lift, cm[N - 2:], axes=(1, 0))
+    return np.moveaxis(u, 0, axis)
+
+
+def forward_bc(family, kind, vals, f):
+    """1D forward with boundary values ``vals``: scalar product, slots :=
+    vals, minus the boundary mass matrix times vals (_add_mass_rhs,
+    tensorproductspace.py:2091-2106), then the dense mass solve."""
+    N = f.shape[0]
+    s = basis_scalar_product(family, kind, f, 0)
+    s[N - 2:] = vals
+    _, Bbc = bc_matrices(family, kind, N)
+    rhs = s[:N - 2] - Bbc @ np.asarray(vals, dtype=float)
+    Bf = {('chebyshev', 'D'): cheb_B_SD, ('chebyshev', 'N'): cheb_B_SN,
+          ('legendre', 'D'): leg_B_SD, ('legendre', 'N'): leg_B_SN}[(family, kind)]
+    s[:N - 2] = np.linalg.solve(dense(Bf(N), N - 2), rhs)
+    return s
+
+
+# --------------------------------------------------------------------------
 # Galerkin matrices (closed forms).  Returned as dict offset -> diagonal.
 # --------------------------------------------------------------------------
 

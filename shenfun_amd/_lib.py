@@ -37,6 +37,7 @@ _SIGS = {
     "sfa_pdma_solve": "viiaaivlllv",
     "sfa_diagma_solve": "viiaaivlllv",
     "sfa_pad_axis": "vviiaaaaiiillllv",
+    "sfa_boundary_update": "vviiaaaailllvvlv",
     "sfa_tdma_o_lu": "vllv",
     "sfa_fdma_lu": "vllv",
     "sfa_heptadma_lu": "vllv",

@@ -306,8 +306,95 @@ class _PolyBase(SpectralBase):
 
     # ------------------------------------------------------------ transforms
     def _clone(self, N, padding_factor=1, dealias_direct=False):
-        return type(self)(N, quad=self.quad, dtype=self.dtype, device=self.device,
-                          padding_factor=padding_factor, dealias_direct=dealias_direct)
+        return type(self)(N, quad=self.quad, bc=self.bc_values, dtype=self.dtype,
+                          device=self.device, padding_factor=padding_factor,
+                          dealias_direct=dealias_direct)
+
+    # ------------------------------------------------------ boundary values
+    @property
+    def has_nonhomogeneous_bcs(self):
+        return self.is_composite and any(v != 0 for v in self.bc_values)
+
+    def bc_basis(self):
+        """Orthogonal coefficients (rows: left, right function) of the boundary
+        basis, the lowest-degree polynomials with unit value (Dirichlet) or
+        unit derivative (Neumann) at one end and zero at the other
+        (get_bc_basis, utilities/findbasis.py:68-140): D: (P0 -+ P1)/2;
+        N: P1/2 -+ P2/8 (Chebyshev), P1/2 -+ P2/6 (Legendre)."""
+        if self.bc_kind == "D":
+            return np.array([[0.5, -0.5, 0.0], [0.5, 0.5, 0.0]])
+        c = 1.0 / 8.0 if self.family() == "chebyshev" else 1.0 / 6.0
+        return np.array([[0.0, 0.5, -c], [0.0, 0.5, c]])
+
+    def _vander_at(self, x, n, deriv=0):
+        """P_k^(deriv)(x), k < n (host, numpy's polynomial classes)."""
+        P = np.polynomial.Chebyshev if self.family() == "chebyshev" else np.polynomial.Legendre
+        x = np.asarray(x, dtype=float)
+        return np.stack([P.basis(k).deriv(deriv)(x) for k in range(n)], axis=1)
+
+    def _bc_functions(self, x, deriv=0):
+        """phi_bc_j^(deriv)(x), shape (len(x), 2)."""
+        return self._vander_at(x, 3, deriv) @ self.bc_basis().T
+
+    def _test_functions(self, x):
+        """phi_k(x) = P_k + K2(k) P_{k+2}, k < N-2, shape (len(x), N-2)."""
+        V = self._vander_at(x, self.N)
+        return V[:, :-2] + self.stencil_k2()[None, :] * V[:, 2:]
+
+    def bc_matrices(self):
+        """Boundary mass and stiffness matrices (N-2, 2): B[k, j] =
+        (phi_bc_j, phi_k)_w and, in the units of this space's stiffness matrix,
+        A[k, j] = (phi_bc_j'', phi_k)_w (Chebyshev) or -(phi_bc_j'', phi_k)
+        (Legendre, whose A is (phi', phi')); quadrature on the space's own N
+        points is exact for these degrees (host, one-time, like the
+        reference's matrix assembly)."""
+        key = "bcmats"
+        if key not in self._cache:
+            x, w = self.points_and_weights(self.N)
+            Phi = self._test_functions(x) * w[:, None]
+            B = Phi.T @ self._bc_functions(x)
+            A = Phi.T @ self._bc_functions(x, 2)
+            if self.family() == "legendre":
+                A = -A
+            self._cache[key] = (A, B)
+        return self._cache[key]
+
+    def _bc_update(self, src, dst, W, nrows, scales=None):
+        """sfa_boundary_update with host matrices W (nmat, nrows, 2)."""
+        Wd = torch.from_numpy(np.ascontiguousarray(W, dtype=float)).to(dst.device)
+        code = _lib.C128 if dst.dtype == torch.complex128 else _lib.F64
+        sshp, sst = _geom(src)
+        dshp, dst_st = _geom(dst)
+        sc = None
+        stride = 0
+        if scales is not None:
+            sc = torch.from_numpy(np.ascontiguousarray(scales, dtype=float)).to(dst.device)
+            stride = sc.shape[-1]
+        _lib.call("sfa_boundary_update", _lib.ptr(src), _lib.ptr(dst), code, dst.dim(), sshp, sst,
+                  dshp, dst_st, self.axis, 2, nrows, W.shape[0], _lib.ptr(Wd),
+                  _lib.ptr(sc) if sc is not None else None, stride, _lib.stream_ptr())
+        return dst
+
+    def set_boundary_dofs(self, u, final=False):
+        """Boundary slots N-2, N-1 of ``u`` along the axis := the boundary
+        values (BoundaryValues.set_boundary_dofs, tensorproductspace.py:2108-2128).
+        In a tensor-product space a constant boundary value is the Fourier
+        mode 0 of the axes already transformed (all later axes when
+        ``final`` is False, every other axis when True), the same value along
+        the others."""
+        if not self.is_composite:
+            return u
+        for j, v in enumerate(self.bc_values):
+            sl = [slice(None)] * u.dim()
+            sl[self.axis] = self.N - 2 + j
+            u[tuple(sl)] = 0
+            if v == 0:
+                continue
+            for i in range(u.dim()):
+                if i != self.axis and (final or i > self.axis):
+                    sl[i] = 0
+            u[tuple(sl)] = v
+        return u
 
     def scalar_product(self, input_array=None, output_array=None, kind=None):
         kind = self._kind(kind)
@@ -336,6 +423,7 @@ class _PolyBase(SpectralBase):
         if out.data_ptr() == inp.data_ptr():
             raise ValueError("backward cannot run in place")
         nbc = self.N - self.dim()
+        src = inp
         if self.is_padded:
             # zero-padded coefficients, boundary slots moved to the end
             # (_padding_backward, spectralbase.py:1368-1389), then the
@@ -344,10 +432,19 @@ class _PolyBase(SpectralBase):
             shp[self.axis] = self._phys_len()
             tmp = self._pad(inp, self._tmp("bw", shp, inp.dtype), 0, nbc)
             self._twin()._backward(tmp, out, kind)
-            return out
-        if self.dealias_direct:
-            inp = self._dealias_input(inp, nbc)
-        self._backward(inp, out, kind)
+        else:
+            if self.dealias_direct:
+                inp = self._dealias_input(inp, nbc)
+            self._backward(inp, out, kind)
+        if self.has_nonhomogeneous_bcs:
+            # the boundary functions weighted by the input's boundary slots,
+            # on the physical points (to_ortho + _add_to_orthogonal,
+            # spectralbase.py:1616-1634, tensorproductspace.py:2070-2089)
+            lift = self._cache.get("bclift")
+            if lift is None:
+                lift = self._bc_functions(self.mesh())[None]
+                self._cache["bclift"] = lift
+            self._bc_update(src, out, lift, self._phys_len())
         return out
 
     def forward(self, input_array=None, output_array=None, kind=None):
@@ -357,9 +454,11 @@ class _PolyBase(SpectralBase):
         tensorproductspace.py:2091-2106)."""
         out = self.scalar_product(input_array, output_array, kind)
         if self.is_composite:
-            sl = [slice(None)] * out.dim()
-            sl[self.axis] = slice(self.N - 2, self.N)
-            out[tuple(sl)] = 0
+            self.set_boundary_dofs(out)
+            if self.has_nonhomogeneous_bcs:
+                # _add_mass_rhs (tensorproductspace.py:2091-2106): minus the
+                # boundary mass matrix times the boundary values
+                self._bc_update(out, out, -self.bc_matrices()[1][None], self.N - 2)
         self._mass_solver()(out, axis=self.axis)
         return out
 
@@ -688,6 +787,10 @@ class _Init:
     def __init__(self, N, quad=None, bc=None, dtype=np.float64, device=None, padding_factor=1,
                  dealias_direct=False, **kw):
         self.quad = quad
+        # boundary values (left, right) of a composite space; a string or
+        # dict bc is parsed by FunctionSpace
+        self.bc_values = tuple(float(v) for v in bc) if isinstance(bc, (tuple, list)) \
+            else (0.0, 0.0)
         SpectralBase.__init__(self, N, dtype, device, padding_factor, dealias_direct)
         if self.is_padded and quad == "GL":
             raise NotImplementedError("padding is supported on Gauss (GC / LG) points")
@@ -717,34 +820,40 @@ class LegendreShenNeumann(_Init, LegendreBase):
     bc_kind = "N"
 
 
-def _bc_kind(bc):
+def _parse_bc(bc):
+    """Boundary conditions -> (kind, (left value, right value)).
+
+    ``'Dirichlet'`` / ``'Neumann'`` (homogeneous), ``(a, b)`` (Dirichlet
+    u(-1) = a, u(1) = b) or the dict form ``{'left': {'D': a}, 'right': {'D': b}}``
+    with 'D' or 'N' at both ends (spectralbase.py BoundaryConditions); values
+    are numbers.
+    """
     if bc is None:
-        return None
+        return None, (0.0, 0.0)
     if isinstance(bc, str):
         b = bc.lower()
         if b in ("dirichlet", "d"):
-            return "D"
+            return "D", (0.0, 0.0)
         if b in ("neumann", "n"):
-            return "N"
+            return "N", (0.0, 0.0)
         raise ValueError(f"unsupported bc {bc!r}")
     if isinstance(bc, (tuple, list)):
-        if len(bc) == 2 and all(float(v) == 0 for v in bc):
-            return "D"
-        raise NotImplementedError("only homogeneous boundary conditions are on the hot path")
+        if len(bc) == 2:
+            return "D", (float(bc[0]), float(bc[1]))
+        raise ValueError(f"unsupported bc {bc!r}")
     if isinstance(bc, dict):
-        kinds = set()
-        vals = []
-        for side in bc.values():
-            for k, v in side.items():
-                kinds.add(k)
-                vals.append(v)
-        if any(float(v) != 0 for v in vals):
-            raise NotImplementedError("only homogeneous boundary conditions are on the hot path")
-        if kinds == {"D"}:
-            return "D"
-        if kinds == {"N"}:
-            return "N"
+        if set(bc) != {"left", "right"} or any(len(bc[s]) != 1 for s in bc):
+            raise NotImplementedError("one condition at each end, 'left' and 'right'")
+        (kl, vl), = bc["left"].items()
+        (kr, vr), = bc["right"].items()
+        if kl != kr or kl not in ("D", "N"):
+            raise NotImplementedError("Dirichlet or Neumann at both ends")
+        return kl, (float(vl), float(vr))
     raise ValueError(f"unsupported bc {bc!r}")
+
+
+def _bc_kind(bc):
+    return _parse_bc(bc)[0]
 
 
 def FunctionSpace(N, family="Fourier", bc=None, dtype="d", quad=None, domain=None, device=None,
@@ -760,15 +869,15 @@ def FunctionSpace(N, family="Fourier", bc=None, dtype="d", quad=None, domain=Non
         if np.dtype(dtype).kind == "c":
             return C2C(N, device=device, **pad)
         return R2C(N, device=device, **pad)
-    kind = _bc_kind(bc)
+    kind, vals = _parse_bc(bc)
     if fam in ("chebyshev", "c"):
         if quad not in (None, "GC", "GL"):
             raise NotImplementedError("Chebyshev-Gauss (GC) and Gauss-Lobatto (GL) quadratures only")
         cls = {None: ChebyshevOrthogonal, "D": ChebyshevShenDirichlet, "N": ChebyshevShenNeumann}[kind]
-        return cls(N, quad=quad or "GC", device=device, **pad)
+        return cls(N, quad=quad or "GC", bc=vals, device=device, **pad)
     if fam in ("legendre", "l"):
         if quad not in (None, "LG"):
             raise NotImplementedError("Legendre-Gauss (LG) quadrature only")
         cls = {None: LegendreOrthogonal, "D": LegendreShenDirichlet, "N": LegendreShenNeumann}[kind]
-        return cls(N, quad="LG", device=device, **pad)
+        return cls(N, quad="LG", bc=vals, device=device, **pad)
     raise NotImplementedError(f"family {family!r} is not on the hot path")

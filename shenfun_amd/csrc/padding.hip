@@ -165,3 +165,89 @@ extern "C" int sfa_pad_axis(const void* in, void* out, int dtype, int ndim, cons
     SFA_LAUNCH_CHECK();
     return SFA_OK;
 }
+
+// ----------------------------------------------------------------- boundary values
+// dst(.., k, ..) += sum_m s_m(line) sum_j W[m][k][j] src(.., n_src - nbc + j, ..)
+// for k < nrows: the boundary functions' part of a composite space with
+// nonhomogeneous boundary values, read from the nbc boundary slots at the end
+// of each src line.  Backward adds the lift on the physical points
+// (BoundaryValues._add_to_orthogonal, tensorproductspace.py:2070-2089), forward
+// subtracts the boundary mass matrix (_add_mass_rhs :2091-2106), the solvers
+// subtract the boundary stiffness and mass matrices scaled per mode
+// (chebyshev/la.py:307-311, la.py:1212-1229).  One thread per (line, k).
+namespace sfa {
+template <typename T>
+__global__ void boundary_update_kernel(const T* __restrict__ src, T* dst, PadMap g, int n_src,
+                                       int nbc, int nrows, int nmat, const double* __restrict__ W,
+                                       const double* __restrict__ scales, int64_t scale_stride)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= g.nlines * nrows) return;
+    const int k = (int)(gid % nrows);
+    const int64_t L = gid / nrows;
+    const int64_t i1 = L / g.d2, i2 = L - i1 * g.d2;
+    const T* s = src + i1 * g.s1_in + i2 * g.s2_in;
+    T acc = pz<T>();
+    for (int m = 0; m < nmat; ++m) {
+        const double sc = scales ? scales[m * scale_stride + L] : 1.0;
+        const double* Wm = W + ((int64_t)m * nrows + k) * nbc;
+        for (int j = 0; j < nbc; ++j)
+            acc = padd(acc, pscale(s[(int64_t)(n_src - nbc + j) * g.sa_in], sc * Wm[j]));
+    }
+    T* d = dst + i1 * g.s1_out + i2 * g.s2_out + (int64_t)k * g.sa_out;
+    *d = padd(*d, acc);
+}
+}  // namespace sfa
+
+extern "C" int sfa_boundary_update(const void* src, void* dst, int dtype, int ndim,
+                                   const int64_t* src_shape, const int64_t* src_strides,
+                                   const int64_t* dst_shape, const int64_t* dst_strides, int axis,
+                                   int64_t nbc, int64_t nrows, int64_t nmat, const double* W,
+                                   const double* scales, int64_t scale_stride, void* stream)
+{
+    Geom gi, go;
+    int rc = make_geom(ndim, src_shape, src_strides, axis, &gi);
+    if (rc) return rc;
+    rc = make_geom(ndim, dst_shape, dst_strides, axis, &go);
+    if (rc) return rc;
+    SFA_CHECK(src != nullptr && dst != nullptr && W != nullptr, SFA_ERR_ARG,
+              "boundary_update: null array");
+    SFA_CHECK(dtype == SFA_F64 || dtype == SFA_C128, SFA_ERR_ARG, "boundary_update: bad dtype %d",
+              dtype);
+    for (int d = 0; d < ndim; ++d)
+        SFA_CHECK(d == axis || src_shape[d] == dst_shape[d], SFA_ERR_ARG,
+                  "boundary_update: shapes differ off the axis (dim %d)", d);
+    SFA_CHECK(nbc >= 1 && nbc <= gi.n && nrows >= 0 && nrows <= go.n && nmat >= 1 && nmat <= 8,
+              SFA_ERR_ARG, "boundary_update: bad nbc %lld / nrows %lld / nmat %lld",
+              (long long)nbc, (long long)nrows, (long long)nmat);
+    // in place: the rows written must not be the boundary slots read
+    SFA_CHECK(src != dst || nrows <= gi.n - nbc, SFA_ERR_ARG,
+              "boundary_update: in place, rows must stay below the boundary slots");
+    const int64_t nlines = go.d1 * go.d2;
+    SFA_CHECK(scales == nullptr || scale_stride >= nlines, SFA_ERR_ARG,
+              "boundary_update: scale_stride %lld < lines %lld", (long long)scale_stride,
+              (long long)nlines);
+    PadMap g;
+    g.sa_in = gi.sa;
+    g.s1_in = gi.s1;
+    g.s2_in = gi.s2;
+    g.sa_out = go.sa;
+    g.s1_out = go.s1;
+    g.s2_out = go.s2;
+    g.d2 = go.d2;
+    g.nlines = nlines;
+    const int64_t total = nlines * nrows;
+    if (total == 0) return SFA_OK;
+    const int bs = 256;
+    const dim3 grid((unsigned)((total + bs - 1) / bs));
+    if (dtype == SFA_C128)
+        hipLaunchKernelGGL(boundary_update_kernel<double2>, grid, dim3(bs), 0, (hipStream_t)stream,
+                           (const double2*)src, (double2*)dst, g, (int)gi.n, (int)nbc, (int)nrows,
+                           (int)nmat, W, scales, scale_stride);
+    else
+        hipLaunchKernelGGL(boundary_update_kernel<double>, grid, dim3(bs), 0, (hipStream_t)stream,
+                           (const double*)src, (double*)dst, g, (int)gi.n, (int)nbc, (int)nrows,
+                           (int)nmat, W, scales, scale_stride);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}

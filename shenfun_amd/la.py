@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .matrices import SparseMatrix, TPMatrix
+from .matrices import SparseMatrix, TPMatrix, _split_bc, apply_bc_matrices
 
 
 def _geom(t):
@@ -249,6 +249,7 @@ class SolverGeneric1ND:
 
     def __init__(self, mats):
         assert isinstance(mats, (list, tuple)) and all(isinstance(m, TPMatrix) for m in mats)
+        mats, self.bc_mats = _split_bc(list(mats))
         self.mats = list(mats)
         self.naxes = mats[0].naxes[0]
         self.space = mats[0].space
@@ -299,6 +300,9 @@ class SolverGeneric1ND:
         return self._dev
 
     def __call__(self, b, u=None, constraints=(), fast=True):
+        if self.bc_mats:
+            # la.py:1212-1229: u's boundary slots set, b -= bc matrices x slots
+            apply_bc_matrices(self.bc_mats, b, b if u is None else u)
         u = _prepare(b, u)
         data = self.perform_lu(u.device)
         ax = self.naxes
@@ -311,6 +315,8 @@ class SolverGeneric1ND:
         shp, st = _geom(u)
         _lib.call(self._solve_fn, _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, ax, _lib.ptr(data),
                   self.n, data.shape[1] * data.shape[2], 1, _lib.stream_ptr())
+        if self.bc_mats:
+            self.bc_mats[0].pmat.space.set_boundary_dofs(u, final=True)
         return u
 
 
@@ -341,6 +347,7 @@ class Helmholtz:
     def __init__(self, *args, device="cuda"):
         if len(args) == 1:
             args = args[0]
+        args, self.bc_mats = _split_bc(list(args))
         assert len(args) in (2, 4)
         Am, Bm = args[:2]
         byk = {m.get_key(): m for m in (Am, Bm)}
@@ -407,6 +414,8 @@ class Helmholtz:
         _check(u, "u")
         if tuple(b.shape) != tuple(self.shape) or u.shape != b.shape:
             raise ValueError(f"arrays must have shape {tuple(self.shape)}")
+        # nonhomogeneous boundary values: b -= bc matrices x u's boundary slots
+        apply_bc_matrices(self.bc_mats, b, u)
         shp, bst = _geom(b)
         _, ust = _geom(u)
         _, fst = _geom(self.u0)

@@ -297,6 +297,32 @@ class TPMatrix:
         return self.space
 
 
+class BCMatrix:
+    """Boundary matrix (N-2, 2) of a composite space with nonhomogeneous
+    boundary values: rows the test functions, columns the boundary functions
+    (the trial-side boundary matrices that extract_bc_matrices separates,
+    matrixbase.py:1769-1793).  ``which`` is 'A' (stiffness units of the space)
+    or 'B' (mass)."""
+
+    is_bc_matrix = True
+
+    def __init__(self, space, which):
+        self.space = space
+        self.which = which
+        A, B = space.bc_matrices()
+        self.mat = A if which == "A" else B
+        self.shape = self.mat.shape
+        self.scale = 1.0
+        self.testfunction = (space, 0)
+
+    def get_key(self):
+        return f"BC{self.which}mat"
+
+    @property
+    def axis(self):
+        return self.space.axis
+
+
 def helmholtz_tpmatrices(T, alpha=0.0):
     """The simplified TPMatrix list of inner(v, alpha*u - div(grad(u))) for a
     tensor-product space T with one Chebyshev/Legendre axis and Fourier
@@ -319,4 +345,38 @@ def helmholtz_tpmatrices(T, alpha=0.0):
     B = mass_matrix(S)
     # Chebyshev: -lap -> -(phi'', phi)_w = -A ; Legendre: (phi', phi') = +A
     a_scale = -1.0 if S.family() == "chebyshev" else 1.0
-    return [TPMatrix(A, T, np.full(shape, a_scale), ax), TPMatrix(B, T, alpha + k2, ax)]
+    mats = [TPMatrix(A, T, np.full(shape, a_scale), ax), TPMatrix(B, T, alpha + k2, ax)]
+    if getattr(S, "has_nonhomogeneous_bcs", False):
+        # the boundary functions' part, moved to the right-hand side by the solvers
+        mats += [TPMatrix(BCMatrix(S, "A"), T, np.full(shape, a_scale), ax),
+                 TPMatrix(BCMatrix(S, "B"), T, alpha + k2, ax)]
+    return mats
+
+
+def _split_bc(mats):
+    """(regular matrices, boundary TPMatrices) (extract_bc_matrices, matrixbase.py:1769-1793)."""
+    reg = [m for m in mats if not getattr(getattr(m, "pmat", m), "is_bc_matrix", False)]
+    bc = [m for m in mats if getattr(getattr(m, "pmat", m), "is_bc_matrix", False)]
+    return reg, bc
+
+
+def apply_bc_matrices(bc_mats, b, u):
+    """b -= sum of the boundary matrices times u's boundary slots, each scaled
+    per mode (Helmholtz.__call__, chebyshev/la.py:307-311; SolverGeneric1ND,
+    la.py:1390-1410).  u's boundary slots are set first (final values)."""
+    if not bc_mats:
+        return b
+    S = bc_mats[0].pmat.space
+    S.set_boundary_dofs(u, final=True)
+    ax = S.axis
+    W = np.stack([m.pmat.mat * float(m.pmat.scale) for m in bc_mats])
+    rows = []
+    for m in bc_mats:
+        sc = np.asarray(m.scale, dtype=float)
+        if b.dim() == 1:
+            rows.append(np.full(1, float(sc.reshape(-1)[0])))
+        else:
+            full = np.broadcast_to(sc, [1 if i == ax else n for i, n in enumerate(b.shape)])
+            rows.append(np.moveaxis(full, ax, -1).reshape(-1))  # C order over the other dims
+    scales = np.stack(rows)
+    return S._bc_update(u, b, -W, S.N - 2, scales)

@@ -29,6 +29,10 @@ class HelmholtzSolver3D:
         if b[0].quad != "GC":
             raise NotImplementedError("the fused solver is built for Chebyshev-Gauss (GC) points; "
                                       "use T.scalar_product -> la.Helmholtz -> T.backward for GL")
+        if b[0].has_nonhomogeneous_bcs or any(x.is_padded for x in b):
+            raise NotImplementedError("the fused solver takes homogeneous boundary values on the "
+                                      "quadrature grid; use T.scalar_product -> la.Helmholtz -> "
+                                      "T.backward")
         self.T = T
         self.alpha = float(alpha)
         self.neumann = b[0].bc_kind == "N"

@@ -182,7 +182,20 @@ def Array(space, buffer=None, val=None):
 
 
 def Function(space, buffer=None, val=None):
-    """Spectral-space array of ``space`` (torch tensor on the GPU)."""
+    """Spectral-space array of ``space`` (torch tensor on the GPU); the
+    boundary slots of a space with nonhomogeneous boundary values hold those
+    values (BoundaryValues.set_boundary_dofs(final=True), as the reference's
+    Function does on creation)."""
+    t = _function(space, buffer, val)
+    bases = space.bases if isinstance(space, TensorProductSpace) else [space]
+    if buffer is None:
+        for b in bases:
+            if getattr(b, "has_nonhomogeneous_bcs", False):
+                b.set_boundary_dofs(t, final=True)
+    return t
+
+
+def _function(space, buffer=None, val=None):
     if isinstance(space, TensorProductSpace):
         shape, dtype = space.shape(True), space.spectral_dtype()
         dev = space.device

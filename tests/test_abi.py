@@ -130,7 +130,12 @@ def test_functionspace_factory():
     assert sfa.FunctionSpace(8, "chebyshev", bc=(0, 0)).bc_kind == "D"
     assert sfa.FunctionSpace(8, "chebyshev", bc={"left": {"N": 0}, "right": {"N": 0}}).bc_kind == "N"
     assert sfa.FunctionSpace(8, "legendre").bc_kind is None
+    V = sfa.FunctionSpace(8, "chebyshev", bc=(1, -2))
+    assert V.bc_kind == "D" and V.bc_values == (1.0, -2.0) and V.has_nonhomogeneous_bcs
+    W = sfa.FunctionSpace(8, "legendre", bc={"left": {"N": 0.5}, "right": {"N": 0}})
+    assert W.bc_kind == "N" and W.bc_values == (0.5, 0.0)
+    assert not sfa.FunctionSpace(8, "chebyshev", bc=(0, 0)).has_nonhomogeneous_bcs
     with pytest.raises(NotImplementedError):
-        sfa.FunctionSpace(8, "chebyshev", bc=(1, 0))
+        sfa.FunctionSpace(8, "chebyshev", bc={"left": {"D": 1}, "right": {"N": 0}})
     with pytest.raises(NotImplementedError):
         sfa.FunctionSpace(8, "laguerre")

@@ -594,3 +594,52 @@ def test_dealias_direct_keeps_the_lower_two_thirds(N):
     assert np.array_equal(r, np.arange(N // 2 + 1) < N // 3)
     p = S.dealias_direct_zero(np.ones(N), "chebyshev", N) != 0
     assert np.array_equal(p, np.arange(N) < 2 * N // 3)
+
+
+# ----------------------------------------------------------------- boundary values
+@pytest.mark.parametrize("family", ["chebyshev", "legendre"])
+@pytest.mark.parametrize("kind", ["D", "N"])
+def test_bc_basis_functionals_are_the_identity(family, kind):
+    """The boundary functions take the unit value (D) or unit derivative (N)
+    at their own end and zero at the other, checked with numpy's polynomial
+    classes at x = -1, 1; their degree is the lowest possible (<= 2)."""
+    d = 0 if kind == "D" else 1
+    F = S.bc_functions(family, kind, np.array([-1.0, 1.0]), d)
+    assert np.allclose(F, np.eye(2), atol=1e-15)
+    assert S.bc_basis(family, kind).shape[1] == (2 if kind == "D" else 3)
+
+
+@pytest.mark.parametrize("family", ["chebyshev", "legendre"])
+@pytest.mark.parametrize("kind", ["D", "N"])
+def test_bc_forward_backward_reproduce_a_polynomial(family, kind):
+    """A polynomial of degree < N meeting the boundary values: forward (the
+    oracle's restatement of _add_mass_rhs + mass solve) puts the values in the
+    slots, and backward_bc returns the polynomial on the points."""
+    N = 14
+    rng = np.random.default_rng(11)
+    P = S._poly_cls(family)
+    vals = (0.7, -1.3)
+    p = P(rng.standard_normal(N))
+    d = 0 if kind == "D" else 1
+    b = np.array([(p.deriv(d) if d else p)(x) for x in (-1.0, 1.0)])
+    C = S.bc_basis(family, kind)
+    for j in range(2):  # correct the boundary functionals with the boundary basis
+        p = p + (vals[j] - b[j]) * P(C[j])
+    x = S.cheb_points_weights(N)[0] if family == "chebyshev" else S.leg_points_weights(N)[0]
+    c = S.forward_bc(family, kind, vals, p(x))
+    assert np.allclose(c[N - 2:], vals)
+    assert np.allclose(S.backward_bc(family, kind, c, x), p(x), rtol=1e-12, atol=1e-12)
+
+
+def test_bc_space_host_side_matches_the_oracle():
+    """The spaces' boundary basis and boundary matrices (host, one-time)
+    against the oracle's (findbasis solve; quadrature on N + 6 points)."""
+    import shenfun_amd as sfa
+    for fam, f in (("C", "chebyshev"), ("L", "legendre")):
+        for bc, kind in (((1.0, 2.0), "D"), ({"left": {"N": 1.0}, "right": {"N": 0.0}}, "N")):
+            V = sfa.FunctionSpace(16, fam, bc=bc)
+            C = S.bc_basis(f, kind)
+            assert np.allclose(V.bc_basis()[:, :C.shape[1]], C, atol=1e-15)
+            A, B = V.bc_matrices()
+            Ao, Bo = S.bc_matrices(f, kind, 16)
+            assert np.allclose(A, Ao, atol=1e-13) and np.allclose(B, Bo, atol=1e-13)
