@@ -48,15 +48,17 @@ int sfa_fft_c2c(const void* in, void* out, int ndim, const int64_t* shape,
                 const int64_t* in_strides, const int64_t* out_strides, int axis,
                 int sign, double scale, void* stream);
 
-/* Real-to-complex DFT along `axis` (real length shape[axis] = N, even),
- * out has N/2+1 entries along axis.  Replaces fftw.rfftn of
+/* Real-to-complex DFT along `axis` (real length shape[axis] = N; odd N runs
+ * as a full-length complex FFT), out has N/2+1 entries along axis.  Replaces fftw.rfftn of
  * fourier/bases.py:249-265 (R2C scalar product, scale = 1/N). */
 int sfa_fft_r2c(const double* in, void* out, int ndim, const int64_t* real_shape,
                 const int64_t* in_strides, const int64_t* out_strides, int axis,
                 double scale, void* stream);
 
-/* Complex-to-real inverse DFT along `axis` (real length real_shape[axis] = N,
- * even; in has N/2+1 entries, imaginary parts of entries 0 and N/2 ignored).
+/* Complex-to-real inverse DFT along `axis` (real length real_shape[axis] = N;
+ * in has N/2+1 entries, the imaginary parts of entry 0 and, for even N, of
+ * entry N/2 ignored; odd N runs as a full-length complex FFT of the Hermitian
+ * extension).
  * Replaces fftw.irfftn of fourier/bases.py:380-407 (normalise_idft=False, scale 1). */
 int sfa_fft_c2r(const void* in, double* out, int ndim, const int64_t* real_shape,
                 const int64_t* in_strides, const int64_t* out_strides, int axis,
@@ -126,6 +128,20 @@ int sfa_boundary_update(const void* src, void* dst, int dtype, int ndim, const i
                         const int64_t* dst_strides, int axis, int64_t nbc, int64_t nrows,
                         int64_t nmat, const double* W, const double* scales, int64_t scale_stride,
                         void* stream);
+
+/* Pointwise product out = a * b of two contiguous arrays of n elements
+ * (float64 or complex128): the physical-space step of the dealiased
+ * convolution (TensorProductSpace.convolve tensorproductspace.py:637-666,
+ * R2C/C2C.convolve(fast=True) fourier/bases.py:437-475, 600-640). */
+int sfa_pointwise_mul(const void* a, const void* b, void* out, int dtype, int64_t n, void* stream);
+
+/* Direct convolution sums of two Fourier coefficient vectors (complex128,
+ * contiguous): real = 0, C2C, N coefficients in FFT order, out 2N entries
+ * (convolve_1D, optimization/cython/convolve.pyx:9-30); real = 1, R2C, N/2+1
+ * coefficients of a real length-N signal, out N+1 entries (convolve_real_1D
+ * convolve.pyx:34-75).  Nyquist entries halved for even N.  The fast=False
+ * path of R2C/C2C.convolve. */
+int sfa_convolve_direct(const void* u, const void* v, void* out, int64_t N, int real, void* stream);
 
 /* Build the dense Legendre transform matrix on the device (the plan step of
  * the recursive transforms, spectralbase.py:484-491, 550-558).

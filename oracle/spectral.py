@@ -451,6 +451,33 @@ def padded_backward(family, bc, c, N, M, axis=0):
     return basis_backward(family, bc, pad_backward(c, family, bc, N, M, axis), axis, N=M)
 
 
+def convolve_direct(u, v, N, real):
+    """Direct convolution sums (optimization/cython/convolve.pyx): C2C
+    convolve_1D :9-30 (out 2N, python-style negative indices) or R2C
+    convolve_real_1D :34-75 (out N+1, p >= 0, negative wavenumbers by
+    conjugation); wavenumbers of a length Np = N (odd) / N + 1 (even) grid in
+    fftfreq order, Nyquist entries halved for even N.  Pure-Python loops:
+    small N only."""
+    Np = N if N % 2 else N + 1
+    k = np.fft.fftfreq(Np, 1.0 / Np).astype(int)
+    uv = np.zeros(N + 1 if real else 2 * N, dtype=complex)
+
+    def coef(a, m):
+        if N % 2 == 0 and abs(m) == N // 2:
+            return a[abs(m)] * 0.5
+        if real:
+            return a[m] if m >= 0 else np.conj(a[-m])
+        return a[m]
+
+    for m in k:
+        for n in k:
+            p = m + n
+            if real and p < 0:
+                continue
+            uv[p] += coef(u, m) * coef(v, n)
+    return uv
+
+
 # --------------------------------------------------------------------------
 # Nonhomogeneous boundary values (Dirichlet or Neumann at both ends)
 # --------------------------------------------------------------------------

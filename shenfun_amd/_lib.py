@@ -38,6 +38,8 @@ _SIGS = {
     "sfa_diagma_solve": "viiaaivlllv",
     "sfa_pad_axis": "vviiaaaaiiillllv",
     "sfa_boundary_update": "vviiaaaailllvvlv",
+    "sfa_pointwise_mul": "vvvilv",
+    "sfa_convolve_direct": "vvvliv",
     "sfa_tdma_o_lu": "vllv",
     "sfa_fdma_lu": "vllv",
     "sfa_heptadma_lu": "vllv",

@@ -659,14 +659,59 @@ class FourierBase(SpectralBase):
     def forward(self, input_array=None, output_array=None, kind=None):
         return self.scalar_product(input_array, output_array, kind)
 
+    def convolve(self, u, v, uv=None, fast=True):
+        """Convolution of the coefficient vectors u and v (R2C fourier/bases.py:437-475,
+        C2C :600-640).  fast: backward both on the padded grid, multiply,
+        forward (the space must be padded; with padding_factor >= 3/2 + 1/N
+        the result is alias-free); otherwise the direct O(N^2) sums, N + 1
+        (R2C) or 2N (C2C) entries."""
+        u = self._tensor(u)
+        v = self._tensor(v)
+        _check(u, "u", torch.complex128)
+        _check(v, "v", torch.complex128)
+        if fast:
+            if not self.padding_factor > 1.0:
+                raise ValueError("padding factor must be > 3/2+1/N to perform convolution "
+                                 "without aliasing")
+            shp = list(u.shape)
+            shp[self.axis] = self._phys_len()
+            dt = torch.float64 if isinstance(self, R2C) else torch.complex128
+            a = self.backward(u, self._tmp("cv_a", shp, dt))
+            b = self.backward(v, self._tmp("cv_b", shp, dt))
+            return self.forward(pointwise_mul(a, b, self._tmp("cv_ab", shp, dt)), uv)
+        if u.dim() != 1 or v.shape != u.shape:
+            raise ValueError("the direct convolution is 1D")
+        real = isinstance(self, R2C)
+        n_out = self.N + 1 if real else 2 * self.N
+        if uv is None:
+            uv = torch.zeros(n_out, dtype=torch.complex128, device=u.device)
+        _check(uv, "uv", torch.complex128)
+        if tuple(uv.shape) != (n_out,) or not (u.is_contiguous() and v.is_contiguous()
+                                                and uv.is_contiguous()):
+            raise ValueError(f"uv must be a contiguous ({n_out},) complex128 array")
+        if u.shape[0] != self.shape(True):
+            raise ValueError("u has the wrong length")
+        _lib.call("sfa_convolve_direct", _lib.ptr(u), _lib.ptr(v), _lib.ptr(uv), self.N, int(real),
+                  _lib.stream_ptr())
+        return uv
+
+
+def pointwise_mul(a, b, out):
+    """out = a * b on the device (contiguous arrays of one dtype)."""
+    for t in (a, b, out):
+        if not t.is_contiguous() or t.shape != a.shape or t.dtype != a.dtype:
+            raise ValueError("pointwise_mul takes contiguous arrays of one shape and dtype")
+    code = _lib.C128 if a.dtype == torch.complex128 else _lib.F64
+    _lib.call("sfa_pointwise_mul", _lib.ptr(a), _lib.ptr(b), _lib.ptr(out), code, a.numel(),
+              _lib.stream_ptr())
+    return out
+
 
 class R2C(FourierBase):
     _pad_kind = 1
 
     def __init__(self, N, device=None, padding_factor=1, dealias_direct=False):
         super().__init__(N, np.float64, device, padding_factor, dealias_direct)
-        if self._phys_len() % 2:
-            raise NotImplementedError(f"R2C on {self._phys_len()} points: even lengths only")
 
     def short_name(self):
         return "R2C"

@@ -251,3 +251,106 @@ extern "C" int sfa_boundary_update(const void* src, void* dst, int dtype, int nd
     SFA_LAUNCH_CHECK();
     return SFA_OK;
 }
+
+// ----------------------------------------------------------------- convolution
+// Pointwise product of two contiguous arrays (the physical-space step of a
+// dealiased convolution, TensorProductSpace.convolve, tensorproductspace.py:637-666).
+namespace sfa {
+template <typename T>
+__device__ inline T pmul(T a, T b);
+template <>
+__device__ inline double pmul<double>(double a, double b) { return a * b; }
+template <>
+__device__ inline double2 pmul<double2>(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <typename T>
+__global__ void pointwise_mul_kernel(const T* __restrict__ a, const T* __restrict__ b,
+                                     T* __restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = pmul(a[i], b[i]);
+}
+
+// Direct convolution sums of two Fourier coefficient vectors (the fast=False
+// path).  C2C (convolve_1D, convolve.pyx:9-30): N coefficients in FFT order,
+// out 2N entries, out[(m + n) mod 2N] += u_m v_n over the wavenumbers of a
+// length Np = N (odd) or N + 1 (even) grid, the Nyquist entries halved.
+// R2C (convolve_real_1D, convolve.pyx:34-75): N/2+1 coefficients, negative
+// wavenumbers by conjugation, out N + 1 entries, p = m + n >= 0 only.
+// One thread per output entry, the wavenumber pairs enumerated in the
+// reference's order for that entry.
+__device__ inline double2 conv_coef(const double2* u, int m, int N, bool real)
+{
+    const int am = m < 0 ? -m : m;
+    if ((N & 1) == 0 && am == N / 2) {  // Nyquist: halved, never conjugated
+        const double2 c = u[am];
+        return make_double2(c.x * 0.5, c.y * 0.5);
+    }
+    if (real) {
+        const double2 c = u[am];
+        return m < 0 ? make_double2(c.x, -c.y) : c;
+    }
+    return u[m < 0 ? N + m : m];
+}
+
+__global__ void convolve_direct_kernel(const double2* __restrict__ u, const double2* __restrict__ v,
+                                       double2* __restrict__ out, int N, int nout, int real)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nout) return;
+    const int Np = (N & 1) ? N : N + 1;
+    const int kmax = Np / 2;  // wavenumbers -kmax .. kmax
+    double2 acc = make_double2(0.0, 0.0);
+    // outputs receiving p: C2C q = p mod 2N (p and p - 2N), R2C q = p
+    for (int j = 0; j < Np; ++j) {
+        const int m = j <= kmax ? j : j - Np;  // fftfreq order, as the reference loops
+        for (int r = 0; r < (real ? 1 : 2); ++r) {
+            const int p = real ? q : (r == 0 ? q : q - 2 * N);
+            const int n = p - m;
+            if (n < -kmax || n > kmax) continue;
+            const double2 a = conv_coef(u, m, N, real != 0);
+            const double2 b = conv_coef(v, n, N, real != 0);
+            acc.x += a.x * b.x - a.y * b.y;
+            acc.y += a.x * b.y + a.y * b.x;
+        }
+    }
+    out[q] = acc;
+}
+}  // namespace sfa
+
+extern "C" int sfa_pointwise_mul(const void* a, const void* b, void* out, int dtype, int64_t n,
+                                 void* stream)
+{
+    SFA_CHECK(a != nullptr && b != nullptr && out != nullptr && n >= 0, SFA_ERR_ARG,
+              "pointwise_mul: bad arguments");
+    SFA_CHECK(dtype == SFA_F64 || dtype == SFA_C128, SFA_ERR_ARG, "pointwise_mul: bad dtype %d",
+              dtype);
+    if (n == 0) return SFA_OK;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (dtype == SFA_C128)
+        hipLaunchKernelGGL(pointwise_mul_kernel<double2>, grid, dim3(256), 0, (hipStream_t)stream,
+                           (const double2*)a, (const double2*)b, (double2*)out, n);
+    else
+        hipLaunchKernelGGL(pointwise_mul_kernel<double>, grid, dim3(256), 0, (hipStream_t)stream,
+                           (const double*)a, (const double*)b, (double*)out, n);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+extern "C" int sfa_convolve_direct(const void* u, const void* v, void* out, int64_t N, int real,
+                                   void* stream)
+{
+    SFA_CHECK(u != nullptr && v != nullptr && out != nullptr, SFA_ERR_ARG,
+              "convolve_direct: null array");
+    SFA_CHECK(N >= 1 && N <= (1 << 20) && (real == 0 || real == 1), SFA_ERR_ARG,
+              "convolve_direct: bad N %lld / real %d", (long long)N, real);
+    const int nout = real ? (int)N + 1 : 2 * (int)N;
+    hipLaunchKernelGGL(convolve_direct_kernel, dim3((nout + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, (const double2*)u, (const double2*)v, (double2*)out,
+                       (int)N, nout, real);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}

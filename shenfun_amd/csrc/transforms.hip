@@ -17,7 +17,13 @@ namespace sfa {
 // DCT-I (REDFT00) of length n, computed as the complex FFT of the line's even
 // extension (length 2(n-1)): the sine terms cancel, so the first n outputs
 // are the DCT-I of the real and imaginary parts at once.
-enum LineOp { OP_C2C = 0, OP_R2C = 1, OP_C2R = 2, OP_DCT2 = 3, OP_DCT3 = 4, OP_DCT1F = 5, OP_DCT1B = 6 };
+// OP_R2CF / OP_C2RF: odd-length R2C / C2R as a full-length complex FFT (real
+// input widened; Hermitian spectrum completed on load), the even lengths pack
+// pairs of reals instead (OP_R2C / OP_C2R)
+enum LineOp {
+    OP_C2C = 0, OP_R2C = 1, OP_C2R = 2, OP_DCT2 = 3, OP_DCT3 = 4, OP_DCT1F = 5, OP_DCT1B = 6,
+    OP_R2CF = 7, OP_C2RF = 8
+};
 
 struct LineArgs {
     const char* in;
@@ -88,7 +94,7 @@ __global__ void __launch_bounds__(256) line_kernel(LineArgs a)
     const int nf = a.plan.nfft;
 
     // ------------------------------------------------------------ load
-    if (OP == OP_C2C) {
+    if (OP == OP_C2C || OP == OP_R2CF) {
         for (int e = tid; e < Tv * n; e += nth) {
             int t, i;
             tile_index(e, Tv, n, a.t_fast_in, t, i);
@@ -102,6 +108,17 @@ __global__ void __launch_bounds__(256) line_kernel(LineArgs a)
             const double* p = reinterpret_cast<const double*>(inb);
             const int64_t o = ibase + t * a.in_s2 + (int64_t)(2 * m) * a.in_sa;
             A[t * ldn + m] = make_double2(p[o], p[o + a.in_sa]);
+        }
+    } else if (OP == OP_C2RF) {
+        // X_k for k <= n/2 from the input, X_{n-k} = conj(X_k) above; Im X_0 dropped
+        const int h = n / 2;
+        for (int e = tid; e < Tv * n; e += nth) {
+            int t, i;
+            tile_index(e, Tv, n, a.t_fast_in, t, i);
+            const int k = i <= h ? i : n - i;
+            double2 v = ld_elem(inb, ibase + t * a.in_s2 + k * a.in_sa, 1);
+            if (i == 0) v.y = 0.0;
+            A[t * ldn + i] = i <= h ? v : cconj(v);
         }
     } else if (OP == OP_C2R) {
         const int h = n / 2;
@@ -214,7 +231,7 @@ __global__ void __launch_bounds__(256) line_kernel(LineArgs a)
     double2* F;
     if (OP == OP_C2C)
         F = run_fft<INV>(A, B, a.plan, ldn, Tv, a.tw, tid, nth);
-    else if (OP == OP_R2C || OP == OP_DCT2 || OP == OP_DCT1F || OP == OP_DCT1B)
+    else if (OP == OP_R2C || OP == OP_R2CF || OP == OP_DCT2 || OP == OP_DCT1F || OP == OP_DCT1B)
         F = run_fft<false>(A, B, a.plan, ldn, Tv, a.tw, tid, nth);
     else
         F = run_fft<true>(A, B, a.plan, ldn, Tv, a.tw, tid, nth);
@@ -222,7 +239,14 @@ __global__ void __launch_bounds__(256) line_kernel(LineArgs a)
 
     // ------------------------------------------------------------ store
     const double sc = a.scale;
-    if (OP == OP_C2C) {
+    if (OP == OP_R2CF) {
+        const int h = n / 2;
+        for (int e = tid; e < Tv * (h + 1); e += nth) {
+            int t, k;
+            tile_index(e, Tv, h + 1, a.t_fast_out, t, k);
+            st_elem(outb, obase + t * a.out_s2 + k * a.out_sa, 1, cscale(F[t * ldn + k], sc));
+        }
+    } else if (OP == OP_C2C || OP == OP_C2RF) {
         for (int e = tid; e < Tv * n; e += nth) {
             int t, i;
             tile_index(e, Tv, n, a.t_fast_out, t, i);
@@ -349,6 +373,8 @@ static int launch_line(int op, bool inv, LineArgs& a, hipStream_t stream)
             break;
         case OP_R2C: SFA_LK(OP_R2C, false); break;
         case OP_C2R: SFA_LK(OP_C2R, true); break;
+        case OP_R2CF: SFA_LK(OP_R2CF, false); break;
+        case OP_C2RF: SFA_LK(OP_C2RF, true); break;
         case OP_DCT2: SFA_LK(OP_DCT2, false); break;
         case OP_DCT3: SFA_LK(OP_DCT3, true); break;
         case OP_DCT1F: SFA_LK(OP_DCT1F, false); break;
@@ -429,7 +455,14 @@ extern "C" int sfa_fft_r2c(const double* in, void* out, int ndim, const int64_t*
     if ((rc = make_geom(ndim, real_shape, in_strides, axis, &gi))) return rc;
     if ((rc = make_geom(ndim, real_shape, out_strides, axis, &go))) return rc;
     const int n = (int)gi.n;
-    SFA_CHECK(n >= 2 && n % 2 == 0, SFA_ERR_UNSUPPORTED, "R2C length must be even (got %d)", n);
+    SFA_CHECK(n >= 1, SFA_ERR_ARG, "R2C length must be >= 1");
+    SFA_CHECK(in != nullptr && out != nullptr, SFA_ERR_ARG, "R2C: null array");
+    if (n % 2) {
+        LineArgs a;
+        if ((rc = setup_line(a, in, out, gi, go, n, n, 0, 1))) return rc;
+        a.scale = scale;
+        return launch_line(OP_R2CF, false, a, (hipStream_t)stream);
+    }
     if (aligned16(in) && aligned16(out) && fast_rc(true, gi, go, in, out, scale, (hipStream_t)stream, &rc))
         return rc;
     LineArgs a;
@@ -449,7 +482,14 @@ extern "C" int sfa_fft_c2r(const void* in, double* out, int ndim, const int64_t*
     if ((rc = make_geom(ndim, real_shape, in_strides, axis, &gi))) return rc;
     if ((rc = make_geom(ndim, real_shape, out_strides, axis, &go))) return rc;
     const int n = (int)gi.n;
-    SFA_CHECK(n >= 2 && n % 2 == 0, SFA_ERR_UNSUPPORTED, "C2R length must be even (got %d)", n);
+    SFA_CHECK(n >= 1, SFA_ERR_ARG, "C2R length must be >= 1");
+    SFA_CHECK(in != nullptr && out != nullptr, SFA_ERR_ARG, "C2R: null array");
+    if (n % 2) {
+        LineArgs a;
+        if ((rc = setup_line(a, in, out, gi, go, n, n, 1, 0))) return rc;
+        a.scale = scale;
+        return launch_line(OP_C2RF, true, a, (hipStream_t)stream);
+    }
     if (aligned16(in) && aligned16(out) && fast_rc(false, gi, go, in, out, scale, (hipStream_t)stream, &rc))
         return rc;
     LineArgs a;

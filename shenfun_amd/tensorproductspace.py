@@ -10,7 +10,7 @@ HIP transform on the strided device array.
 import numpy as np
 import torch
 
-from .bases import FourierBase, R2C, _Planned
+from .bases import FourierBase, R2C, _Planned, pointwise_mul
 
 
 class TensorProductSpace(_Planned):
@@ -162,6 +162,16 @@ class TensorProductSpace(_Planned):
     def backward(self, input_array=None, output_array=None, kind=None, **kw):
         """(tensorproductspace.py:46-96): first axis first."""
         return self._chain(input_array, output_array, "backward", list(range(self.dimensions)), kind)
+
+    def convolve(self, a_hat, b_hat, ab_hat=None):
+        """Convolution of two spectral arrays (tensorproductspace.py:637-666):
+        backward both (on this space's, possibly padded, grid), pointwise
+        product, forward into ``ab_hat``; alias-free when the Fourier bases
+        are padded by at least 3/2 (``get_dealiased``)."""
+        shp, dt = self.shape(False), self.physical_dtype()
+        a = self.backward(a_hat, self._buf(("conv", 0), shp, dt))
+        b = self.backward(b_hat, self._buf(("conv", 1), shp, dt))
+        return self.forward(pointwise_mul(a, b, self._buf(("conv", 2), shp, dt)), ab_hat)
 
 
 def Array(space, buffer=None, val=None):

@@ -38,7 +38,10 @@ def test_error_reporting_without_gpu_work():
     assert rc == -1
     assert b"TDMA" in L.sfa_last_error()
     rc = L.sfa_fft_r2c(None, None, 1, _lib.arr([7]), _lib.arr([1]), _lib.arr([1]), 0, 1.0, None)
-    assert rc == -3  # odd R2C length unsupported
+    assert rc == -1 and b"null" in L.sfa_last_error()
+    rc = L.sfa_pad_axis(None, None, 0, 1, _lib.arr([4]), _lib.arr([1]), _lib.arr([6]), _lib.arr([1]),
+                        0, 0, 0, 4, 0, 0, 0, None)
+    assert rc == -1
     rc = L.sfa_helmholtz3d(None, None, 4, 4, 4, 1.0, 0, 0, 0, None, None, 0, None)
     assert rc == -1
 

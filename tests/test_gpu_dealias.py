@@ -82,8 +82,6 @@ def test_padded_1d_transforms(sfa, ofam, fam, bc, dt, N, axis):
 @pytest.mark.parametrize("N", [12, 17])
 def test_dealias_direct_backward(sfa, ofam, fam, bc, dt, N):
     print(f"[dealias_direct {ofam} {bc} N={N}]", flush=True)
-    if ofam == "fourier-r2c" and N % 2:
-        pytest.skip("R2C lengths are even")
     V = sfa.FunctionSpace(N, fam, bc=bc, dtype=dt).get_dealiased(1, dealias_direct=True)
     rng = np.random.default_rng(N)
     c = _coeffs(ofam, N, (4,), rng)
@@ -124,7 +122,7 @@ def test_tensor_product_dealiased_product(sfa):
     for ax, (fam, bc) in enumerate(bases):
         ref = S.padded_backward(fam, bc, ref, N[ax], Ms[ax], ax)
     assert rel(u, ref) < TOL
-    w = Tp.forward(u * u)
+    w = Tp.convolve(dev(a_hat), dev(a_hat))
     wr = ref * ref
     for ax in range(2, -1, -1):
         fam, bc = bases[ax]
@@ -137,3 +135,60 @@ def test_tensor_product_dealiased_product(sfa):
                 wm[:N[ax] - 2].shape)
             wr = np.moveaxis(sol, 0, ax)
     assert rel(w, wr) < 1e-11
+
+
+@pytest.mark.parametrize("N", [9, 15, 27, 13])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_r2c_odd_lengths(sfa, N, axis):
+    """Odd R2C / C2R lengths (a full-length complex FFT) vs numpy's rfft/irfft."""
+    print(f"[r2c odd N={N} axis={axis}]", flush=True)
+    V = sfa.FunctionSpace(N, "F", dtype="d")
+    V.axis = axis
+    V.dimensions = 2
+    rng = np.random.default_rng(N)
+    f = rng.standard_normal((N, 6) if axis == 0 else (6, N))
+    c = V.scalar_product(dev(f))
+    assert rel(c, S.basis_scalar_product("fourier-r2c", None, f, axis)) < 1e-13
+    u = V.backward(c)
+    assert rel(u, f) < 1e-13
+    assert rel(u, S.basis_backward("fourier-r2c", None, c.cpu().numpy(), axis, N=N)) < 1e-13
+
+
+@pytest.mark.parametrize("fam,dt", [("fourier-r2c", "d"), ("fourier-c2c", "D")])
+@pytest.mark.parametrize("N", [8, 9, 16, 33])
+def test_convolve(sfa, fam, dt, N):
+    """R2C/C2C.convolve: direct sums vs the oracle's restatement, and the
+    padded (3/2 + 1/N, the reference test's factor) fast path vs the oracle
+    and vs the truncated direct sums (test_transforms.py:84-116)."""
+    print(f"[convolve {fam} N={N}]", flush=True)
+    rng = np.random.default_rng(N)
+    M0 = N // 2 + 1 if fam == "fourier-r2c" else N
+    u0 = rng.random(M0) + 1j * rng.random(M0)
+    u1 = rng.random(M0) + 1j * rng.random(M0)
+    if fam == "fourier-r2c":
+        u0[0] = u0[0].real
+        u1[0] = u1[0].real
+        if N % 2 == 0:
+            u0[-1] = u0[-1].real
+            u1[-1] = u1[-1].real
+    V = sfa.FunctionSpace(N, "F", dtype=dt)
+    uv1 = V.convolve(dev(u0), dev(u1), fast=False)
+    ref1 = S.convolve_direct(u0, u1, N, fam == "fourier-r2c")
+    assert rel(uv1, ref1) < 1e-13
+    P = sfa.FunctionSpace(N, "F", dtype=dt, padding_factor=1.5 + 1.00001 / N)
+    M = P.shape(False)
+    uv2 = P.convolve(dev(u0), dev(u1), fast=True)
+    ref2 = S.padded_scalar_product(fam, None, S.padded_backward(fam, None, u0, N, M)
+                                   * S.padded_backward(fam, None, u1, N, M), N)
+    assert rel(uv2, ref2) < 1e-12
+    uv3 = np.zeros_like(ref2)
+    if fam == "fourier-r2c":
+        uv3[:] = ref1[:N // 2 + 1]
+        if N % 2 == 0:
+            uv3[-1] = uv3[-1].real * 2
+    else:
+        uv3[:N // 2 + 1] = ref1[:N // 2 + 1]
+        uv3[-(N // 2):] += ref1[-(N // 2):]
+    assert rel(uv2, uv3) < 1e-12
+    with pytest.raises(ValueError):
+        V.convolve(dev(u0), dev(u1), fast=True)  # unpadded: aliasing

@@ -495,11 +495,7 @@ def test_padded_backward_is_the_series_on_the_fine_grid(fam, bc, N):
     """Padded backward = the N-term expansion evaluated at the M padded points
     (chebvander / legvander / explicit exponential sums with the symmetric
     Nyquist term), independent of the pad slices."""
-    if fam == "fourier-r2c" and N % 2:
-        pytest.skip("R2C lengths are even")
     M = int(np.floor(1.5 * N))
-    if fam == "fourier-r2c" and M % 2:
-        pytest.skip("R2C lengths are even")
     rng = np.random.default_rng(N)
     c = _spec(fam, N, rng)
     if bc is not None:
@@ -643,3 +639,37 @@ def test_bc_space_host_side_matches_the_oracle():
             A, B = V.bc_matrices()
             Ao, Bo = S.bc_matrices(f, kind, 16)
             assert np.allclose(A, Ao, atol=1e-13) and np.allclose(B, Bo, atol=1e-13)
+
+
+# ----------------------------------------------------------------- convolution
+@pytest.mark.parametrize("fam", ["fourier-r2c", "fourier-c2c"])
+@pytest.mark.parametrize("N", [8, 9])
+def test_convolve_fast_equals_direct(fam, N):
+    """tests/test_transforms.py:84-116 of the reference on the oracle: the
+    padded (3/2 + 1/N) transform product equals the direct sums after
+    truncation."""
+    rng = np.random.default_rng(N)
+    M0 = N // 2 + 1 if fam == "fourier-r2c" else N
+    u0 = rng.random(M0) + 1j * rng.random(M0)
+    u1 = rng.random(M0) + 1j * rng.random(M0)
+    if fam == "fourier-r2c":
+        u0[0] = u0[0].real
+        u1[0] = u1[0].real
+        if N % 2 == 0:
+            u0[-1] = u0[-1].real
+            u1[-1] = u1[-1].real
+    uv1 = S.convolve_direct(u0, u1, N, fam == "fourier-r2c")
+    pf = np.floor(N * (1.5 + 1.00001 / N)) / N
+    M = int(np.floor(pf * N))
+    uv2 = S.padded_scalar_product(fam, None, S.padded_backward(fam, None, u0, N, M)
+                                  * S.padded_backward(fam, None, u1, N, M), N)
+    uv3 = np.zeros_like(uv2)
+    if fam == "fourier-r2c":
+        uv3[:] = uv1[:N // 2 + 1]
+        if N % 2 == 0:
+            uv3[-1] *= 2
+            uv3[-1] = uv3[-1].real
+    else:
+        uv3[:N // 2 + 1] = uv1[:N // 2 + 1]
+        uv3[-(N // 2):] += uv1[-(N // 2):]
+    assert np.allclose(uv3, uv2)
