@@ -479,6 +479,52 @@ def convolve_direct(u, v, N, real):
 
 
 # --------------------------------------------------------------------------
+# Petrov-Galerkin test space Phi2
+# --------------------------------------------------------------------------
+
+
+def phi2_stencil(family, N):
+    """{offset: S(k)}, k < N-4: Chebyshev chebyshev/bases.py:1441-1445,
+    Legendre legendre/bases.py:631 (the ``_stencil`` dicts)."""
+    k = np.arange(N - 4, dtype=float)
+    if family == 'chebyshev':
+        return {0: 1 / (2 * PI * (k + 1) * (k + 2)), 2: -1 / (PI * (k + 1) * (k + 3)),
+                4: 1 / (2 * PI * (k + 2) * (k + 3))}
+    return {0: 1 / (2 * (2 * k + 3)), 2: -(2 * k + 5) / (2 * k + 7) / (2 * k + 3),
+            4: 1 / (2 * (2 * k + 7))}
+
+
+def phi2_scalar_product(family, f, axis=0):
+    """(f, phi_k)_w on the N points of f: the orthogonal scalar product, then
+    the stencil (spectralbase.py:1462-1474 with Phi2's stencil matrix,
+    :1504-1526); slots N-4 .. N-1 zero."""
+    N = f.shape[axis]
+    t = np.moveaxis(cheb_scalar_product(f, axis) if family == 'chebyshev'
+                    else leg_scalar_product(f, axis), axis, 0)
+    out = np.zeros_like(t)
+    for off, v in phi2_stencil(family, N).items():
+        out[:N - 4] += _bcast(v, t.ndim, 0) * t[off:off + N - 4]
+    return np.moveaxis(out, 0, axis)
+
+
+def pg_matrices(family, bc, N):
+    """Dense Petrov-Galerkin matrices (n x n, n = N-2) of an N-point trial space
+    and its Phi2 test space on N + 2 points: A[k, j] = (phi_k, psi_j'')_w,
+    B[k, j] = (phi_k, psi_j)_w, by Gauss quadrature on N + 8 points."""
+    n = N - 2
+    Q = N + 8
+    x, w = cheb_points_weights(Q) if family == 'chebyshev' else leg_points_weights(Q)
+    P = _poly_cls(family)
+    S = phi2_stencil(family, N + 2)
+    phi = np.stack([sum(S[o][k] * P.basis(k + o)(x) for o in S) for k in range(n)], axis=1)
+    K2 = stencil_k2(family, bc, N)
+    psi = [P.basis(j) + K2[j] * P.basis(j + 2) for j in range(n)]
+    A = (phi * w[:, None]).T @ np.stack([p.deriv(2)(x) for p in psi], axis=1)
+    B = (phi * w[:, None]).T @ np.stack([p(x) for p in psi], axis=1)
+    return A, B
+
+
+# --------------------------------------------------------------------------
 # Nonhomogeneous boundary values (Dirichlet or Neumann at both ends)
 # --------------------------------------------------------------------------
 

@@ -310,6 +310,18 @@ class _PolyBase(SpectralBase):
                           device=self.device, padding_factor=padding_factor,
                           dealias_direct=dealias_direct)
 
+    def get_testspace(self, kind="Galerkin", **kw):
+        """self for Galerkin; for 'PG' the Phi2 space on N + 2 points when self
+        has two boundary conditions (spectralbase.py:1148-1204)."""
+        if kind in ("Galerkin", "G"):
+            return self
+        if kind not in ("Petrov-Galerkin", "PG"):
+            raise ValueError(f"unknown test space kind {kind!r}")
+        if self.N - self.dim() != 2:
+            return self._clone(self.N, self.padding_factor, self.dealias_direct)
+        cls = ChebyshevPhi2 if self.family() == "chebyshev" else LegendrePhi2
+        return cls(self.N + 2, quad=self.quad, dtype=self.dtype, device=self.device)
+
     # ------------------------------------------------------ boundary values
     @property
     def has_nonhomogeneous_bcs(self):
@@ -656,6 +668,12 @@ class FourierBase(SpectralBase):
         return type(self)(N, device=self.device, padding_factor=padding_factor,
                           dealias_direct=dealias_direct)
 
+    def get_testspace(self, kind="Galerkin", **kw):
+        """self for Galerkin; a copy for 'PG' (Fourier spaces are their own test space)."""
+        if kind in ("Galerkin", "G"):
+            return self
+        return self._clone(self.N, self.padding_factor, self.dealias_direct)
+
     def forward(self, input_array=None, output_array=None, kind=None):
         return self.scalar_product(input_array, output_array, kind)
 
@@ -863,6 +881,82 @@ class LegendreShenDirichlet(_Init, LegendreBase):
 
 class LegendreShenNeumann(_Init, LegendreBase):
     bc_kind = "N"
+
+
+class _Phi2:
+    """Petrov-Galerkin test space of a two-condition composite space: phi_k =
+    (1-x^2)^2 P''_{k+2} / h, k < N-4, as the orthogonal combination
+    S0(k) P_k + S2(k) P_{k+2} + S4(k) P_{k+4} (chebyshev/bases.py:1357-1455,
+    legendre/bases.py:575-660, the ``_stencil`` dicts); get_testspace('PG')
+    (spectralbase.py:1148-1204) builds it on N + 2 points for an N-point
+    trial space.  Transforms are dense per-line contractions with the
+    stencil folded into the quadrature matrices (built once on the host)."""
+
+    bc_kind = "B"
+
+    def dim(self):
+        return self.N - 4
+
+    def boundary_condition(self):
+        return "Biharmonic"
+
+    def short_name(self):
+        return "P2"
+
+    def stencil(self):
+        """{offset: S_offset(k)} for k < N-4 (zero above)."""
+        k = np.arange(self.N - 4, dtype=float)
+        if self.family() == "chebyshev":
+            return {0: 1 / (2 * PI * (k + 1) * (k + 2)), 2: -1 / (PI * (k + 1) * (k + 3)),
+                    4: 1 / (2 * PI * (k + 2) * (k + 3))}
+        return {0: 1 / (2 * (2 * k + 3)), 2: -(2 * k + 5) / (2 * k + 7) / (2 * k + 3),
+                4: 1 / (2 * (2 * k + 7))}
+
+    def stencil_matrix(self):
+        S = np.zeros((self.N, self.N))
+        k = np.arange(self.N - 4)
+        for off, v in self.stencil().items():
+            S[k, k + off] = v
+        return S
+
+    def _ortho_vander(self):
+        x, w = self.points_and_weights(self.N)
+        return self._vander_at(x, self.N), w
+
+    def _analysis_matrix(self):
+        def build():
+            V, w = self._ortho_vander()
+            return self.stencil_matrix() @ (V * w[:, None]).T  # M[k, j] = w_j phi_k(x_j)
+        return self._dev_array("P2a", build)
+
+    def _synthesis_matrix(self):
+        def build():
+            V, _ = self._ortho_vander()
+            return V @ self.stencil_matrix().T  # M[j, k] = phi_k(x_j)
+        return self._dev_array("P2s", build)
+
+    def _scalar_product(self, inp, out, kind):
+        if out.data_ptr() == inp.data_ptr():
+            raise ValueError("dense transform cannot run in place")
+        self._matmul(inp, out, self._analysis_matrix())
+
+    def _backward(self, inp, out, kind):
+        self._matmul(inp, out, self._synthesis_matrix())
+
+    def forward(self, input_array=None, output_array=None, kind=None):
+        raise NotImplementedError("Phi2 is a test space: scalar_product and backward only")
+
+    def _clone(self, N, padding_factor=1, dealias_direct=False):
+        return type(self)(N, quad=self.quad, dtype=self.dtype, device=self.device,
+                          padding_factor=padding_factor, dealias_direct=dealias_direct)
+
+
+class ChebyshevPhi2(_Phi2, _Init, ChebyshevBase):
+    pass
+
+
+class LegendrePhi2(_Phi2, _Init, LegendreBase):
+    pass
 
 
 def _parse_bc(bc):

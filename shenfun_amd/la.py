@@ -303,9 +303,20 @@ class SolverGeneric1ND:
         if self.bc_mats:
             # la.py:1212-1229: u's boundary slots set, b -= bc matrices x slots
             apply_bc_matrices(self.bc_mats, b, b if u is None else u)
+        ax = self.naxes
+        if u is not None and u.shape != b.shape:
+            # a Petrov-Galerkin right-hand side lives on the test space (longer
+            # along the axis): its first n entries are the equations
+            # (BandedMatrixSolver.solve, la.py:256-260: u[sl] = b[sl])
+            _check(b, "b")
+            _check(u, "u")
+            if b.dim() != u.dim() or any(b.shape[i] != u.shape[i] for i in range(b.dim()) if i != ax) \
+                    or b.shape[ax] < self.n or b.dtype != u.dtype:
+                raise ValueError("b and u must agree off the solve axis")
+            u.narrow(ax, 0, self.n).copy_(b.narrow(ax, 0, self.n))
+            b = u
         u = _prepare(b, u)
         data = self.perform_lu(u.device)
-        ax = self.naxes
         # mode 0 line: index 0 along every non-axis dim
         s = [0] * u.dim()
         s[ax] = slice(None)

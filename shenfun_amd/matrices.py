@@ -323,14 +323,66 @@ class BCMatrix:
         return self.space.axis
 
 
-def helmholtz_tpmatrices(T, alpha=0.0):
+class PGMatrix(SparseMatrix):
+    """Petrov-Galerkin matrix (Phi2 test functions x the trial basis or its
+    second derivative) assembled once on the host by Gauss quadrature on
+    N + 6 points (exact for these degrees) and kept as its nonzero diagonals
+    (the reference's sympy/quadrature assembly of the same matrices)."""
+
+    def __init__(self, test, trial, deriv, key):
+        n = trial.N - 2
+        Q = trial.N + 6
+        x, w = trial.points_and_weights(Q)
+        V = trial._vander_at(x, trial.N + 2)
+        Phi = V[:, :test.N] @ test.stencil_matrix().T[:, :n]  # phi_k(x_q), k < n
+        K2 = trial.stencil_k2()
+        Vd = trial._vander_at(x, trial.N, deriv)
+        Psi = Vd[:, :n] + K2[None, :] * Vd[:, 2:n + 2]  # trial_j^(deriv)(x_q)
+        D = (Phi * w[:, None]).T @ Psi
+        # the bands are structural (phi_k spans P_k..P_{k+4}, the trial psi_j
+        # P_j, P_{j+2}; (1-x^2)^2 P''_{k+2} is orthogonal to degrees below k):
+        # -2..4 for the mass, 0 and 2 for the second derivative.  Outside
+        # them the quadrature leaves roundoff of the high-degree terms.
+        tol = 1e-11 * np.abs(D).max()
+        d = {}
+        for off in range(max(-4, -n + 1), min(9, n)):
+            v = np.diagonal(D, off)
+            if np.any(np.abs(v) > tol):
+                d[off] = v.copy()
+        super().__init__(d, (n, n))
+        self.testfunction = (test, 0)
+        self.trialfunction = (trial, deriv)
+        self._key = key
+
+    def get_key(self):
+        return self._key
+
+
+def helmholtz_tpmatrices(T, alpha=0.0, test=None):
     """The simplified TPMatrix list of inner(v, alpha*u - div(grad(u))) for a
     tensor-product space T with one Chebyshev/Legendre axis and Fourier
     elsewhere: [A (stiffness) with its scale, B (mass) with scale
     alpha + |k|^2] (forms/inner.py:374-493 -> get_simplified_tpmatrices,
-    matrixbase.py:1648-1677)."""
+    matrixbase.py:1648-1677).  With ``test = T.get_testspace('PG')`` the
+    Petrov-Galerkin pair (Phi2, u'') with scale -1 and (Phi2, u) with
+    scale alpha + |k|^2 (bands 0, 2 and -2..4: solved as FDMA)."""
     ax = T.nonperiodic_axis()
     S = T.bases[ax]
+    if test is not None and test is not T and getattr(test.bases[ax], "bc_kind", None) == "B":
+        if getattr(S, "has_nonhomogeneous_bcs", False):
+            raise NotImplementedError("Petrov-Galerkin with nonhomogeneous boundary values")
+        P2 = test.bases[ax]
+        shape = list(T.shape(True))
+        shape[ax] = 1
+        k2 = np.zeros(shape)
+        for i, b in enumerate(T.bases):
+            if i != ax:
+                k = b.wavenumbers().astype(float)
+                s = [1] * T.dimensions
+                s[i] = len(k)
+                k2 = k2 + (k ** 2).reshape(s)
+        return [TPMatrix(PGMatrix(P2, S, 2, "PGAmat"), T, np.full(shape, -1.0), ax),
+                TPMatrix(PGMatrix(P2, S, 0, "PGBmat"), T, alpha + k2, ax)]
     shape = list(T.shape(True))
     shape[ax] = 1
     k2 = np.zeros(shape)

@@ -73,8 +73,18 @@ class TensorProductSpace(_Planned):
 
     local_wavenumbers = wavenumbers
 
-    def get_testspace(self, kind="G"):
-        return self
+    def get_testspace(self, kind="Galerkin", **kw):
+        """self for Galerkin; for 'PG' the space with each basis replaced by
+        its Petrov-Galerkin test space (Phi2 on N + 2 points for the
+        two-condition axis, tensorproductspace.py get_testspace)."""
+        if kind in ("Galerkin", "G"):
+            return self
+        key = ("testspace", kind)
+        if key not in self._padded:
+            bases = [b.get_testspace(kind) if hasattr(b, "get_testspace") else b._clone(b.N)
+                     for b in self.bases]
+            self._padded[key] = TensorProductSpace(None, bases, device=self.device)
+        return self._padded[key]
 
     def get_dealiased(self, padding_factor=1.5, dealias_direct=False):
         """Space otherwise as self for dealiasing (tensorproductspace.py:543-578):

@@ -1,0 +1,90 @@
+"""GPU parity: Petrov-Galerkin (Phi2 test space) scalar products and
+Helmholtz solves against the oracle and manufactured solutions."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import spectral as S
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if isinstance(a, torch.Tensor) else a
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def dev(a):
+    return torch.from_numpy(np.array(a)).cuda()  # a copy: broadcast views are read-only
+
+
+@pytest.fixture(scope="module")
+def sfa():
+    import shenfun_amd
+    return shenfun_amd
+
+
+@pytest.mark.parametrize("family,fam", [("chebyshev", "C"), ("legendre", "L")])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_phi2_scalar_product(sfa, family, fam, axis):
+    print(f"[phi2 sp {family} axis={axis}]", flush=True)
+    P2 = sfa.FunctionSpace(20, fam, bc="D").get_testspace("PG")
+    assert P2.N == 22 and P2.dim() == 18
+    P2.axis = axis
+    P2.dimensions = 2
+    rng = np.random.default_rng(axis)
+    f = rng.standard_normal((22, 5) if axis == 0 else (5, 22))
+    assert rel(P2.scalar_product(dev(f)), S.phi2_scalar_product(family, f, axis)) < 1e-12
+
+
+@pytest.mark.parametrize("family,fam", [("chebyshev", "C"), ("legendre", "L")])
+@pytest.mark.parametrize("bc", ["D", "N"])
+def test_pg_helmholtz2d_manufactured(sfa, family, fam, bc):
+    """alpha u - lap u = f, Petrov-Galerkin (demo/poisson2D.py's test space):
+    f on the test space's N + 2 points, f_hat its Phi2 scalar product,
+    SolverGeneric1ND on the FDMA system, backward on the trial space."""
+    print(f"[pg helmholtz2d {family} {bc}]", flush=True)
+    alpha = 1.5
+    SD = sfa.FunctionSpace(24, fam, bc=bc)
+    K = sfa.FunctionSpace(16, "F", dtype="d")
+    T = sfa.TensorProductSpace(None, (SD, K))
+    B = T.get_testspace("PG")
+    Pn = np.polynomial.Polynomial
+    q = Pn([1, 0, -1]) if bc == "D" else Pn([1, 0, -2, 0, 1])  # q(+-1) = 0 / q'(+-1) = 0
+    r = Pn([0.3, 0.5, 0, 0.2]) * q
+    def fields(X):
+        x, y = X
+        u = q(x) * np.cos(2 * y) + r(x) * np.sin(3 * y) + (0.4 * q(x) if bc == "N" else 0)
+        f = (alpha * u - (q.deriv(2)(x) * np.cos(2 * y) - 4 * q(x) * np.cos(2 * y)
+                          + r.deriv(2)(x) * np.sin(3 * y) - 9 * r(x) * np.sin(3 * y)
+                          + (0.4 * q.deriv(2)(x) if bc == "N" else 0)))
+        return u, f
+    _, f = fields(B.mesh())
+    u_exact, _ = fields(T.mesh())
+    f_hat = B.scalar_product(dev(np.broadcast_to(f, B.shape(False))))
+    mats = sfa.helmholtz_tpmatrices(T, alpha, test=B)
+    sol = sfa.la.SolverGeneric1ND(mats)
+    assert sol.kind == "fdma"
+    u_hat = sol(f_hat, sfa.Function(T))
+    assert rel(T.backward(u_hat), np.broadcast_to(u_exact, T.shape(False))) < 1e-11
+
+
+def test_pg_helmholtz3d_chebyshev(sfa):
+    """Chebyshev x C2C x R2C, Petrov-Galerkin, per-(ky, kz) FDMA solves."""
+    print("[pg helmholtz3d]", flush=True)
+    alpha = 0.5
+    SD = sfa.FunctionSpace(20, "C", bc="D")
+    T = sfa.TensorProductSpace(None, (SD, sfa.FunctionSpace(12, "F", dtype="D"),
+                                      sfa.FunctionSpace(16, "F", dtype="d")))
+    B = T.get_testspace("PG")
+    q = np.polynomial.Polynomial([1, 0.2, -1, -0.2])  # (1 - x^2)(1 + 0.2 x)
+    def fields(X):
+        x, y, z = X
+        c = np.cos(y + 0.2) * np.cos(3 * z)
+        return q(x) * c, alpha * q(x) * c - (q.deriv(2)(x) * c - 10 * q(x) * c)
+    _, f = fields(B.mesh())
+    u_exact, _ = fields(T.mesh())
+    f_hat = B.scalar_product(dev(np.broadcast_to(f, B.shape(False))))
+    u_hat = sfa.la.SolverGeneric1ND(sfa.helmholtz_tpmatrices(T, alpha, test=B))(
+        f_hat, sfa.Function(T))
+    assert rel(T.backward(u_hat), np.broadcast_to(u_exact, T.shape(False))) < 1e-11

@@ -673,3 +673,59 @@ def test_convolve_fast_equals_direct(fam, N):
         uv3[:N // 2 + 1] = uv1[:N // 2 + 1]
         uv3[-(N // 2):] += uv1[-(N // 2):]
     assert np.allclose(uv3, uv2)
+
+
+# ----------------------------------------------------------------- Petrov-Galerkin
+@pytest.mark.parametrize("family", ["chebyshev", "legendre"])
+def test_phi2_functions_vanish_with_their_slope(family):
+    """phi_k = (1-x^2)^2 P''_{k+2} / h: phi_k(+-1) = phi_k'(+-1) = 0, and phi_k
+    is proportional to (1-x^2)^2 P''_{k+2} (checked by division)."""
+    N = 14
+    P = S._poly_cls(family)
+    St = S.phi2_stencil(family, N)
+    x = np.linspace(-0.9, 0.9, 7)
+    for k in range(N - 4):
+        phi = sum(St[o][k] * P.basis(k + o) for o in St)
+        for xe in (-1.0, 1.0):
+            assert abs(phi(xe)) < 1e-13 and abs(phi.deriv(1)(xe)) < 1e-12
+        ratio = phi(x) / ((1 - x * x) ** 2 * P.basis(k + 2).deriv(2)(x))
+        assert np.allclose(ratio, ratio[0], rtol=1e-10)
+
+
+@pytest.mark.parametrize("family", ["chebyshev", "legendre"])
+@pytest.mark.parametrize("bc", ["D", "N"])
+def test_pg_matrices_bands(family, bc):
+    """(Phi2, psi'') has bands 0, 2 (diagonal -1 for Dirichlet) and (Phi2, psi)
+    bands -2..4: what makes the Petrov-Galerkin Poisson system FDMA."""
+    A, B = S.pg_matrices(family, bc, 14)
+    nz = lambda M: sorted({j - k for k in range(M.shape[0]) for j in range(M.shape[1])
+                           if abs(M[k, j]) > 1e-12 * abs(M).max()})
+    assert nz(A) == [0, 2] and nz(B) == [-2, 0, 2, 4]
+    if bc == "D":
+        assert np.allclose(np.diag(A), -1.0)
+
+
+def test_phi2_scalar_product_matches_quadrature():
+    """The stencil route equals the quadrature sum of f_j w_j phi_k(x_j)."""
+    for family in ("chebyshev", "legendre"):
+        N = 16
+        x, w = S.cheb_points_weights(N) if family == "chebyshev" else S.leg_points_weights(N)
+        f = np.sin(2 * x) + x ** 3
+        P = S._poly_cls(family)
+        St = S.phi2_stencil(family, N)
+        ref = [np.sum(w * f * sum(St[o][k] * P.basis(k + o)(x) for o in St)) for k in range(N - 4)]
+        s = S.phi2_scalar_product(family, f)
+        assert np.allclose(s[:N - 4], ref, rtol=1e-12, atol=1e-14) and np.all(s[N - 4:] == 0)
+
+
+def test_pg_tpmatrices_host_side_match_the_oracle():
+    import shenfun_amd as sfa
+    for fam, family in (("C", "chebyshev"), ("L", "legendre")):
+        for bc in ("D", "N"):
+            SD = sfa.FunctionSpace(14, fam, bc=bc)
+            T = sfa.TensorProductSpace(None, (SD, sfa.FunctionSpace(8, "F", dtype="d")))
+            mats = sfa.helmholtz_tpmatrices(T, 2.0, test=T.get_testspace("PG"))
+            A, B = S.pg_matrices(family, bc, 14)
+            assert np.allclose(mats[0].pmat.todense(), A, atol=1e-12)
+            assert np.allclose(mats[1].pmat.todense(), B, atol=1e-12)
+            assert np.allclose(mats[1].scale[0], 2.0 + np.arange(5) ** 2)
