@@ -494,6 +494,43 @@ def phi2_stencil(family, N):
             4: 1 / (2 * (2 * k + 7))}
 
 
+def biharmonic_stencil(family, kind, N):
+    """{offset: S(k)}, k < N-4 (SB) or N-8 (P4): ShenBiharmonic
+    chebyshev/bases.py ShenBiharmonic ``_stencil`` {0: 1, 2: -(2n+4)/(n+3),
+    4: (n+1)/(n+3)}, legendre/bases.py {0: 1, 2: -(4n+10)/(2n+7),
+    4: (2n+3)/(2n+7)}; Phi4 chebyshev/bases.py / legendre/bases.py Phi4."""
+    if kind == 'SB':
+        k = np.arange(N - 4, dtype=float)
+        if family == 'chebyshev':
+            return {0: np.ones_like(k), 2: -(2 * k + 4) / (k + 3), 4: (k + 1) / (k + 3)}
+        return {0: np.ones_like(k), 2: -(4 * k + 10) / (2 * k + 7), 4: (2 * k + 3) / (2 * k + 7)}
+    n = np.arange(N - 8, dtype=float)
+    if family == 'chebyshev':
+        return {0: 1 / (8 * PI * (n + 1) * (n + 2) * (n + 3) * (n + 4)),
+                2: -1 / (2 * PI * (n + 1) * (n + 3) * (n + 4) * (n + 5)),
+                4: 3 / (4 * PI * (n + 2) * (n + 3) * (n + 5) * (n + 6)),
+                6: -1 / (2 * PI * (n + 3) * (n + 4) * (n + 5) * (n + 7)),
+                8: 1 / (8 * PI * (n + 4) * (n + 5) * (n + 6) * (n + 7))}
+    return {0: 1 / (2 * (8 * n ** 3 + 60 * n ** 2 + 142 * n + 105)),
+            2: -2 / (8 * n ** 3 + 84 * n ** 2 + 262 * n + 231),
+            4: 3 * (2 * n + 9) / ((2 * n + 5) * (2 * n + 7) * (2 * n + 11) * (2 * n + 13)),
+            6: -2 / (8 * n ** 3 + 132 * n ** 2 + 694 * n + 1155),
+            8: 1 / (2 * (8 * n ** 3 + 156 * n ** 2 + 1006 * n + 2145))}
+
+
+def stencil_polys(family, st):
+    """The basis functions of a stencil dict as numpy polynomial objects."""
+    P = _poly_cls(family)
+    n = len(next(iter(st.values())))
+    return [sum(st[o][k] * P.basis(k + o) for o in st) for k in range(n)]
+
+
+def sb_backward(family, c, x):
+    """ShenBiharmonic synthesis sum_k c_k phi_k(x) (dofs k < N-4)."""
+    N = len(c)
+    return sum(c[k] * p(x) for k, p in enumerate(stencil_polys(family, biharmonic_stencil(family, 'SB', N))))
+
+
 def phi2_scalar_product(family, f, axis=0):
     """(f, phi_k)_w on the N points of f: the orthogonal scalar product, then
     the stencil (spectralbase.py:1462-1474 with Phi2's stencil matrix,

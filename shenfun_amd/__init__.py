@@ -12,7 +12,7 @@ from .bases import (C2C, R2C, ChebyshevOrthogonal, ChebyshevShenDirichlet,  # no
                     ChebyshevShenNeumann, FunctionSpace, LegendreOrthogonal,
                     LegendreShenDirichlet, LegendreShenNeumann)
 from .config import config  # noqa: F401
-from .matrices import (SparseMatrix, TPMatrix, helmholtz_tpmatrices, mass_matrix,  # noqa: F401
+from .matrices import (SparseMatrix, TPMatrix, biharmonic_tpmatrices, helmholtz_tpmatrices, mass_matrix,  # noqa: F401
                        stiffness_matrix)
 from .batched import BatchedHelmholtz2D  # noqa: F401
 from .pipeline import HelmholtzSolver3D  # noqa: F401

@@ -310,6 +310,29 @@ class _PolyBase(SpectralBase):
                           device=self.device, padding_factor=padding_factor,
                           dealias_direct=dealias_direct)
 
+    def stencil(self):
+        """{offset: coefficients} of the basis in the orthogonal polynomials."""
+        if not self.is_composite:
+            return {0: np.ones(self.N)}
+        return {0: np.ones(self.N - 2), 2: self.stencil_k2()}
+
+    def stencil_matrix(self):
+        """(N, N): row k < dim() holds phi_k's orthogonal coefficients."""
+        S = np.zeros((self.N, self.N))
+        k = np.arange(self.dim())
+        for off, v in self.stencil().items():
+            S[k, k + off] = v
+        return S
+
+    def ortho_norms(self, n):
+        """Exact (P_m, P_m)_w, m < n: Chebyshev pi (m = 0), pi/2; Legendre 2/(2m+1)."""
+        m = np.arange(n, dtype=float)
+        if self.family() == "chebyshev":
+            h = np.full(n, PI / 2)
+            h[0] = PI
+            return h
+        return 2.0 / (2 * m + 1)
+
     def get_testspace(self, kind="Galerkin", **kw):
         """self for Galerkin; for 'PG' the Phi2 space on N + 2 points when self
         has two boundary conditions (spectralbase.py:1148-1204)."""
@@ -883,41 +906,64 @@ class LegendreShenNeumann(_Init, LegendreBase):
     bc_kind = "N"
 
 
-class _Phi2:
-    """Petrov-Galerkin test space of a two-condition composite space: phi_k =
-    (1-x^2)^2 P''_{k+2} / h, k < N-4, as the orthogonal combination
-    S0(k) P_k + S2(k) P_{k+2} + S4(k) P_{k+4} (chebyshev/bases.py:1357-1455,
-    legendre/bases.py:575-660, the ``_stencil`` dicts); get_testspace('PG')
-    (spectralbase.py:1148-1204) builds it on N + 2 points for an N-point
-    trial space.  Transforms are dense per-line contractions with the
-    stencil folded into the quadrature matrices (built once on the host)."""
+def _stencil_dict(family, kind, k):
+    """{offset: coefficient(k)} of phi_k in the orthogonal polynomials.
+    ShenBiharmonic (u = u' = 0 at both ends): chebyshev/bases.py:1175-1260
+    (``_stencil``), legendre/bases.py ShenBiharmonic; Phi2 (test space of
+    two-condition bases): chebyshev/bases.py:1441-1445, legendre/bases.py:631;
+    Phi4 (test space of four-condition bases): chebyshev/bases.py Phi4,
+    legendre/bases.py Phi4."""
+    n = np.asarray(k, dtype=float)
+    if kind == "SB":
+        if family == "chebyshev":
+            return {0: np.ones_like(n), 2: -(2 * n + 4) / (n + 3), 4: (n + 1) / (n + 3)}
+        return {0: np.ones_like(n), 2: -(4 * n + 10) / (2 * n + 7), 4: (2 * n + 3) / (2 * n + 7)}
+    if kind == "P2":
+        if family == "chebyshev":
+            return {0: 1 / (2 * PI * (n + 1) * (n + 2)), 2: -1 / (PI * (n + 1) * (n + 3)),
+                    4: 1 / (2 * PI * (n + 2) * (n + 3))}
+        return {0: 1 / (2 * (2 * n + 3)), 2: -(2 * n + 5) / (2 * n + 7) / (2 * n + 3),
+                4: 1 / (2 * (2 * n + 7))}
+    if family == "chebyshev":  # P4
+        return {0: 1 / (8 * PI * (n + 1) * (n + 2) * (n + 3) * (n + 4)),
+                2: -1 / (2 * PI * (n + 1) * (n + 3) * (n + 4) * (n + 5)),
+                4: 3 / (4 * PI * (n + 2) * (n + 3) * (n + 5) * (n + 6)),
+                6: -1 / (2 * PI * (n + 3) * (n + 4) * (n + 5) * (n + 7)),
+                8: 1 / (8 * PI * (n + 4) * (n + 5) * (n + 6) * (n + 7))}
+    return {0: 1 / (2 * (8 * n ** 3 + 60 * n ** 2 + 142 * n + 105)),
+            2: -2 / (8 * n ** 3 + 84 * n ** 2 + 262 * n + 231),
+            4: 3 * (2 * n + 9) / ((2 * n + 5) * (2 * n + 7) * (2 * n + 11) * (2 * n + 13)),
+            6: -2 / (8 * n ** 3 + 132 * n ** 2 + 694 * n + 1155),
+            8: 1 / (2 * (8 * n ** 3 + 156 * n ** 2 + 1006 * n + 2145))}
 
-    bc_kind = "B"
+
+class _StencilSpace:
+    """Composite spaces given by a stencil of several bands: the
+    ShenBiharmonic trial space (4 boundary slots) and the Petrov-Galerkin
+    test spaces Phi2 (4) and Phi4 (8) that get_testspace('PG') builds on N + 2
+    / N + 4 points (spectralbase.py:1148-1204).  Transforms are dense
+    per-line contractions with the stencil folded into the quadrature
+    matrices (built once on the host); forward of ShenBiharmonic solves its
+    pentadiagonal mass matrix."""
+
+    kind = None
+    _nbc = {"SB": 4, "P2": 4, "P4": 8}
+
+    @property
+    def bc_kind(self):
+        return self.kind
 
     def dim(self):
-        return self.N - 4
+        return self.N - self._nbc[self.kind]
 
     def boundary_condition(self):
         return "Biharmonic"
 
     def short_name(self):
-        return "P2"
+        return self.kind
 
     def stencil(self):
-        """{offset: S_offset(k)} for k < N-4 (zero above)."""
-        k = np.arange(self.N - 4, dtype=float)
-        if self.family() == "chebyshev":
-            return {0: 1 / (2 * PI * (k + 1) * (k + 2)), 2: -1 / (PI * (k + 1) * (k + 3)),
-                    4: 1 / (2 * PI * (k + 2) * (k + 3))}
-        return {0: 1 / (2 * (2 * k + 3)), 2: -(2 * k + 5) / (2 * k + 7) / (2 * k + 3),
-                4: 1 / (2 * (2 * k + 7))}
-
-    def stencil_matrix(self):
-        S = np.zeros((self.N, self.N))
-        k = np.arange(self.N - 4)
-        for off, v in self.stencil().items():
-            S[k, k + off] = v
-        return S
+        return _stencil_dict(self.family(), self.kind, np.arange(self.dim()))
 
     def _ortho_vander(self):
         x, w = self.points_and_weights(self.N)
@@ -927,13 +973,13 @@ class _Phi2:
         def build():
             V, w = self._ortho_vander()
             return self.stencil_matrix() @ (V * w[:, None]).T  # M[k, j] = w_j phi_k(x_j)
-        return self._dev_array("P2a", build)
+        return self._dev_array("STa", build)
 
     def _synthesis_matrix(self):
         def build():
             V, _ = self._ortho_vander()
             return V @ self.stencil_matrix().T  # M[j, k] = phi_k(x_j)
-        return self._dev_array("P2s", build)
+        return self._dev_array("STs", build)
 
     def _scalar_product(self, inp, out, kind):
         if out.data_ptr() == inp.data_ptr():
@@ -944,19 +990,75 @@ class _Phi2:
         self._matmul(inp, out, self._synthesis_matrix())
 
     def forward(self, input_array=None, output_array=None, kind=None):
-        raise NotImplementedError("Phi2 is a test space: scalar_product and backward only")
+        if self.kind != "SB":
+            raise NotImplementedError(f"{self.kind} is a test space: scalar_product and backward only")
+        out = self.scalar_product(input_array, output_array, kind)
+        sl = [slice(None)] * out.dim()
+        sl[self.axis] = slice(self.dim(), self.N)
+        out[tuple(sl)] = 0
+        self._mass_solver()(out, axis=self.axis)
+        return out
+
+    def _mass_solver(self):
+        if "mass" not in self._cache:
+            from .matrices import GalerkinMatrix
+            B = GalerkinMatrix(self, self, 0, "BSBSBmat")
+            self._cache["mass"] = B.get_solver()(B)
+        return self._cache["mass"]
+
+    def get_testspace(self, kind="Galerkin", **kw):
+        if kind in ("Galerkin", "G"):
+            return self
+        if kind not in ("Petrov-Galerkin", "PG"):
+            raise ValueError(f"unknown test space kind {kind!r}")
+        if self.kind != "SB":
+            return self
+        cls = ChebyshevPhi4 if self.family() == "chebyshev" else LegendrePhi4
+        return cls(self.N + 4, quad=self.quad, dtype=self.dtype, device=self.device)
 
     def _clone(self, N, padding_factor=1, dealias_direct=False):
-        return type(self)(N, quad=self.quad, dtype=self.dtype, device=self.device,
-                          padding_factor=padding_factor, dealias_direct=dealias_direct)
+        if padding_factor != 1 or dealias_direct:
+            raise NotImplementedError("dealiasing of the biharmonic / Petrov-Galerkin spaces")
+        return type(self)(N, quad=self.quad, dtype=self.dtype, device=self.device)
 
 
-class ChebyshevPhi2(_Phi2, _Init, ChebyshevBase):
-    pass
+class ChebyshevPhi2(_StencilSpace, _Init, ChebyshevBase):
+    kind = "P2"
 
 
-class LegendrePhi2(_Phi2, _Init, LegendreBase):
-    pass
+class LegendrePhi2(_StencilSpace, _Init, LegendreBase):
+    kind = "P2"
+
+
+class ChebyshevPhi4(_StencilSpace, _Init, ChebyshevBase):
+    kind = "P4"
+
+
+class LegendrePhi4(_StencilSpace, _Init, LegendreBase):
+    kind = "P4"
+
+
+class ChebyshevShenBiharmonic(_StencilSpace, _Init, ChebyshevBase):
+    kind = "SB"
+
+
+class LegendreShenBiharmonic(_StencilSpace, _Init, LegendreBase):
+    kind = "SB"
+
+
+def _is_biharmonic(bc):
+    if isinstance(bc, str):
+        return bc.lower() in ("biharmonic", "b")
+    if isinstance(bc, (tuple, list)) and len(bc) == 4:
+        if any(float(v) != 0 for v in bc):
+            raise NotImplementedError("biharmonic spaces take homogeneous boundary values")
+        return True
+    if isinstance(bc, dict) and set(bc) == {"left", "right"} and \
+            all(set(bc[sd]) == {"D", "N"} for sd in bc):
+        if any(float(v) != 0 for sd in bc for v in bc[sd].values()):
+            raise NotImplementedError("biharmonic spaces take homogeneous boundary values")
+        return True
+    return False
 
 
 def _parse_bc(bc):
@@ -1008,6 +1110,12 @@ def FunctionSpace(N, family="Fourier", bc=None, dtype="d", quad=None, domain=Non
         if np.dtype(dtype).kind == "c":
             return C2C(N, device=device, **pad)
         return R2C(N, device=device, **pad)
+    if bc is not None and _is_biharmonic(bc):
+        if fam in ("chebyshev", "c"):
+            return ChebyshevShenBiharmonic(N, quad=quad or "GC", device=device)
+        if fam in ("legendre", "l"):
+            return LegendreShenBiharmonic(N, quad="LG", device=device)
+        raise NotImplementedError(f"biharmonic {family!r} space")
     kind, vals = _parse_bc(bc)
     if fam in ("chebyshev", "c"):
         if quad not in (None, "GC", "GL"):

@@ -323,39 +323,103 @@ class BCMatrix:
         return self.space.axis
 
 
-class PGMatrix(SparseMatrix):
-    """Petrov-Galerkin matrix (Phi2 test functions x the trial basis or its
-    second derivative) assembled once on the host by Gauss quadrature on
-    N + 6 points (exact for these degrees) and kept as its nonzero diagonals
-    (the reference's sympy/quadrature assembly of the same matrices)."""
+def _clean_diagonals(D, reltol=1e-8):
+    """Nonzero diagonals of the dense matrix D as {offset: diagonal}: a
+    diagonal is dropped when its 2-norm over max|row sum| / ncols is below
+    reltol (SparseMatrix.clean_diagonals, matrixbase.py:510-528)."""
+    relmax = np.abs(D.sum(axis=1)).max() / D.shape[1]
+    if relmax == 0:
+        relmax = 1.0
+    d = {}
+    for off in range(-D.shape[0] + 1, D.shape[1]):
+        v = np.diagonal(D, off)
+        if np.linalg.norm(v) / relmax >= reltol:
+            d[off] = v.copy()
+    return d
+
+
+class GalerkinMatrix(SparseMatrix):
+    """(test_k, d^deriv/dx^deriv trial_j)_w for composite test/trial spaces
+    given by their stencils, assembled once on the host by exact algebra in
+    the orthogonal basis: the trial functions' derivative coefficients (numpy
+    chebder / legder), weighted by the exact norms (P_m, P_m)_w, against the
+    test stencil; then the nonzero diagonals (the reference assembles the
+    same matrices by quadrature or closed forms and cleans them the same
+    way).  Used for the Petrov-Galerkin and biharmonic systems."""
 
     def __init__(self, test, trial, deriv, key):
-        n = trial.N - 2
-        Q = trial.N + 6
-        x, w = trial.points_and_weights(Q)
-        V = trial._vander_at(x, trial.N + 2)
-        Phi = V[:, :test.N] @ test.stencil_matrix().T[:, :n]  # phi_k(x_q), k < n
-        K2 = trial.stencil_k2()
-        Vd = trial._vander_at(x, trial.N, deriv)
-        Psi = Vd[:, :n] + K2[None, :] * Vd[:, 2:n + 2]  # trial_j^(deriv)(x_q)
-        D = (Phi * w[:, None]).T @ Psi
-        # the bands are structural (phi_k spans P_k..P_{k+4}, the trial psi_j
-        # P_j, P_{j+2}; (1-x^2)^2 P''_{k+2} is orthogonal to degrees below k):
-        # -2..4 for the mass, 0 and 2 for the second derivative.  Outside
-        # them the quadrature leaves roundoff of the high-degree terms.
-        tol = 1e-11 * np.abs(D).max()
-        d = {}
-        for off in range(max(-4, -n + 1), min(9, n)):
-            v = np.diagonal(D, off)
-            if np.any(np.abs(v) > tol):
-                d[off] = v.copy()
-        super().__init__(d, (n, n))
+        n = trial.dim()
+        if test.dim() != n:
+            raise ValueError("test and trial spaces must have the same number of unknowns")
+        # band window.  test_k spans P_k .. P_{k+s_te}, trial_j P_j .. P_{j+s_tr}:
+        # the mass matrix lies in [-s_tr, s_te].  A Petrov-Galerkin test
+        # function is (1-x^2)^q times a polynomial, so deriv integrations by
+        # parts move the derivative onto it and the window is
+        # [deriv - s_tr, s_te]; Legendre Galerkin (weight 1, trial functions
+        # vanishing with their slopes) is symmetric, [-s_tr, s_te].  Outside
+        # the window the algebra below would only leave the cancellation
+        # error of very large derivative coefficients.
+        s_te, s_tr = max(test.stencil()), max(trial.stencil())
+        if deriv == 0:
+            lo = -s_tr
+        elif getattr(test, "kind", None) in ("P2", "P4"):
+            lo = deriv - s_tr
+        elif trial.family() == "legendre" and test.bc_kind == trial.bc_kind:
+            lo = -s_tr
+        else:
+            raise NotImplementedError("Chebyshev Galerkin matrices of derivatives are not banded; "
+                                      "use the Petrov-Galerkin test space get_testspace('PG')")
+        L = max(test.N, trial.N)
+        P = np.polynomial.chebyshev if trial.family() == "chebyshev" else np.polynomial.legendre
+        der = P.chebder if trial.family() == "chebyshev" else P.legder
+        St = trial.stencil_matrix()[:n]
+        Dc = np.zeros((n, L))
+        for j in range(n):
+            c = der(St[j], deriv) if deriv else St[j]
+            Dc[j, :len(c)] = c
+        Sv = np.zeros((n, L))
+        Sv[:, :test.N] = test.stencil_matrix()[:n]
+        D = (Sv * trial.ortho_norms(L)[None, :]) @ Dc.T
+        off = np.arange(n)[None, :] - np.arange(n)[:, None]
+        D[(off < lo) | (off > s_te)] = 0.0
+        super().__init__(_clean_diagonals(D), (n, n))
         self.testfunction = (test, 0)
         self.trialfunction = (trial, deriv)
         self._key = key
 
     def get_key(self):
         return self._key
+
+
+def _k2(T, ax):
+    shape = list(T.shape(True))
+    shape[ax] = 1
+    k2 = np.zeros(shape)
+    for i, b in enumerate(T.bases):
+        if i != ax:
+            k = b.wavenumbers().astype(float)
+            s = [1] * T.dimensions
+            s[i] = len(k)
+            k2 = k2 + (k ** 2).reshape(s)
+    return k2
+
+
+def biharmonic_tpmatrices(T, a=1.0, b=0.0, c=0.0, test=None):
+    """TPMatrix list of inner(v, a*biharmonic(u) + b*div(grad(u)) + c*u) for T
+    with one ShenBiharmonic axis and Fourier elsewhere, v in ``test``
+    (T.get_testspace('PG'): Phi4, the HeptaDMA systems; or T itself,
+    Galerkin, banded for Legendre): (v, 4th x-derivative of u) with scale a,
+    (v, 2nd x-derivative) with b - 2a|k|^2, (v, u) with a|k|^4 - b|k|^2 + c
+    (forms/inner.py -> get_simplified_tpmatrices, matrixbase.py:1648-1677)."""
+    ax = T.nonperiodic_axis()
+    S = T.bases[ax]
+    if getattr(S, "kind", None) != "SB":
+        raise ValueError("biharmonic_tpmatrices needs a ShenBiharmonic axis")
+    V = (test if test is not None else T).bases[ax]
+    k2 = _k2(T, ax)
+    return [TPMatrix(GalerkinMatrix(V, S, 4, "SBBmat"), T, np.full(k2.shape, float(a)), ax),
+            TPMatrix(GalerkinMatrix(V, S, 2, "ABBmat"), T, b - 2 * a * k2, ax),
+            TPMatrix(GalerkinMatrix(V, S, 0, "BBBmat"), T, a * k2 ** 2 - b * k2 + c, ax)]
 
 
 def helmholtz_tpmatrices(T, alpha=0.0, test=None):
@@ -368,21 +432,13 @@ def helmholtz_tpmatrices(T, alpha=0.0, test=None):
     scale alpha + |k|^2 (bands 0, 2 and -2..4: solved as FDMA)."""
     ax = T.nonperiodic_axis()
     S = T.bases[ax]
-    if test is not None and test is not T and getattr(test.bases[ax], "bc_kind", None) == "B":
+    if test is not None and test is not T and getattr(test.bases[ax], "kind", None) == "P2":
         if getattr(S, "has_nonhomogeneous_bcs", False):
             raise NotImplementedError("Petrov-Galerkin with nonhomogeneous boundary values")
         P2 = test.bases[ax]
-        shape = list(T.shape(True))
-        shape[ax] = 1
-        k2 = np.zeros(shape)
-        for i, b in enumerate(T.bases):
-            if i != ax:
-                k = b.wavenumbers().astype(float)
-                s = [1] * T.dimensions
-                s[i] = len(k)
-                k2 = k2 + (k ** 2).reshape(s)
-        return [TPMatrix(PGMatrix(P2, S, 2, "PGAmat"), T, np.full(shape, -1.0), ax),
-                TPMatrix(PGMatrix(P2, S, 0, "PGBmat"), T, alpha + k2, ax)]
+        k2 = _k2(T, ax)
+        return [TPMatrix(GalerkinMatrix(P2, S, 2, "PGAmat"), T, np.full(k2.shape, -1.0), ax),
+                TPMatrix(GalerkinMatrix(P2, S, 0, "PGBmat"), T, alpha + k2, ax)]
     shape = list(T.shape(True))
     shape[ax] = 1
     k2 = np.zeros(shape)

@@ -23,7 +23,7 @@ from .bases import C2C, ChebyshevBase, R2C
 class HelmholtzSolver3D:
     def __init__(self, T, alpha, precision="fp64", fp64_planes=0):
         b = T.bases
-        if not (len(b) == 3 and isinstance(b[0], ChebyshevBase) and b[0].is_composite
+        if not (len(b) == 3 and isinstance(b[0], ChebyshevBase) and b[0].bc_kind in ("D", "N")
                 and isinstance(b[1], C2C) and isinstance(b[2], R2C)):
             raise ValueError("expects Chebyshev(SD/SN) x C2C x R2C")
         if b[0].quad != "GC":

@@ -88,3 +88,63 @@ def test_pg_helmholtz3d_chebyshev(sfa):
     u_hat = sfa.la.SolverGeneric1ND(sfa.helmholtz_tpmatrices(T, alpha, test=B))(
         f_hat, sfa.Function(T))
     assert rel(T.backward(u_hat), np.broadcast_to(u_exact, T.shape(False))) < 1e-11
+
+
+@pytest.mark.parametrize("family,fam", [("chebyshev", "C"), ("legendre", "L")])
+def test_shen_biharmonic_transforms(sfa, family, fam):
+    """ShenBiharmonic backward vs the oracle's synthesis, and forward (scalar
+    product + its pentadiagonal mass solve) of a polynomial in the space
+    recovers its coefficients."""
+    print(f"[sb transforms {family}]", flush=True)
+    N = 24
+    V = sfa.FunctionSpace(N, fam, bc="Biharmonic")
+    assert V.dim() == N - 4
+    rng = np.random.default_rng(1)
+    c = np.zeros(N)
+    c[:N - 4] = rng.standard_normal(N - 4)
+    x = np.asarray(V.mesh())
+    u = V.backward(dev(c))
+    assert rel(u, S.sb_backward(family, c, x)) < 1e-12
+    assert rel(V.forward(u), c) < 1e-11
+
+
+def _biharmonic_fields(X, a, b, c, kx):
+    """u = g(x) cos(kx y) + h(x) with g, h vanishing with their slopes at +-1,
+    and f = a biharm(u) + b lap(u) + c u."""
+    Pn = np.polynomial.Polynomial
+    w = Pn([1, 0, -2, 0, 1])  # (1 - x^2)^2
+    g = w * Pn([0.5, 0.3, -0.2])
+    h = w * Pn([0.2, 0, 0.4])
+    x, y = X[0], X[1]
+    cy = np.cos(kx * y)
+    u = g(x) * cy + h(x)
+    k2 = kx * kx
+    f = (a * (g.deriv(4)(x) - 2 * k2 * g.deriv(2)(x) + k2 * k2 * g(x)) + b * (g.deriv(2)(x) - k2 * g(x))
+         + c * g(x)) * cy + a * h.deriv(4)(x) + b * h.deriv(2)(x) + c * h(x)
+    return u, f
+
+
+@pytest.mark.parametrize("family,fam,kind", [("chebyshev", "C", "PG"), ("legendre", "L", "PG"),
+                                             ("legendre", "L", "G")])
+def test_biharmonic2d_manufactured(sfa, family, fam, kind):
+    """a biharm(u) + b lap(u) + c u = f with u = u' = 0 at x = +-1, Petrov-Galerkin
+    (Phi4 test space, HeptaDMA per mode) or Legendre Galerkin (PDMA)."""
+    print(f"[biharmonic2d {family} {kind}]", flush=True)
+    a, b, c = 1.0, -3.0, 2.0
+    SB = sfa.FunctionSpace(28, fam, bc=(0, 0, 0, 0))
+    T = sfa.TensorProductSpace(None, (SB, sfa.FunctionSpace(16, "F", dtype="d")))
+    B = T.get_testspace(kind)
+    _, f = _biharmonic_fields(B.mesh(), a, b, c, 3)
+    u_exact, _ = _biharmonic_fields(T.mesh(), a, b, c, 3)
+    f_hat = B.scalar_product(dev(np.broadcast_to(f, B.shape(False))))
+    sol = sfa.la.SolverGeneric1ND(sfa.biharmonic_tpmatrices(T, a, b, c, test=B))
+    assert sol.kind == ("heptadma" if kind == "PG" else "pdma")
+    u_hat = sol(f_hat, sfa.Function(T))
+    assert rel(T.backward(u_hat), np.broadcast_to(u_exact, T.shape(False))) < 1e-10
+
+
+def test_biharmonic_chebyshev_galerkin_is_refused(sfa):
+    SB = sfa.FunctionSpace(20, "C", bc="Biharmonic")
+    T = sfa.TensorProductSpace(None, (SB, sfa.FunctionSpace(8, "F", dtype="d")))
+    with pytest.raises(NotImplementedError):
+        sfa.biharmonic_tpmatrices(T)

@@ -729,3 +729,38 @@ def test_pg_tpmatrices_host_side_match_the_oracle():
             assert np.allclose(mats[0].pmat.todense(), A, atol=1e-12)
             assert np.allclose(mats[1].pmat.todense(), B, atol=1e-12)
             assert np.allclose(mats[1].scale[0], 2.0 + np.arange(5) ** 2)
+
+
+@pytest.mark.parametrize("family", ["chebyshev", "legendre"])
+def test_biharmonic_bases_boundary_conditions(family):
+    """ShenBiharmonic functions vanish with their slope at +-1; Phi4 functions
+    are (1-x^2)^4 P''''_{k+4} up to a constant (so vanish with 3 derivatives)."""
+    N = 16
+    P = S._poly_cls(family)
+    for p in S.stencil_polys(family, S.biharmonic_stencil(family, "SB", N)):
+        for xe in (-1.0, 1.0):
+            assert abs(p(xe)) < 1e-12 and abs(p.deriv(1)(xe)) < 1e-11
+    x = np.linspace(-0.9, 0.9, 7)
+    for k, p in enumerate(S.stencil_polys(family, S.biharmonic_stencil(family, "P4", N + 4))):
+        ratio = p(x) / ((1 - x * x) ** 4 * P.basis(k + 4).deriv(4)(x))
+        assert np.allclose(ratio, ratio[0], rtol=1e-9)
+
+
+def test_biharmonic_matrices_host_side_match_quadrature():
+    """The exact-algebra band matrices of the biharmonic PG system against a
+    quadrature on N + 12 points (small N, where the quadrature of 4th
+    derivatives is accurate)."""
+    import shenfun_amd as sfa
+    for fam, family in (("C", "chebyshev"), ("L", "legendre")):
+        N = 16
+        SB = sfa.FunctionSpace(N, fam, bc=(0, 0, 0, 0))
+        T = sfa.TensorProductSpace(None, (SB, sfa.FunctionSpace(8, "F", dtype="d")))
+        mats = sfa.biharmonic_tpmatrices(T, test=T.get_testspace("PG"))
+        Q = N + 12
+        x, w = S.cheb_points_weights(Q) if family == "chebyshev" else S.leg_points_weights(Q)
+        phi = S.stencil_polys(family, S.biharmonic_stencil(family, "P4", N + 4))
+        psi = S.stencil_polys(family, S.biharmonic_stencil(family, "SB", N))
+        for m, d in zip(mats, (4, 2, 0)):
+            D = np.array([[np.sum(w * phi[k](x) * (psi[j].deriv(d)(x) if d else psi[j](x)))
+                           for j in range(N - 4)] for k in range(N - 4)])
+            assert np.allclose(m.pmat.todense(), D, rtol=1e-10, atol=1e-10 * np.abs(D).max())
