@@ -13,39 +13,68 @@ namespace sfa {
 constexpr int TL = 16;  // lines per block
 
 // ----------------------------------------------------------------- C2C strided
-template <int N, bool INV, typename C2 = double2>
-__global__ void __launch_bounds__(TL * Pow2<N>::Q)
-    c2c_strided_kernel(const C2* in, C2* out, int64_t in_sa,
-                       int64_t in_s1, int64_t in_s2, int64_t out_sa, int64_t out_s1,
-                       int64_t out_s2, int64_t d2, int64_t nlines, double scale,
-                       const double2* __restrict__ twg)
+struct StridedLines {
+    const void* in;
+    void* out;
+    int64_t in_sa, in_s1, in_s2, out_sa, out_s1, out_s2, d2, nlines;
+    double scale;
+};
+
+// one block: TL lines starting at line blk * TL
+template <int N, bool INV, typename C2>
+__device__ __forceinline__ void c2c_strided_body(const StridedLines& g, const double2* __restrict__ twg,
+                                                 int64_t blk)
 {
     using Pl = Pow2<N>;
     constexpr int P = Pl::P, Q = Pl::Q, LS = Pl::LS;
     extern __shared__ __align__(16) unsigned char sm_raw[];
     C2* tw = reinterpret_cast<C2*>(sm_raw);
     C2* ex = tw + N;
+    const C2* in = reinterpret_cast<const C2*>(g.in);
+    C2* out = reinterpret_cast<C2*>(g.out);
     const int tid = threadIdx.x, nth = blockDim.x;
     const int t = tid % TL, j = tid / TL;
-    const int64_t L = (int64_t)blockIdx.x * TL + t;
-    const bool valid = L < nlines;
-    const int64_t i1 = valid ? L / d2 : 0, i2 = valid ? L - i1 * d2 : 0;
+    const int64_t L = blk * TL + t;
+    const bool valid = L < g.nlines;
+    const int64_t i1 = valid ? L / g.d2 : 0, i2 = valid ? L - i1 * g.d2 : 0;
     for (int i = tid; i < N; i += nth) tw[i] = from_c64<C2>(twg[i]);
     C2 a[P];
-    const C2* src = in + i1 * in_s1 + i2 * in_s2;
+    const C2* src = in + i1 * g.in_s1 + i2 * g.in_s2;
 #pragma unroll
-    for (int r = 0; r < P; ++r) a[r] = valid ? src[(int64_t)(j + Q * r) * in_sa] : cmk<C2>(0.0, 0.0);
+    for (int r = 0; r < P; ++r) a[r] = valid ? src[(int64_t)(j + Q * r) * g.in_sa] : cmk<C2>(0.0, 0.0);
     __syncthreads();
     pow2_fft<N, INV>(a, j, ex + t * LS, tw);
     if (valid) {
-        C2* dst = out + i1 * out_s1 + i2 * out_s2;
-        const typename CT<C2>::R sc = (typename CT<C2>::R)scale;
+        C2* dst = out + i1 * g.out_s1 + i2 * g.out_s2;
+        const typename CT<C2>::R sc = (typename CT<C2>::R)g.scale;
 #pragma unroll
         for (int h = 0; h < P / Q; ++h)
 #pragma unroll
             for (int k2 = 0; k2 < Q; ++k2)
-                dst[(int64_t)pow2_out_index<N>(j, h, k2) * out_sa] = cscale(a[h * Q + k2], sc);
+                dst[(int64_t)pow2_out_index<N>(j, h, k2) * g.out_sa] = cscale(a[h * Q + k2], sc);
     }
+}
+
+template <int N, bool INV, typename C2 = double2>
+__global__ void __launch_bounds__(TL * Pow2<N>::Q)
+    c2c_strided_kernel(StridedLines g, const double2* __restrict__ twg)
+{
+    c2c_strided_body<N, INV, C2>(g, twg, blockIdx.x);
+}
+
+// the mixed form's two sub-arrays in one grid: the complex128 planes' blocks
+// first, then the complex64 planes' blocks (two launches left the GPU
+// under-filled during the few complex128 blocks)
+template <int N, bool INV>
+__global__ void __launch_bounds__(TL * Pow2<N>::Q)
+    c2c_strided_dual_kernel(StridedLines g64, StridedLines g32, int64_t nblk64,
+                            const double2* __restrict__ twg)
+{
+    const int64_t b = blockIdx.x;
+    if (b < nblk64)
+        c2c_strided_body<N, INV, double2>(g64, twg, b);
+    else
+        c2c_strided_body<N, INV, float2>(g32, twg, b - nblk64);
 }
 
 // ----------------------------------------------------------------- R2C contiguous
@@ -180,6 +209,24 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 }
 
 // ----------------------------------------------------------------- host dispatch
+static StridedLines strided_lines(const Geom& gi, const Geom& go, const void* in, void* out,
+                                  double scale)
+{
+    StridedLines g;
+    g.in = in;
+    g.out = out;
+    g.in_sa = gi.sa;
+    g.in_s1 = gi.s1;
+    g.in_s2 = gi.s2;
+    g.out_sa = go.sa;
+    g.out_s1 = go.s1;
+    g.out_s2 = go.s2;
+    g.d2 = gi.d2;
+    g.nlines = gi.d1 * gi.d2;
+    g.scale = scale;
+    return g;
+}
+
 template <int N, bool INV, typename C2 = double2>
 static int launch_c2c_strided(const Geom& gi, const Geom& go, const void* in, void* out,
                               double scale, hipStream_t s)
@@ -193,10 +240,51 @@ static int launch_c2c_strided(const Geom& gi, const Geom& go, const void* in, vo
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k, dim3((unsigned)((nl + TL - 1) / TL)), dim3(TL * Pl::Q), lds, s,
-                       (const C2*)in, (C2*)out, gi.sa, gi.s1, gi.s2, go.sa, go.s1, go.s2,
-                       gi.d2, nl, scale, tw);
+                       strided_lines(gi, go, in, out, scale), tw);
     SFA_LAUNCH_CHECK();
     return SFA_OK;
+}
+
+template <int N, bool INV>
+static int launch_c2c_strided_dual(const Geom& g64, void* W64, const Geom& g32, void* W32,
+                                   double scale, hipStream_t s)
+{
+    using Pl = Pow2<N>;
+    const double2* tw = fft_table(N);
+    SFA_CHECK(tw, SFA_ERR_HIP, "twiddle table allocation failed");
+    const int64_t nb64 = (g64.d1 * g64.d2 + TL - 1) / TL, nb32 = (g32.d1 * g32.d2 + TL - 1) / TL;
+    SFA_CHECK(nb64 + nb32 < (1ll << 31), SFA_ERR_UNSUPPORTED, "too many lines");
+    const size_t lds = sizeof(double2) * ((size_t)N + (size_t)TL * Pl::LS);  // the larger body
+    auto k = c2c_strided_dual_kernel<N, INV>;
+    if (lds > 64 * 1024)
+        hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, dim3((unsigned)(nb64 + nb32)), dim3(TL * Pl::Q), lds, s,
+                       strided_lines(g64, g64, W64, W64, scale), strided_lines(g32, g32, W32, W32, scale),
+                       nb64, tw);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+// in-place y C2C of the mixed workspace's complex128 and complex64 planes in
+// one launch; returns 1 if handled (power-of-two Ny up to 256).  At 256^3 it
+// takes 38.5 -> 34.2 us per pass: the 272 complex128 blocks alone leave a
+// tail on 256 CUs.  Not at Ny = 512, where the complex128 body's 256 VGPRs
+// would hold the complex64 blocks to one wave per SIMD (217 -> 261 us).
+int fast_c2c_mixed(const Geom& g64, void* W64, const Geom& g32, void* W32, int sign, double scale,
+                   hipStream_t s, int* rc)
+{
+    if (!(g64.s2 == 1 && g32.s2 == 1 && g64.sa != 1 && g32.sa != 1 && g64.n == g32.n)) return 0;
+    const bool inv = sign > 0;
+#define SFA_C2CM(NN)                                                                      \
+    case NN:                                                                              \
+        *rc = inv ? launch_c2c_strided_dual<NN, true>(g64, W64, g32, W32, scale, s)       \
+                  : launch_c2c_strided_dual<NN, false>(g64, W64, g32, W32, scale, s);     \
+        return 1;
+    switch (g64.n) {
+        SFA_C2CM(16) SFA_C2CM(32) SFA_C2CM(64) SFA_C2CM(128) SFA_C2CM(256)
+        default: return 0;
+    }
+#undef SFA_C2CM
 }
 
 // returns 1 if handled (rc in *rc), 0 if the shape needs the generic kernel

@@ -240,6 +240,8 @@ int fast_c2c(const Geom& gi, const Geom& go, const void* in, void* out, int sign
              hipStream_t s, int* rc);
 int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
                  hipStream_t s, int* rc);
+int fast_c2c_mixed(const Geom& g64, void* W64, const Geom& g32, void* W32, int sign, double scale,
+                   hipStream_t s, int* rc);
 int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
              double scale, hipStream_t s, int* rc);
 
@@ -384,12 +386,14 @@ int xpass_launch(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int 
     return SFA_OK;
 }
 
-// y C2C of both sub-arrays of the mixed workspace (complex128 planes, then
-// complex64 planes; running them on two streams measured slower).
+// y C2C of both sub-arrays of the mixed workspace: one launch whose first
+// blocks take the complex128 planes (running the two on two streams measured
+// slower); two launches for the lengths the dual kernel does not cover.
 static int y_pass_mixed(const Geom& g64, const Geom& g32, double2* W64, float2* W32, int sign,
                         double scale, hipStream_t s)
 {
     int rc = 0;
+    if (fast_c2c_mixed(g64, W64, g32, W32, sign, scale, s, &rc)) return rc;
     if (!fast_c2c(g64, g64, W64, W64, sign, scale, s, &rc) || rc ||
         !fast_c2c_f32(g32, g32, W32, W32, sign, scale, s, &rc) || rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Ny=%lld unsupported", (long long)g64.n);
