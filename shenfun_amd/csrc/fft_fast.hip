@@ -448,7 +448,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     const int64_t Lb = L / a.nlb;
     const int64_t Lo = Lb * a.bstride + (L - Lb * a.nlb);  // this line's element x = 0
     for (int i = tid; i < N; i += NTH) {
-        tw[i] = from_c64<C2>(a.tw[i]);
+        tw[i] = from_c64<C2>(a.tw[(i % Q) * (i / Q)]);  // k1-major: tw[k1 Q + j] = w^(j k1)
         tqs[i] = from_c64<C2>(a.tq[i]);
         k2s[pd(i)] = (i < N - 2) ? a.K2[i] : 0.0;
     }
@@ -479,7 +479,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         ug[q] = uq;
     }
     __syncthreads();  // tables
-    pow2_fft_p1<N, false>(v, j, ex + t * LA, tw);
+    pow2_fft_p1t<N, false>(v, j, ex + t * LA, tw);
     __syncthreads();  // exchange, and the switch to mapping B
     C2* ln = ex + tb * LA;
     pow2_fft_p2<N, false>(v, jb, ln);
@@ -575,7 +575,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         v[r] = cmul(cconj(tqs[k]), z);
     }
     wave_sync();
-    pow2_fft_p1<N, true>(v, jb, ln, tw);
+    pow2_fft_p1t<N, true>(v, jb, ln, tw);
     __syncthreads();  // exchange, and the switch back to mapping A
     pow2_fft_p2<N, true>(v, j, ex + t * LA);
     const C2 t0a = t0s[t];

@@ -95,6 +95,25 @@ __device__ inline void pow2_fft_p1(C2 (&a)[Pow2<N>::P], int j, C2* ex, const C2*
     for (int k1 = 0; k1 < P; ++k1) ex[k1 * (Q + 1) + j] = a[k1];
 }
 
+// First half with the twiddle table laid out k1-major, twT[k1 Q + j] =
+// exp(-2 pi i j k1 / N) (same N entries): when the Q threads of a line are
+// adjacent lanes they read consecutive entries (tw[j k1] strides by k1, a
+// bank conflict for every even k1); lanes that share j still read one entry.
+template <int N, bool INV, typename C2>
+__device__ inline void pow2_fft_p1t(C2 (&a)[Pow2<N>::P], int j, C2* ex, const C2* twT)
+{
+    constexpr int P = Pow2<N>::P, Q = Pow2<N>::Q;
+    dft<P, INV>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < P; ++k1) {
+        C2 w = twT[k1 * Q + j];
+        if (INV) w.y = -w.y;
+        a[k1] = cmul(a[k1], w);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) ex[k1 * (Q + 1) + j] = a[k1];
+}
+
 // Second half: after the exchange is visible, thread j reads the DFT-Q inputs
 // of k1 = h Q + j from ex and transforms them.  The thread j of this half may
 // be a different thread of the block than the one of the first half (the
