@@ -348,7 +348,8 @@ class _PolyBase(SpectralBase):
     # ------------------------------------------------------ boundary values
     @property
     def has_nonhomogeneous_bcs(self):
-        return self.is_composite and any(v != 0 for v in self.bc_values)
+        return self.is_composite and any(not (isinstance(v, float) and v == 0)
+                                         for v in self.bc_values)
 
     def bc_basis(self):
         """Orthogonal coefficients (rows: left, right function) of the boundary
@@ -420,6 +421,9 @@ class _PolyBase(SpectralBase):
         if not self.is_composite:
             return u
         for j, v in enumerate(self.bc_values):
+            if not isinstance(v, float):
+                u.narrow(self.axis, self.N - 2 + j, 1).copy_(self._bc_slot_array(j, u, final))
+                continue
             sl = [slice(None)] * u.dim()
             sl[self.axis] = self.N - 2 + j
             u[tuple(sl)] = 0
@@ -430,6 +434,51 @@ class _PolyBase(SpectralBase):
                     sl[i] = 0
             u[tuple(sl)] = v
         return u
+
+    def _bc_slot_array(self, j, u, final):
+        """Boundary slot j for a boundary value that varies along the other
+        axes: the function on their mesh, forward-transformed along the axes
+        already transformed (BoundaryValues.set_tensor_bcs,
+        tensorproductspace.py:1909-2068, projects it onto the other bases the
+        same way); shape of u with 1 along this axis."""
+        T = self.tensorproductspace
+        if T is None:
+            raise ValueError("a boundary value that varies needs the space in a TensorProductSpace")
+        key = ("bcslot", j, bool(final), tuple(u.shape), u.dtype)
+        if key in self._cache:
+            return self._cache[key]
+        ax = self.axis
+        nd = T.dimensions
+        X = [np.asarray(b.mesh(), dtype=float) for b in T.bases]
+        coords = [X[i].reshape([X[i].size if k == i else 1 for k in range(nd)]) for i in range(nd)]
+        others = [i for i in range(nd) if i != ax]
+        v = self.bc_values[j]
+        if hasattr(v, "free_symbols"):
+            import sympy as sp
+            names = "xyz"[:nd]  # the coordinates of axes 0, 1, 2
+            byname = {str(q): q for q in v.free_symbols}
+            if any(n not in names or names.index(n) == ax for n in byname):
+                raise ValueError(f"boundary value {v} must be a function of the other coordinates")
+            fn = sp.lambdify([byname.get(names[i], sp.Symbol(names[i])) for i in others], v, "numpy")
+        else:
+            fn = v
+        shape = [1 if i == ax else X[i].size for i in range(nd)]
+        arr = np.array(np.broadcast_to(np.asarray(fn(*[coords[i] for i in others]), dtype=float),
+                                       shape))
+        t = torch.from_numpy(arr).to(u.device)
+        for i in range(nd - 1, -1, -1):
+            if i == ax or not (final or i > ax):
+                continue
+            b = T.bases[i]
+            if not isinstance(b, FourierBase):
+                raise NotImplementedError("boundary values varying along a non-periodic axis")
+            t = t if isinstance(b, R2C) else t.to(torch.complex128)
+            shp = list(t.shape)
+            shp[i] = b.shape(True)
+            t = b.forward(t, torch.empty(shp, dtype=torch.complex128, device=u.device))
+        t = t.to(u.dtype) if u.dtype.is_complex or not t.dtype.is_complex else t.real.contiguous()
+        self._cache[key] = t
+        return t
 
     def scalar_product(self, input_array=None, output_array=None, kind=None):
         kind = self._kind(kind)
@@ -875,7 +924,7 @@ class _Init:
         self.quad = quad
         # boundary values (left, right) of a composite space; a string or
         # dict bc is parsed by FunctionSpace
-        self.bc_values = tuple(float(v) for v in bc) if isinstance(bc, (tuple, list)) \
+        self.bc_values = tuple(_bcval(v) for v in bc) if isinstance(bc, (tuple, list)) \
             else (0.0, 0.0)
         SpectralBase.__init__(self, N, dtype, device, padding_factor, dealias_direct)
         if self.is_padded and quad == "GL":
@@ -1061,6 +1110,19 @@ def _is_biharmonic(bc):
     return False
 
 
+def _bcval(v):
+    """A boundary value: a number (float), or a function of the other
+    coordinates (a Python callable taking them in axis order, or a sympy
+    expression in x, y, z) kept as given."""
+    if callable(v) and not hasattr(v, "free_symbols"):
+        return v
+    if hasattr(v, "free_symbols"):
+        if not v.free_symbols:
+            return float(v)
+        return v
+    return float(v)
+
+
 def _parse_bc(bc):
     """Boundary conditions -> (kind, (left value, right value)).
 
@@ -1080,7 +1142,7 @@ def _parse_bc(bc):
         raise ValueError(f"unsupported bc {bc!r}")
     if isinstance(bc, (tuple, list)):
         if len(bc) == 2:
-            return "D", (float(bc[0]), float(bc[1]))
+            return "D", (_bcval(bc[0]), _bcval(bc[1]))
         raise ValueError(f"unsupported bc {bc!r}")
     if isinstance(bc, dict):
         if set(bc) != {"left", "right"} or any(len(bc[s]) != 1 for s in bc):
@@ -1089,7 +1151,7 @@ def _parse_bc(bc):
         (kr, vr), = bc["right"].items()
         if kl != kr or kl not in ("D", "N"):
             raise NotImplementedError("Dirichlet or Neumann at both ends")
-        return kl, (float(vl), float(vr))
+        return kl, (_bcval(vl), _bcval(vr))
     raise ValueError(f"unsupported bc {bc!r}")
 
 
