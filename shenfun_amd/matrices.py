@@ -306,14 +306,26 @@ class BCMatrix:
 
     is_bc_matrix = True
 
-    def __init__(self, space, which):
+    def __init__(self, space, which, test=None):
         self.space = space
         self.which = which
-        A, B = space.bc_matrices()
-        self.mat = A if which == "A" else B
+        if test is None:
+            A, B = space.bc_matrices()
+            self.mat = A if which == "A" else B
+        else:
+            # Petrov-Galerkin: (test_k, phi_bc_j'') or (test_k, phi_bc_j) by exact
+            # algebra in the orthogonal basis, as GalerkinMatrix
+            P = np.polynomial.chebyshev if space.family() == "chebyshev" else np.polynomial.legendre
+            der = P.chebder if space.family() == "chebyshev" else P.legder
+            n, L = test.dim(), test.N
+            Dc = np.zeros((2, L))
+            for j, c in enumerate(space.bc_basis()):
+                c = der(c, 2) if which == "A" else c
+                Dc[j, :len(c)] = c
+            self.mat = (test.stencil_matrix()[:n] * space.ortho_norms(L)[None, :]) @ Dc.T
         self.shape = self.mat.shape
         self.scale = 1.0
-        self.testfunction = (space, 0)
+        self.testfunction = (space if test is None else test, 0)
 
     def get_key(self):
         return f"BC{self.which}mat"
@@ -433,12 +445,14 @@ def helmholtz_tpmatrices(T, alpha=0.0, test=None):
     ax = T.nonperiodic_axis()
     S = T.bases[ax]
     if test is not None and test is not T and getattr(test.bases[ax], "kind", None) == "P2":
-        if getattr(S, "has_nonhomogeneous_bcs", False):
-            raise NotImplementedError("Petrov-Galerkin with nonhomogeneous boundary values")
         P2 = test.bases[ax]
         k2 = _k2(T, ax)
-        return [TPMatrix(GalerkinMatrix(P2, S, 2, "PGAmat"), T, np.full(k2.shape, -1.0), ax),
+        mats = [TPMatrix(GalerkinMatrix(P2, S, 2, "PGAmat"), T, np.full(k2.shape, -1.0), ax),
                 TPMatrix(GalerkinMatrix(P2, S, 0, "PGBmat"), T, alpha + k2, ax)]
+        if getattr(S, "has_nonhomogeneous_bcs", False):
+            mats += [TPMatrix(BCMatrix(S, "A", test=P2), T, np.full(k2.shape, -1.0), ax),
+                     TPMatrix(BCMatrix(S, "B", test=P2), T, alpha + k2, ax)]
+        return mats
     shape = list(T.shape(True))
     shape[ax] = 1
     k2 = np.zeros(shape)

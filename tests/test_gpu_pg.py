@@ -148,3 +148,41 @@ def test_biharmonic_chebyshev_galerkin_is_refused(sfa):
     T = sfa.TensorProductSpace(None, (SB, sfa.FunctionSpace(8, "F", dtype="d")))
     with pytest.raises(NotImplementedError):
         sfa.biharmonic_tpmatrices(T)
+
+
+@pytest.mark.parametrize("family,fam", [("chebyshev", "C"), ("legendre", "L")])
+@pytest.mark.parametrize("kind", ["D", "N"])
+def test_pg_with_boundary_values(sfa, family, fam, kind):
+    """demo/poisson2D.py's setting: Petrov-Galerkin with nonhomogeneous
+    Dirichlet or Neumann values (constant and varying along y)."""
+    print(f"[pg bc {family} {kind}]", flush=True)
+    import sympy as sp
+    y = sp.symbols("y", real=True)
+    alpha = 1.5
+    aL = 0.4 + 0.3 * sp.cos(2 * y)
+    aR = -0.6
+    bc = (aL, aR) if kind == "D" else {"left": {"N": aL}, "right": {"N": aR}}
+    SD = sfa.FunctionSpace(24, fam, bc=bc)
+    T = sfa.TensorProductSpace(None, (SD, sfa.FunctionSpace(16, "F", dtype="d")))
+    B = T.get_testspace("PG")
+    Pn = np.polynomial.Polynomial
+    # u = aL(y) phiL(x) + aR phiR(x) + q(x) cos(3y), phi the boundary functions
+    C = S.bc_basis(family, kind)
+    Pc = S._poly_cls(family)
+    phiL, phiR = Pc(C[0]).convert(kind=Pn), Pc(C[1]).convert(kind=Pn)
+    q = Pn([1, 0, -1]) if kind == "D" else Pn([1, 0, -2, 0, 1])
+    aLf = sp.lambdify(y, aL, "numpy")
+    aLyy = sp.lambdify(y, sp.diff(aL, y, 2), "numpy")
+    def fields(X):
+        x, Y = X
+        u = aLf(Y) * phiL(x) + aR * phiR(x) + q(x) * np.cos(3 * Y)
+        lap = (aLyy(Y) * phiL(x) + aLf(Y) * phiL.deriv(2)(x) + aR * phiR.deriv(2)(x)
+               + q.deriv(2)(x) * np.cos(3 * Y) - 9 * q(x) * np.cos(3 * Y))
+        return u, alpha * u - lap
+    _, f = fields(B.mesh())
+    u_exact, _ = fields(T.mesh())
+    f_hat = B.scalar_product(dev(np.broadcast_to(f, B.shape(False))))
+    mats = sfa.helmholtz_tpmatrices(T, alpha, test=B)
+    assert len(mats) == 4
+    u_hat = sfa.la.SolverGeneric1ND(mats)(f_hat, sfa.Function(T))
+    assert rel(T.backward(u_hat), np.broadcast_to(u_exact, T.shape(False))) < 1e-11
