@@ -430,10 +430,15 @@ class _PolyBase(SpectralBase):
             if v == 0:
                 continue
             for i in range(u.dim()):
-                if i != self.axis and (final or i > self.axis):
+                if i != self.axis and (final or self._transformed_before(i)):
                     sl[i] = 0
             u[tuple(sl)] = v
         return u
+
+    def _transformed_before(self, i):
+        """Axis i is already in spectral space when forward reaches this axis."""
+        T = self.tensorproductspace
+        return T.transformed_before(i, self.axis) if T is not None else i > self.axis
 
     def _bc_slot_array(self, j, u, final):
         """Boundary slot j for a boundary value that varies along the other
@@ -466,8 +471,8 @@ class _PolyBase(SpectralBase):
         arr = np.array(np.broadcast_to(np.asarray(fn(*[coords[i] for i in others]), dtype=float),
                                        shape))
         t = torch.from_numpy(arr).to(u.device)
-        for i in range(nd - 1, -1, -1):
-            if i == ax or not (final or i > ax):
+        for i in T.axes[::-1]:  # forward order: an R2C axis first
+            if i == ax or not (final or self._transformed_before(i)):
                 continue
             b = T.bases[i]
             if not isinstance(b, FourierBase):

@@ -26,6 +26,8 @@ class HelmholtzSolver3D:
         if not (len(b) == 3 and isinstance(b[0], ChebyshevBase) and b[0].bc_kind in ("D", "N")
                 and isinstance(b[1], C2C) and isinstance(b[2], R2C)):
             raise ValueError("expects Chebyshev(SD/SN) x C2C x R2C")
+        if tuple(getattr(T, "axes", (0, 1, 2))) != (0, 1, 2):
+            raise NotImplementedError("the fused solver takes the natural transform order axes=(0, 1, 2)")
         if b[0].quad != "GC":
             raise NotImplementedError("the fused solver is built for Chebyshev-Gauss (GC) points; "
                                       "use T.scalar_product -> la.Helmholtz -> T.backward for GL")

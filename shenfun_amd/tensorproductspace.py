@@ -21,12 +21,15 @@ class TensorProductSpace(_Planned):
         self.dimensions = len(self.bases)
         if not 1 <= self.dimensions <= 3:
             raise ValueError("1..3 dimensional spaces are supported")
-        if axes is not None and tuple(axes) != tuple(range(self.dimensions)) and \
-                tuple(axes) != tuple(range(self.dimensions))[::-1]:
-            raise NotImplementedError("custom axis orders are not supported")
+        # transform order (tensorproductspace.py:314-469): backward runs
+        # axes[0], axes[1], ..., forward and scalar_product the reverse, so an
+        # R2C basis (real physical data) must be axes[-1], transformed first
+        self.axes = tuple(int(a) for a in axes) if axes is not None else tuple(range(self.dimensions))
+        if sorted(self.axes) != list(range(self.dimensions)):
+            raise ValueError(f"axes {axes} is not a permutation of the dimensions")
         for i, b in enumerate(self.bases):
-            if isinstance(b, R2C) and i != self.dimensions - 1:
-                raise ValueError("an R2C space must be the last axis")
+            if isinstance(b, R2C) and i != self.axes[-1]:
+                raise ValueError("an R2C space must be the first axis transformed forward, axes[-1]")
             b.axis = i
             b.dimensions = self.dimensions
             b.tensorproductspace = self
@@ -54,7 +57,7 @@ class TensorProductSpace(_Planned):
         return tuple(slice(0, s) for s in self.shape(forward_output))
 
     def physical_dtype(self):
-        return torch.float64 if isinstance(self.bases[-1], R2C) or not any(
+        return torch.float64 if any(isinstance(b, R2C) for b in self.bases) or not any(
             isinstance(b, FourierBase) for b in self.bases) else torch.complex128
 
     def spectral_dtype(self):
@@ -83,7 +86,7 @@ class TensorProductSpace(_Planned):
         if key not in self._padded:
             bases = [b.get_testspace(kind) if hasattr(b, "get_testspace") else b._clone(b.N)
                      for b in self.bases]
-            self._padded[key] = TensorProductSpace(None, bases, device=self.device)
+            self._padded[key] = TensorProductSpace(None, bases, axes=self.axes, device=self.device)
         return self._padded[key]
 
     def get_dealiased(self, padding_factor=1.5, dealias_direct=False):
@@ -102,7 +105,7 @@ class TensorProductSpace(_Planned):
             bases = [b.get_dealiased(padding_factor=pf, dealias_direct=dealias_direct)
                      for b, pf in zip(self.bases, padding_factor)]
             bases = [b._clone(b.N) if b is o else b for b, o in zip(bases, self.bases)]
-            self._padded[key] = TensorProductSpace(None, bases, device=self.device)
+            self._padded[key] = TensorProductSpace(None, bases, axes=self.axes, device=self.device)
         return self._padded[key]
 
     # ------------------------------------------------------------ helpers
@@ -112,12 +115,9 @@ class TensorProductSpace(_Planned):
             self._work[k] = torch.empty(shape, dtype=dtype, device=self.device)
         return self._work[k]
 
-    def _shape_after_forward(self, axis):
-        """Array shape after transforming axes >= axis."""
-        s = list(self.shape(False))
-        for i in range(axis, self.dimensions):
-            s[i] = self.shape(True)[i]
-        return s
+    def transformed_before(self, i, ax):
+        """True if forward transforms axis i before axis ax."""
+        return self.axes.index(i) > self.axes.index(ax)
 
     def _chain(self, inp, output_array, method, order, kind):
         cur = inp
@@ -161,17 +161,15 @@ class TensorProductSpace(_Planned):
     # ---------------------------------------------------------- transforms
     def scalar_product(self, input_array=None, output_array=None, kind=None, **kw):
         """(tensorproductspace.py:146-190): last axis first."""
-        return self._chain(input_array, output_array, "scalar_product",
-                           list(range(self.dimensions - 1, -1, -1)), kind)
+        return self._chain(input_array, output_array, "scalar_product", list(self.axes[::-1]), kind)
 
     def forward(self, input_array=None, output_array=None, kind=None, **kw):
         """(tensorproductspace.py:205-259): last axis first."""
-        return self._chain(input_array, output_array, "forward",
-                           list(range(self.dimensions - 1, -1, -1)), kind)
+        return self._chain(input_array, output_array, "forward", list(self.axes[::-1]), kind)
 
     def backward(self, input_array=None, output_array=None, kind=None, **kw):
         """(tensorproductspace.py:46-96): first axis first."""
-        return self._chain(input_array, output_array, "backward", list(range(self.dimensions)), kind)
+        return self._chain(input_array, output_array, "backward", list(self.axes), kind)
 
     def convolve(self, a_hat, b_hat, ab_hat=None):
         """Convolution of two spectral arrays (tensorproductspace.py:637-666):

@@ -129,3 +129,73 @@ def test_helmholtz3d_mixed(sfa, bc, shape, planes):
     # all-FP64 solve of the same input agrees with the oracle to round-off
     u64 = sfa.HelmholtzSolver3D(T, 1.0)(dev(f))
     assert rel(u64, u_ref) < 1e-12
+
+
+ALLAXES3D = {0: (0, 1, 2), 1: (1, 0, 2), 2: (2, 0, 1)}
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_helmholtz3d_any_axis(sfa, axis):
+    """tests/test_matrices.py:559-593 of the reference: the Chebyshev axis at
+    position 0, 1 or 2 (transform order ``axes``, the R2C axis transformed
+    first); H(H.matvec(uh)) == uh, and backward -> forward round trips
+    against the oracle chain."""
+    print(f"[helmholtz3d axis={axis}]", flush=True)
+    a = ALLAXES3D[axis]
+    N = (8, 9, 10)
+    bases = [None] * 3
+    bases[a[0]] = sfa.FunctionSpace(N[a[0]], "C", bc=(0, 0))
+    bases[a[1]] = sfa.FunctionSpace(N[a[1]], "F", dtype="D")
+    bases[a[2]] = sfa.FunctionSpace(N[a[2]], "F", dtype="d")
+    T = sfa.TensorProductSpace(None, bases, axes=a)
+    H = sfa.la.Helmholtz(*sfa.helmholtz_tpmatrices(T, 0.0))
+    rng = np.random.default_rng(axis)
+    shp = T.shape(True)
+    # coefficients of real data (Hermitian across the Fourier axes): the
+    # forward transform of a random real field
+    uh = T.forward(dev(rng.standard_normal(T.shape(False)))).cpu().numpy()
+    f = H.matvec(dev(uh), torch.empty(shp, dtype=torch.complex128, device="cuda"))
+    uc = H(f, torch.zeros(shp, dtype=torch.complex128, device="cuda"))
+    assert rel(uc, uh) < 1e-12
+    kinds = [None] * 3
+    kinds[a[0]] = ("chebyshev", "D")
+    kinds[a[1]] = ("fourier-c2c", None)
+    kinds[a[2]] = ("fourier-r2c", None)
+    u = T.backward(dev(uh))
+    ref = uh
+    for ax in a:  # backward order
+        ref = S.basis_backward(kinds[ax][0], kinds[ax][1], ref, ax, N=N[ax])
+    assert u.dtype == torch.float64 and rel(u, ref) < 1e-12
+    assert rel(T.forward(u), uh) < 1e-11
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("fam", ["C", "L"])
+def test_helmholtz2d_any_axis(sfa, axis, fam):
+    """tests/test_matrices.py:594-621: 2D with the non-periodic axis first or
+    second (R2C transformed first); f = (sA A + sB B) uh per mode from the
+    dense matrices on the host, then the solve (la.Helmholtz for Chebyshev,
+    SolverGeneric1ND for Legendre) returns uh."""
+    print(f"[helmholtz2d {fam} axis={axis}]", flush=True)
+    N = (8, 10)
+    a = (0, 1) if axis == 0 else (1, 0)
+    bases = [None] * 2
+    bases[axis] = sfa.FunctionSpace(N[axis], fam, bc=(0, 0))
+    bases[1 - axis] = sfa.FunctionSpace(N[1 - axis], "F", dtype="d")
+    T = sfa.TensorProductSpace(None, bases, axes=a)
+    mats = sfa.helmholtz_tpmatrices(T, 0.5)
+    rng = np.random.default_rng(7)
+    shp = T.shape(True)
+    n = N[axis] - 2
+    uh = np.zeros(shp, dtype=complex)
+    um = np.moveaxis(uh, axis, 0)
+    um[:n] = rng.random((n, shp[1 - axis])) + 1j * rng.random((n, shp[1 - axis]))
+    fm = np.zeros_like(um)
+    for m in range(shp[1 - axis]):
+        M = sum(np.moveaxis(t.scale, axis, 0).reshape(-1)[m] * float(t.pmat.scale) * t.pmat.todense()
+                for t in mats)
+        fm[:n, m] = M @ um[:n, m]
+    f = np.moveaxis(fm, 0, axis)
+    sol = sfa.la.Helmholtz(*mats) if fam == "C" else sfa.la.SolverGeneric1ND(mats)
+    uc = sol(dev(f), torch.zeros(shp, dtype=torch.complex128, device="cuda"))
+    assert rel(uc, uh) < 1e-12
