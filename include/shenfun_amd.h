@@ -228,6 +228,18 @@ int sfa_heptadma_solve(void* u, int dtype, int ndim, const int64_t* shape,
                        const int64_t* strides, int axis, const double* data, int64_t n,
                        int64_t data_line_stride, int64_t skip_lines, void* stream);
 
+/* Banded matrix-vector product along `axis`: c_i (+)= scales[line] *
+ * sum_r D[r][i] v_{i + offsets[r]} for i < nrows and 0 <= i + offsets[r] < ncols;
+ * D device (noffs, nrows), the diagonals by row (D[r][i] = A[i][i + offsets[r]]),
+ * offsets device (noffs int64), scales device per line (C order over the
+ * non-axis dims) or NULL.  accumulate = 0 writes c (rows >= nrows zeroed),
+ * 1 adds to rows < nrows.  Replaces SparseMatrix.matvec (matrixbase.py:87-144)
+ * and TPMatrix.matvec (:1437-1449, one non-periodic axis).  v != c. */
+int sfa_band_matvec(const void* v, void* c, int dtype, int ndim, const int64_t* v_shape,
+                    const int64_t* v_strides, const int64_t* c_shape, const int64_t* c_strides,
+                    int axis, int64_t nrows, int64_t ncols, int noffs, const int64_t* offsets,
+                    const double* D, const double* scales, int accumulate, void* stream);
+
 /* Chebyshev Helmholtz LU (alfa*A + beta*B), N basis size (M = N-2 unknowns).
  * A0 (M), A2 (M-2), A4 (M-4): stiffness diagonals 0, +2, +4; Bm2 (M-2), B0 (M),
  * B2 (M-2): mass diagonals (unscaled; the Neumann 1/j^2 column scaling and

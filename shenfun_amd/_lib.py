@@ -40,6 +40,7 @@ _SIGS = {
     "sfa_boundary_update": "vviiaaaailllvvlv",
     "sfa_pointwise_mul": "vvvilv",
     "sfa_convolve_direct": "vvvliv",
+    "sfa_band_matvec": "vviiaaaaillivvviv",
     "sfa_tdma_o_lu": "vllv",
     "sfa_fdma_lu": "vllv",
     "sfa_heptadma_lu": "vllv",

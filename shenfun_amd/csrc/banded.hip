@@ -888,3 +888,72 @@ extern "C" int sfa_poisson_add_solve(const void* b, void* u, int dtype, int ndim
     SFA_LAUNCH_CHECK();
     return SFA_OK;
 }
+
+// ----------------------------------------------------------------- banded matvec
+// c(.., i, ..) (+)= s(line) sum_r D[r][i] v(.., i + off_r, ..) for i < nrows,
+// 0 <= i + off_r < ncols (SparseMatrix.matvec, matrixbase.py:87-144, with the
+// per-mode scale of TPMatrix.matvec :1437-1449).  D holds the diagonals by row
+// (D[r][i] = A[i][i + off_r]); without accumulate, rows >= nrows of c are zeroed.
+namespace sfa {
+template <typename T>
+__global__ void band_matvec_kernel(const T* __restrict__ v, T* __restrict__ c, LineMap gv,
+                                   LineMap gc, int nout, int nrows, int ncols, int noffs,
+                                   const int64_t* __restrict__ offs, const double* __restrict__ D,
+                                   const double* __restrict__ scales, int accumulate)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= gc.nlines * nout) return;
+    const int i = (int)(gid % nout);
+    const int64_t L = gid / nout;
+    const T* vl = v + gv.off(L);
+    T* cl = c + gc.off(L) + (int64_t)i * gc.sa;
+    T acc = zero<T>();
+    if (i < nrows) {
+        for (int r = 0; r < noffs; ++r) {
+            const int64_t j = i + offs[r];
+            if (j >= 0 && j < ncols) acc = add(acc, mul(vl[(int64_t)j * gv.sa], D[(int64_t)r * nrows + i]));
+        }
+        if (scales) acc = mul(acc, scales[L]);
+    } else if (accumulate) {
+        return;
+    }
+    *cl = accumulate ? add(*cl, acc) : acc;
+}
+}  // namespace sfa
+
+extern "C" int sfa_band_matvec(const void* v, void* c, int dtype, int ndim, const int64_t* v_shape,
+                               const int64_t* v_strides, const int64_t* c_shape,
+                               const int64_t* c_strides, int axis, int64_t nrows, int64_t ncols,
+                               int noffs, const int64_t* offsets, const double* D,
+                               const double* scales, int accumulate, void* stream)
+{
+    LineMap gv, gc;
+    int64_t nv, nc;
+    int rc = line_map(ndim, v_shape, v_strides, axis, &gv, &nv);
+    if (rc) return rc;
+    rc = line_map(ndim, c_shape, c_strides, axis, &gc, &nc);
+    if (rc) return rc;
+    SFA_CHECK(v != nullptr && c != nullptr && D != nullptr && offsets != nullptr && v != c,
+              SFA_ERR_ARG, "band_matvec: null or aliased arrays");
+    SFA_CHECK(dtype == SFA_F64 || dtype == SFA_C128, SFA_ERR_ARG, "band_matvec: bad dtype %d", dtype);
+    for (int d = 0; d < ndim; ++d)
+        SFA_CHECK(d == axis || v_shape[d] == c_shape[d], SFA_ERR_ARG,
+                  "band_matvec: shapes differ off the axis (dim %d)", d);
+    SFA_CHECK(noffs >= 1, SFA_ERR_ARG, "band_matvec: no diagonals");
+    SFA_CHECK(nrows >= 1 && nrows <= nc && ncols >= 1 && ncols <= nv && nc <= (1 << 28), SFA_ERR_ARG,
+              "band_matvec: matrix %lld x %lld does not fit the arrays (%lld, %lld)", (long long)nrows,
+              (long long)ncols, (long long)nc, (long long)nv);
+    const int64_t total = gc.nlines * nc;
+    if (total == 0) return SFA_OK;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (dtype == SFA_C128)
+        hipLaunchKernelGGL(band_matvec_kernel<double2>, grid, dim3(256), 0, (hipStream_t)stream,
+                           (const double2*)v, (double2*)c, gv, gc, (int)nc, (int)nrows, (int)ncols, noffs,
+                           offsets, D, scales, accumulate);
+    else
+        hipLaunchKernelGGL(band_matvec_kernel<double>, grid, dim3(256), 0, (hipStream_t)stream,
+                           (const double*)v, (double*)c, gv, gc, (int)nc, (int)nrows, (int)ncols, noffs,
+                           offsets, D, scales, accumulate);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}

@@ -291,6 +291,15 @@ class SolverGeneric1ND:
                              for r, off in enumerate(self.offsets)}, (n, n))
         self.sol0 = cls(mat0)
 
+    def matvec(self, u, c):
+        """c = sum of the matrices times u (la.py:1212-1229), plus the boundary
+        matrices times u's boundary slots when there are any."""
+        for j, m in enumerate(self.mats):
+            m.matvec(u, c, accumulate=j > 0)
+        if self.bc_mats:
+            apply_bc_matrices(self.bc_mats, c, u, sign=+1.0)
+        return c
+
     def perform_lu(self, device):
         if self._dev is None or self._dev.device != device:
             d = torch.from_numpy(self._data).to(device)

@@ -15,6 +15,9 @@ from numbers import Number
 
 import numpy as np
 import scipy.sparse
+import torch
+
+from . import _lib
 
 PI = np.pi
 
@@ -65,6 +68,48 @@ class SparseMatrix(dict):
                 v = m.diagonal_array(k) * float(m.scale)
                 out[k] = out[k] + v if k in out else v
         return SparseMatrix(out, self.shape, scale=1.0)
+
+    def band_rows(self):
+        """(offsets, D) with D[r][i] = A[i][i + offsets[r]] (scale applied), the
+        layout of sfa_band_matvec."""
+        N = self.shape[0]
+        offs = sorted(self.keys())
+        D = np.zeros((len(offs), N))
+        for r, o in enumerate(offs):
+            v = self.diagonal_array(o) * float(np.asarray(self.scale).reshape(-1)[0])
+            if o >= 0:
+                D[r, :len(v)] = v
+            else:
+                D[r, -o:-o + len(v)] = v
+        return offs, D
+
+    def _device_rows(self, device):
+        key = ("_band_dev", str(device))
+        cache = self.__dict__.setdefault("_dev_cache", {})
+        if key not in cache:
+            offs, D = self.band_rows()
+            cache[key] = (torch.tensor(offs, dtype=torch.int64, device=device),
+                          torch.from_numpy(D).to(device))
+        return cache[key]
+
+    def matvec(self, v, c, format=None, axis=0, scales=None, accumulate=False):
+        """c = self . v along ``axis`` on the device (matrixbase.py:87-144):
+        rows >= the matrix size are zeroed; ``scales`` (device, one per line
+        of the non-axis dims in C order) multiplies each line; ``accumulate``
+        adds into c instead."""
+        for t, name in ((v, "v"), (c, "c")):
+            if not isinstance(t, torch.Tensor) or not t.is_cuda:
+                raise TypeError(f"{name} must be a CUDA (HIP) torch tensor")
+        if v.dtype != c.dtype or v.dtype not in (torch.float64, torch.complex128):
+            raise TypeError("v and c must both be float64 or both complex128")
+        offs, D = self._device_rows(v.device)
+        code = _lib.C128 if v.dtype == torch.complex128 else _lib.F64
+        _lib.call("sfa_band_matvec", _lib.ptr(v), _lib.ptr(c), code, v.dim(), _lib.arr(v.shape),
+                  _lib.arr(v.stride()), _lib.arr(c.shape), _lib.arr(c.stride()), axis,
+                  self.shape[0], self.shape[1], offs.numel(), _lib.ptr(offs), _lib.ptr(D),
+                  _lib.ptr(scales) if scales is not None else None, int(bool(accumulate)),
+                  _lib.stream_ptr())
+        return c
 
     def get_solver(self):
         """Band pattern -> solver class (matrixbase.py:439-474)."""
@@ -296,6 +341,22 @@ class TPMatrix:
     def tensorproductspace(self):
         return self.space
 
+    def line_scales(self, shape, device):
+        """The per-mode scale broadcast over the non-axis dims of ``shape``,
+        one value per line in C order (device)."""
+        ax = self.naxes[0]
+        full = np.broadcast_to(np.asarray(self.scale, dtype=float),
+                               [1 if i == ax else n for i, n in enumerate(shape)])
+        return torch.from_numpy(np.array(np.moveaxis(full, ax, -1)).reshape(-1)).to(device)
+
+    def matvec(self, v, c, format=None, accumulate=False):
+        """c = self . v: the matrix along the non-periodic axis times the
+        per-mode scale (TPMatrix.matvec, matrixbase.py:1437-1449)."""
+        if getattr(self.pmat, "is_bc_matrix", False):
+            raise TypeError("boundary matrices act on the boundary slots: use the solvers' matvec")
+        return self.pmat.matvec(v, c, axis=self.naxes[0], scales=self.line_scales(v.shape, v.device),
+                                accumulate=accumulate)
+
 
 class BCMatrix:
     """Boundary matrix (N-2, 2) of a composite space with nonhomogeneous
@@ -482,10 +543,12 @@ def _split_bc(mats):
     return reg, bc
 
 
-def apply_bc_matrices(bc_mats, b, u):
-    """b -= sum of the boundary matrices times u's boundary slots, each scaled
-    per mode (Helmholtz.__call__, chebyshev/la.py:307-311; SolverGeneric1ND,
-    la.py:1390-1410).  u's boundary slots are set first (final values)."""
+def apply_bc_matrices(bc_mats, b, u, sign=-1.0):
+    """b += sign * (sum of the boundary matrices times u's boundary slots, each
+    scaled per mode); sign -1 moves them to the right-hand side
+    (Helmholtz.__call__, chebyshev/la.py:307-311; SolverGeneric1ND,
+    la.py:1390-1410), +1 adds them to a matvec (la.py:1212-1229).  u's
+    boundary slots are set first (final values)."""
     if not bc_mats:
         return b
     S = bc_mats[0].pmat.space
@@ -501,4 +564,4 @@ def apply_bc_matrices(bc_mats, b, u):
             full = np.broadcast_to(sc, [1 if i == ax else n for i, n in enumerate(b.shape)])
             rows.append(np.moveaxis(full, ax, -1).reshape(-1))  # C order over the other dims
     scales = np.stack(rows)
-    return S._bc_update(u, b, -W, S.N - 2, scales)
+    return S._bc_update(u, b, sign * W, S.N - 2, scales)

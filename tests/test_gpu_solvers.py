@@ -202,3 +202,67 @@ def test_poisson_add(sfa, N):
     assert rel(u[:N - 2], ref[:N - 2]) < TOL
     dense = S.dense(S.cheb_A_SD(N), N - 2)
     assert rel(u[:N - 2], np.linalg.solve(dense, b[:N - 2])) < 1e-11
+
+
+@pytest.mark.parametrize("di", MATS)
+@pytest.mark.parametrize("axis", [0, 1])
+def test_sparse_matvec(sfa, di, axis):
+    """SparseMatrix.matvec (matrixbase.py:87-144) along an axis vs numpy."""
+    print(f"[matvec {sorted(di)} axis={axis}]", flush=True)
+    M = sfa.SparseMatrix(di, (N10, N10), scale=1.5)
+    rng = np.random.default_rng(3)
+    shp = [12, 4, 3]
+    shp[axis] = N10 + 2  # the vector is longer than the matrix: rows >= N come out zero
+    v = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+    c = torch.full(shp, 7.0 + 1j, dtype=torch.complex128, device="cuda")
+    M.matvec(dev(v), c, axis=axis)
+    vm = np.moveaxis(v, axis, 0)
+    ref = np.zeros_like(vm)
+    ref[:N10] = np.tensordot(1.5 * S.dense(di, N10), vm[:N10], axes=(1, 0))
+    assert rel(c, np.moveaxis(ref, 0, axis)) < 1e-14
+
+
+def test_matvec_dense_tail_and_tpmatrix(sfa):
+    """The Chebyshev stiffness matrix (N/2 diagonals) and TPMatrix.matvec with
+    its per-mode scale (matrixbase.py:1437-1449)."""
+    SD = sfa.FunctionSpace(40, "C", bc="D")
+    K = sfa.FunctionSpace(8, "F", dtype="d")
+    T = sfa.TensorProductSpace(None, (SD, K))
+    A, B = sfa.helmholtz_tpmatrices(T, 2.0)
+    rng = np.random.default_rng(4)
+    shp = T.shape(True)
+    u = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+    u[38:] = 0
+    c = torch.zeros(shp, dtype=torch.complex128, device="cuda")
+    B.matvec(dev(u), c)
+    A.matvec(dev(u), c, accumulate=True)
+    ref = np.zeros_like(u)
+    for m in range(shp[1]):
+        Mm = A.scale.reshape(-1)[0] * A.pmat.todense() * float(A.pmat.scale) + \
+            B.scale.reshape(-1)[m] * B.pmat.todense() * float(B.pmat.scale)
+        ref[:38, m] = Mm @ u[:38, m]
+    assert rel(c, ref) < 1e-13
+
+
+@pytest.mark.parametrize("case", ["legendre", "pg", "bc"])
+def test_solver_generic_matvec_roundtrip(sfa, case):
+    """SolverGeneric1ND.matvec then the solve returns u (the reference's
+    G.matvec check, tests/test_matrices.py:594-621)."""
+    print(f"[generic matvec {case}]", flush=True)
+    bc = (0.5, -1.0) if case == "bc" else (0, 0)
+    fam = "L" if case != "pg" else "C"
+    SD = sfa.FunctionSpace(24, fam, bc=bc)
+    T = sfa.TensorProductSpace(None, (SD, sfa.FunctionSpace(12, "F", dtype="d")))
+    test = T.get_testspace("PG") if case == "pg" else None
+    G = sfa.la.SolverGeneric1ND(sfa.helmholtz_tpmatrices(T, 1.5, test=test))
+    rng = np.random.default_rng(5)
+    shp = T.shape(True)
+    u = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+    u[22:] = 0
+    ud = dev(u)
+    if case == "bc":
+        SD.set_boundary_dofs(ud, final=True)
+    c = torch.zeros(shp, dtype=torch.complex128, device="cuda")
+    G.matvec(ud, c)
+    uc = G(c, sfa.Function(T))
+    assert rel(uc, ud.cpu().numpy()) < 1e-11
