@@ -164,6 +164,16 @@ int sfa_line_matmul(const void* in, void* out, int dtype, int ndim,
                     const int64_t* out_strides, int axis, int64_t nout,
                     const double* M, void* stream);
 
+/* Evaluation of a 1-3D expansion at scattered points: out[p] = sum over
+ * (k0, k1, k2) of c[k0][k1][k2] E0[p][k0] E1[p][k1] E2[p][k2]; E_d device
+ * complex128 (npts, n_d), each axis' basis functions at the points'
+ * coordinate d (stencil, R2C weights and boundary functions folded in);
+ * c contiguous complex128 (n0, n1, n2), n2 = 1 and E2 = ones for 2D, n1 = n2 = 1
+ * for 1D.  Replaces TensorProductSpace.eval (tensorproductspace.py:668-760)
+ * and SpectralBase.eval (spectralbase.py eval -> evaluate_basis_all). */
+int sfa_tensor_eval(const void* c, int64_t n0, int64_t n1, int64_t n2, const void* E0,
+                    const void* E1, const void* E2, void* out, int64_t npts, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Banded solves (la.py + optimization/cython/la.pyx)                  */
 /* ------------------------------------------------------------------ */

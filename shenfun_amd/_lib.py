@@ -41,6 +41,7 @@ _SIGS = {
     "sfa_pointwise_mul": "vvvilv",
     "sfa_convolve_direct": "vvvliv",
     "sfa_band_matvec": "vviiaaaaillivvviv",
+    "sfa_tensor_eval": "vlllvvvvlv",
     "sfa_tdma_o_lu": "vllv",
     "sfa_fdma_lu": "vllv",
     "sfa_heptadma_lu": "vllv",

@@ -216,6 +216,39 @@ class SpectralBase(_Planned):
         z0, z1 = self._direct_range()
         return self._pad(inp, self._tmp("dd", inp.shape, inp.dtype), 2, nbc, z0, max(z0, z1))
 
+    # ------------------------------------------------------------ evaluation
+    def map_reference_domain(self, x):
+        """The hot-path spaces live on their reference domains ([-1, 1], [0, 2 pi))."""
+        return np.asarray(x, dtype=float)
+
+    def evaluate_basis_all(self, x):
+        """(len(x), N) trial basis functions at x (spectralbase.py:372,
+        argument=1; composite spaces: _composite, spectralbase.py:1589-1603)."""
+        raise NotImplementedError
+
+    def _eval_weights(self):
+        return None
+
+    def _eval_matrix(self, x):
+        """evaluate_basis_all as complex128, with the R2C weights folded in."""
+        E = np.asarray(self.evaluate_basis_all(x), dtype=np.complex128)
+        w = self._eval_weights()
+        return E if w is None else E * w[None, :]
+
+    def eval(self, x, u, output_array=None):
+        """u(x) at the points x from the expansion coefficients u
+        (spectralbase.py:493-514, kind='vandermonde'): the basis functions at
+        x are built on the host, the sum runs on the device
+        (sfa_tensor_eval).  1D spaces only: a multi-dimensional array is
+        evaluated by TensorProductSpace.eval."""
+        x = np.atleast_1d(self.map_reference_domain(x)).ravel()
+        u = self._tensor(u)
+        if u.dim() != 1 or u.shape[0] != (self.shape(True) if isinstance(self, FourierBase)
+                                          else self.N):
+            raise ValueError("eval takes the 1D coefficient array of this space")
+        real = not isinstance(self, C2C) and self.dtype.kind != "c"
+        return tensor_eval(u, [self._eval_matrix(x)], x.size, real, output_array)
+
     @property
     def is_composite(self):
         return False
@@ -367,6 +400,20 @@ class _PolyBase(SpectralBase):
         P = np.polynomial.Chebyshev if self.family() == "chebyshev" else np.polynomial.Legendre
         x = np.asarray(x, dtype=float)
         return np.stack([P.basis(k).deriv(deriv)(x) for k in range(n)], axis=1)
+
+    def evaluate_basis_all(self, x):
+        x = np.asarray(x, dtype=float).ravel()
+        vander = (np.polynomial.chebyshev.chebvander if self.family() == "chebyshev"
+                  else np.polynomial.legendre.legvander)
+        V = vander(x, self.N - 1)
+        d = self.dim()
+        P = np.zeros((x.size, self.N))
+        for off, v in self.stencil().items():
+            P[:, :d] += V[:, off:off + d] * v[None, :]
+        if self.bc_kind in ("D", "N"):
+            # boundary slots: the boundary basis functions carrying the values
+            P[:, d:] = self._bc_functions(x)
+        return P
 
     def _bc_functions(self, x, deriv=0):
         """phi_bc_j^(deriv)(x), shape (len(x), 2)."""
@@ -754,6 +801,11 @@ class FourierBase(SpectralBase):
     def forward(self, input_array=None, output_array=None, kind=None):
         return self.scalar_product(input_array, output_array, kind)
 
+    def evaluate_basis_all(self, x):
+        """exp(i k x) for the stored wavenumbers k (fourier/bases.py:186-197)."""
+        x = np.asarray(x, dtype=float).ravel()
+        return np.exp(1j * x[:, None] * self.wavenumbers()[None, :].astype(float))
+
     def convolve(self, u, v, uv=None, fast=True):
         """Convolution of the coefficient vectors u and v (R2C fourier/bases.py:437-475,
         C2C :600-640).  fast: backward both on the padded grid, multiply,
@@ -802,6 +854,32 @@ def pointwise_mul(a, b, out):
     return out
 
 
+def tensor_eval(c, mats, npts, real, output_array=None):
+    """sum over k of c[k0, k1, k2] E0[p, k0] E1[p, k1] E2[p, k2] for each
+    point p (one sfa_tensor_eval launch); ``mats`` are host (npts, n_d)
+    basis-function matrices for the leading axes of the device array ``c``.
+    Returns the real part when ``real``."""
+    if c.dim() != len(mats) or not 1 <= c.dim() <= 3:
+        raise ValueError("one evaluation matrix per axis of the coefficient array")
+    for d, E in enumerate(mats):
+        if E.shape != (npts, c.shape[d]):
+            raise ValueError(f"axis {d}: evaluation matrix {E.shape} does not match "
+                             f"({npts}, {c.shape[d]})")
+    dev = c.device
+    cc = c.to(torch.complex128).contiguous()
+    n = list(c.shape) + [1] * (3 - c.dim())
+    Es = [torch.from_numpy(np.ascontiguousarray(E, dtype=np.complex128)).to(dev) for E in mats]
+    Es += [torch.ones((npts, 1), dtype=torch.complex128, device=dev)] * (3 - len(Es))
+    out = torch.empty(npts, dtype=torch.complex128, device=dev)
+    _lib.call("sfa_tensor_eval", _lib.ptr(cc), n[0], n[1], n[2], _lib.ptr(Es[0]), _lib.ptr(Es[1]),
+              _lib.ptr(Es[2]), _lib.ptr(out), npts, _lib.stream_ptr())
+    res = out.real if real else out
+    if output_array is not None:
+        output_array.copy_(res.reshape(output_array.shape))
+        return output_array
+    return res.contiguous()
+
+
 class R2C(FourierBase):
     _pad_kind = 1
 
@@ -822,6 +900,15 @@ class R2C(FourierBase):
 
     def _direct_range(self):
         return self.N // 3, self.N // 2 + 1  # fourier/bases.py:430-433
+
+    def _eval_weights(self):
+        """u(x) = Re sum_k w_k c_k e^{ikx} over the stored half: w = 2 but for
+        k = 0 and the Nyquist mode of even N (fourier/bases.py:380-394)."""
+        w = np.full(self.N // 2 + 1, 2.0)
+        w[0] = 1.0
+        if self.N % 2 == 0:
+            w[-1] = 1.0
+        return w
 
     def scalar_product(self, input_array=None, output_array=None, kind=None):
         inp = self._tensor(input_array)

@@ -278,3 +278,65 @@ extern "C" int sfa_line_matmul(const void* in, void* out, int dtype, int ndim,
     SFA_LAUNCH_CHECK();
     return SFA_OK;
 }
+
+// ----------------------------------------------------------------- point evaluation
+// out[p] = sum_{k0,k1,k2} c[k0][k1][k2] E0[p][k0] E1[p][k1] E2[p][k2] for
+// scattered points p (TensorProductSpace.eval, tensorproductspace.py:668-760):
+// E_d (npts, n_d) complex128 are each axis' basis functions at the points'
+// coordinate d (stencils, weights and boundary functions folded in by the
+// caller), c contiguous complex128 (n0, n1, n2) (n2 = 1, E2 = 1 for 2D).  One
+// workgroup per point: threads stride the (k0, k1) pairs, each sums over k2,
+// then a workgroup reduction.
+namespace sfa {
+__global__ void __launch_bounds__(256)
+    tensor_eval_kernel(const double2* __restrict__ c, int n0, int n1, int n2,
+                       const double2* __restrict__ E0, const double2* __restrict__ E1,
+                       const double2* __restrict__ E2, double2* __restrict__ out)
+{
+    __shared__ double2 red[256];
+    const int p = blockIdx.x, tid = threadIdx.x;
+    const double2* e0 = E0 + (int64_t)p * n0;
+    const double2* e1 = E1 + (int64_t)p * n1;
+    const double2* e2 = E2 + (int64_t)p * n2;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int q = tid; q < n0 * n1; q += blockDim.x) {
+        const int k0 = q / n1, k1 = q - k0 * n1;
+        const double2* row = c + ((int64_t)k0 * n1 + k1) * n2;
+        double2 s = make_double2(0.0, 0.0);
+        for (int k2 = 0; k2 < n2; ++k2) {
+            const double2 a = row[k2], b = e2[k2];
+            s.x += a.x * b.x - a.y * b.y;
+            s.y += a.x * b.y + a.y * b.x;
+        }
+        const double2 a = e0[k0], b = e1[k1];
+        const double2 w = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+        acc.x += w.x * s.x - w.y * s.y;
+        acc.y += w.x * s.y + w.y * s.x;
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (tid < h) {
+            red[tid].x += red[tid + h].x;
+            red[tid].y += red[tid + h].y;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) out[p] = red[0];
+}
+}  // namespace sfa
+
+extern "C" int sfa_tensor_eval(const void* c, int64_t n0, int64_t n1, int64_t n2, const void* E0,
+                               const void* E1, const void* E2, void* out, int64_t npts, void* stream)
+{
+    SFA_CHECK(c && E0 && E1 && E2 && out, SFA_ERR_ARG, "tensor_eval: null array");
+    SFA_CHECK(n0 >= 1 && n1 >= 1 && n2 >= 1 && n0 * n1 * n2 < (1ll << 30) && npts >= 0 &&
+                  npts < (1ll << 31),
+              SFA_ERR_ARG, "tensor_eval: bad sizes");
+    if (npts == 0) return SFA_OK;
+    hipLaunchKernelGGL(tensor_eval_kernel, dim3((unsigned)npts), dim3(256), 0, (hipStream_t)stream,
+                       (const double2*)c, (int)n0, (int)n1, (int)n2, (const double2*)E0,
+                       (const double2*)E1, (const double2*)E2, (double2*)out);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}

@@ -10,7 +10,7 @@ HIP transform on the strided device array.
 import numpy as np
 import torch
 
-from .bases import FourierBase, R2C, _Planned, pointwise_mul
+from .bases import FourierBase, R2C, _Planned, pointwise_mul, tensor_eval
 
 
 class TensorProductSpace(_Planned):
@@ -180,6 +180,25 @@ class TensorProductSpace(_Planned):
         a = self.backward(a_hat, self._buf(("conv", 0), shp, dt))
         b = self.backward(b_hat, self._buf(("conv", 1), shp, dt))
         return self.forward(pointwise_mul(a, b, self._buf(("conv", 2), shp, dt)), ab_hat)
+
+
+    def eval(self, points, coefficients, output_array=None, method=1):
+        """Function values at scattered points (tensorproductspace.py:668-760):
+        ``points`` of shape (D, npts).  Each axis' basis functions at the
+        points' coordinates are built on the host (R2C weights folded in),
+        the sum over all coefficients runs on the device in one launch
+        (sfa_tensor_eval); ``method`` is accepted and ignored."""
+        pts = np.atleast_2d(np.asarray(points, dtype=float))
+        if pts.shape[0] != self.dimensions:
+            raise ValueError(f"points must have shape ({self.dimensions}, npts)")
+        npts = pts.shape[1]
+        c = coefficients
+        if tuple(c.shape) != tuple(self.shape(True)):
+            raise ValueError(f"coefficients of shape {tuple(c.shape)}, expected {self.shape(True)}")
+        if not isinstance(c, torch.Tensor):
+            c = torch.from_numpy(np.ascontiguousarray(c)).to(self.device)
+        mats = [b._eval_matrix(b.map_reference_domain(pts[i])) for i, b in enumerate(self.bases)]
+        return tensor_eval(c, mats, npts, self.physical_dtype() == torch.float64, output_array)
 
 
 def Array(space, buffer=None, val=None):
