@@ -203,6 +203,19 @@ int sfa_diagma_solve(void* u, int dtype, int ndim, const int64_t* shape,
                      const int64_t* strides, int axis, const double* d, int64_t n,
                      int64_t data_line_stride, int64_t skip_lines, void* stream);
 
+/* General banded systems with offsets -nl..nu (0 <= nl <= 4, 1 <= nu <= 4):
+ * Doolittle LU without pivoting in place on nbatch stacked DIA blocks of
+ * (nl + 1 + nu) rows x n (A(i, j) at row nl + j - i, column j), then the
+ * forward/backward substitution along axis like the other band solves.
+ * Replaces the reference's general sparse fallback Solve (la.py:823-844,
+ * chosen by matrixbase.py:439-474 for any other pattern) on the banded
+ * systems of the Galerkin spaces; nonzero pivots required (SPD or
+ * diagonally dominant matrices). */
+int sfa_band_lu(double* data, int64_t n, int nl, int nu, int64_t nbatch, void* stream);
+int sfa_band_solve(void* u, int dtype, int ndim, const int64_t* shape, const int64_t* strides,
+                   int axis, const double* data, int64_t n, int64_t data_line_stride,
+                   int64_t skip_lines, int nl, int nu, void* stream);
+
 /* The other band patterns of SparseMatrix.get_solver (matrixbase.py:439-474),
  * DIA rows in offset order, same layout and line conventions as above.
  * LU in place, nbatch systems of (rows, n):

@@ -42,6 +42,8 @@ _SIGS = {
     "sfa_convolve_direct": "vvvliv",
     "sfa_band_matvec": "vviiaaaaillivvviv",
     "sfa_tensor_eval": "vlllvvvvlv",
+    "sfa_band_lu": "vliilv",
+    "sfa_band_solve": "viiaaivllliiv",
     "sfa_tdma_o_lu": "vllv",
     "sfa_fdma_lu": "vllv",
     "sfa_heptadma_lu": "vllv",

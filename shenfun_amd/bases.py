@@ -1055,6 +1055,16 @@ def _stencil_dict(family, kind, k):
     Phi4 (test space of four-condition bases): chebyshev/bases.py Phi4,
     legendre/bases.py Phi4."""
     n = np.asarray(k, dtype=float)
+    if kind in ("DN", "ND"):
+        # phi_k = P_k + b_k P_{k+1} + c_k P_{k+2}: DN u(-1) = u'(1) = 0, ND
+        # u'(-1) = u(1) = 0 (chebyshev/bases.py DirichletNeumann /
+        # NeumannDirichlet, legendre/bases.py:1084-1272)
+        sgn = 1.0 if kind == "DN" else -1.0
+        if family == "chebyshev":
+            q = 2 * n * n + 6 * n + 5
+            return {0: np.ones_like(n), 1: sgn * 4 * (n + 1) / q, 2: -(2 * n * n + 2 * n + 1) / q}
+        return {0: np.ones_like(n), 1: sgn * (2 * n + 3) / (n + 2) ** 2,
+                2: -((n + 1) / (n + 2)) ** 2}
     if kind == "SB":
         if family == "chebyshev":
             return {0: np.ones_like(n), 2: -(2 * n + 4) / (n + 3), 4: (n + 1) / (n + 3)}
@@ -1088,7 +1098,9 @@ class _StencilSpace:
     pentadiagonal mass matrix."""
 
     kind = None
-    _nbc = {"SB": 4, "P2": 4, "P4": 8}
+    _nbc = {"SB": 4, "P2": 4, "P4": 8, "DN": 2, "ND": 2}
+    _names = {"SB": "Biharmonic", "P2": "Biharmonic", "P4": "Biharmonic",
+              "DN": "DirichletNeumann", "ND": "NeumannDirichlet"}
 
     @property
     def bc_kind(self):
@@ -1098,7 +1110,7 @@ class _StencilSpace:
         return self.N - self._nbc[self.kind]
 
     def boundary_condition(self):
-        return "Biharmonic"
+        return self._names[self.kind]
 
     def short_name(self):
         return self.kind
@@ -1131,7 +1143,7 @@ class _StencilSpace:
         self._matmul(inp, out, self._synthesis_matrix())
 
     def forward(self, input_array=None, output_array=None, kind=None):
-        if self.kind != "SB":
+        if self.kind in ("P2", "P4"):
             raise NotImplementedError(f"{self.kind} is a test space: scalar_product and backward only")
         out = self.scalar_product(input_array, output_array, kind)
         sl = [slice(None)] * out.dim()
@@ -1143,7 +1155,7 @@ class _StencilSpace:
     def _mass_solver(self):
         if "mass" not in self._cache:
             from .matrices import GalerkinMatrix
-            B = GalerkinMatrix(self, self, 0, "BSBSBmat")
+            B = GalerkinMatrix(self, self, 0, f"B{self.kind}{self.kind}mat")
             self._cache["mass"] = B.get_solver()(B)
         return self._cache["mass"]
 
@@ -1152,6 +1164,8 @@ class _StencilSpace:
             return self
         if kind not in ("Petrov-Galerkin", "PG"):
             raise ValueError(f"unknown test space kind {kind!r}")
+        if self.kind in ("DN", "ND"):
+            raise NotImplementedError("Petrov-Galerkin test spaces of the mixed bases")
         if self.kind != "SB":
             return self
         cls = ChebyshevPhi4 if self.family() == "chebyshev" else LegendrePhi4
@@ -1161,6 +1175,22 @@ class _StencilSpace:
         if padding_factor != 1 or dealias_direct:
             raise NotImplementedError("dealiasing of the biharmonic / Petrov-Galerkin spaces")
         return type(self)(N, quad=self.quad, dtype=self.dtype, device=self.device)
+
+
+class ChebyshevDirichletNeumann(_StencilSpace, _Init, ChebyshevBase):
+    kind = "DN"
+
+
+class LegendreDirichletNeumann(_StencilSpace, _Init, LegendreBase):
+    kind = "DN"
+
+
+class ChebyshevNeumannDirichlet(_StencilSpace, _Init, ChebyshevBase):
+    kind = "ND"
+
+
+class LegendreNeumannDirichlet(_StencilSpace, _Init, LegendreBase):
+    kind = "ND"
 
 
 class ChebyshevPhi2(_StencilSpace, _Init, ChebyshevBase):
@@ -1247,6 +1277,21 @@ def _parse_bc(bc):
     raise ValueError(f"unsupported bc {bc!r}")
 
 
+def _mixed_bc(bc):
+    """'DN' for {'left': {'D': 0}, 'right': {'N': 0}}, 'ND' for the mirror
+    (DirichletNeumann / NeumannDirichlet), None otherwise; homogeneous only."""
+    if not isinstance(bc, dict) or set(bc) != {"left", "right"} or \
+            any(not isinstance(bc[s], dict) or len(bc[s]) != 1 for s in bc):
+        return None
+    (kl, vl), = bc["left"].items()
+    (kr, vr), = bc["right"].items()
+    if {kl, kr} != {"D", "N"}:
+        return None
+    if any(not (isinstance(_bcval(v), float) and _bcval(v) == 0) for v in (vl, vr)):
+        raise NotImplementedError("nonhomogeneous mixed Dirichlet/Neumann values")
+    return kl + kr
+
+
 def _bc_kind(bc):
     return _parse_bc(bc)[0]
 
@@ -1270,6 +1315,16 @@ def FunctionSpace(N, family="Fourier", bc=None, dtype="d", quad=None, domain=Non
         if fam in ("legendre", "l"):
             return LegendreShenBiharmonic(N, quad="LG", device=device)
         raise NotImplementedError(f"biharmonic {family!r} space")
+    mixed = _mixed_bc(bc)
+    if mixed is not None:
+        cls = {("chebyshev", "DN"): ChebyshevDirichletNeumann, ("c", "DN"): ChebyshevDirichletNeumann,
+               ("chebyshev", "ND"): ChebyshevNeumannDirichlet, ("c", "ND"): ChebyshevNeumannDirichlet,
+               ("legendre", "DN"): LegendreDirichletNeumann, ("l", "DN"): LegendreDirichletNeumann,
+               ("legendre", "ND"): LegendreNeumannDirichlet, ("l", "ND"): LegendreNeumannDirichlet}
+        if (fam, mixed) not in cls:
+            raise NotImplementedError(f"mixed boundary conditions on {family!r}")
+        return cls[(fam, mixed)](N, quad=quad or ("GC" if fam in ("chebyshev", "c") else "LG"),
+                                 device=device)
     kind, vals = _parse_bc(bc)
     if fam in ("chebyshev", "c"):
         if quad not in (None, "GC", "GL"):

@@ -242,6 +242,38 @@ __global__ void band_solve_kernel(T* __restrict__ u, LineMap g, const double* __
     }
 }
 
+// Doolittle LU without pivoting of a general band matrix with offsets
+// -NL..NU (unit stride), in place on its DIA rows (A(i, j) at row NL + j - i,
+// column j): the multipliers replace the lower bands, U the diagonal and the
+// upper bands, the layout band_solve_kernel<T, NL, NU, 1> reads.  One thread
+// per system.  Stands in for the reference's general sparse fallback
+// (la.py:823-844, SuperLU) on the banded systems the Galerkin spaces give;
+// without pivoting it needs nonzero pivots (SPD or diagonally dominant
+// matrices, as the mass and Helmholtz matrices are).
+template <int NL, int NU>
+__global__ void band_lu_kernel(double* __restrict__ data, int n, int64_t nbatch)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbatch) return;
+    double* r = data + b * (int64_t)(NL + 1 + NU) * n;
+    for (int k = 0; k + 1 < n; ++k) {
+        const double piv = r[(int64_t)NL * n + k];
+#pragma unroll
+        for (int l = 1; l <= NL; ++l) {
+            if (k + l >= n) break;
+            double* lik = r + (int64_t)(NL - l) * n + k;  // A(k + l, k)
+            const double m = *lik / piv;
+            *lik = m;
+#pragma unroll
+            for (int q = 1; q <= NU; ++q) {
+                if (k + q >= n) break;
+                // A(k + l, k + q) -= m A(k, k + q); offset q - l in [1 - NL, NU - 1]
+                r[(int64_t)(NL + q - l) * n + k + q] -= m * r[(int64_t)(NL + q) * n + k + q];
+            }
+        }
+    }
+}
+
 // la.pyx:254-263 TDMA_O_LU, one thread per system
 __global__ void tdma_o_lu_kernel(double* __restrict__ data, int n, int64_t nbatch)
 {
@@ -741,6 +773,38 @@ static int band_lu(const char* name, void (*kern)(double*, int, int64_t), int ch
                        data, (int)n, nbatch);
     SFA_LAUNCH_CHECK();
     return SFA_OK;
+}
+
+// general banded systems, offsets -nl..nu (0 <= nl <= 4, 1 <= nu <= 4), unit stride
+using BandSolveFn = int (*)(const char*, int64_t, void*, int, int, const int64_t*,
+                            const int64_t*, int, const double*, int64_t, int64_t, int64_t, void*);
+#define SFA_ROW(NL) {band_solve<NL, 1, 1>, band_solve<NL, 2, 1>, band_solve<NL, 3, 1>,           \
+                     band_solve<NL, 4, 1>}
+static const BandSolveFn kBandSolve[5][4] = {SFA_ROW(0), SFA_ROW(1), SFA_ROW(2), SFA_ROW(3),
+                                             SFA_ROW(4)};
+#undef SFA_ROW
+#define SFA_ROW(NL) {band_lu_kernel<NL, 1>, band_lu_kernel<NL, 2>, band_lu_kernel<NL, 3>,        \
+                     band_lu_kernel<NL, 4>}
+static void (*const kBandLU[5][4])(double*, int, int64_t) = {SFA_ROW(0), SFA_ROW(1), SFA_ROW(2),
+                                                               SFA_ROW(3), SFA_ROW(4)};
+#undef SFA_ROW
+
+extern "C" int sfa_band_lu(double* data, int64_t n, int nl, int nu, int64_t nbatch, void* stream)
+{
+    SFA_CHECK(nl >= 0 && nl <= 4 && nu >= 1 && nu <= 4, SFA_ERR_ARG,
+              "band_lu: bands -%d..%d outside -4..4 (nu >= 1)", nl, nu);
+    return band_lu("band", kBandLU[nl][nu - 1], 1, 1, data, n, nbatch, stream);
+}
+
+extern "C" int sfa_band_solve(void* u, int dtype, int ndim, const int64_t* shape,
+                              const int64_t* strides, int axis, const double* data, int64_t n,
+                              int64_t data_line_stride, int64_t skip_lines, int nl, int nu,
+                              void* stream)
+{
+    SFA_CHECK(nl >= 0 && nl <= 4 && nu >= 1 && nu <= 4, SFA_ERR_ARG,
+              "band_solve: bands -%d..%d outside -4..4 (nu >= 1)", nl, nu);
+    return kBandSolve[nl][nu - 1]("sfa_band_solve", 1, u, dtype, ndim, shape, strides, axis, data,
+                                  n, data_line_stride, skip_lines, stream);
 }
 
 extern "C" int sfa_tdma_o_lu(double* data, int64_t n, int64_t nbatch, void* stream)

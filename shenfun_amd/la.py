@@ -87,11 +87,19 @@ class _Banded:
     def _dev(self, device):
         if self._lu is None or self._device != device:
             d = torch.from_numpy(self._data.copy()).to(device)
-            if self.lu_fn is not None:
-                _lib.call(self.lu_fn, _lib.ptr(d), d.shape[1], 1, _lib.stream_ptr())
+            self._call_lu(d, d.shape[1], 1)
             self._lu = d
             self._device = device
         return self._lu
+
+    def _call_lu(self, d, n, nbatch):
+        if self.lu_fn is not None:
+            _lib.call(self.lu_fn, _lib.ptr(d), n, nbatch, _lib.stream_ptr())
+
+    def _call_solve(self, u, axis, data, n, dls, skip):
+        shp, st = _geom(u)
+        _lib.call(self.solve_fn, _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, axis, _lib.ptr(data),
+                  n, dls, skip, _lib.stream_ptr())
 
     def perform_lu(self, device="cuda"):
         return self._dev(torch.device(device))
@@ -113,9 +121,7 @@ class _Banded:
         u = _prepare(b, u)
         self.apply_constraints(u, constraints, axis)
         data = self._dev(u.device)
-        shp, st = _geom(u)
-        _lib.call(self.solve_fn, _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, axis, _lib.ptr(data),
-                  data.shape[1], 0, 0, _lib.stream_ptr())
+        self._call_solve(u, axis, data, data.shape[1], 0, 0)
         return u
 
     solve = __call__
@@ -230,12 +236,45 @@ class HeptaDMA(_Banded):
 _PATTERNS = (DiagMA, TwoDMA, TDMA_O, TDMA, ThreeDMA, FDMA, PDMA, HeptaDMA)
 
 
-class Solve:
-    """General sparse fallback of the reference (SuperLU, la.py:823-844) is not
-    part of the GPU hot path."""
+class Solve(_Banded):
+    """The reference's general fallback (SuperLU, la.py:823-844) for the band
+    patterns without a dedicated solver, e.g. the -2..2 mass matrices of the
+    mixed Dirichlet/Neumann bases: banded LU without pivoting on the device
+    (sfa_band_lu / sfa_band_solve), offsets -nl..nu within -4..4.  The
+    Galerkin mass and Helmholtz matrices are SPD, so no pivoting is needed; a
+    zero or non-finite pivot raises."""
 
     def __init__(self, mats):
-        raise NotImplementedError("general sparse systems are not on the HIP hot path")
+        mat = _sum_mats(mats)
+        if float(mat.scale) == 0:
+            raise ValueError("Solve: the matrix is zero")
+        self.nl, self.nu = _band_extent(mat.keys())
+        self.offsets = tuple(range(-self.nl, self.nu + 1))
+        super().__init__(mat)
+
+    def _call_lu(self, d, n, nbatch):
+        _band_lu(d, n, self.nl, self.nu, nbatch)
+
+    def _call_solve(self, u, axis, data, n, dls, skip):
+        shp, st = _geom(u)
+        _lib.call("sfa_band_solve", _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, axis,
+                  _lib.ptr(data), n, dls, skip, self.nl, self.nu, _lib.stream_ptr())
+
+
+def _band_extent(keys):
+    keys = [int(k) for k in keys]
+    nl, nu = max(0, -min(keys)), max(1, max(keys))
+    if nl > 4 or nu > 4:
+        raise NotImplementedError(f"band offsets {sorted(keys)} outside -4..4")
+    return nl, nu
+
+
+def _band_lu(d, n, nl, nu, nbatch):
+    """In-place banded LU of nbatch DIA blocks; checks the pivots."""
+    _lib.call("sfa_band_lu", _lib.ptr(d), n, nl, nu, nbatch, _lib.stream_ptr())
+    piv = d.view(nbatch, nl + 1 + nu, n)[:, nl, :]
+    if not bool(torch.isfinite(d).all()) or bool((piv == 0).any()):
+        raise ZeroDivisionError("banded LU without pivoting met a zero pivot")
 
 
 def Solver(mats):
@@ -263,13 +302,17 @@ class SolverGeneric1ND:
         for m in mats:
             keys |= set(m.pmat.keys())
         keys = tuple(sorted(keys))
+        self._band = None
         for cls in _PATTERNS:
             if set(keys) <= set(cls.offsets):
                 break
         else:
-            raise NotImplementedError(f"band pattern {keys} not supported")
+            # no dedicated pattern: the general banded fallback (Solve)
+            cls = Solve
+            self._band = _band_extent(keys)
         self.kind = cls.__name__.lower().replace("diagma", "diag")
-        self.offsets = cls.offsets
+        self.offsets = cls.offsets if self._band is None else \
+            tuple(range(-self._band[0], self._band[1] + 1))
         self._lu_fn, self._solve_fn = cls.lu_fn, cls.solve_fn
         # per-mode DIA data (nmodes, rows, n), C order over the non-axis dims
         data = np.zeros((self.nmodes, len(self.offsets), n))
@@ -303,7 +346,9 @@ class SolverGeneric1ND:
     def perform_lu(self, device):
         if self._dev is None or self._dev.device != device:
             d = torch.from_numpy(self._data).to(device)
-            if self._lu_fn is not None:
+            if self._band is not None:
+                _band_lu(d, self.n, self._band[0], self._band[1], self.nmodes)
+            elif self._lu_fn is not None:
                 _lib.call(self._lu_fn, _lib.ptr(d), self.n, self.nmodes, _lib.stream_ptr())
             self._dev = d
         return self._dev
@@ -333,8 +378,13 @@ class SolverGeneric1ND:
         self.sol0(line0.unsqueeze(0) if line0.dim() == 0 else line0, axis=0,
                   constraints=constraints)
         shp, st = _geom(u)
-        _lib.call(self._solve_fn, _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, ax, _lib.ptr(data),
-                  self.n, data.shape[1] * data.shape[2], 1, _lib.stream_ptr())
+        if self._band is not None:
+            _lib.call("sfa_band_solve", _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, ax,
+                      _lib.ptr(data), self.n, data.shape[1] * data.shape[2], 1, self._band[0],
+                      self._band[1], _lib.stream_ptr())
+        else:
+            _lib.call(self._solve_fn, _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, ax,
+                      _lib.ptr(data), self.n, data.shape[1] * data.shape[2], 1, _lib.stream_ptr())
         if self.bc_mats:
             self.bc_mats[0].pmat.space.set_boundary_dofs(u, final=True)
         return u

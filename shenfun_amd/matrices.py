@@ -524,6 +524,12 @@ def helmholtz_tpmatrices(T, alpha=0.0, test=None):
         s = [1] * T.dimensions
         s[i] = len(k)
         k2 = k2 + (k ** 2).reshape(s)
+    if getattr(S, "kind", None) in ("DN", "ND"):
+        # mixed bases: (phi_k, phi_j'') and the mass matrix by exact algebra
+        # (Legendre; Chebyshev raises: its Galerkin stiffness is not banded)
+        A = GalerkinMatrix(S, S, 2, f"A{S.kind}{S.kind}mat")
+        B = GalerkinMatrix(S, S, 0, f"B{S.kind}{S.kind}mat")
+        return [TPMatrix(A, T, np.full(shape, -1.0), ax), TPMatrix(B, T, alpha + k2, ax)]
     A = stiffness_matrix(S)
     B = mass_matrix(S)
     # Chebyshev: -lap -> -(phi'', phi)_w = -A ; Legendre: (phi', phi') = +A

@@ -1,0 +1,69 @@
+"""Host side of the mixed Dirichlet/Neumann bases and of the general banded
+fallback (no GPU): the stencils against the boundary conditions they encode,
+the Galerkin matrices' band structure, solver selection and the C-ABI's
+argument checks."""
+import numpy as np
+import pytest
+
+import shenfun_amd as sfa
+from shenfun_amd import _lib
+
+DN = {"left": {"D": 0}, "right": {"N": 0}}
+ND = {"left": {"N": 0}, "right": {"D": 0}}
+
+
+@pytest.mark.parametrize("fam", ["C", "L"])
+@pytest.mark.parametrize("bc,kind", [(DN, "DN"), (ND, "ND")])
+def test_mixed_stencils_satisfy_their_conditions(fam, bc, kind):
+    """DN: phi_k(-1) = phi_k'(1) = 0; ND: phi_k'(-1) = phi_k(1) = 0, for every
+    k < N - 2, each phi_k of exact degree k + 2 (numpy polynomial classes)."""
+    V = sfa.FunctionSpace(16, fam, bc=bc)
+    assert V.dim() == 14 and V.short_name() == kind
+    P = np.polynomial.Chebyshev if fam == "C" else np.polynomial.Legendre
+    for row in V.stencil_matrix()[:V.dim()]:
+        p = P(row)
+        left, right = (p(-1.0), p.deriv()(1.0)) if kind == "DN" else (p.deriv()(-1.0), p(1.0))
+        assert abs(left) < 1e-12 and abs(right) < 1e-11
+        assert row[np.nonzero(row)[0][-1]] != 0
+
+
+@pytest.mark.parametrize("bc", [DN, ND])
+def test_mixed_legendre_galerkin_matrices(bc):
+    """Legendre: (phi_k, phi_j'') is diagonal (the bases are built for it) and
+    equals -(phi_k', phi_j') by quadrature; the mass matrix has bands -2..2,
+    symmetric, and goes to the general banded solver."""
+    V = sfa.FunctionSpace(14, "L", bc=bc)
+    A = sfa.matrices.GalerkinMatrix(V, V, 2, "A")
+    B = sfa.matrices.GalerkinMatrix(V, V, 0, "B")
+    assert sorted(A.keys()) == [0]
+    assert sorted(B.keys()) == [-2, -1, 0, 1, 2]
+    x, w = np.polynomial.legendre.leggauss(40)
+    S = V.stencil_matrix()[:V.dim()]
+    Vx = np.polynomial.legendre.legvander(x, V.N - 1)
+    D1 = np.stack([np.polynomial.legendre.legval(x, np.polynomial.legendre.legder(r)) for r in S])
+    phi = Vx @ S.T
+    assert np.allclose(A.diags().toarray(), -(D1 * w) @ D1.T, rtol=0, atol=1e-10)
+    assert np.allclose(B.diags().toarray(), (phi.T * w) @ phi, rtol=0, atol=1e-12)
+    assert B.get_solver() is sfa.la.Solve
+
+
+def test_mixed_chebyshev_stiffness_refused():
+    V = sfa.FunctionSpace(12, "C", bc=DN)
+    T = sfa.TensorProductSpace(None, (V, sfa.FunctionSpace(8, "F", dtype="d")))
+    with pytest.raises(NotImplementedError):
+        sfa.helmholtz_tpmatrices(T, 1.0)
+    with pytest.raises(NotImplementedError):
+        sfa.FunctionSpace(12, "L", bc={"left": {"D": 1.0}, "right": {"N": 0}})
+
+
+def test_general_band_extent_and_abi_checks():
+    assert sfa.la._band_extent([-2, -1, 0, 1, 2]) == (2, 2)
+    assert sfa.la._band_extent([0, 3]) == (0, 3)
+    assert sfa.la._band_extent([-1, 0]) == (1, 1)
+    with pytest.raises(NotImplementedError):
+        sfa.la._band_extent([-5, 0, 1])
+    L = _lib.lib()
+    assert L.sfa_band_lu(None, 8, 5, 1, 1, None) == -1  # nl > 4
+    assert L.sfa_band_lu(None, 8, 1, 1, 1, None) == -1  # null data
+    assert L.sfa_band_solve(None, 0, 1, _lib.arr([8]), _lib.arr([1]), 0, None, 8, 0, 0, 2, 0,
+                            None) == -1  # nu < 1
