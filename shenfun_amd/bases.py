@@ -392,6 +392,10 @@ class _PolyBase(SpectralBase):
         N: P1/2 -+ P2/8 (Chebyshev), P1/2 -+ P2/6 (Legendre)."""
         if self.bc_kind == "D":
             return np.array([[0.5, -0.5, 0.0], [0.5, 0.5, 0.0]])
+        if self.bc_kind == "DN":  # P0 (u(-1)), P0 + P1 (u'(1)): chebyshev/bases.py:1977-1978
+            return np.array([[1.0, 0.0, 0.0], [1.0, 1.0, 0.0]])
+        if self.bc_kind == "ND":  # -P0 + P1 (u'(-1)), P0 (u(1)): legendre/bases.py:1216-1217
+            return np.array([[-1.0, 1.0, 0.0], [1.0, 0.0, 0.0]])
         c = 1.0 / 8.0 if self.family() == "chebyshev" else 1.0 / 6.0
         return np.array([[0.0, 0.5, -c], [0.0, 0.5, c]])
 
@@ -410,7 +414,7 @@ class _PolyBase(SpectralBase):
         P = np.zeros((x.size, self.N))
         for off, v in self.stencil().items():
             P[:, :d] += V[:, off:off + d] * v[None, :]
-        if self.bc_kind in ("D", "N"):
+        if self.bc_kind in ("D", "N", "DN", "ND"):
             # boundary slots: the boundary basis functions carrying the values
             P[:, d:] = self._bc_functions(x)
         return P
@@ -420,9 +424,9 @@ class _PolyBase(SpectralBase):
         return self._vander_at(x, 3, deriv) @ self.bc_basis().T
 
     def _test_functions(self, x):
-        """phi_k(x) = P_k + K2(k) P_{k+2}, k < N-2, shape (len(x), N-2)."""
-        V = self._vander_at(x, self.N)
-        return V[:, :-2] + self.stencil_k2()[None, :] * V[:, 2:]
+        """phi_k(x), k < dim(), shape (len(x), dim()) (the stencil on the
+        orthogonal polynomials)."""
+        return self.evaluate_basis_all(x)[:, :self.dim()]
 
     def bc_matrices(self):
         """Boundary mass and stiffness matrices (N-2, 2): B[k, j] =
@@ -1143,6 +1147,9 @@ class _StencilSpace:
         self._matmul(inp, out, self._synthesis_matrix())
 
     def forward(self, input_array=None, output_array=None, kind=None):
+        if self.kind in ("DN", "ND"):
+            # boundary slots and their mass right-hand side as the D/N bases
+            return super().forward(input_array, output_array, kind)
         if self.kind in ("P2", "P4"):
             raise NotImplementedError(f"{self.kind} is a test space: scalar_product and backward only")
         out = self.scalar_product(input_array, output_array, kind)
@@ -1174,7 +1181,7 @@ class _StencilSpace:
     def _clone(self, N, padding_factor=1, dealias_direct=False):
         if padding_factor != 1 or dealias_direct:
             raise NotImplementedError("dealiasing of the biharmonic / Petrov-Galerkin spaces")
-        return type(self)(N, quad=self.quad, dtype=self.dtype, device=self.device)
+        return type(self)(N, quad=self.quad, bc=self.bc_values, dtype=self.dtype, device=self.device)
 
 
 class ChebyshevDirichletNeumann(_StencilSpace, _Init, ChebyshevBase):
@@ -1278,8 +1285,8 @@ def _parse_bc(bc):
 
 
 def _mixed_bc(bc):
-    """'DN' for {'left': {'D': 0}, 'right': {'N': 0}}, 'ND' for the mirror
-    (DirichletNeumann / NeumannDirichlet), None otherwise; homogeneous only."""
+    """('DN', (a, b)) for {'left': {'D': a}, 'right': {'N': b}}, ('ND', (a, b))
+    for the mirror (DirichletNeumann / NeumannDirichlet), None otherwise."""
     if not isinstance(bc, dict) or set(bc) != {"left", "right"} or \
             any(not isinstance(bc[s], dict) or len(bc[s]) != 1 for s in bc):
         return None
@@ -1287,9 +1294,7 @@ def _mixed_bc(bc):
     (kr, vr), = bc["right"].items()
     if {kl, kr} != {"D", "N"}:
         return None
-    if any(not (isinstance(_bcval(v), float) and _bcval(v) == 0) for v in (vl, vr)):
-        raise NotImplementedError("nonhomogeneous mixed Dirichlet/Neumann values")
-    return kl + kr
+    return kl + kr, (_bcval(vl), _bcval(vr))
 
 
 def _bc_kind(bc):
@@ -1321,10 +1326,10 @@ def FunctionSpace(N, family="Fourier", bc=None, dtype="d", quad=None, domain=Non
                ("chebyshev", "ND"): ChebyshevNeumannDirichlet, ("c", "ND"): ChebyshevNeumannDirichlet,
                ("legendre", "DN"): LegendreDirichletNeumann, ("l", "DN"): LegendreDirichletNeumann,
                ("legendre", "ND"): LegendreNeumannDirichlet, ("l", "ND"): LegendreNeumannDirichlet}
-        if (fam, mixed) not in cls:
+        if (fam, mixed[0]) not in cls:
             raise NotImplementedError(f"mixed boundary conditions on {family!r}")
-        return cls[(fam, mixed)](N, quad=quad or ("GC" if fam in ("chebyshev", "c") else "LG"),
-                                 device=device)
+        return cls[(fam, mixed[0])](N, quad=quad or ("GC" if fam in ("chebyshev", "c") else "LG"),
+                                    bc=mixed[1], device=device)
     kind, vals = _parse_bc(bc)
     if fam in ("chebyshev", "c"):
         if quad not in (None, "GC", "GL"):

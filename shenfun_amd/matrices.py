@@ -529,7 +529,12 @@ def helmholtz_tpmatrices(T, alpha=0.0, test=None):
         # (Legendre; Chebyshev raises: its Galerkin stiffness is not banded)
         A = GalerkinMatrix(S, S, 2, f"A{S.kind}{S.kind}mat")
         B = GalerkinMatrix(S, S, 0, f"B{S.kind}{S.kind}mat")
-        return [TPMatrix(A, T, np.full(shape, -1.0), ax), TPMatrix(B, T, alpha + k2, ax)]
+        mats = [TPMatrix(A, T, np.full(shape, -1.0), ax), TPMatrix(B, T, alpha + k2, ax)]
+        if S.has_nonhomogeneous_bcs:
+            # (phi_k, psi_j'') and (phi_k, psi_j) of the boundary functions psi_j
+            mats += [TPMatrix(BCMatrix(S, "A", test=S), T, np.full(shape, -1.0), ax),
+                     TPMatrix(BCMatrix(S, "B", test=S), T, alpha + k2, ax)]
+        return mats
     A = stiffness_matrix(S)
     B = mass_matrix(S)
     # Chebyshev: -lap -> -(phi'', phi)_w = -A ; Legendre: (phi', phi') = +A

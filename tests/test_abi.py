@@ -138,7 +138,7 @@ def test_functionspace_factory():
     W = sfa.FunctionSpace(8, "legendre", bc={"left": {"N": 0.5}, "right": {"N": 0}})
     assert W.bc_kind == "N" and W.bc_values == (0.5, 0.0)
     assert not sfa.FunctionSpace(8, "chebyshev", bc=(0, 0)).has_nonhomogeneous_bcs
-    with pytest.raises(NotImplementedError):
-        sfa.FunctionSpace(8, "chebyshev", bc={"left": {"D": 1}, "right": {"N": 0}})
+    M = sfa.FunctionSpace(8, "chebyshev", bc={"left": {"D": 1}, "right": {"N": 0}})
+    assert M.bc_kind == "DN" and M.bc_values == (1.0, 0.0) and M.has_nonhomogeneous_bcs
     with pytest.raises(NotImplementedError):
         sfa.FunctionSpace(8, "laguerre")

@@ -133,3 +133,55 @@ def test_mixed_legendre_helmholtz3d(sfa):
     f_hat = T.scalar_product(dev(np.broadcast_to(f, T.shape(False)).copy()))
     u_hat = sfa.la.SolverGeneric1ND(sfa.helmholtz_tpmatrices(T, alpha))(f_hat, sfa.Function(T))
     assert rel(T.backward(u_hat), np.broadcast_to(u, T.shape(False))) < 1e-11
+
+
+@pytest.mark.parametrize("fam", ["C", "L"])
+@pytest.mark.parametrize("kind", ["DN", "ND"])
+def test_mixed_nonhomogeneous_1d(sfa, fam, kind):
+    """Any polynomial of degree < N lies in the space with its own boundary
+    values: forward puts them in the slots and recovers the rest, backward
+    and eval give the polynomial back."""
+    print(f"[mixed nonhomogeneous 1d {fam} {kind}]", flush=True)
+    N = 20
+    P = np.polynomial.Chebyshev if fam == "C" else np.polynomial.Legendre
+    p = P(np.random.default_rng(5).standard_normal(N))
+    cond = {"D": lambda e: p(e), "N": lambda e: p.deriv()(e)}
+    vals = (cond[kind[0]](-1.0), cond[kind[1]](1.0))
+    V = sfa.FunctionSpace(N, fam, bc={"left": {kind[0]: vals[0]}, "right": {kind[1]: vals[1]}})
+    x = np.asarray(V.mesh())
+    c = V.forward(dev(p(x)))
+    assert np.allclose(c[N - 2:].cpu().numpy(), vals)
+    assert rel(V.backward(c), p(x)) < 1e-12
+    xs = np.random.default_rng(6).uniform(-1, 1, 25)
+    assert rel(V.eval(xs, c), p(xs)) < 1e-11
+
+
+@pytest.mark.parametrize("kind", ["DN", "ND"])
+def test_mixed_nonhomogeneous_helmholtz2d(sfa, kind):
+    """alpha u - lap u = f on Legendre-DN/ND(a, b) x R2C with
+    u = g(x) + q(x) cos(2y): g carries the boundary values, q is homogeneous;
+    the boundary matrices move g's part to the right-hand side."""
+    print(f"[mixed nonhomogeneous helmholtz2d {kind}]", flush=True)
+    a, b = 0.8, -0.6
+    alpha = 1.5
+    SD = sfa.FunctionSpace(24, "L", bc={"left": {kind[0]: a}, "right": {kind[1]: b}})
+    T = sfa.TensorProductSpace(None, (SD, sfa.FunctionSpace(16, "F", dtype="d")))
+    x, y = T.mesh()
+    Pn = np.polynomial.Polynomial
+    if kind == "DN":
+        g = Pn([a + b, b]) + 0.2 * Pn([3, 2, -1])  # g(-1) = a, g'(1) = b
+        q = Pn([3, 2, -1])
+    else:
+        g = Pn([b - a, a]) + 0.2 * Pn([3, -2, -1])  # g'(-1) = a, g(1) = b
+        q = Pn([3, -2, -1])
+    cy = np.cos(2 * y)
+    u = g(x) + q(x) * cy
+    f = alpha * u - (g.deriv(2)(x) + q.deriv(2)(x) * cy - 4 * q(x) * cy)
+    f_hat = T.scalar_product(dev(np.broadcast_to(f, T.shape(False)).copy()))
+    mats = sfa.helmholtz_tpmatrices(T, alpha)
+    assert len(mats) == 4
+    sol = sfa.la.SolverGeneric1ND(mats)
+    u_hat = sol(f_hat, sfa.Function(T))
+    ue = np.broadcast_to(u, T.shape(False))
+    assert rel(T.backward(u_hat), ue) < 1e-11
+    assert rel(T.forward(dev(ue.copy())), u_hat.cpu().numpy()) < 1e-11

@@ -52,8 +52,6 @@ def test_mixed_chebyshev_stiffness_refused():
     T = sfa.TensorProductSpace(None, (V, sfa.FunctionSpace(8, "F", dtype="d")))
     with pytest.raises(NotImplementedError):
         sfa.helmholtz_tpmatrices(T, 1.0)
-    with pytest.raises(NotImplementedError):
-        sfa.FunctionSpace(12, "L", bc={"left": {"D": 1.0}, "right": {"N": 0}})
 
 
 def test_general_band_extent_and_abi_checks():
@@ -67,3 +65,19 @@ def test_general_band_extent_and_abi_checks():
     assert L.sfa_band_lu(None, 8, 1, 1, 1, None) == -1  # null data
     assert L.sfa_band_solve(None, 0, 1, _lib.arr([8]), _lib.arr([1]), 0, None, 8, 0, 0, 2, 0,
                             None) == -1  # nu < 1
+
+
+@pytest.mark.parametrize("fam", ["C", "L"])
+@pytest.mark.parametrize("kind", ["DN", "ND"])
+def test_mixed_boundary_functions(fam, kind):
+    """Boundary functions: slot 0 carries the left condition, slot 1 the right
+    one, each with the other end's condition zero (DN: P0, P0 + P1; ND:
+    -P0 + P1, P0, the reference's phi_{N-2}, phi_{N-1})."""
+    V = sfa.FunctionSpace(10, fam, bc={"left": {kind[0]: 0.7}, "right": {kind[1]: -0.4}})
+    assert V.bc_values == (0.7, -0.4) and V.has_nonhomogeneous_bcs
+    P = np.polynomial.Chebyshev if fam == "C" else np.polynomial.Legendre
+    cond = {"D": lambda p, e: p(e), "N": lambda p, e: p.deriv()(e)}
+    for j, row in enumerate(V.bc_basis()):
+        p = P(row)
+        assert np.isclose(cond[kind[0]](p, -1.0), 1.0 if j == 0 else 0.0)
+        assert np.isclose(cond[kind[1]](p, 1.0), 1.0 if j == 1 else 0.0)
