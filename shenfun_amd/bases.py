@@ -1305,9 +1305,12 @@ def FunctionSpace(N, family="Fourier", bc=None, dtype="d", quad=None, domain=Non
                   **kw):
     """shenfun.FunctionSpace (forms/arguments.py:30) for the hot-path spaces."""
     fam = family.lower()
-    if domain is not None and fam not in ("fourier", "f"):
-        if tuple(float(d) for d in domain) != (-1.0, 1.0):
-            raise NotImplementedError("only the reference domain [-1, 1] is on the hot path")
+    if domain is not None:
+        ref = (0.0, 2 * PI) if fam in ("fourier", "f") else (-1.0, 1.0)
+        if not np.allclose([float(d) for d in domain], ref, rtol=0, atol=1e-14):
+            # other domains scale the wavenumbers / derivatives (domain_factor);
+            # refused rather than silently solved on the reference domain
+            raise NotImplementedError(f"only the reference domain {ref} is on the hot path")
     pad = dict(padding_factor=kw.get("padding_factor", 1),
                dealias_direct=kw.get("dealias_direct", False))
     if fam in ("fourier", "f"):

@@ -81,3 +81,14 @@ def test_mixed_boundary_functions(fam, kind):
         p = P(row)
         assert np.isclose(cond[kind[0]](p, -1.0), 1.0 if j == 0 else 0.0)
         assert np.isclose(cond[kind[1]](p, 1.0), 1.0 if j == 1 else 0.0)
+
+
+def test_non_reference_domains_refused():
+    """A domain other than the reference one would scale wavenumbers and
+    derivatives; it is refused instead of silently ignored."""
+    assert isinstance(sfa.FunctionSpace(8, "F", dtype="d", domain=(0, 2 * np.pi)), sfa.R2C)
+    assert sfa.FunctionSpace(8, "C", domain=(-1, 1)).N == 8
+    with pytest.raises(NotImplementedError):
+        sfa.FunctionSpace(8, "F", dtype="D", domain=(0, 1))
+    with pytest.raises(NotImplementedError):
+        sfa.FunctionSpace(8, "L", bc=(0, 0), domain=(0, 1))
