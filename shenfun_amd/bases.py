@@ -246,7 +246,8 @@ class SpectralBase(_Planned):
         if u.dim() != 1 or u.shape[0] != (self.shape(True) if isinstance(self, FourierBase)
                                           else self.N):
             raise ValueError("eval takes the 1D coefficient array of this space")
-        real = not isinstance(self, C2C) and self.dtype.kind != "c"
+        # real values for R2C, and for real coefficients of a polynomial space
+        real = isinstance(self, R2C) or (not isinstance(self, FourierBase) and not u.is_complex())
         return tensor_eval(u, [self._eval_matrix(x)], x.size, real, output_array)
 
     @property

@@ -198,7 +198,9 @@ class TensorProductSpace(_Planned):
         if not isinstance(c, torch.Tensor):
             c = torch.from_numpy(np.ascontiguousarray(c)).to(self.device)
         mats = [b._eval_matrix(b.map_reference_domain(pts[i])) for i, b in enumerate(self.bases)]
-        return tensor_eval(c, mats, npts, self.physical_dtype() == torch.float64, output_array)
+        real = any(isinstance(b, R2C) for b in self.bases) or \
+            (self.physical_dtype() == torch.float64 and not c.is_complex())
+        return tensor_eval(c, mats, npts, real, output_array)
 
 
 def Array(space, buffer=None, val=None):

@@ -130,3 +130,17 @@ def test_eval_tps_at_mesh_equals_backward(sfa):
     u_hat = T.forward(u)
     X = [np.broadcast_to(m, T.shape(False)).ravel() for m in T.mesh()]
     assert rel(T.eval(np.stack(X), u_hat), u.cpu().numpy().ravel()) < 1e-12
+
+
+def test_eval_complex_coefficients_on_polynomial_space(sfa):
+    """Complex coefficients of a Chebyshev space (a line of a tensor-product
+    array) keep their imaginary part."""
+    print("[eval complex poly]", flush=True)
+    V = sfa.FunctionSpace(12, "C", bc=(0, 0))
+    rng = np.random.default_rng(4)
+    c = np.zeros(12, complex)
+    c[:10] = rng.standard_normal(10) + 1j * rng.standard_normal(10)
+    xs = rng.uniform(-1, 1, 30)
+    ref = V.evaluate_basis_all(xs) @ c
+    out = V.eval(xs, dev(c))
+    assert out.dtype == torch.complex128 and rel(out, ref) < 1e-12
