@@ -340,7 +340,7 @@ class _PolyBase(SpectralBase):
 
     # ------------------------------------------------------------ transforms
     def _clone(self, N, padding_factor=1, dealias_direct=False):
-        return type(self)(N, quad=self.quad, bc=self.bc_values, dtype=self.dtype,
+        return type(self)(N, quad=self.quad, bc=self._bc_orig, dtype=self.dtype,
                           device=self.device, padding_factor=padding_factor,
                           dealias_direct=dealias_direct)
 
@@ -382,8 +382,10 @@ class _PolyBase(SpectralBase):
     # ------------------------------------------------------ boundary values
     @property
     def has_nonhomogeneous_bcs(self):
+        # decided by the values as given: a time-dependent value that happens
+        # to vanish at the current time still counts
         return self.is_composite and any(not (isinstance(v, float) and v == 0)
-                                         for v in self.bc_values)
+                                         for v in getattr(self, "_bc_orig", self.bc_values))
 
     def bc_basis(self):
         """Orthogonal coefficients (rows: left, right function) of the boundary
@@ -1015,15 +1017,51 @@ def _poly(cls_base, bc_kind, name):
     return type(name, (cls_base,), {"bc_kind": bc_kind})
 
 
+class BoundaryValues:
+    """``space.bc`` (BoundaryValues.update, tensorproductspace.py:1859-1907):
+    ``update(time)`` substitutes ``time`` for the sympy symbol t in the
+    boundary values as given and makes the results the current values; the
+    boundary slots of later forward transforms, Functions and solves use them
+    (cached slot arrays of varying values are dropped).  Without ``time`` the
+    last time is kept (initially 0)."""
+
+    def __init__(self, base):
+        self.base = base
+        self.bc_time = 0.0
+
+    def update(self, time=None, update_tensor=False):
+        vals = []
+        changed = bool(update_tensor)
+        if time is not None:
+            self.bc_time = float(time)
+        for v in self.base._bc_orig:
+            if hasattr(v, "free_symbols"):
+                ts = [q for q in v.free_symbols if q.name == "t"]
+                if ts:
+                    v = v.subs(ts[0], self.bc_time)
+                    changed = True
+            vals.append(_bcval(v))
+        self.base.bc_values = tuple(vals)
+        if changed:
+            cache = self.base._cache
+            for k in [k for k in cache if isinstance(k, tuple) and k[:1] == ("bcslot",)]:
+                del cache[k]
+        return self.base.bc_values
+
+
 class _Init:
     def __init__(self, N, quad=None, bc=None, dtype=np.float64, device=None, padding_factor=1,
                  dealias_direct=False, **kw):
         self.quad = quad
         # boundary values (left, right) of a composite space; a string or
         # dict bc is parsed by FunctionSpace
-        self.bc_values = tuple(_bcval(v) for v in bc) if isinstance(bc, (tuple, list)) \
+        self._bc_orig = tuple(_bcval(v) for v in bc) if isinstance(bc, (tuple, list)) \
             else (0.0, 0.0)
+        self.bc_values = self._bc_orig
         SpectralBase.__init__(self, N, dtype, device, padding_factor, dealias_direct)
+        # time-dependent values (sympy expressions in t) start at t = 0
+        self.bc = BoundaryValues(self)
+        self.bc.update()
         if self.is_padded and quad == "GL":
             raise NotImplementedError("padding is supported on Gauss (GC / LG) points")
 
@@ -1182,7 +1220,7 @@ class _StencilSpace:
     def _clone(self, N, padding_factor=1, dealias_direct=False):
         if padding_factor != 1 or dealias_direct:
             raise NotImplementedError("dealiasing of the biharmonic / Petrov-Galerkin spaces")
-        return type(self)(N, quad=self.quad, bc=self.bc_values, dtype=self.dtype, device=self.device)
+        return type(self)(N, quad=self.quad, bc=self._bc_orig, dtype=self.dtype, device=self.device)
 
 
 class ChebyshevDirichletNeumann(_StencilSpace, _Init, ChebyshevBase):

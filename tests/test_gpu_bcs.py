@@ -161,3 +161,35 @@ def test_bc_varying_along_fourier_axes(sfa, form):
     ue = np.broadcast_to(u, T.shape(False))
     assert rel(T.backward(u_hat), ue) < 1e-11
     assert rel(T.forward(dev(ue.copy())), u_hat.cpu().numpy()) < 1e-11
+
+
+def test_bc_time_dependent_helmholtz2d(sfa):
+    """u(x, y, t) = a(t, y) (1 - x)/2 + b(t) (1 + x)/2 + q(x) cos(2y) on
+    Chebyshev-D x R2C with sympy boundary values in t (and y): one solver,
+    built once, recovers u at several times after bc.update(t) (the
+    RayleighBenard demo pattern, demo/RayleighBenard.py:80)."""
+    print("[bc time-dependent]", flush=True)
+    import sympy as sp
+    t, y = sp.symbols("t,y", real=True)
+    aS = sp.sin(t) * sp.cos(y) + 0.5
+    bS = sp.cos(2 * t)
+    alpha = 2.0
+    SD = sfa.FunctionSpace(20, "C", bc=(aS, bS))
+    T = sfa.TensorProductSpace(None, (SD, sfa.FunctionSpace(16, "F", dtype="d")))
+    X = T.mesh()
+    x, Y = X
+    q = np.polynomial.Polynomial([0, 0, 1, 0, -1])
+    H = sfa.la.Helmholtz(*sfa.helmholtz_tpmatrices(T, alpha))
+    for tt in (0.0, 0.7, 1.9):
+        SD.bc.update(tt)
+        a = np.sin(tt) * np.cos(Y) + 0.5
+        b = np.cos(2 * tt)
+        u = a * (1 - x) / 2 + b * (1 + x) / 2 + q(x) * np.cos(2 * Y)
+        lap = -np.sin(tt) * np.cos(Y) * (1 - x) / 2 + q.deriv(2)(x) * np.cos(2 * Y) \
+            - 4 * q(x) * np.cos(2 * Y)
+        f = alpha * u - lap
+        f_hat = T.scalar_product(dev(np.broadcast_to(f, T.shape(False)).copy()))
+        u_hat = H(f_hat, sfa.Function(T))
+        ue = np.broadcast_to(u, T.shape(False))
+        assert rel(T.backward(u_hat), ue) < 1e-11, tt
+        assert rel(T.forward(dev(ue.copy())), u_hat.cpu().numpy()) < 1e-11, tt

@@ -92,3 +92,20 @@ def test_non_reference_domains_refused():
         sfa.FunctionSpace(8, "F", dtype="D", domain=(0, 1))
     with pytest.raises(NotImplementedError):
         sfa.FunctionSpace(8, "L", bc=(0, 0), domain=(0, 1))
+
+
+def test_time_dependent_boundary_values_update():
+    """bc.update(t) substitutes t in the values as given (BoundaryValues.update);
+    a value vanishing at the current time still makes the space nonhomogeneous."""
+    import sympy as sp
+    t, y = sp.symbols("t,y", real=True)
+    V = sfa.FunctionSpace(10, "C", bc=(sp.sin(t), 2 * sp.cos(t)))
+    assert V.bc_values == (0.0, 2.0) and V.has_nonhomogeneous_bcs
+    assert np.allclose(V.bc.update(0.5), (np.sin(0.5), 2 * np.cos(0.5)))
+    assert np.allclose(V.bc.update(), (np.sin(0.5), 2 * np.cos(0.5)))  # time kept
+    W = sfa.FunctionSpace(10, "L", bc=(sp.sin(t) * sp.cos(y), 0))
+    assert W.bc_values == (0.0, 0.0) and W.has_nonhomogeneous_bcs
+    a, b = W.bc.update(1.0)
+    assert b == 0.0 and sp.simplify(a - np.sin(1.0) * sp.cos(y)) == 0
+    # a clone (dealiased / test spaces) starts from the values as given
+    assert V._clone(10).bc_values == (0.0, 2.0)
