@@ -1218,9 +1218,12 @@ class _StencilSpace:
         return cls(self.N + 4, quad=self.quad, dtype=self.dtype, device=self.device)
 
     def _clone(self, N, padding_factor=1, dealias_direct=False):
-        if padding_factor != 1 or dealias_direct:
-            raise NotImplementedError("dealiasing of the biharmonic / Petrov-Galerkin spaces")
-        return type(self)(N, quad=self.quad, bc=self._bc_orig, dtype=self.dtype, device=self.device)
+        if (padding_factor != 1 or dealias_direct) and self.kind in ("P2", "P4"):
+            raise NotImplementedError("dealiasing of the Petrov-Galerkin test spaces")
+        # the stencils depend on k only, so a padded space runs on an
+        # unpadded twin of the padded length like the D/N bases
+        return type(self)(N, quad=self.quad, bc=self._bc_orig, dtype=self.dtype, device=self.device,
+                          padding_factor=padding_factor, dealias_direct=dealias_direct)
 
 
 class ChebyshevDirichletNeumann(_StencilSpace, _Init, ChebyshevBase):
