@@ -6,7 +6,8 @@ A drop-in for shenfun's per-axis transforms (``forward``/``backward``/
 all compute in hand-written HIP kernels for gfx950 behind the C ABI of
 ``include/shenfun_amd.h``.  torch-ROCm tensors are the buffer type.
 """
-from . import la  # noqa: F401
+from . import io, la  # noqa: F401
+from .io import Checkpoint  # noqa: F401
 from ._lib import lib as _load_lib  # noqa: F401
 from .bases import (C2C, R2C, ChebyshevOrthogonal, ChebyshevShenDirichlet,  # noqa: F401
                     ChebyshevShenNeumann, FunctionSpace, LegendreOrthogonal,
