@@ -255,6 +255,15 @@ class Solve(_Banded):
     def _call_lu(self, d, n, nbatch):
         _band_lu(d, n, self.nl, self.nu, nbatch)
 
+    def _pin_row0(self):
+        # row 0 := the unit row (SparseMatrixSolver.apply_constraints, la.py:216-249)
+        n = self._data.shape[1]
+        self._data[self.nl, 0] = 1.0
+        for q in range(1, self.nu + 1):
+            if q < n:
+                self._data[self.nl + q, q] = 0.0
+        self._lu = None
+
     def _call_solve(self, u, axis, data, n, dls, skip):
         shp, st = _geom(u)
         _lib.call("sfa_band_solve", _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, axis,

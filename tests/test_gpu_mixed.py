@@ -185,3 +185,21 @@ def test_mixed_nonhomogeneous_helmholtz2d(sfa, kind):
     ue = np.broadcast_to(u, T.shape(False))
     assert rel(T.backward(u_hat), ue) < 1e-11
     assert rel(T.forward(dev(ue.copy())), u_hat.cpu().numpy()) < 1e-11
+
+
+def test_general_band_constraint_row0(sfa):
+    """apply_constraints((0, val)) on the general banded solver: row 0 becomes
+    the unit row and b[0] = val, against a dense solve."""
+    print("[band generic constraint]", flush=True)
+    n = 15
+    rng = np.random.default_rng(9)
+    d = _rand_bands(rng, (-2, -1, 0, 1, 2), n)
+    sol = sfa.la.Solve(sfa.SparseMatrix(d, (n, n)))
+    b = rng.standard_normal((n, 3))
+    u = sol(dev(b), axis=0, constraints=((0, 0.5),))
+    D = S.dense(d, n)
+    D[0, :] = 0.0
+    D[0, 0] = 1.0
+    bb = b.copy()
+    bb[0] = 0.5
+    assert rel(u, np.linalg.solve(D, bb)) < 1e-12
