@@ -332,12 +332,19 @@ int sfa_poisson_add_solve(const void* b, void* u, int dtype, int ndim, const int
  * 17 (kz < 17 in FP64); the value is clamped to [2, Nz/2 - 1].  The error of
  * the complex64 part is about 6e-8 times the largest coefficient above the
  * cut relative to the largest of f, so the cut is a wavenumber, not a
- * fraction of Nz.  Mixed precision needs a plan and
- * power-of-two Nx in {64, 128, 256}, Ny in [16, 512], Nz/2 in [16, 256].
- * work: device workspace of at least sfa_helmholtz3d_workspace() bytes. */
+ * fraction of Nz.  precision 1 applies no check of that error: the guarded
+ * form (sfa_helmholtz3d_guarded) is the safe one.  Mixed precision needs a
+ * plan and power-of-two Nx in {64, 128, 256, 512}, Ny in [16, 512], Nz/2 in
+ * [16, 256].
+ * work: device workspace of at least sfa_helmholtz3d_workspace() bytes (the
+ * complex workspace plus a small statistics area used by the guarded form). */
 int64_t sfa_helmholtz3d_plan_bytes(int64_t Nx, int64_t Ny, int64_t Nz);
 int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double alpha, int neumann,
                          void* plan, int64_t plan_bytes, void* stream);
+/* Forget a plan buffer registered by sfa_helmholtz3d_plan (call before its
+ * memory is freed or reused: a plan is recognised by its address).  No
+ * reference counterpart (chebyshev.la.Helmholtz's factors are Python-owned). */
+int sfa_helmholtz3d_plan_release(const void* plan);
 int64_t sfa_helmholtz3d_workspace(int64_t Nx, int64_t Ny, int64_t Nz, int precision);
 int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                     double alpha, int neumann, int precision, int fp64_planes, const void* plan,
@@ -352,11 +359,35 @@ int sfa_helmholtz3d_many(const double* f, double* u, int64_t nbatch, int64_t Nx,
                          int64_t Nz, double alpha, int neumann, int precision, int fp64_planes,
                          const void* plan, void* work, int64_t work_bytes, void* stream);
 
+/* Guarded mixed precision: the mixed pipeline of sfa_helmholtz3d_many with
+ * the first complex64 plane kh_hint (<= 0: 17), whose z R2C and x-pass also
+ * sum, per band of 16 kz planes, the energy of f's z spectrum and of u's
+ * Chebyshev coefficients.  After the x-pass the host reads these sums (the
+ * inverse passes are already queued) and predicts the relative max error of
+ * the complex64 part: 64 eps32 sqrt((w^2 E_f,hi + E_u,hi) / E_u), E_hi the
+ * energies of the planes at and above the cut, w = 1 / (alpha + Kh^2) the
+ * solve's gain there.  If it exceeds tol the solve is redone from f with the
+ * smallest cut 16m + 1 whose prediction passes, or in FP64 when none does
+ * (f is left unchanged).  Out (each may be NULL): kh_used = first complex64
+ * plane of the returned u (Nz/2 + 1: all FP64), kh_next = the smallest
+ * passing cut for this f (0: none), err_pred = the prediction for kh_used (0
+ * for FP64).  Synchronises with the host once per call, after the x-pass.
+ * work: at least nbatch x the complex workspace + the statistics area
+ * (sfa_helmholtz3d_workspace() bytes for nbatch = 1).  Same reference sequence
+ * as sfa_helmholtz3d; the guard has no reference counterpart (shenfun is FP64
+ * only). */
+int sfa_helmholtz3d_guarded(const double* f, double* u, int64_t nbatch, int64_t Nx, int64_t Ny,
+                            int64_t Nz, double alpha, int neumann, int kh_hint, double tol,
+                            const void* plan, void* work, int64_t work_bytes, int* kh_used,
+                            int* kh_next, double* err_pred, void* stream);
+
 /* Profiling variant of sfa_helmholtz3d: runs the pipeline (precision and
  * fp64_planes as there) nrep times with hipEvents recorded on `stream` between
  * its stages and adds each stage's milliseconds into stage_ms[5] (z R2C, y
  * C2C, x-pass, y C2C inverse, z C2R; in mixed precision a stage covers its
- * complex128 and complex64 launches).  Synchronises the stream.  No reference
+ * complex128 and complex64 launches; precision 2 times the guarded form's
+ * kernels, with their statistics, at the cut fp64_planes and without the
+ * host check).  Synchronises the stream.  No reference
  * counterpart: it exists to attribute the pipeline's time to its kernels. */
 int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                                 double alpha, int neumann, int precision, int fp64_planes,

@@ -60,6 +60,8 @@ _SIGS = {
     "sfa_helmholtz3d": "vvllldiiivvlv",
     "sfa_helmholtz3d_many": "vvlllldiiivvlv",
     "sfa_helmholtz3d_stage_times": "vvllldiiivvlivv",
+    "sfa_helmholtz3d_guarded": "vvlllldiidvvlvvvv",
+    "sfa_helmholtz3d_plan_release": "v",
     "sfa_batch2d_plan": "ivvvvvvllivlv",
     "sfa_batch2d_solve": "vvvvliv",
     "sfa_batch2d_release": "v",

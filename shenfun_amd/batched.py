@@ -228,8 +228,22 @@ class BatchedHelmholtz2D:
         name = torch.cuda.get_device_properties(self.device).name
         return f"{name}|{self.nprob}|{int(self.total)}|" + ",".join(map(str, h))
 
+    def algorithmic_flops(self):
+        """Algorithmic FP64 flops per batch: the Legendre analysis and
+        synthesis contractions over the Ny + 2 real columns of the R2C
+        spectrum, 2 (Nx - 2) Nx (Ny + 2) each (the recursive transforms of
+        transforms.pyx:219-323 do this work), plus a real FFT pair along y,
+        2.5 Ny log2(Ny) per line and direction.  The kernel's dense DFT
+        contractions along y do more (``flops``)."""
+        nx = self.Nx.astype(float)
+        ny = self.Ny.astype(float)
+        leg = 4.0 * (nx - 2) * nx * (ny + 2)
+        fft = 2 * 2.5 * nx * ny * np.log2(ny)
+        return float(np.sum(leg + fft))
+
     def flops(self):
-        """FP64 flops of the four contractions per batch (2 per multiply-add)."""
+        """FP64 flops the kernel executes per batch: the four contractions
+        (two Legendre, two dense real DFT), 2 per multiply-add."""
         nx = self.Nx.astype(float)
         ny = self.Ny.astype(float)
         m = nx - 2

@@ -5,6 +5,17 @@
 
 namespace sfa {
 
+// Spectrum statistics of the guarded mixed-precision solve: per band of 16
+// kz planes (band 0 = kz 0..16, band b >= 1 = kz 16b+1 .. 16b+16, so the
+// band edges are the admissible first complex64 planes 16m+1), the energy of
+// f's z spectrum (kStatBands accumulators) followed by that of u's Chebyshev
+// coefficients (kStatBands more), each accumulator kStatSpread doubles apart
+// so that concurrent atomics of different bands go to different cache lines.
+constexpr int kStatBands = 16;
+constexpr int kStatSpread = 32;
+constexpr int kStatBytes = 2 * kStatBands * kStatSpread * 8;
+__host__ __device__ constexpr int stat_band(int kz) { return kz == 0 ? 0 : (kz - 1) / 16; }
+
 struct FFTPlan {
     int nfft;       // complex FFT length
     int nst;        // number of stages

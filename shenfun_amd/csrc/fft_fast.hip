@@ -407,11 +407,16 @@ struct XPass2Args {
     XPlan pl;
     const double2* tw;   // exp(-2 pi i m/N)
     const double2* tq;   // exp(-i pi k/(2N))
+    double* stats;       // STATS kernels: band energies of the solution's coefficients
 };
 
 // C2 = double2: FP64 everywhere.  C2 = float2: complex64 storage and DCT/FFT
 // arithmetic, the Helmholtz solve still FP64 (values widened on entry).
-template <int N, typename C2>
+// STATS: also accumulate, per band of 16 kz planes, the energy of the
+// solution's Chebyshev coefficients t (after to_ortho),
+// w_kz (|t_0|^2 + 1/2 sum_{k>=1} |t_k|^2) per line (w_kz = 2 for 0 < kz < Nz/2,
+// else 1), into a.stats[(kStatBands + band) * kStatSpread].
+template <int N, typename C2, bool STATS = false>
 __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
 {
     using Pl = Pow2<N>;
@@ -562,6 +567,25 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         }
         v[r] = cc;
     }
+    if (STATS) {
+        double el = 0.0;
+#pragma unroll
+        for (int r = 0; r < P; ++r) {
+            const double2 c = to_c64(v[r]);
+            el += ((jb + Q * r) == 0 ? 1.0 : 0.5) * (c.x * c.x + c.y * c.y);
+        }
+#pragma unroll
+        for (int off = Q / 2; off >= 1; off >>= 1) el += __shfl_xor(el, off);
+        const int64_t Ls = bid * TL + tb;
+        if (jb == 0 && Ls < a.nl) {
+            const int64_t Lr = Ls % a.nlb;
+            const int64_t ky = Lr / a.Kp, kz = a.kz0 + (Lr - ky * a.Kp);
+            if (kz < a.Nzh) {
+                const double wz = (kz == 0 || kz == a.Nzh - 1) ? 1.0 : 2.0;
+                atomicAdd(a.stats + (kStatBands + stat_band((int)kz)) * kStatSpread, wz * el);
+            }
+        }
+    }
     wave_sync();
 #pragma unroll
     for (int r = 0; r < P; ++r) ln[pd(jb + Q * r)] = v[r];
@@ -608,15 +632,15 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_kerne
 // Mixed precision: the complex128 planes' blocks [0, nb64) and the complex64
 // planes' blocks after them in one grid, so the few FP64 blocks run beside the
 // FP32 ones instead of in a launch of their own.
-template <int N>
+template <int N, bool STATS>
 __global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_dual_kernel(XPass2Args a64, XPass2Args a32,
                                                                         int64_t nb64)
 {
     const int64_t b = blockIdx.x;
     if (b < nb64)
-        xpass2_body<N, double2>(a64, b);
+        xpass2_body<N, double2, STATS>(a64, b);
     else
-        xpass2_body<N, float2>(a32, b - nb64);
+        xpass2_body<N, float2, STATS>(a32, b - nb64);
 }
 
 template <int N, typename C2>
@@ -637,7 +661,7 @@ static int launch_xpass2_dual(XPass2Args& a64, XPass2Args& a32, hipStream_t s)
 {
     using Pl = Pow2<N>;
     const size_t lds = XLay<N>::lds(sizeof(double2));
-    auto k = xpass2_dual_kernel<N>;
+    auto k = a64.stats ? xpass2_dual_kernel<N, true> : xpass2_dual_kernel<N, false>;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const int64_t nb64 = (a64.nl + TL - 1) / TL, nb32 = (a32.nl + TL - 1) / TL;
@@ -666,6 +690,7 @@ static bool xpass_args(void* W, int Nx, int Ny, int Nzh, int Kp, int kz0, double
     a->pl = pl;
     a->tw = fft_table(Nx);
     a->tq = quarter_table(Nx);
+    a->stats = nullptr;
     return a->tw && a->tq;
 }
 
@@ -694,7 +719,8 @@ int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, 
 // [Kh, Nzh) complex64 in W32 (pitch Nzh - Kh), one launch; nb arrays back to
 // back in each of W64 and W32.
 int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, double alpha,
-                     int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc)
+                     int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc,
+                     double* stats)
 {
     if (!(Nx == 64 || Nx == 128 || Nx == 256 || Nx == 512) || pl.linemap == nullptr || pl.bs == nullptr)
         return 0;
@@ -704,6 +730,7 @@ int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, doub
         *rc = fail(SFA_ERR_HIP, "table allocation failed");
         return 1;
     }
+    a64.stats = a32.stats = stats;
     switch (Nx) {
         case 64: *rc = launch_xpass2_dual<64>(a64, a32, s); break;
         case 128: *rc = launch_xpass2_dual<128>(a64, a32, s); break;
@@ -729,11 +756,14 @@ namespace sfa {
 // ----------------------------------------------------------------- kz-split z passes
 // R2C of contiguous rows of 2H reals (FP64 arithmetic), output split by kz:
 // k < Kh -> o64 (row pitch Kh, complex128), k >= Kh -> o32 (row pitch H+1-Kh, complex64).
-template <int H>
+// STATS: also accumulate the energy sum_k w_k |X_k|^2 of the output per band
+// of 16 kz planes (w_k = 2 for 0 < k < H, else 1: the R2C half spectrum)
+// into stats[band * kStatSpread]: one vector atomic per wave.
+template <int H, bool STATS>
 __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     r2c_split_kernel(const double* __restrict__ in, double2* __restrict__ o64,
                      float2* __restrict__ o32, int64_t nrows, int Kh, double scale,
-                     const double2* __restrict__ twg)
+                     const double2* __restrict__ twg, double* __restrict__ stats)
 {
     using Pl = Pow2<H>;
     constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
@@ -766,6 +796,9 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 #pragma unroll
         for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<H>(j, h, k2)] = a[h * Q + k2];
     __syncthreads();
+    double en[P + 2];
+#pragma unroll
+    for (int i = 0; i < P + 2; ++i) en[i] = 0.0;
     if (valid) {
 #pragma unroll
         for (int i = 0; i <= P; ++i) {
@@ -777,7 +810,34 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
             const double2 X = make_double2(0.5 * (zk.x + zc.x + d.y) * scale, 0.5 * (zk.y + zc.y - d.x) * scale);
             if (k < Kh) o64[R * Kh + k] = X;
             else o32[R * Kl + (k - Kh)] = from_c64<float2>(X);
+            if (STATS) en[i] = ((k == 0 || k == H) ? 1.0 : 2.0) * (X.x * X.x + X.y * X.y);
         }
+    }
+    if (STATS) {
+        // thread j holds k = j + Q i.  With G = 16 / Q elements per band,
+        // element i lies in band i / G for j >= 1; for j == 0 (k = Q i) in
+        // band (i - 1) / G, so its elements are shifted down by one and k = 0
+        // is added to band 0.
+        constexpr int G = 16 / Q;
+        constexpr int NBL = P / G + 1;
+        constexpr int NB = (H - 1) / 16 + 1;
+        static_assert(NB <= kStatBands && NB <= 64, "band count");
+        double acc[NBL];
+#pragma unroll
+        for (int b = 0; b < NBL; ++b) acc[b] = 0.0;
+#pragma unroll
+        for (int i = 0; i <= P; ++i) acc[i / G] += (j == 0) ? en[i + 1] : en[i];
+        if (j == 0) acc[0] += en[0];
+#pragma unroll
+        for (int b = 0; b < NBL; ++b)
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) acc[b] += __shfl_xor(acc[b], off);
+        const int lane = threadIdx.x & 63;
+        double mine = 0.0;
+#pragma unroll
+        for (int b = 0; b < NBL; ++b)
+            if (lane == b) mine = acc[b];
+        if (lane < NB) atomicAdd(stats + lane * kStatSpread, mine);
     }
 }
 
@@ -879,7 +939,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 
 template <int H, bool SPLIT>
 static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int Kh, double scale,
-                        hipStream_t s)
+                        hipStream_t s, double* stats)
 {
     using Pl = Pow2<H>;
     const double2* tw = fft_table(2 * H);
@@ -887,11 +947,11 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
     const size_t lds = sizeof(double2) * ((size_t)TL * zrow_stride<H>() + H);
     dim3 grid((unsigned)((nrows + TL - 1) / TL)), block(TL * Pl::Q);
     if (SPLIT) {
-        auto k = r2c_split_kernel<H>;
+        auto k = stats ? r2c_split_kernel<H, true> : r2c_split_kernel<H, false>;
         if (lds > 64 * 1024)
             hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double*)in, (double2*)o64, (float2*)o32,
-                           nrows, Kh, scale, tw);
+                           nrows, Kh, scale, tw, stats);
     } else {
         auto k = Kh <= 3 * Pl::Q ? c2r_merge_kernel<H, true> : c2r_merge_kernel<H, false>;
         if (lds > 64 * 1024)
@@ -903,14 +963,15 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
     return SFA_OK;
 }
 
-// split (r2c=true: in -> o64/o32) or merge (r2c=false: o64/o32 -> in) of contiguous rows
+// split (r2c=true: in -> o64/o32) or merge (r2c=false: o64/o32 -> in) of
+// contiguous rows; stats (split only, may be NULL): band energies of the output
 int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
-             double scale, hipStream_t s, int* rc)
+             double scale, hipStream_t s, int* rc, double* stats)
 {
 #define SFA_SP(HH)                                                                            \
     case HH:                                                                                  \
-        *rc = r2c ? launch_split<HH, true>(in, o64, o32, nrows, Kh, scale, s)                 \
-                  : launch_split<HH, false>(in, o64, o32, nrows, Kh, scale, s);               \
+        *rc = r2c ? launch_split<HH, true>(in, o64, o32, nrows, Kh, scale, s, stats)          \
+                  : launch_split<HH, false>(in, o64, o32, nrows, Kh, scale, s, nullptr);      \
         return 1;
     switch (H) {
         SFA_SP(16) SFA_SP(32) SFA_SP(64) SFA_SP(128) SFA_SP(256)

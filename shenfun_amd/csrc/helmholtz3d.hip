@@ -235,7 +235,8 @@ int fast_xpass(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int ne
 int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha,
                    int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc);
 int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, double alpha,
-                     int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc);
+                     int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc,
+                     double* stats);
 int fast_c2c(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
              hipStream_t s, int* rc);
 int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
@@ -243,7 +244,7 @@ int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int 
 int fast_c2c_mixed(const Geom& g64, void* W64, const Geom& g32, void* W32, int sign, double scale,
                    hipStream_t s, int* rc);
 int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
-             double scale, hipStream_t s, int* rc);
+             double scale, hipStream_t s, int* rc, double* stats);
 
 // ------------------------------------------------------------------ plan
 // Device plan of the fused solve, the counterpart of chebyshev.la.Helmholtz's
@@ -406,9 +407,12 @@ static int y_pass_mixed(const Geom& g64, const Geom& g32, double2* W64, float2* 
 // Helmholtz solve are FP64.
 // nb arrays (Nx, Ny, Nz) back to back in f and u; the workspace holds the nb
 // arrays' complex128 planes, then their complex64 planes.
+// stats (may be NULL): zeroed band-energy accumulators (kStatBytes) that the
+// z R2C and the x-pass add to; evx (may be NULL): recorded after the x-pass.
 static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                              double alpha, int neumann, int Kh, const void* plan, void* work,
-                             hipStream_t s, hipEvent_t* ev, int64_t nb = 1)
+                             hipStream_t s, hipEvent_t* ev, int64_t nb = 1, double* stats = nullptr,
+                             hipEvent_t evx = nullptr)
 {
     auto mark = [&](int i) {
         if (ev) hipEventRecord(ev[i], s);
@@ -436,20 +440,21 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
     };
     const Geom g64 = geom(Kh), g32 = geom(Kl);
     mark(0);
-    if (!split_rc(true, (int)(Nz / 2), f, W64, W32, rows, Kh, 1.0 / (double)Nz, s, &rc))
+    if (!split_rc(true, (int)(Nz / 2), f, W64, W32, rows, Kh, 1.0 / (double)Nz, s, &rc, stats))
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz=%lld unsupported", (long long)Nz);
     if (rc) return rc;
     mark(1);
     if ((rc = y_pass_mixed(g64, g32, W64, W32, -1, 1.0 / (double)Ny, s))) return rc;
     mark(2);
     if (!fast_xpass_mixed(W64, W32, (int)Nx, (int)Ny, (int)Nzh, Kh, alpha, neumann, K2, pv, nb, s,
-                          &rc) ||
+                          &rc, stats) ||
         rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nx=%lld unsupported", (long long)Nx);
     mark(3);
+    if (evx) SFA_HIP(hipEventRecord(evx, s));
     if ((rc = y_pass_mixed(g64, g32, W64, W32, +1, 1.0, s))) return rc;
     mark(4);
-    if (!split_rc(false, (int)(Nz / 2), u, W64, W32, rows, Kh, 1.0, s, &rc))
+    if (!split_rc(false, (int)(Nz / 2), u, W64, W32, rows, Kh, 1.0, s, &rc, nullptr))
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz unsupported");
     mark(5);
     return rc;
@@ -517,12 +522,20 @@ static int mixed_planes(int64_t Nz, int fp64_planes)
 
 using namespace sfa;
 
-extern "C" int64_t sfa_helmholtz3d_workspace(int64_t Nx, int64_t Ny, int64_t Nz, int precision)
+// Complex workspace of one array (FP64 layout at its padded pitch; the mixed
+// layout is smaller); the guarded solve keeps its statistics behind the nb
+// arrays' workspaces.
+static int64_t core_workspace(int64_t Nx, int64_t Ny, int64_t Nz)
 {
-    (void)precision;
     const int64_t Nzh = Nz / 2 + 1;
     const int64_t Kp = Nzh >= 64 ? (Nzh + 7) / 8 * 8 : Nzh;  // covers fp64_pitch()
     return Nx * Ny * Kp * (int64_t)sizeof(double2);
+}
+
+extern "C" int64_t sfa_helmholtz3d_workspace(int64_t Nx, int64_t Ny, int64_t Nz, int precision)
+{
+    (void)precision;
+    return core_workspace(Nx, Ny, Nz) + kStatBytes;
 }
 
 extern "C" int64_t sfa_helmholtz3d_plan_bytes(int64_t Nx, int64_t Ny, int64_t Nz)
@@ -564,6 +577,13 @@ extern "C" int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double a
     return SFA_OK;
 }
 
+extern "C" int sfa_helmholtz3d_plan_release(const void* plan)
+{
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    g_plans.erase(plan);
+    return SFA_OK;
+}
+
 extern "C" int sfa_helmholtz3d_many(const double* f, double* u, int64_t nbatch, int64_t Nx,
                                     int64_t Ny, int64_t Nz, double alpha, int neumann,
                                     int precision, int fp64_planes, const void* plan, void* work,
@@ -576,8 +596,7 @@ extern "C" int sfa_helmholtz3d_many(const double* f, double* u, int64_t nbatch, 
     SFA_CHECK(nbatch >= 1, SFA_ERR_ARG, "nbatch must be >= 1");
     SFA_CHECK(f != nullptr && u != nullptr, SFA_ERR_ARG, "NULL f or u");
     SFA_CHECK(precision == 0 || precision == 1, SFA_ERR_ARG, "precision must be 0 or 1");
-    SFA_CHECK(work_bytes / nbatch >= sfa_helmholtz3d_workspace(Nx, Ny, Nz, precision), SFA_ERR_ARG,
-              "workspace too small");
+    SFA_CHECK(work_bytes / nbatch >= core_workspace(Nx, Ny, Nz), SFA_ERR_ARG, "workspace too small");
     hipStream_t s = (hipStream_t)stream;
     if (precision == 1) {
         SFA_CHECK(Nz / 2 + 1 >= 4, SFA_ERR_UNSUPPORTED, "mixed precision needs Nz >= 6");
@@ -595,6 +614,149 @@ extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t N
                                 work, work_bytes, stream);
 }
 
+namespace sfa {
+
+__global__ void zero_doubles_kernel(double* p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+// Error model of the mixed form.  Complex64 storage and FP32 transforms of
+// the planes kz >= Kh add relative rounding errors of a few eps32 per stage to
+// those planes' values: forward (the stored z spectrum of f, the y FFT, the
+// DCT-II) before the solve, whose gain on those planes is at most
+// 1 / (alpha + Kh^2), and backward (DCT-III, inverse y FFT, the stored
+// spectrum) on the solution's own values.  Rounding errors add like noise, so
+// the rms error of u is about eps32 sqrt(w^2 E_f,hi + E_u,hi) with E the grid
+// energies of f and u in those planes (Parseval: E_f = Nz sum w_kz |X|^2 over
+// f's z spectrum X; E_u = Nx Ny Nz sum w_kz (|t_0|^2 + 1/2 sum |t_k|^2) over
+// u's Chebyshev coefficients t, the DCT-II being orthogonal on the
+// Gauss-Chebyshev points), and max|u| >= rms(u) = sqrt(E_u / (Nx Ny Nz)).
+// kGuardK (64) covers the number of rounding stages (about 30, adding like a
+// random walk) and the ratio of the largest to the rms error.  Bands
+// straddling Kh count as complex64 (conservative).
+constexpr double kEps32 = 5.9604644775390625e-08;
+constexpr double kGuardK = 64.0;
+
+static double guard_predict(const double* h, int Kh, int64_t Nx, int64_t Ny, int64_t Nz, double alpha)
+{
+    const double* sf = h;
+    const double* st = h + kStatBands * kStatSpread;
+    const int b0 = stat_band(Kh);
+    double ef_hi = 0.0, eu_hi = 0.0, eu = 0.0;
+    for (int b = 0; b < kStatBands; ++b) {
+        const double ef = sf[b * kStatSpread], et = st[b * kStatSpread];
+        eu += et;
+        if (b >= b0) {
+            ef_hi += ef;
+            eu_hi += et;
+        }
+    }
+    const double vol = (double)Nx * (double)Ny * (double)Nz;
+    ef_hi *= (double)Nz;
+    eu_hi *= vol;
+    eu *= vol;
+    const double den = alpha + (double)Kh * (double)Kh;
+    const double w = den > 1.0 ? 1.0 / den : 1.0;
+    if (!(eu > 0.0)) return (ef_hi + eu_hi > 0.0) ? INFINITY : 0.0;
+    const double r = (w * w * ef_hi + eu_hi) / eu;
+    return r >= 0.0 ? kGuardK * kEps32 * sqrt(r) : INFINITY;
+}
+
+// Per host thread and device: a side stream, two events and a pinned buffer
+// that carry the statistics to the host while the backward kernels run.
+// They live until the process ends.
+struct GuardCtx {
+    hipStream_t side;
+    hipEvent_t evx, evc;
+    double* host;
+};
+
+static GuardCtx* guard_ctx()
+{
+    static thread_local std::map<int, GuardCtx> ctxs;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto it = ctxs.find(dev);
+    if (it != ctxs.end()) return &it->second;
+    GuardCtx c{};
+    if (hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&c.evx, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.evc, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void**)&c.host, kStatBytes, hipHostMallocDefault) != hipSuccess)
+        return nullptr;
+    return &(ctxs[dev] = c);
+}
+
+}  // namespace sfa
+
+extern "C" int sfa_helmholtz3d_guarded(const double* f, double* u, int64_t nbatch, int64_t Nx,
+                                       int64_t Ny, int64_t Nz, double alpha, int neumann,
+                                       int kh_hint, double tol, const void* plan, void* work,
+                                       int64_t work_bytes, int* kh_used, int* kh_next,
+                                       double* err_pred, void* stream)
+{
+    SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
+              "need Nx >= 8, Ny >= 1, even Nz >= 2");
+    SFA_CHECK(plan != nullptr && plan_ok(plan, Nx, Ny, Nz, alpha, neumann), SFA_ERR_ARG,
+              "plan was not built by sfa_helmholtz3d_plan for these sizes/alpha/bc");
+    SFA_CHECK(nbatch >= 1, SFA_ERR_ARG, "nbatch must be >= 1");
+    SFA_CHECK(f != nullptr && u != nullptr && work != nullptr, SFA_ERR_ARG, "NULL f, u or work");
+    SFA_CHECK(tol > 0.0, SFA_ERR_ARG, "tol must be > 0");
+    const int64_t Nzh = Nz / 2 + 1;
+    SFA_CHECK(Nzh >= 4, SFA_ERR_UNSUPPORTED, "mixed precision needs Nz >= 6");
+    SFA_CHECK(Nz <= 2 * 16 * kStatBands, SFA_ERR_UNSUPPORTED, "guarded mixed precision: Nz <= 512");
+    const int64_t core = core_workspace(Nx, Ny, Nz);
+    SFA_CHECK(work_bytes >= nbatch * core + kStatBytes, SFA_ERR_ARG,
+              "workspace too small (need nbatch * %lld + %d bytes)", (long long)core, kStatBytes);
+    hipStream_t s = (hipStream_t)stream;
+    GuardCtx* g = guard_ctx();
+    SFA_CHECK(g != nullptr, SFA_ERR_HIP, "guard stream/event/pinned buffer creation failed");
+    double* stats = (double*)((char*)work + nbatch * core);
+    hipLaunchKernelGGL(zero_doubles_kernel, dim3(1), dim3(256), 0, s, stats, kStatBytes / 8);
+    SFA_LAUNCH_CHECK();
+    const int Kh = mixed_planes(Nz, kh_hint);
+    int rc = helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann, Kh, plan, work, s, nullptr, nbatch,
+                               stats, g->evx);
+    if (rc) return rc;
+    // the statistics are complete after the x-pass: copy them on the side
+    // stream while the inverse y and z passes run
+    SFA_HIP(hipStreamWaitEvent(g->side, g->evx, 0));
+    SFA_HIP(hipMemcpyAsync(g->host, stats, kStatBytes, hipMemcpyDeviceToHost, g->side));
+    SFA_HIP(hipEventRecord(g->evc, g->side));
+    SFA_HIP(hipEventSynchronize(g->evc));
+    double e = guard_predict(g->host, Kh, Nx, Ny, Nz, alpha);
+    // admissible first complex64 planes 16m + 1 (the band edges)
+    int next = 0;
+    double enext = 0.0;
+    for (int m = 1; 16 * m + 1 <= Nzh - 2; ++m) {
+        const double em = guard_predict(g->host, 16 * m + 1, Nx, Ny, Nz, alpha);
+        if (em <= tol) {
+            next = 16 * m + 1;
+            enext = em;
+            break;
+        }
+    }
+    int used = Kh;
+    if (!(e <= tol)) {
+        // redo from f: with more complex128 planes, or in FP64
+        if (next > Kh) {
+            used = next;
+            e = enext;
+            rc = helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann, used, plan, work, s, nullptr, nbatch);
+        } else {
+            used = (int)Nzh;
+            e = 0.0;
+            rc = helmholtz3d_fp64(f, u, Nx, Ny, Nz, alpha, neumann, plan, work, s, nullptr, nbatch);
+        }
+        if (rc) return rc;
+    }
+    if (kh_used) *kh_used = used;
+    if (kh_next) *kh_next = next;
+    if (err_pred) *err_pred = e;
+    return SFA_OK;
+}
+
 // Same pipeline as sfa_helmholtz3d, run nrep times with hipEvents recorded on
 // `stream` between the five kernels; stage_ms[5] receives the summed
 // milliseconds of each stage (z R2C, y C2C, x-pass, y C2C inverse, z C2R).
@@ -608,10 +770,11 @@ extern "C" int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t N
               "plan was not built by sfa_helmholtz3d_plan for these sizes/alpha/bc");
     SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
               "need Nx >= 8, Ny >= 1, even Nz >= 2");
-    SFA_CHECK(precision == 0 || precision == 1, SFA_ERR_ARG, "precision must be 0 or 1");
+    SFA_CHECK(precision >= 0 && precision <= 2, SFA_ERR_ARG, "precision must be 0, 1 or 2");
     SFA_CHECK(precision == 0 || Nz / 2 + 1 >= 4, SFA_ERR_UNSUPPORTED, "mixed precision needs Nz >= 6");
     SFA_CHECK(work_bytes >= sfa_helmholtz3d_workspace(Nx, Ny, Nz, precision), SFA_ERR_ARG,
               "workspace too small");
+    double* stats = precision == 2 ? (double*)((char*)work + core_workspace(Nx, Ny, Nz)) : nullptr;
     SFA_CHECK(nrep >= 1 && stage_ms != nullptr, SFA_ERR_ARG, "bad nrep/stage_ms");
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t ev[6];
@@ -619,8 +782,9 @@ extern "C" int sfa_helmholtz3d_stage_times(const double* f, double* u, int64_t N
     for (int i = 0; i < 5; ++i) stage_ms[i] = 0.f;
     int rc = SFA_OK;
     for (int r = 0; r < nrep && rc == SFA_OK; ++r) {
-        rc = precision == 1 ? helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann,
-                                                mixed_planes(Nz, fp64_planes), plan, work, s, ev)
+        rc = precision >= 1 ? helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann,
+                                                mixed_planes(Nz, fp64_planes), plan, work, s, ev, 1,
+                                                stats)
                             : helmholtz3d_fp64(f, u, Nx, Ny, Nz, alpha, neumann, plan, work, s, ev);
         if (rc) break;
         hipEventSynchronize(ev[5]);
