@@ -155,6 +155,18 @@ int sfa_convolve_direct(const void* u, const void* v, void* out, int64_t N, int 
 int sfa_legendre_matrix(const double* x, const double* w, int64_t Nx, int64_t N,
                         const double* K2, int analysis, double* M, void* stream);
 
+/* Dense transform matrix from the caller's recursion rows: the plan step of
+ * the recursive seam lib.scalar_product(input, output, x, w, axis, a) /
+ * lib.evaluate_expansion_all(input, output, x, axis, a) (spectralbase.py:
+ * 484-491, 552-558; transforms.pyx:175-323).  a: device (a_rows, a_len)
+ * row-major DIA rows of the recursion matrix, a_rows = 2 (anm, anp) or 3
+ * (anm, ann, anp), x P_n = anm_n P_{n+1} + ann_n P_n + anp_n P_{n-1}, P_0 = 1,
+ * a_len >= N + 2.  analysis=1: M[i][j] = P_i(x_j) w_j, shape (N, Nx);
+ * analysis=0: M[j][i] = P_i(x_j), shape (Nx, N).  x (Nx), w (Nx): device. */
+int sfa_recursion_matrix(const double* x, const double* w, int64_t Nx, const double* a,
+                         int64_t a_rows, int64_t a_len, int64_t N, int analysis, double* M,
+                         void* stream);
+
 /* out(.., k, ..) = sum_j M[k][j] in(.., j, ..) along `axis` (M row-major
  * nout x shape[axis], FP64).  With the matrices above this replaces the
  * recursive Legendre kernels transforms.pyx:175-190 (scalar_product) and

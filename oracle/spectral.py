@@ -939,20 +939,21 @@ def twodma_solve(u, data, axis=0):
     return np.moveaxis(um, 0, axis)
 
 
-def threedma_solve(u, data, axis=0):
+def threedma_solve(u, data, axis=0, compat=False):
     """la.pyx:569-580 ThreeDMA_inner_solve_ptr (offsets 0, 2, 4; no LU).
 
-    Deviation, deliberate: the reference takes the +4 band from DIA row 1
-    (``u2 = &data[m1+4]``, la.pyx:574; ``data[1, 4:]``, la.py:688), i.e. the +2
-    band shifted by two rows, so its result is the solution of the matrix only
-    when those two bands coincide.  This restatement reads row 2, the +4 band,
-    and so solves the matrix the class documents (la.py:661-668).
+    The reference takes the +4 band from DIA row 1 (``u2 = &data[m1+4]``,
+    la.pyx:574; ``data[1, 4:]``, la.py:688), i.e. the +2 band shifted by two
+    rows, so its result is the solution of the matrix only when those two
+    bands coincide.  compat=True restates exactly that; the default reads row
+    2, the +4 band, and so solves the matrix the class documents
+    (la.py:661-668).
     """
     um = np.moveaxis(np.array(u, copy=True), axis, 0)
     n = data.shape[-1]
     d = data[0]
     u1 = data[1, 2:]
-    u2 = data[2, 4:]
+    u2 = data[1, 4:] if compat else data[2, 4:]
     um[n - 1] = um[n - 1] / d[n - 1]
     um[n - 2] = um[n - 2] / d[n - 2]
     um[n - 3] = (um[n - 3] - u1[n - 3] * um[n - 1]) / d[n - 3]
@@ -1236,6 +1237,43 @@ def poisson_add_solve(b, N, scale=1.0, axis=0):
     if not abs(scale - 1) < 1e-8:
         u[:M] = u[:M] / scale
     return np.moveaxis(u, 0, axis)
+
+
+def ann_solve(b, N, scale=1.0, axis=0, constraints=((0, 0),)):
+    """Chebyshev Neumann Poisson A u = b: ANNSolver.__call__ (chebyshev/la.py:
+    74-128) with u = b.  The SN stiffness is upper triangular with row-constant
+    entries after the column scaling m^2, so the unknowns v_m = m^2 u_m follow
+    from a back substitution with running parity sums for k = M-1 .. 1; u_0
+    keeps b_0 until the constraints set their rows; u /= scale; the boundary
+    slots are set to the (homogeneous) boundary values."""
+    A = cheb_A_SN(N)
+    M = N - 2
+    um = np.moveaxis(np.array(b, copy=True), axis, 0)
+    bs = um[:M].copy()
+    us = um[:M]
+    j2 = np.arange(M, dtype=float) ** 2
+    j2[0] = 1
+    j2 = 1.0 / j2
+    d = A[0] * j2
+    d1 = A[2] * j2[2:]
+    se = np.zeros(um.shape[1:], dtype=um.dtype)
+    so = np.zeros(um.shape[1:], dtype=um.dtype)
+    us[M - 1] = bs[M - 1] / d[M - 1]
+    us[M - 2] = bs[M - 2] / d[M - 2]
+    for k in range(M - 3, 0, -1):
+        if k % 2 == 0:
+            se = se + us[k + 2]
+            us[k] = bs[k] - d1[k] * se
+        else:
+            so = so + us[k + 2]
+            us[k] = bs[k] - d1[k] * so
+        us[k] = us[k] / d[k]
+    us *= j2.reshape((M,) + (1,) * (um.ndim - 1))
+    um /= scale
+    for row, val in constraints:
+        um[row] = val
+    um[M:] = 0
+    return np.moveaxis(um, 0, axis)
 
 
 # --------------------------------------------------------------------------

@@ -31,6 +31,7 @@ _SIGS = {
     "sfa_cheb_backward_gl": "vviiaaaivv",
     "sfa_legendre_matrix": "vvllvivv",
     "sfa_line_matmul": "vviiaaailvv",
+    "sfa_recursion_matrix": "vvlvlllivv",
     "sfa_tdma_lu": "vllv",
     "sfa_pdma_lu": "vllv",
     "sfa_tdma_solve": "viiaaivlllv",

@@ -41,6 +41,38 @@ __global__ void legendre_matrix_kernel(const double* __restrict__ x, const doubl
     }
 }
 
+// One thread per node x_j: the generic three-term recurrence of
+// transforms.pyx:219-323 on the caller's DIA rows of the recursion matrix,
+// x P_n = anm_n P_{n+1} + ann_n P_n + anp_n P_{n-1} (ann = 0 without a middle
+// row), P_0 = 1, in the reference's operation order.
+__global__ void recursion_matrix_kernel(const double* __restrict__ x, const double* __restrict__ w,
+                                        int Nx, int N, const double* __restrict__ anm,
+                                        const double* __restrict__ ann,
+                                        const double* __restrict__ anp, int analysis,
+                                        double* __restrict__ M)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= Nx) return;
+    const double xj = x[j];
+    const double wj = analysis ? w[j] : 1.0;
+    const double a0 = ann ? ann[0] : 0.0, a1 = ann ? ann[1] : 0.0;
+    double Lnm = 1.0;
+    double Ln = (xj - a0) / anm[0];
+    double Lnp = (xj - a1) / anm[1] * Ln - anp[1] / anm[1] * Lnm;
+    for (int i = 0; i < N; ++i) {
+        if (analysis)
+            M[(int64_t)i * Nx + j] = Lnm * wj;
+        else
+            M[(int64_t)j * N + i] = Lnm;
+        const double s1 = 1.0 / anm[i + 2];
+        const double s2 = anp[i + 2] / anm[i + 2];
+        const double a00 = ann ? ann[i + 2] : 0.0;
+        Lnm = Ln;
+        Ln = Lnp;
+        Lnp = s1 * (xj - a00) * Ln - s2 * Lnm;
+    }
+}
+
 // out(line, k) = sum_j M[k][j] in(line, j); tile 64 k x 16 lines, j chunks of 32.
 template <bool CPLX>
 __global__ void __launch_bounds__(256) line_matmul_kernel(
@@ -219,6 +251,25 @@ extern "C" int sfa_legendre_matrix(const double* x, const double* w, int64_t Nx,
     dim3 grid((unsigned)((Nx + 127) / 128)), block(128);
     hipLaunchKernelGGL(legendre_matrix_kernel, grid, block, 0, (hipStream_t)stream, x, w, (int)Nx,
                        (int)N, K2, analysis, M);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
+extern "C" int sfa_recursion_matrix(const double* x, const double* w, int64_t Nx, const double* a,
+                                    int64_t a_rows, int64_t a_len, int64_t N, int analysis,
+                                    double* M, void* stream)
+{
+    SFA_CHECK(Nx >= 1 && N >= 1, SFA_ERR_ARG, "Nx, N must be >= 1");
+    SFA_CHECK(a != nullptr && (a_rows == 2 || a_rows == 3), SFA_ERR_ARG,
+              "a must hold 2 (anm, anp) or 3 (anm, ann, anp) DIA rows");
+    SFA_CHECK(a_len >= N + 2 && a_len >= 2, SFA_ERR_ARG, "recursion rows must have >= N + 2 entries");
+    SFA_CHECK(!analysis || w != nullptr, SFA_ERR_ARG, "analysis matrix needs weights");
+    const double* anm = a;
+    const double* ann = a_rows == 3 ? a + a_len : nullptr;
+    const double* anp = a + (a_rows - 1) * a_len;
+    dim3 grid((unsigned)((Nx + 127) / 128)), block(128);
+    hipLaunchKernelGGL(recursion_matrix_kernel, grid, block, 0, (hipStream_t)stream, x, w, (int)Nx,
+                       (int)N, anm, ann, anp, analysis, M);
     SFA_LAUNCH_CHECK();
     return SFA_OK;
 }

@@ -17,6 +17,12 @@ import torch
 
 from . import _lib
 from .matrices import SparseMatrix, TPMatrix, _split_bc, apply_bc_matrices
+from .optimization import optimizer, runtimeoptimizer
+
+
+def _no_host(*args, **kwargs):
+    raise NotImplementedError("no host implementation: config['optimization']['mode'] must name a "
+                              "backend ('hip')")
 
 
 def _geom(t):
@@ -83,6 +89,7 @@ class _Banded:
         self._data = data  # host DIA rows (offsets sorted), scipy dia layout
         self._lu = None
         self._device = None
+        self._pinned = False
 
     def _dev(self, device):
         if self._lu is None or self._device != device:
@@ -94,7 +101,10 @@ class _Banded:
 
     def _call_lu(self, d, n, nbatch):
         if self.lu_fn is not None:
-            _lib.call(self.lu_fn, _lib.ptr(d), n, nbatch, _lib.stream_ptr())
+            if self._dispatch and nbatch == 1:
+                type(self).LU(d)
+            else:
+                _lib.call(self.lu_fn, _lib.ptr(d), n, nbatch, _lib.stream_ptr())
 
     def _call_solve(self, u, axis, data, n, dls, skip):
         shp, st = _geom(u)
@@ -108,7 +118,9 @@ class _Banded:
         if len(constraints) > 0:
             assert len(constraints) == 1
             assert constraints[0][0] == 0, "Can only fix first row"
-            self._pin_row0()
+            if not self._pinned:  # once: the factors then keep the unit row
+                self._pin_row0()
+                self._pinned = True
             s = [slice(None)] * b.dim()
             s[axis] = 0
             b[tuple(s)] = constraints[0][1]
@@ -117,11 +129,17 @@ class _Banded:
     def _pin_row0(self):
         raise NotImplementedError
 
+    # solves go through the static Solve resolved by the backend dispatch
+    _dispatch = True
+
     def __call__(self, b, u=None, axis=0, constraints=()):
         u = _prepare(b, u)
         self.apply_constraints(u, constraints, axis)
         data = self._dev(u.device)
-        self._call_solve(u, axis, data, data.shape[1], 0, 0)
+        if self._dispatch:
+            type(self).Solve(u, data, axis)
+        else:
+            self._call_solve(u, axis, data, data.shape[1], 0, 0)
         return u
 
     solve = __call__
@@ -190,11 +208,25 @@ class TwoDMA(_Banded):
 
 class ThreeDMA(_Banded):
     """Upper 3-diagonal solver, offsets 0, 2, 4 (la.py:661-709); no LU.  The +4
-    band enters the back substitution from its own DIA row (the reference reads
-    the +2 row there, la.py:688)."""
+    band enters the back substitution from its own DIA row.  The reference
+    reads the +2 row there (la.py:688, la.pyx:574), which solves the matrix
+    only when the two bands coincide; compat=True reproduces that reading, so
+    that results equal shenfun's for any matrix."""
     offsets = (0, 2, 4)
     lu_fn = None
     solve_fn = "sfa_threedma_solve"
+    _dispatch = False
+
+    def __init__(self, mats, compat=False):
+        super().__init__(mats)
+        self.compat = bool(compat)
+
+    def _dev(self, device):
+        d = super()._dev(device)
+        if self.compat:
+            d = d.clone()
+            d[2, 4:] = d[1, 4:]
+        return d
 
     def _pin_row0(self):
         # la.py:674-684
@@ -231,6 +263,23 @@ class HeptaDMA(_Banded):
         self._lu = None
 
 
+# shenfun's static LU/Solve methods of the banded solvers (la.py:291-820),
+# resolved by name through the configured backend (optimization/hip.py:
+# TDMA_LU, TDMA_Solve, ...) like the reference's @optimizer methods
+for _cls, _lu_name in ((DiagMA, None), (TDMA, "LU"), (PDMA, "LU"), (TDMA_O, "LU"), (TwoDMA, None),
+                       (ThreeDMA, None), (FDMA, "LU"), (HeptaDMA, "LU")):
+    def _mk(cls, name):
+        f = lambda *a, **k: _no_host(*a, **k)  # noqa: E731
+        f.__name__ = name
+        f.__qualname__ = f"{cls.__name__}.{name}"
+        return staticmethod(optimizer(f))
+    _cls.Solve = _mk(_cls, "Solve")
+    _cls.inner_solve = _mk(_cls, "inner_solve")
+    if _lu_name:
+        _cls.LU = _mk(_cls, "LU")
+del _cls, _lu_name, _mk
+
+
 # band patterns of SparseMatrix.get_solver (matrixbase.py:439-474), narrowest
 # first: SolverGeneric1ND takes the first whose offsets cover every mode's keys
 _PATTERNS = (DiagMA, TwoDMA, TDMA_O, TDMA, ThreeDMA, FDMA, PDMA, HeptaDMA)
@@ -243,6 +292,8 @@ class Solve(_Banded):
     (sfa_band_lu / sfa_band_solve), offsets -nl..nu within -4..4.  The
     Galerkin mass and Helmholtz matrices are SPD, so no pivoting is needed; a
     zero or non-finite pivot raises."""
+
+    _dispatch = False
 
     def __init__(self, mats):
         mat = _sum_mats(mats)
@@ -352,6 +403,13 @@ class SolverGeneric1ND:
             apply_bc_matrices(self.bc_mats, c, u, sign=+1.0)
         return c
 
+    @staticmethod
+    @runtimeoptimizer
+    def solve_data(u, data, sol, naxes, is_rank_zero):
+        """Per-mode solves of all modes but (0[, 0]) (la.py:1321-1344),
+        resolved to the backend's SolverGeneric1ND_solve_data."""
+        _no_host()
+
     def perform_lu(self, device):
         if self._dev is None or self._dev.device != device:
             d = torch.from_numpy(self._data).to(device)
@@ -391,9 +449,13 @@ class SolverGeneric1ND:
             _lib.call("sfa_band_solve", _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, ax,
                       _lib.ptr(data), self.n, data.shape[1] * data.shape[2], 1, self._band[0],
                       self._band[1], _lib.stream_ptr())
-        else:
+        elif self.kind == "threedma":
             _lib.call(self._solve_fn, _lib.ptr(u), _dtype_code(u), u.dim(), shp, st, ax,
                       _lib.ptr(data), self.n, data.shape[1] * data.shape[2], 1, _lib.stream_ptr())
+        else:
+            # all modes but the first, one launch (la.py:1346-1364 -> la.pyx:371)
+            name = {"diag": "DiagMA"}.get(self.kind, self.sol0.__class__.__name__) + "_inner_solve"
+            self.solve_data(u, data, name, ax, True)
         if self.bc_mats:
             self.bc_mats[0].pmat.space.set_boundary_dofs(u, final=True)
         return u
@@ -479,12 +541,31 @@ class Helmholtz:
         self.u1 = torch.zeros(shape, dtype=torch.float64, device=dev)
         self.u2 = torch.zeros(shape, dtype=torch.float64, device=dev)
         self.L = torch.zeros(shape, dtype=torch.float64, device=dev)
-        fshape, fst = _geom(self.u0)
-        A0, A2d, A4, Bm2, B0, B2 = self._diags
-        _lib.call("sfa_helmholtz_lu", int(self.neumann), N, _lib.ptr(A0), _lib.ptr(A2d), _lib.ptr(A4),
-                  _lib.ptr(Bm2), _lib.ptr(B0), _lib.ptr(B2), _lib.ptr(self._ab[0]),
-                  _lib.ptr(self._ab[1]), 1, _lib.ptr(self.u0), _lib.ptr(self.u1), _lib.ptr(self.u2),
-                  _lib.ptr(self.L), self.u0.dim(), fshape, fst, ax, _lib.stream_ptr())
+        self._Apm, self._Bpm = Apm, Bpm
+        self.LU_Helmholtz(Apm, Bpm, self.alfa, self.beta, self.neumann, self.u0, self.u1, self.u2,
+                          self.L, ax)
+
+    # shenfun's static methods (chebyshev/la.py:278-286, 343-351), resolved at
+    # every call through the configured backend (optimization/hip.py)
+    @staticmethod
+    @runtimeoptimizer
+    def LU_Helmholtz(A, B, As, Bs, neumann, u0, u1, u2, L, axis=0):
+        _no_host()
+
+    @staticmethod
+    @runtimeoptimizer
+    def Solve_Helmholtz(b, u, neumann, u0, u1, u2, L, axis=0):
+        _no_host()
+
+    @staticmethod
+    @runtimeoptimizer
+    def Helmholtz_matvec(v, b, alfa, beta, A, B, axis=0):
+        _no_host()
+
+    @staticmethod
+    @runtimeoptimizer
+    def Helmholtz_Neumann_matvec(v, b, alfa, beta, A, B, axis=0):
+        _no_host()
 
     def __call__(self, b, u=None, constraints=()):
         _check(b, "b")
@@ -495,12 +576,7 @@ class Helmholtz:
             raise ValueError(f"arrays must have shape {tuple(self.shape)}")
         # nonhomogeneous boundary values: b -= bc matrices x u's boundary slots
         apply_bc_matrices(self.bc_mats, b, u)
-        shp, bst = _geom(b)
-        _, ust = _geom(u)
-        _, fst = _geom(self.u0)
-        _lib.call("sfa_helmholtz_solve", _lib.ptr(b), _lib.ptr(u), _dtype_code(u), u.dim(), shp, bst,
-                  ust, self.axis, int(self.neumann), _lib.ptr(self.u0), _lib.ptr(self.u1),
-                  _lib.ptr(self.u2), _lib.ptr(self.L), fst, _lib.stream_ptr())
+        self.Solve_Helmholtz(b, u, self.neumann, self.u0, self.u1, self.u2, self.L, self.axis)
         if constraints != ():
             assert len(constraints) == 1
             u[(0,) * u.dim()] = constraints[0][1]
@@ -544,21 +620,39 @@ class ADDSolver:
 
 
 class ANNSolver:
-    """Chebyshev Neumann Poisson (chebyshev/la.py:66-128): solved as the
-    Neumann Helmholtz system with beta = 0, whose k = 0 gauge fixes u_0; the
-    constraint (default (0, 0)) then sets u_0."""
+    """Chebyshev Neumann Poisson (chebyshev/la.py:66-128) along any axis of a
+    1-3 dimensional array: the Neumann Helmholtz system with beta = 0, whose
+    k = 0 gauge fixes u_0 (la.pyx:735-738), factorised once for the 1D
+    operator and broadcast over the lines (zero factor stride); each
+    constraint (row, value), default (0, 0), then sets that row along the
+    axis, as the reference's u[row] = value after moving the axis first."""
 
     def __init__(self, mats, device="cuda"):
         A = _sum_mats(mats)
         assert A.get_key() == "ASNSNmat"
         from .matrices import mass_matrix
         space = A.testfunction[0]
+        self.mat = A
+        self.N = A.shape[0] + 2
         self._H = Helmholtz(A, mass_matrix(space), float(A.scale), 0.0, device=device)
 
     def __call__(self, b, u=None, axis=0, constraints=((0, 0),)):
-        if b.dim() != 1:
-            raise NotImplementedError("ANNSolver on multi-dimensional arrays: use Helmholtz")
-        u = self._H(b, u)
+        _check(b, "b")
+        if u is None:
+            u = b
+        _check(u, "u")
+        ax = axis % b.dim()
+        if b.shape[ax] != self.N or u.shape != b.shape or u.dtype != b.dtype:
+            raise ValueError(f"b and u need {self.N} entries along axis {ax} and equal shapes")
+        H = self._H
+        shp, bst = _geom(b)
+        _, ust = _geom(u)
+        fst = [0] * b.dim()
+        fst[ax] = 1
+        _lib.call("sfa_helmholtz_solve", _lib.ptr(b), _lib.ptr(u), _dtype_code(u), u.dim(), shp, bst,
+                  ust, ax, 1, _lib.ptr(H.u0), _lib.ptr(H.u1), _lib.ptr(H.u2), _lib.ptr(H.L),
+                  _lib.arr(fst), _lib.stream_ptr())
         for row, val in constraints:
-            u[row] = val
+            u.select(ax, int(row)).fill_(val)
+        u.narrow(ax, self.N - 2, 2).zero_()  # homogeneous boundary slots (set_boundary_dofs)
         return u

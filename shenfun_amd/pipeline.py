@@ -31,8 +31,11 @@ class HelmholtzSolver3D:
     # the mixed form is tried again
     FP64_STREAK = 8
 
-    def __init__(self, T, alpha, precision="fp64", fp64_planes=0, tol=1e-10, guard=True,
+    def __init__(self, T, alpha, precision=None, fp64_planes=0, tol=1e-10, guard=True,
                  own_workspace=True):
+        if precision is None:
+            from .config import config
+            precision = config["precision"]
         b = T.bases
         if not (len(b) == 3 and isinstance(b[0], ChebyshevBase) and b[0].bc_kind in ("D", "N")
                 and isinstance(b[1], C2C) and isinstance(b[2], R2C)):

@@ -364,6 +364,52 @@ def test_threedma_reads_its_own_plus4_band():
     assert not np.allclose(S.threedma_solve(b, wrong, 0), u)
 
 
+@pytest.mark.parametrize("n", [9, 12])
+def test_threedma_compat_reading_is_the_shifted_band_solve(n):
+    """compat=True (the reference's reading, la.pyx:574) is the exact solve of
+    the matrix whose +4 band is replaced by the +2 row's entries 4.. (its
+    values shifted by two rows); with coinciding bands it is the plain solve."""
+    rng = np.random.default_rng(n)
+    d = {0: rng.uniform(4, 5, n), 2: rng.uniform(0.2, 1, n - 2), 4: rng.uniform(-1, -0.2, n - 4)}
+    data = S.band_dia(d, n, (0, 2, 4))
+    b = rng.standard_normal((n, 3))
+    shifted = dict(d)
+    shifted[4] = data[1, 4:].copy()
+    ref = np.linalg.solve(S.dense(shifted, n), b)
+    assert np.allclose(S.threedma_solve(b, data, 0, compat=True), ref, rtol=1e-13, atol=1e-13)
+    assert not np.allclose(S.threedma_solve(b, data, 0, compat=True), S.threedma_solve(b, data, 0))
+    same = dict(d)
+    same[4] = data[1, 4:].copy()
+    dsame = S.band_dia(same, n, (0, 2, 4))
+    assert np.allclose(S.threedma_solve(b, dsame, 0, compat=True), S.threedma_solve(b, dsame, 0),
+                       rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_ann_solve_vs_dense(axis):
+    """ANNSolver (chebyshev/la.py:74-128): rows 1..M-1 of scale * A_SN u equal
+    b's, u_0 is the constraint value, the boundary slots are zero -- the dense
+    solve of A_SN with row 0 replaced by the unit row."""
+    N, scale = 20, -2.5
+    M = N - 2
+    A = S.dense(S.cheb_A_SN(N), M) * scale
+    rng = np.random.default_rng(axis)
+    b = rng.standard_normal((N, 5)) + 1j * rng.standard_normal((N, 5))
+    Ac = A.copy()
+    Ac[0] = 0.0
+    Ac[0, 0] = 1.0
+    rhs = b[:M].copy()
+    rhs[0] = 0.0
+    ref = np.zeros_like(b)
+    ref[:M] = np.linalg.solve(Ac, rhs)
+    bb = b if axis == 0 else b.T.copy()
+    u = S.ann_solve(bb, N, scale, axis=axis)
+    u = u if axis == 0 else u.T
+    assert np.allclose(u, ref, rtol=1e-12, atol=1e-12)
+    # the equations of rows 1..M-1 hold
+    assert np.max(np.abs(A[1:] @ u[:M] - b[1:M])) < 1e-10
+
+
 @pytest.mark.parametrize("bc", ["D", "N"])
 def test_helmholtz_lu_solve_vs_dense(bc):
     """LU_Helmholtz + Solve_Helmholtz == dense solve of alfa A + beta B;
