@@ -311,6 +311,38 @@ int sfa_helmholtz_matvec(const void* v, void* c, int dtype, int ndim, const int6
                          const double* alfa, const double* beta, int64_t ab_stride,
                          void* stream);
 
+/* Chebyshev Galerkin biharmonic a0 S + alfa A + beta B on the ShenBiharmonic
+ * basis along `axis` of an array of `shape` (N = shape[axis] >= 8, N - 4
+ * unknowns in two parity chains).  diags: 11 device pointers, the diagonals
+ * as the reference passes them: S[0], S[2], S[4], A[-2], A[0], A[2], B[-4],
+ * B[-2], B[0], B[2], B[4] (a lower diagonal -k stored from its first entry,
+ * D[i] = M(i + k, i)).  alfa/beta per line (ab_stride 1, C order over the
+ * non-axis dims) or one value (0).  Factor arrays u0, u1, u2, l0, l1, ak, bk:
+ * C-contiguous (2,) + shape with the axis length (N - 3) / 2 (the reference's
+ * layout, chebyshev/la.py:480-492).  ak, bk may be NULL (else written too).
+ * S's dense upper tail is rank two per row and kept so through the
+ * elimination, so the LU is O(N) per chain.
+ * Replaces LU_Biharmonic la.pyx:975-1138 (+ Biharmonic_factor_pr :1141-1233). */
+int sfa_biharmonic_lu(int ndim, const int64_t* shape, int axis, double a0, const double* alfa,
+                      const double* beta, int64_t ab_stride, const double* const* diags,
+                      double* u0, double* u1, double* u2, double* l0, double* l1, double* ak,
+                      double* bk, void* stream);
+/* ak, bk from l0, l1 alone.  Replaces Biharmonic_factor_pr la.pyx:1141-1233. */
+int sfa_biharmonic_factor(int ndim, const int64_t* shape, int axis, const double* l0,
+                          const double* l1, double* ak, double* bk, void* stream);
+/* u = (a0 S + alfa A + beta B)^-1 b along axis; b may alias u; entries
+ * N-4 .. N-1 of u are not written.  Replaces Biharmonic_Solve la.pyx:1235-1310. */
+int sfa_biharmonic_solve(const void* b, void* u, int dtype, int ndim, const int64_t* shape,
+                         const int64_t* b_strides, const int64_t* u_strides, int axis, double a0,
+                         const double* u0, const double* u1, const double* u2, const double* l0,
+                         const double* l1, const double* ak, const double* bk, void* stream);
+/* c = (a0 S + alfa A + beta B) v along axis (entries N-4 .. N-1 of c zero);
+ * v != c.  Replaces Biharmonic_matvec Matvec.pyx:820-923. */
+int sfa_biharmonic_matvec(const void* v, void* c, int dtype, int ndim, const int64_t* shape,
+                          const int64_t* v_strides, const int64_t* c_strides, int axis, double a0,
+                          const double* alfa, const double* beta, int64_t ab_stride,
+                          const double* const* diags, void* stream);
+
 /* Chebyshev Dirichlet Poisson A u = b (alpha = 0), A0 (M) diagonal, A2 (M-2)
  * row-constant tail; u /= scale unless |scale-1| < 1e-8.  b may alias u.
  * Replaces Poisson_Solve_ADD la.pyx:614-706 (chebyshev/la.py:11-63). */

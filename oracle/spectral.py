@@ -1276,6 +1276,168 @@ def ann_solve(b, N, scale=1.0, axis=0, constraints=((0, 0),)):
     return np.moveaxis(um, 0, axis)
 
 
+def cheb_SB_matrices(N, quad='GC'):
+    """Galerkin matrices of the Chebyshev ShenBiharmonic basis (N-4 unknowns),
+    dicts offset -> diagonal: SSBSBmat (phi_j^(4), phi_k)_w, upper with every
+    even offset (chebyshev/matrices.py:1444-1459), ASBSBmat (phi_j'', phi_k)_w,
+    offsets -2, 0, 2 (:1008-1017), BSBSBmat (phi_j, phi_k)_w, offsets -4..4
+    from the norms and the stencil (:515-528)."""
+    ki = np.arange(N - 4)
+    k = ki.astype(float)
+    S = {0: PI * 8 * (k + 1) ** 2 * (k + 2) * (k + 4)}
+    for j in range(2, N - 4, 2):
+        m = 8 * (ki[:-j] + 1) * (ki[:-j] + 2) * (ki[:-j] * (ki[:-j] + 4) + 3 * (ki[j:] + 2) ** 2)
+        S[j] = m * PI / (k[j:] + 3)
+    A = {-2: 2 * (k[2:] - 1) * (k[2:] + 2) * PI,
+         0: -4 * ((k + 1) * (k + 2) ** 2) / (k + 3) * PI,
+         2: 2 * (k[:-2] + 1) * (k[:-2] + 2) * PI}
+    h = cheb_norm_sq(N, quad)
+    st = biharmonic_stencil('chebyshev', 'SB', N)
+    S2 = np.concatenate([st[2], np.zeros(2)])  # the reference's S[2] has N-2 entries
+    S4 = st[4]
+    B = {0: h[:-4] + h[2:-2] * S2[:-2] ** 2 + h[4:] * S4 ** 2,
+         2: h[2:-4] * S2[:-4] + h[4:-2] * S4[:-2] * S2[2:-2],
+         4: h[4:-4] * S4[:-4]}
+    B[-2] = B[2].copy()
+    B[-4] = B[4].copy()
+    return S, A, B
+
+
+def _lu_oe_biharmonic(odd, a, b, c, sii, siu, siuu, ail, aii, aiu, bill, bil, bii, biu, biuu):
+    """LU_oe_Biharmonic_1D (la.pyx:1018-1138) of one parity chain: three rolling
+    rows c0, c1, c2 of a S + b A + c B, the dense upper tail of S from its
+    closed form, eliminated with the two lower bands."""
+    M = sii.shape[0]
+    c0, c1, c2 = np.zeros(M), np.zeros(M), np.zeros(M)
+    u0, u1, u2, l0, l1 = (np.zeros(M) for _ in range(5))
+
+    def tail(kr, j):
+        return 8 * (kr + 1) * (kr + 2) * (kr * (kr + 4) + 3 * (j + 2) * (j + 2)) * a * PI / (j + 3.)
+
+    c0[0] = a * sii[0] + b * aii[0] + c * bii[0]
+    c0[1] = a * siu[0] + b * aiu[0] + c * biu[0]
+    c0[2] = a * siuu[0] + c * biuu[0]
+    c0[3] = tail(odd, 6 + odd)
+    c0[4] = tail(odd, 8 + odd)
+    c1[0] = b * ail[0] + c * bil[0]
+    c1[1] = a * sii[1] + b * aii[1] + c * bii[1]
+    c1[2] = a * siu[1] + b * aiu[1] + c * biu[1]
+    c1[3] = a * siuu[1] + c * biuu[1]
+    c1[4] = tail(odd + 2, 8 + odd)
+    c2[0] = c * bill[0]
+    c2[1] = b * ail[1] + c * bil[1]
+    c2[2] = a * sii[2] + b * aii[2] + c * bii[2]
+    c2[3] = a * siu[2] + b * aiu[2] + c * biu[2]
+    c2[4] = a * siuu[2] + c * biuu[2]
+    for i in range(5, M):
+        j = 2 * i + odd
+        c0[i] = tail(odd, j)
+        c1[i] = tail(odd + 2, j)
+        c2[i] = tail(odd + 4, j)
+    u0[0], u1[0], u2[0] = c0[0], c0[1], c0[2]
+    for kk in range(1, M):
+        l0[kk - 1] = c1[kk - 1] / u0[kk - 1]
+        if kk < M - 1:
+            l1[kk - 1] = c2[kk - 1] / u0[kk - 1]
+        c1[kk:] = c1[kk:] - l0[kk - 1] * c0[kk:]
+        if kk < M - 1:
+            c2[kk:] = c2[kk:] - l1[kk - 1] * c0[kk:]
+        c0[kk:] = c1[kk:]
+        c1[kk:] = c2[kk:]
+        if kk < M - 2:
+            c2[kk] = c * bill[kk]
+            c2[kk + 1] = b * ail[kk + 1] + c * bil[kk + 1]
+            c2[kk + 2] = a * sii[kk + 2] + b * aii[kk + 2] + c * bii[kk + 2]
+            if kk < M - 3:
+                c2[kk + 3] = a * siu[kk + 2] + b * aiu[kk + 2] + c * biu[kk + 2]
+            if kk < M - 4:
+                c2[kk + 4] = a * siuu[kk + 2] + c * biuu[kk + 2]
+            if kk < M - 5:
+                kr = 2 * (kk + 2) + odd
+                for i in range(kk + 5, M):
+                    c2[i] = tail(kr, 2 * i + odd)
+        u0[kk] = c0[kk]
+        if kk < M - 1:
+            u1[kk] = c0[kk + 1]
+        if kk < M - 2:
+            u2[kk] = c0[kk + 2]
+    return u0, u1, u2, l0, l1
+
+
+def _factor_oe_pr(odd, l0, l1):
+    """Biharmonic_factor_oe_pr (la.pyx:1211-1233): the two coefficients of U's
+    tail rows, (a_k / (j+3) + b_k (j+2)^2 / (j+3)) a0, eliminated like the rows."""
+    M = l0.shape[0] + 1
+    a = np.zeros(M - 1)
+    b = np.zeros(M - 1)
+    k = odd
+    a[0] = 8 * k * (k + 1) * (k + 2) * (k + 4) * PI
+    b[0] = 24 * (k + 1) * (k + 2) * PI
+    k = 2 + odd
+    a[1] = 8 * k * (k + 1) * (k + 2) * (k + 4) * PI - l0[0] * a[0]
+    b[1] = 24 * (k + 1) * (k + 2) * PI - l0[0] * b[0]
+    for k in range(2, M - 3):
+        kk = 2 * k + odd
+        pp = 8 * kk * (kk + 1) * (kk + 2) * (kk + 4)
+        rr = 24 * (kk + 1) * (kk + 2)
+        a[k] = pp * PI - l0[k - 1] * a[k - 1] - l1[k - 2] * a[k - 2]
+        b[k] = rr * PI - l0[k - 1] * b[k - 1] - l1[k - 2] * b[k - 2]
+    return a, b
+
+
+def biharmonic_lu(N, a0, alfa, beta):
+    """LU_Biharmonic_1D + Biharmonic_factor_pr_1D (la.pyx:991-1016, 1203-1209)
+    for a0 S + alfa A + beta B of the Chebyshev ShenBiharmonic basis: factors
+    (u0, u1, u2, l0, l1, ak, bk), each (2, M) with M the even chain's length."""
+    S, A, B = cheb_SB_matrices(N)
+    dg = [S[0], S[2], S[4], A[-2], A[0], A[2], B[-4], B[-2], B[0], B[2], B[4]]
+    M = dg[0][::2].shape[0]
+    out = [np.zeros((2, M)) for _ in range(7)]
+    for odd in (0, 1):
+        f = _lu_oe_biharmonic(odd, a0, alfa, beta, *[d[odd::2] for d in dg])
+        ak, bk = _factor_oe_pr(odd, f[3], f[4])
+        for t, v in zip(out, list(f) + [ak, bk]):
+            t[odd, :len(v)] = v
+    return tuple(out)
+
+
+def biharmonic_solve(b, factors, a0, axis=0):
+    """Solve_oe_Biharmonic_1D (la.pyx:1286-1310): forward with the two lower
+    bands, back substitution with the three upper bands and the tail through
+    running sums s1 = sum u/(j+3), s2 = sum u (j+2)^2/(j+3), per parity chain,
+    along ``axis``; factors as biharmonic_lu returns them (a tuple: one mode
+    for every line) or a list of such per line (C order over the other dims).  Entries
+    N-4.. of u are zero."""
+    bm = np.moveaxis(np.asarray(b), axis, 0)
+    shp = bm.shape
+    lines = bm.reshape(shp[0], -1)
+    out = np.zeros_like(lines)
+    for li in range(lines.shape[1]):
+        u0, u1, u2, l0, l1, ak, bk = factors[li] if isinstance(factors, list) else factors
+        for odd in (0, 1):
+            fk = lines[odd:shp[0] - 4:2, li]
+            M = fk.shape[0]
+            y = np.zeros(M, dtype=lines.dtype)
+            y[0] = fk[0]
+            y[1] = fk[1] - l0[odd, 0] * y[0]
+            for i in range(2, M):
+                y[i] = fk[i] - l0[odd, i - 1] * y[i - 1] - l1[odd, i - 2] * y[i - 2]
+            uk = np.zeros(M, dtype=lines.dtype)
+            U0, U1, U2 = u0[odd], u1[odd], u2[odd]
+            uk[M - 1] = y[M - 1] / U0[M - 1]
+            uk[M - 2] = (y[M - 2] - U1[M - 2] * uk[M - 1]) / U0[M - 2]
+            uk[M - 3] = (y[M - 3] - U1[M - 3] * uk[M - 2] - U2[M - 3] * uk[M - 1]) / U0[M - 3]
+            s1 = s2 = 0.0
+            for kk in range(M - 4, -1, -1):
+                j = 2 * kk + odd + 6
+                s1 = s1 + uk[kk + 3] / (j + 3.)
+                s2 = s2 + (uk[kk + 3] / (j + 3.)) * ((j + 2) * (j + 2))
+                uk[kk] = (y[kk] - U1[kk] * uk[kk + 1] - U2[kk] * uk[kk + 2]
+                          - ak[odd, kk] * a0 * s1 - bk[odd, kk] * a0 * s2) / U0[kk]
+            out[odd:shp[0] - 4:2, li] = uk
+    return np.moveaxis(out.reshape(shp), 0, axis)
+
+
 # --------------------------------------------------------------------------
 # End-to-end pipelines (the configurations' work)
 # --------------------------------------------------------------------------

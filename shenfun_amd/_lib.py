@@ -63,6 +63,10 @@ _SIGS = {
     "sfa_helmholtz3d_stage_times": "vvllldiiivvlivv",
     "sfa_helmholtz3d_guarded": "vvlllldiidvvlvvvv",
     "sfa_helmholtz3d_plan_release": "v",
+    "sfa_biharmonic_lu": "iaidvvlvvvvvvvvv",
+    "sfa_biharmonic_factor": "iaivvvvv",
+    "sfa_biharmonic_solve": "vviiaaaidvvvvvvvv",
+    "sfa_biharmonic_matvec": "vviiaaaidvvlvv",
     "sfa_batch2d_plan": "ivvvvvvllivlv",
     "sfa_batch2d_solve": "vvvvliv",
     "sfa_batch2d_release": "v",

@@ -656,3 +656,103 @@ class ANNSolver:
             u.select(ax, int(row)).fill_(val)
         u.narrow(ax, self.N - 2, 2).zero_()  # homogeneous boundary slots (set_boundary_dofs)
         return u
+
+
+class Biharmonic:
+    """Chebyshev Galerkin biharmonic solver a0 S + alfa A + beta B along one
+    axis (chebyshev/la.py:354-558): ``Biharmonic(S, A, B)`` with the scales
+    carried by the (TP)matrices (SSBSBmat, ASBSBmat, BSBSBmat in any order,
+    ``biharmonic_tpmatrices`` gives them), or ``Biharmonic(S, A, B, a0, alfa,
+    beta)``.  LU once per mode on the device (two parity chains, the dense
+    upper tail of S kept in rank-two form), ``__call__(b, u=None)`` solves in
+    one launch, ``matvec(v, c)`` multiplies.  The factor arrays u0, u1, u2,
+    l0, l1, ak, bk have the reference's layout."""
+
+    def __init__(self, *args, device="cuda"):
+        if len(args) == 1:
+            args = args[0]
+        args, bc = _split_bc(list(args))
+        if bc:
+            raise NotImplementedError("Biharmonic with nonhomogeneous boundary values")
+        assert len(args) in (3, 6)
+        byk = {m.get_key(): m for m in args[:3]}
+        S, A, B = byk["SSBSBmat"], byk["ASBSBmat"], byk["BSBSBmat"]
+        if len(args) == 3:
+            a0, alfa, beta = S.scale, A.scale, B.scale
+            if isinstance(S, TPMatrix):
+                a0 = np.asarray(a0) * float(S.pmat.scale)
+                alfa = np.asarray(alfa) * float(A.pmat.scale)
+                beta = np.asarray(beta) * float(B.pmat.scale)
+        else:
+            a0, alfa, beta = args[3:6]
+        a0 = np.unique(np.asarray(a0, dtype=float))
+        if a0.size != 1:
+            raise ValueError("the biharmonic scale a0 must be one number")
+        self.a0 = float(a0[0])
+        Spm = S.pmat if isinstance(S, TPMatrix) else S
+        Apm = A.pmat if isinstance(A, TPMatrix) else A
+        Bpm = B.pmat if isinstance(B, TPMatrix) else B
+        space = Spm.testfunction[0]
+        self.axis = S.axis if isinstance(S, TPMatrix) else space.axis
+        T = S.tensorproductspace if isinstance(S, TPMatrix) else space.tensorproductspace
+        N = Spm.shape[0] + 4
+        shape = list(T.shape(True)) if T is not None else [N]
+        ab_shape = list(shape)
+        ab_shape[self.axis] = 1
+        self.alfa = np.ascontiguousarray(np.broadcast_to(np.asarray(alfa, dtype=float), ab_shape))
+        self.beta = np.ascontiguousarray(np.broadcast_to(np.asarray(beta, dtype=float), ab_shape))
+        self.shape = shape
+        M = (N - 4 + 1) // 2
+        fshape = [2] + shape
+        fshape[1 + self.axis] = M
+        dev = torch.device(device)
+        self.u0, self.u1, self.u2, self.l0, self.l1, self.ak, self.bk = (
+            torch.zeros(fshape, dtype=torch.float64, device=dev) for _ in range(7))
+        diag = lambda m, k: np.ascontiguousarray(  # noqa: E731
+            np.broadcast_to(np.asarray(m[k], dtype=float), (m.shape[0] - abs(k),)))
+        self._diags = [torch.from_numpy(diag(m, k)).to(dev) for m, k in
+                       ((Spm, 0), (Spm, 2), (Spm, 4), (Apm, -2), (Apm, 0), (Apm, 2),
+                        (Bpm, -4), (Bpm, -2), (Bpm, 0), (Bpm, 2), (Bpm, 4))]
+        self.LU_Biharmonic(self.a0, self.alfa, self.beta, *self._diags, self.u0, self.u1, self.u2,
+                           self.l0, self.l1, self.axis)
+        self.Biharmonic_factor_pr(self.ak, self.bk, self.l0, self.l1, self.axis)
+
+    # shenfun's static methods (chebyshev/la.py:497-518), resolved at every call
+    @staticmethod
+    @runtimeoptimizer
+    def LU_Biharmonic(a0, alfa, beta, sii, siu, siuu, ail, aii, aiu, bill, bil, bii, biu, biuu,
+                      u0, u1, u2, l0, l1, axis):
+        _no_host()
+
+    @staticmethod
+    @runtimeoptimizer
+    def Biharmonic_factor_pr(ak, bk, l0, l1, axis):
+        _no_host()
+
+    @staticmethod
+    @runtimeoptimizer
+    def Biharmonic_Solve(b, u, u0, u1, u2, l0, l1, ak, bk, a0, axis=0):
+        _no_host()
+
+    @staticmethod
+    @runtimeoptimizer
+    def Biharmonic_matvec(v, b, a0, alfa, beta, sii, siu, siuu, ail, aii, aiu, bill, bil, bii,
+                          biu, biuu, axis=0):
+        _no_host()
+
+    def __call__(self, b, u=None, **kw):
+        _check(b, "b")
+        if u is None:
+            u = b
+        _check(u, "u")
+        if tuple(b.shape) != tuple(self.shape) or u.shape != b.shape:
+            raise ValueError(f"arrays must have shape {tuple(self.shape)}")
+        self.Biharmonic_Solve(b, u, self.u0, self.u1, self.u2, self.l0, self.l1, self.ak, self.bk,
+                              self.a0, self.axis)
+        return u
+
+    def matvec(self, v, c):
+        _check(v, "v")
+        _check(c, "c")
+        self.Biharmonic_matvec(v, c, self.a0, self.alfa, self.beta, *self._diags, self.axis)
+        return c

@@ -288,6 +288,78 @@ class LegASNSNmat(SpectralMatrix):
         return "ASNSNmat"
 
 
+class _ChebSBmat(SpectralMatrix):
+    """Galerkin matrix of the Chebyshev ShenBiharmonic basis (N-4 unknowns)."""
+
+    def __init__(self, test, trial, scale=1.0):
+        self.testfunction = test
+        self.trialfunction = trial
+        d = self.assemble()
+        N = test[0].N - 4
+        SparseMatrix.__init__(self, d, (N, N), scale=scale)
+
+
+class ChebSSBSBmat(_ChebSBmat):
+    """(phi_j^(4), phi_k)_w, chebyshev/matrices.py:1444-1459: upper triangular
+    with every even offset, s_kj = 8 pi (k+1)(k+2)(k(k+4) + 3(j+2)^2)/(j+3) for
+    j > k and 8 pi (k+1)^2 (k+2)(k+4) on the diagonal."""
+
+    def assemble(self):
+        N = self.testfunction[0].N
+        ki = np.arange(N - 4)
+        k = ki.astype(float)
+        d = {0: PI * 8 * (k + 1) ** 2 * (k + 2) * (k + 4)}
+        for j in range(2, N - 4, 2):
+            m = 8 * (ki[:-j] + 1) * (ki[:-j] + 2) * (ki[:-j] * (ki[:-j] + 4) + 3 * (ki[j:] + 2) ** 2)
+            d[j] = m * PI / (k[j:] + 3)
+        return d
+
+    def get_key(self):
+        return "SSBSBmat"
+
+
+class ChebASBSBmat(_ChebSBmat):
+    """(phi_j'', phi_k)_w, chebyshev/matrices.py:1008-1017 (offsets -2, 0, 2)."""
+
+    def assemble(self):
+        k = np.arange(self.testfunction[0].N - 4, dtype=float)
+        return {-2: 2 * (k[2:] - 1) * (k[2:] + 2) * PI,
+                0: -4 * ((k + 1) * (k + 2) ** 2) / (k + 3) * PI,
+                2: 2 * (k[:-2] + 1) * (k[:-2] + 2) * PI}
+
+    def get_key(self):
+        return "ASBSBmat"
+
+
+class ChebBSBSBmat(_ChebSBmat):
+    """(phi_j, phi_k)_w, chebyshev/matrices.py:515-528: the norms against the
+    stencil phi_k = T_k + s2_k T_{k+2} + s4_k T_{k+4}, s2 = -(2k+4)/(k+3),
+    s4 = (k+1)/(k+3) (offsets -4..4)."""
+
+    def assemble(self):
+        N = self.testfunction[0].N
+        h = _cheb_h(N, self.testfunction[0].quad)
+        k = np.arange(N, dtype=float)
+        s2 = -(2 * k + 4) / (k + 3)
+        s4 = (k + 1) / (k + 3)
+        d = {0: h[:-4] + h[2:-2] * s2[:N - 4] ** 2 + h[4:] * s4[:N - 4] ** 2,
+             2: h[2:-4] * s2[:N - 6] + h[4:-2] * s4[:N - 6] * s2[2:N - 4],
+             4: h[4:-4] * s4[:N - 8]}
+        d[-2] = d[2].copy()
+        d[-4] = d[4].copy()
+        return d
+
+    def get_key(self):
+        return "BSBSBmat"
+
+
+def chebyshev_biharmonic_matrices(space):
+    """(S, A, B): the Galerkin biharmonic, stiffness and mass matrices of a
+    Chebyshev ShenBiharmonic space, as chebyshev.la.Biharmonic takes them."""
+    return (ChebSSBSBmat((space, 0), (space, 4)), ChebASBSBmat((space, 0), (space, 2)),
+            ChebBSBSBmat((space, 0), (space, 0)))
+
+
 def mass_matrix(space):
     """(phi_j, phi_k)_w for a composite space."""
     fam, bc = space.family(), space.bc_kind
@@ -490,6 +562,11 @@ def biharmonic_tpmatrices(T, a=1.0, b=0.0, c=0.0, test=None):
         raise ValueError("biharmonic_tpmatrices needs a ShenBiharmonic axis")
     V = (test if test is not None else T).bases[ax]
     k2 = _k2(T, ax)
+    if S.family() == "chebyshev" and V is S:
+        # Galerkin: S is dense above the diagonal; solved by la.Biharmonic
+        Sm, Am, Bm = chebyshev_biharmonic_matrices(S)
+        return [TPMatrix(Sm, T, np.full(k2.shape, float(a)), ax), TPMatrix(Am, T, b - 2 * a * k2, ax),
+                TPMatrix(Bm, T, a * k2 ** 2 - b * k2 + c, ax)]
     return [TPMatrix(GalerkinMatrix(V, S, 4, "SBBmat"), T, np.full(k2.shape, float(a)), ax),
             TPMatrix(GalerkinMatrix(V, S, 2, "ABBmat"), T, b - 2 * a * k2, ax),
             TPMatrix(GalerkinMatrix(V, S, 0, "BBBmat"), T, a * k2 ** 2 - b * k2 + c, ax)]

@@ -14,7 +14,9 @@ Reference entry points replaced (file:line in shenfun/optimization/cython):
   :371 SolverGeneric1ND_solve_data, :462-592 the *_inner_solve line solves,
   :614 Poisson_Solve_ADD, :708 LU_Helmholtz, :845 Solve_Helmholtz;
   transforms.pyx:175 scalar_product, :199 evaluate_expansion_all;
-  Matvec.pyx:697 Helmholtz_matvec, :722-818 Helmholtz_Neumann_matvec.
+  Matvec.pyx:697 Helmholtz_matvec, :722-818 Helmholtz_Neumann_matvec;
+  la.pyx:975 LU_Biharmonic, :1195 Biharmonic_factor_pr, :1235 Biharmonic_Solve,
+  Matvec.pyx:908 Biharmonic_matvec.
 """
 import collections
 import hashlib
@@ -420,3 +422,88 @@ def evaluate_expansion_all(input_array, output_array, x, axis, a):
         raise ValueError("output length along axis must equal the number of nodes")
     n = input_array.shape[axis]
     return _contract(input_array, output_array, axis, _recursion_matrix(x, None, a, n, False))
+
+
+# ---------------------------------------------------------------- Chebyshev biharmonic
+def _diag_ptrs(diags):
+    """Device copies of the 11 diagonals (sii, siu, siuu, ail, aii, aiu, bill,
+    bil, bii, biu, biuu) and the C array of their addresses."""
+    import ctypes
+    ds = [d if isinstance(d, torch.Tensor) and d.is_cuda and d.dtype == torch.float64
+          else torch.from_numpy(np.ascontiguousarray(np.atleast_1d(np.asarray(d, dtype=float)))).cuda()
+          for d in diags]
+    arr = (ctypes.c_void_p * 11)(*[d.data_ptr() for d in ds])
+    return ds, ctypes.cast(arr, ctypes.c_void_p), arr
+
+
+def LU_Biharmonic(a0, alfa, beta, sii, siu, siuu, ail, aii, aiu, bill, bil, bii, biu, biuu,
+                  u0, u1, u2, l0, l1, axis=0):
+    """Factors u0, u1, u2, l0, l1 ((2,) + u's shape with the axis length the
+    even chain's) of a0 S + alfa A + beta B for the Chebyshev ShenBiharmonic
+    basis (la.pyx:975-1138); alfa, beta scalars or arrays with 1 along axis."""
+    outs = [_Dev(x) for x in (u0, u1, u2, l0, l1)]
+    t = outs[0].t
+    ndim = t.dim() - 1
+    ax = axis % ndim
+    N = int(np.size(sii)) + 4
+    shape = list(t.shape[1:])
+    shape[ax] = N
+    al = _per_line(alfa, shape, ax)
+    be = _per_line(beta, shape, ax)
+    ds, dptr, _keep = _diag_ptrs((sii, siu, siuu, ail, aii, aiu, bill, bil, bii, biu, biuu))
+    _lib.call("sfa_biharmonic_lu", ndim, _lib.arr(shape), ax, float(a0), _lib.ptr(al), _lib.ptr(be), 1,
+              dptr, *[_lib.ptr(o.t) for o in outs], None, None, _lib.stream_ptr())
+    for o in outs:
+        o.back()
+
+
+def Biharmonic_factor_pr(ak, bk, l0, l1, axis=0):
+    """ak, bk of U's tail rows from l0, l1 (la.pyx:1141-1233)."""
+    outs = [_Dev(x) for x in (ak, bk)]
+    d0, d1 = _Dev(l0), _Dev(l1)
+    t = outs[0].t
+    ndim = t.dim() - 1
+    ax = axis % ndim
+    shape = list(t.shape[1:])
+    shape[ax] = 2 * t.shape[1 + ax] + 4
+    _lib.call("sfa_biharmonic_factor", ndim, _lib.arr(shape), ax, _lib.ptr(d0.t), _lib.ptr(d1.t),
+              _lib.ptr(outs[0].t), _lib.ptr(outs[1].t), _lib.stream_ptr())
+    for o in outs:
+        o.back()
+
+
+def Biharmonic_Solve(b, u, u0, u1, u2, l0, l1, ak, bk, a0, axis=0):
+    """u = (a0 S + alfa A + beta B)^-1 b along axis with the factors above
+    (la.pyx:1235-1310); u may be b; entries N-4.. of u are not written."""
+    db = _Dev(b)
+    du = db if u is b else _Dev(u)
+    fs = [_Dev(x) for x in (u0, u1, u2, l0, l1, ak, bk)]
+    t = db.t
+    ax = axis % t.dim()
+    shp, bst = _geom(t)
+    _, ust = _geom(du.t)
+    _lib.call("sfa_biharmonic_solve", _lib.ptr(t), _lib.ptr(du.t), _dtype_code(t), t.dim(), shp, bst, ust,
+              ax, float(a0), *[_lib.ptr(f.t) for f in fs], _lib.stream_ptr())
+    du.back()
+    return u
+
+
+def Biharmonic_matvec(v, b, a0, alfa, beta, sii, siu, siuu, ail, aii, aiu, bill, bil, bii, biu, biuu,
+                      axis=0):
+    """b = (a0 S + alfa A + beta B) v along axis (Matvec.pyx:820-923)."""
+    dv = _Dev(v)
+    db = _Dev(b)
+    t = dv.t
+    ax = axis % t.dim()
+    al = _per_line(alfa, t.shape, ax)
+    be = _per_line(beta, t.shape, ax)
+    ds, dptr, _keep = _diag_ptrs((sii, siu, siuu, ail, aii, aiu, bill, bil, bii, biu, biuu))
+    shp, vst = _geom(t)
+    _, cst = _geom(db.t)
+    _lib.call("sfa_biharmonic_matvec", _lib.ptr(t), _lib.ptr(db.t), _dtype_code(t), t.dim(), shp, vst, cst,
+              ax, float(a0), _lib.ptr(al), _lib.ptr(be), 1, dptr, _lib.stream_ptr())
+    db.back()
+    return b
+
+
+__all__ += ["LU_Biharmonic", "Biharmonic_factor_pr", "Biharmonic_Solve", "Biharmonic_matvec"]

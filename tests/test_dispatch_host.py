@@ -16,7 +16,8 @@ CYTHON_NAMES = [
     "TDMA_inner_solve", "PDMA_inner_solve", "TDMA_O_inner_solve", "FDMA_inner_solve",
     "HeptaDMA_inner_solve", "TwoDMA_inner_solve", "ThreeDMA_inner_solve", "DiagMA_inner_solve",
     "SolverGeneric1ND_solve_data", "LU_Helmholtz", "Solve_Helmholtz", "Helmholtz_matvec",
-    "Helmholtz_Neumann_matvec", "Poisson_Solve_ADD", "scalar_product", "evaluate_expansion_all"]
+    "Helmholtz_Neumann_matvec", "Poisson_Solve_ADD", "scalar_product", "evaluate_expansion_all",
+    "LU_Biharmonic", "Biharmonic_factor_pr", "Biharmonic_Solve", "Biharmonic_matvec"]
 
 
 @pytest.fixture

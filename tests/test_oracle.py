@@ -810,3 +810,66 @@ def test_biharmonic_matrices_host_side_match_quadrature():
             D = np.array([[np.sum(w * phi[k](x) * (psi[j].deriv(d)(x) if d else psi[j](x)))
                            for j in range(N - 4)] for k in range(N - 4)])
             assert np.allclose(m.pmat.todense(), D, rtol=1e-10, atol=1e-10 * np.abs(D).max())
+
+
+# ----------------------------------------------------------------- Chebyshev Galerkin biharmonic
+def _sb_galerkin(N, deriv):
+    """(phi_j^(deriv), phi_k)_w for the Chebyshev ShenBiharmonic basis by exact
+    algebra in the Chebyshev basis (chebder, norms pi, pi/2, ...)."""
+    C = np.polynomial.chebyshev
+    st = S.biharmonic_stencil("chebyshev", "SB", N)
+    M = N - 4
+    P = np.zeros((M, N))
+    for k in range(M):
+        P[k, k], P[k, k + 2], P[k, k + 4] = 1.0, st[2][k], st[4][k]
+    h = np.full(N, np.pi / 2)
+    h[0] = np.pi
+    D = np.zeros((M, M))
+    for j in range(M):
+        c = C.chebder(P[j], deriv) if deriv else P[j]
+        cc = np.zeros(N)
+        cc[:len(c)] = c
+        D[:, j] = (P * h) @ cc
+    return D
+
+
+@pytest.mark.parametrize("N", [16, 17, 24, 41])
+def test_cheb_biharmonic_matrices_vs_algebra(N):
+    """SSBSBmat, ASBSBmat, BSBSBmat closed forms (chebyshev/matrices.py:515-528,
+    1008-1017, 1444-1459) equal the Galerkin products computed exactly."""
+    Sm, Am, Bm = S.cheb_SB_matrices(N)
+    M = N - 4
+    for d, deriv in ((Sm, 4), (Am, 2), (Bm, 0)):
+        ref = _sb_galerkin(N, deriv)
+        assert np.allclose(S.dense(d, M), ref, rtol=1e-12, atol=1e-9 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("N", [16, 17, 30])
+@pytest.mark.parametrize("abc", [(1.0, 0.0, 0.0), (1.0, -18.0, 83.0), (2.5, -3.0, 0.5)])
+def test_cheb_biharmonic_lu_solve_vs_dense(N, abc):
+    """LU_Biharmonic + Biharmonic_factor_pr + Biharmonic_Solve (la.pyx:975-1310)
+    restated: the solve equals the dense solve of a0 S + alfa A + beta B,
+    complex right-hand sides along either axis; L U rebuilds the chain
+    matrices."""
+    a0, al, be = abc
+    Sm, Am, Bm = S.cheb_SB_matrices(N)
+    M = N - 4
+    D = a0 * S.dense(Sm, M) + al * S.dense(Am, M) + be * S.dense(Bm, M)
+    fac = S.biharmonic_lu(N, a0, al, be)
+    rng = np.random.default_rng(N)
+    b = rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))
+    ref = np.linalg.solve(D, b[:M])
+    assert np.allclose(S.biharmonic_solve(b, fac, a0, 0)[:M], ref, rtol=1e-11, atol=1e-13)
+    assert np.allclose(S.biharmonic_solve(b.T.copy(), fac, a0, 1)[:, :M], ref.T, rtol=1e-11, atol=1e-13)
+    u0, u1, u2, l0, l1, ak, bk = fac
+    for odd in (0, 1):
+        idx = np.arange(odd, M, 2)
+        m = len(idx)
+        Dc = D[np.ix_(idx, idx)]
+        L = np.eye(m) + np.diag(l0[odd, :m - 1], -1) + np.diag(l1[odd, :m - 2], -2)
+        U = np.diag(u0[odd, :m]) + np.diag(u1[odd, :m - 1], 1) + np.diag(u2[odd, :m - 2], 2)
+        for r in range(m):
+            for c in range(r + 3, m):
+                j = 2 * c + odd
+                U[r, c] = a0 * (ak[odd, r] / (j + 3) + bk[odd, r] * (j + 2) ** 2 / (j + 3))
+        assert np.allclose(L @ U, Dc, rtol=1e-12, atol=1e-10 * np.abs(Dc).max())
