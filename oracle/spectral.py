@@ -1395,9 +1395,13 @@ def biharmonic_lu(N, a0, alfa, beta):
     out = [np.zeros((2, M)) for _ in range(7)]
     for odd in (0, 1):
         f = _lu_oe_biharmonic(odd, a0, alfa, beta, *[d[odd::2] for d in dg])
-        ak, bk = _factor_oe_pr(odd, f[3], f[4])
-        for t, v in zip(out, list(f) + [ak, bk]):
+        for t, v in zip(out, f):
             t[odd, :len(v)] = v
+    for odd in (0, 1):
+        # on the (2, M) rows, as Biharmonic_factor_pr_1D passes them
+        ak, bk = _factor_oe_pr(odd, out[3][odd], out[4][odd])
+        out[5][odd, :len(ak)] = ak
+        out[6][odd, :len(bk)] = bk
     return tuple(out)
 
 
