@@ -8,9 +8,11 @@ once per checkpoint, ``read`` copies back into the caller's array.
 Datasets are named ``name/step`` as the reference's groups, plus the
 attributes ``tstep`` and ``t``.  The previous file is kept as
 ``<filename>.old.chk.npz`` (the reference copies ``.chk.h5`` to
-``.old.chk.h5``), and a new file replaces the old atomically.
+``.old.chk.h5``, io/__init__.py:75-93), and the new file replaces the current
+one by a rename, so a complete checkpoint exists at every moment.
 """
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -56,11 +58,20 @@ class Checkpoint:
                 self.f[f"{name}/{int(step)}"] = np.array(a)
 
     def _flush(self):
+        """Write the new archive to a temporary file (synced to disk), keep a
+        copy of the current one as .old (it stays in place meanwhile), then
+        rename the new one over it: at every moment a complete .chk.npz
+        exists, the old or the new."""
         tmp = self.path + ".tmp.npz"
         with open(tmp, "wb") as fh:
             np.savez(fh, **self.f)
+            fh.flush()
+            os.fsync(fh.fileno())
         if os.path.exists(self.path):
-            os.replace(self.path, self.filename + ".old.chk.npz")
+            old = self.filename + ".old.chk.npz"
+            otmp = old + ".tmp"
+            shutil.copy2(self.path, otmp)
+            os.replace(otmp, old)
         os.replace(tmp, self.path)
 
     def update(self, t, tstep):

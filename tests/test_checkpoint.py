@@ -49,3 +49,28 @@ def test_checkpoint_kill_file(tmp_path, monkeypatch):
     assert not os.path.exists("killshenfun") and chk.attrs() == (7, 1.5)
     out = torch.zeros(6)
     assert torch.equal(chk.read(out, "U"), U)
+
+
+def test_checkpoint_flush_keeps_a_complete_file(tmp_path, monkeypatch):
+    """A stop between writing .old and renaming the new archive leaves the
+    current .chk.npz in place (complete), never no file."""
+    monkeypatch.chdir(tmp_path)
+    U = torch.arange(4.0)
+    chk = Checkpoint("c", checkevery=1, data={"0": {"U": [U]}})
+    chk.update(0.1, 1)
+    real_replace = os.replace
+    calls = []
+
+    def failing_replace(src, dst):
+        calls.append((src, dst))
+        if dst.endswith("c.chk.npz"):
+            raise OSError("stopped before the final rename")
+        return real_replace(src, dst)
+
+    U.add_(1.0)
+    monkeypatch.setattr(os, "replace", failing_replace)
+    with pytest.raises(OSError):
+        chk.update(0.2, 2)
+    monkeypatch.setattr(os, "replace", real_replace)
+    assert os.path.exists("c.chk.npz") and chk.attrs() == (1, 0.1)
+    assert os.path.exists("c.old.chk.npz")
