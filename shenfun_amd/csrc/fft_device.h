@@ -8,13 +8,23 @@ namespace sfa {
 // Spectrum statistics of the guarded mixed-precision solve: per band of 16
 // kz planes (band 0 = kz 0..16, band b >= 1 = kz 16b+1 .. 16b+16, so the
 // band edges are the admissible first complex64 planes 16m+1), the energy of
-// f's z spectrum (kStatBands accumulators) followed by that of u's Chebyshev
-// coefficients (kStatBands more), each accumulator kStatSpread doubles apart
-// so that concurrent atomics of different bands go to different cache lines.
+// f's z spectrum (accumulators 0 .. kStatBands-1) and that of u's Chebyshev
+// coefficients (kStatBands .. 2 kStatBands-1).  Each accumulator is spread
+// over kStatSlots slots of one 128-byte line each, picked by workgroup index:
+// thousands of waves adding into a handful of addresses serialise on them
+// (the z R2C at 256^3 took 117 us instead of 50).  stat_reduce_kernel sums the
+// slots into the 2 kStatBands doubles at kStatSumOff.
 constexpr int kStatBands = 16;
-constexpr int kStatSpread = 32;
-constexpr int kStatBytes = 2 * kStatBands * kStatSpread * 8;
+constexpr int kStatSlots = 64;
+constexpr int kStatLine = 16;  // doubles per slot
+constexpr int kStatAcc = 2 * kStatBands;
+constexpr int kStatSumOff = kStatAcc * kStatSlots * kStatLine;  // doubles
+constexpr int kStatBytes = (kStatSumOff + kStatAcc) * 8;
 __host__ __device__ constexpr int stat_band(int kz) { return kz == 0 ? 0 : (kz - 1) / 16; }
+__host__ __device__ constexpr int stat_index(int acc, int slot)
+{
+    return (acc * kStatSlots + slot) * kStatLine;
+}
 
 struct FFTPlan {
     int nfft;       // complex FFT length

@@ -5,6 +5,8 @@
 //   * R2C / C2R along the contiguous axis (z): rows packed as N/2 complex.
 // Also the v2 fused x-pass (DCT-II -> stencil -> Helmholtz scan solve ->
 // to_ortho -> DCT-III) on 16 flattened (ky, kz) lines per block.
+#include <stdlib.h>
+
 #include "fft_pow2.h"
 #include "helm_scan.h"
 
@@ -20,8 +22,8 @@ struct StridedLines {
     double scale;
 };
 
-// one block: TL lines starting at line blk * TL
-template <int N, bool INV, typename C2>
+// one block of NL * Q threads: NL lines starting at line blk * NL
+template <int N, bool INV, typename C2, int NL = TL>
 __device__ __forceinline__ void c2c_strided_body(const StridedLines& g, const double2* __restrict__ twg,
                                                  int64_t blk)
 {
@@ -33,8 +35,8 @@ __device__ __forceinline__ void c2c_strided_body(const StridedLines& g, const do
     const C2* in = reinterpret_cast<const C2*>(g.in);
     C2* out = reinterpret_cast<C2*>(g.out);
     const int tid = threadIdx.x, nth = blockDim.x;
-    const int t = tid % TL, j = tid / TL;
-    const int64_t L = blk * TL + t;
+    const int t = tid % NL, j = tid / NL;
+    const int64_t L = blk * NL + t;
     const bool valid = L < g.nlines;
     const int64_t i1 = valid ? L / g.d2 : 0, i2 = valid ? L - i1 * g.d2 : 0;
     for (int i = tid; i < N; i += nth) tw[i] = from_c64<C2>(twg[i]);
@@ -89,24 +91,26 @@ constexpr int zrow_stride()
 }
 
 // rows of 2H reals (stride in_rs), output rows of H+1 complex (stride out_rs).
-template <int H>
-__global__ void __launch_bounds__(TL * Pow2<H>::Q)
-    r2c_contig_kernel(const double* __restrict__ in, double2* __restrict__ out, int64_t in_rs,
-                      int64_t out_rs, int64_t d2, int64_t in_s1, int64_t out_s1, int64_t nrows,
-                      double scale, const double2* __restrict__ twg /* 2H entries */)
+// NR rows R0 .. R0 + NR - 1 (of nrows) per workgroup of NR * Q threads.
+template <int H, int NR = TL>
+__device__ __forceinline__ void r2c_contig_body(const double* __restrict__ in, double2* __restrict__ out,
+                                                int64_t in_rs, int64_t out_rs, int64_t d2, int64_t in_s1,
+                                                int64_t out_s1, int64_t nrows, double scale,
+                                                const double2* __restrict__ twg /* 2H entries */,
+                                                int64_t R0)
 {
     using Pl = Pow2<H>;
     constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
-    // LDS: TL lines x LS exchange, then exp(-2 pi i m / H), m < H.  The
+    // LDS: NR lines x LS exchange, then exp(-2 pi i m / H), m < H.  The
     // post-twiddles exp(-2 pi i k / N) are read from the global table twg
     // (L2-resident): keeping them out of LDS lets four blocks share a CU.
     double2* ex = sm;
-    double2* twH = ex + TL * LS;
+    double2* twH = ex + NR * LS;
     const int tid = threadIdx.x, nth = blockDim.x;
     const int j = tid % Q, t = tid / Q;  // j fastest: contiguous row segments
-    const int64_t R = (int64_t)blockIdx.x * TL + t;
+    const int64_t R = R0 + t;
     const bool valid = R < nrows;
     const int64_t i1 = valid ? R / d2 : 0, i2 = valid ? R - i1 * d2 : 0;
     for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
@@ -145,19 +149,30 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 
 template <int H>
 __global__ void __launch_bounds__(TL * Pow2<H>::Q)
-    c2r_contig_kernel(const double2* __restrict__ in, double* __restrict__ out, int64_t in_rs,
+    r2c_contig_kernel(const double* __restrict__ in, double2* __restrict__ out, int64_t in_rs,
                       int64_t out_rs, int64_t d2, int64_t in_s1, int64_t out_s1, int64_t nrows,
                       double scale, const double2* __restrict__ twg /* 2H entries */)
+{
+    r2c_contig_body<H>(in, out, in_rs, out_rs, d2, in_s1, out_s1, nrows, scale, twg,
+                       (int64_t)blockIdx.x * TL);
+}
+
+template <int H, int NR = TL>
+__device__ __forceinline__ void c2r_contig_body(const double2* __restrict__ in, double* __restrict__ out,
+                                                int64_t in_rs, int64_t out_rs, int64_t d2, int64_t in_s1,
+                                                int64_t out_s1, int64_t nrows, double scale,
+                                                const double2* __restrict__ twg /* 2H entries */,
+                                                int64_t R0)
 {
     using Pl = Pow2<H>;
     constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
     double2* ex = sm;
-    double2* twH = ex + TL * LS;
+    double2* twH = ex + NR * LS;
     const int tid = threadIdx.x, nth = blockDim.x;
     const int j = tid % Q, t = tid / Q;
-    const int64_t R = (int64_t)blockIdx.x * TL + t;
+    const int64_t R = R0 + t;
     const bool valid = R < nrows;
     const int64_t i1 = valid ? R / d2 : 0, i2 = valid ? R - i1 * d2 : 0;
     for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
@@ -206,6 +221,16 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
                 *reinterpret_cast<double2*>(dst + 2 * m) = make_double2(z.x * scale, z.y * scale);
             }
     }
+}
+
+template <int H>
+__global__ void __launch_bounds__(TL * Pow2<H>::Q)
+    c2r_contig_kernel(const double2* __restrict__ in, double* __restrict__ out, int64_t in_rs,
+                      int64_t out_rs, int64_t d2, int64_t in_s1, int64_t out_s1, int64_t nrows,
+                      double scale, const double2* __restrict__ twg /* 2H entries */)
+{
+    c2r_contig_body<H>(in, out, in_rs, out_rs, d2, in_s1, out_s1, nrows, scale, twg,
+                       (int64_t)blockIdx.x * TL);
 }
 
 // ----------------------------------------------------------------- host dispatch
@@ -415,7 +440,7 @@ struct XPass2Args {
 // STATS: also accumulate, per band of 16 kz planes, the energy of the
 // solution's Chebyshev coefficients t (after to_ortho),
 // w_kz (|t_0|^2 + 1/2 sum_{k>=1} |t_k|^2) per line (w_kz = 2 for 0 < kz < Nz/2,
-// else 1), into a.stats[(kStatBands + band) * kStatSpread].
+// else 1), into accumulator kStatBands + band of a.stats (slot bid % kStatSlots).
 template <int N, typename C2, bool STATS = false>
 __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
 {
@@ -582,7 +607,8 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
             const int64_t ky = Lr / a.Kp, kz = a.kz0 + (Lr - ky * a.Kp);
             if (kz < a.Nzh) {
                 const double wz = (kz == 0 || kz == a.Nzh - 1) ? 1.0 : 2.0;
-                atomicAdd(a.stats + (kStatBands + stat_band((int)kz)) * kStatSpread, wz * el);
+                atomicAdd(a.stats + stat_index(kStatBands + stat_band((int)kz), (int)(bid % kStatSlots)),
+                          wz * el);
             }
         }
     }
@@ -758,22 +784,23 @@ namespace sfa {
 // k < Kh -> o64 (row pitch Kh, complex128), k >= Kh -> o32 (row pitch H+1-Kh, complex64).
 // STATS: also accumulate the energy sum_k w_k |X_k|^2 of the output per band
 // of 16 kz planes (w_k = 2 for 0 < k < H, else 1: the R2C half spectrum)
-// into stats[band * kStatSpread]: one vector atomic per wave.
-template <int H, bool STATS>
-__global__ void __launch_bounds__(TL * Pow2<H>::Q)
-    r2c_split_kernel(const double* __restrict__ in, double2* __restrict__ o64,
-                     float2* __restrict__ o32, int64_t nrows, int Kh, double scale,
-                     const double2* __restrict__ twg, double* __restrict__ stats)
+// into accumulator band of stats (slot: the row block modulo kStatSlots): one
+// vector atomic per wave.
+template <int H, bool STATS, int NR = TL>
+__device__ __forceinline__ void r2c_split_body(const double* __restrict__ in, double2* __restrict__ o64,
+                                               float2* __restrict__ o32, int64_t nrows, int Kh,
+                                               double scale, const double2* __restrict__ twg,
+                                               double* __restrict__ stats, int64_t R0)
 {
     using Pl = Pow2<H>;
     constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
     double2* ex = sm;
-    double2* twH = ex + TL * LS;
+    double2* twH = ex + NR * LS;
     const int tid = threadIdx.x, nth = blockDim.x;
     const int j = tid % Q, t = tid / Q;
-    const int64_t R = (int64_t)blockIdx.x * TL + t;
+    const int64_t R = R0 + t;
     const bool valid = R < nrows;
     const int Kl = H + 1 - Kh;
     for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
@@ -837,8 +864,17 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 #pragma unroll
         for (int b = 0; b < NBL; ++b)
             if (lane == b) mine = acc[b];
-        if (lane < NB) atomicAdd(stats + lane * kStatSpread, mine);
+        if (lane < NB) atomicAdd(stats + stat_index(lane, (int)((R0 / NR) % kStatSlots)), mine);
     }
+}
+
+template <int H, bool STATS>
+__global__ void __launch_bounds__(TL * Pow2<H>::Q)
+    r2c_split_kernel(const double* __restrict__ in, double2* __restrict__ o64,
+                     float2* __restrict__ o32, int64_t nrows, int Kh, double scale,
+                     const double2* __restrict__ twg, double* __restrict__ stats)
+{
+    r2c_split_body<H, STATS>(in, o64, o32, nrows, Kh, scale, twg, stats, (int64_t)blockIdx.x * TL);
 }
 
 // C2R (FP64 arithmetic) of rows merged from o64 (k < Kh) and o32 (k >= Kh).
@@ -846,21 +882,20 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 // i < 3) can lie in the complex128 planes; every other load is a plain
 // complex64 load, and all loads are issued before any conversion (a
 // per-element select of a converted complex64 load waited on each load in turn).
-template <int H, bool KSMALL>
-__global__ void __launch_bounds__(TL * Pow2<H>::Q)
-    c2r_merge_kernel(const double2* __restrict__ i64, const float2* __restrict__ i32,
-                     double* __restrict__ out, int64_t nrows, int Kh, double scale,
-                     const double2* __restrict__ twg)
+template <int H, bool KSMALL, int NR = TL>
+__device__ __forceinline__ void c2r_merge_body(const double2* __restrict__ i64, const float2* __restrict__ i32,
+                                               double* __restrict__ out, int64_t nrows, int Kh,
+                                               double scale, const double2* __restrict__ twg, int64_t R0)
 {
     using Pl = Pow2<H>;
     constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
     constexpr int N = 2 * H;
     extern __shared__ double2 sm[];
     double2* ex = sm;
-    double2* twH = ex + TL * LS;
+    double2* twH = ex + NR * LS;
     const int tid = threadIdx.x, nth = blockDim.x;
     const int j = tid % Q, t = tid / Q;
-    const int64_t R = (int64_t)blockIdx.x * TL + t;
+    const int64_t R = R0 + t;
     const bool valid = R < nrows;
     const int Kl = H + 1 - Kh;
     for (int i = tid; i < H; i += nth) twH[i] = twg[2 * i];
@@ -935,6 +970,15 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
                 *reinterpret_cast<double2*>(dst + 2 * m) = make_double2(z.x * scale, z.y * scale);
             }
     }
+}
+
+template <int H, bool KSMALL>
+__global__ void __launch_bounds__(TL * Pow2<H>::Q)
+    c2r_merge_kernel(const double2* __restrict__ i64, const float2* __restrict__ i32,
+                     double* __restrict__ out, int64_t nrows, int Kh, double scale,
+                     const double2* __restrict__ twg)
+{
+    c2r_merge_body<H, KSMALL>(i64, i32, out, nrows, Kh, scale, twg, (int64_t)blockIdx.x * TL);
 }
 
 template <int H, bool SPLIT>

@@ -618,7 +618,17 @@ namespace sfa {
 
 __global__ void zero_doubles_kernel(double* p, int n)
 {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// the slots of each statistics accumulator summed into stats[kStatSumOff + acc]
+__global__ void stat_reduce_kernel(double* stats)
+{
+    const int acc = threadIdx.x;
+    if (acc >= kStatAcc) return;
+    double s = 0.0;
+    for (int sl = 0; sl < kStatSlots; ++sl) s += stats[stat_index(acc, sl)];
+    stats[kStatSumOff + acc] = s;
 }
 
 // Error model of the mixed form.  Complex64 storage and FP32 transforms of
@@ -640,12 +650,12 @@ constexpr double kGuardK = 64.0;
 
 static double guard_predict(const double* h, int Kh, int64_t Nx, int64_t Ny, int64_t Nz, double alpha)
 {
-    const double* sf = h;
-    const double* st = h + kStatBands * kStatSpread;
+    const double* sf = h;  // the reduced sums: f's bands, then u's
+    const double* st = h + kStatBands;
     const int b0 = stat_band(Kh);
     double ef_hi = 0.0, eu_hi = 0.0, eu = 0.0;
     for (int b = 0; b < kStatBands; ++b) {
-        const double ef = sf[b * kStatSpread], et = st[b * kStatSpread];
+        const double ef = sf[b], et = st[b];
         eu += et;
         if (b >= b0) {
             ef_hi += ef;
@@ -683,7 +693,7 @@ static GuardCtx* guard_ctx()
     if (hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess) return nullptr;
     if (hipEventCreateWithFlags(&c.evx, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c.evc, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void**)&c.host, kStatBytes, hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc((void**)&c.host, kStatAcc * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return nullptr;
     return &(ctxs[dev] = c);
 }
@@ -713,7 +723,7 @@ extern "C" int sfa_helmholtz3d_guarded(const double* f, double* u, int64_t nbatc
     GuardCtx* g = guard_ctx();
     SFA_CHECK(g != nullptr, SFA_ERR_HIP, "guard stream/event/pinned buffer creation failed");
     double* stats = (double*)((char*)work + nbatch * core);
-    hipLaunchKernelGGL(zero_doubles_kernel, dim3(1), dim3(256), 0, s, stats, kStatBytes / 8);
+    hipLaunchKernelGGL(zero_doubles_kernel, dim3(32), dim3(256), 0, s, stats, kStatBytes / 8);
     SFA_LAUNCH_CHECK();
     const int Kh = mixed_planes(Nz, kh_hint);
     int rc = helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann, Kh, plan, work, s, nullptr, nbatch,
@@ -722,7 +732,10 @@ extern "C" int sfa_helmholtz3d_guarded(const double* f, double* u, int64_t nbatc
     // the statistics are complete after the x-pass: copy them on the side
     // stream while the inverse y and z passes run
     SFA_HIP(hipStreamWaitEvent(g->side, g->evx, 0));
-    SFA_HIP(hipMemcpyAsync(g->host, stats, kStatBytes, hipMemcpyDeviceToHost, g->side));
+    hipLaunchKernelGGL(stat_reduce_kernel, dim3(1), dim3(64), 0, g->side, stats);
+    SFA_LAUNCH_CHECK();
+    SFA_HIP(hipMemcpyAsync(g->host, stats + kStatSumOff, kStatAcc * sizeof(double),
+                           hipMemcpyDeviceToHost, g->side));
     SFA_HIP(hipEventRecord(g->evc, g->side));
     SFA_HIP(hipEventSynchronize(g->evc));
     double e = guard_predict(g->host, Kh, Nx, Ny, Nz, alpha);

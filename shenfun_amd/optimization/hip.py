@@ -445,7 +445,7 @@ def LU_Biharmonic(a0, alfa, beta, sii, siu, siuu, ail, aii, aiu, bill, bil, bii,
     t = outs[0].t
     ndim = t.dim() - 1
     ax = axis % ndim
-    N = int(np.size(sii)) + 4
+    N = (sii.numel() if isinstance(sii, torch.Tensor) else int(np.size(sii))) + 4
     shape = list(t.shape[1:])
     shape[ax] = N
     al = _per_line(alfa, shape, ax)

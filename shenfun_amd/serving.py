@@ -453,7 +453,7 @@ class Server:
             mx = max((r.key[0] * 2) * (r.key[1] + 2) for r in reqs) * 8
             return 2 * n + mx * min(len(reqs), 256)
         shp = reqs[0].key[0]
-        ws = shp[0] * shp[1] * (shp[2] // 2 + 8) * 16  # workspace per solve (padded kz pitch)
+        ws = int(_lib.lib().sfa_helmholtz3d_workspace(*shp, 0))  # workspace per solve
         return 2 * n + len(reqs) * ws
 
     def _finish(self, r, status, t):

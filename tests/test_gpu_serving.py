@@ -77,14 +77,17 @@ def test_alone_equals_in_mix(served):
 
 
 def test_memory_admission_serialises(served):
-    """A memory budget below any batch admits one batch at a time; results
-    are unchanged."""
+    """A memory budget that fits the largest single request but no two of
+    them admits one request at a time (nothing is refused); results are
+    unchanged."""
     from shenfun_amd.serving import Request, Server
     _, reqs = served
     sub = [Request(r.rid, r.family, r.key, r.f, 0.0) for r in reqs[:40]]
-    srv = Server(slots=4, mem_budget=1)
+    one = max(Server._bytes([r]) for r in sub)
+    srv = Server(slots=4, mem_budget=one)
     srv.run(sub, realtime=False)
     for a, b in zip(sub, reqs[:40]):
+        assert a.status == "ok", a.status
         assert _rel(a.u, b.u.cpu().numpy()) <= 1e-13
 
 
