@@ -425,6 +425,25 @@ int sfa_helmholtz3d_guarded(const double* f, double* u, int64_t nbatch, int64_t 
                             const void* plan, void* work, int64_t work_bytes, int* kh_used,
                             int* kh_next, double* err_pred, void* stream);
 
+/* No-allocation variant of sfa_helmholtz3d (precision 0 or 1, no guard).
+ * It has no counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a file-scope device
+ * array of sfa_helmholtz3d_noalloc_max_bytes() bytes (32 MiB) per GPU, held
+ * until the process ends; the call makes no hipMalloc/hipFree and builds no
+ * table: it needs a plan from sfa_helmholtz3d_plan (which also builds the
+ * device tables) on the same device.  Shapes: Nx in {64, 128, 256, 512},
+ * power-of-two Ny in [16, 512] and Nz in [32, 512], whose workspace
+ * (sfa_helmholtz3d_workspace() minus the statistics area) fits the array
+ * (e.g. 128^3, 256 x 128 x 64); larger shapes are refused with
+ * SFA_ERR_ARG.  Every call on one GPU uses that GPU's one array: calls that
+ * overlap on one GPU (other streams or host threads) race.  The result is
+ * bitwise equal to sfa_helmholtz3d's. */
+int64_t sfa_helmholtz3d_noalloc_max_bytes(void);
+int sfa_helmholtz3d_noalloc(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
+                            double alpha, int neumann, int precision, int fp64_planes,
+                            const void* plan, void* stream);
+
 /* Profiling variant of sfa_helmholtz3d: runs the pipeline (precision and
  * fp64_planes as there) nrep times with hipEvents recorded on `stream` between
  * its stages and adds each stage's milliseconds into stage_ms[5] (z R2C, y

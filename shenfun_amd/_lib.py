@@ -63,6 +63,7 @@ _SIGS = {
     "sfa_helmholtz3d_stage_times": "vvllldiiivvlivv",
     "sfa_helmholtz3d_guarded": "vvlllldiidvvlvvvv",
     "sfa_helmholtz3d_plan_release": "v",
+    "sfa_helmholtz3d_noalloc": "vvllldiiivv",
     "sfa_biharmonic_lu": "iaidvvlvvvvvvvvv",
     "sfa_biharmonic_factor": "iaivvvvv",
     "sfa_biharmonic_solve": "vviiaaaidvvvvvvvv",
@@ -101,6 +102,8 @@ def lib():
         L.sfa_version.restype = _int
         L.sfa_helmholtz3d_workspace.restype = _i64
         L.sfa_helmholtz3d_workspace.argtypes = [_i64, _i64, _i64, _int]
+        L.sfa_helmholtz3d_noalloc_max_bytes.restype = _i64
+        L.sfa_helmholtz3d_noalloc_max_bytes.argtypes = []
         L.sfa_helmholtz3d_plan_bytes.restype = _i64
         L.sfa_helmholtz3d_plan_bytes.argtypes = [_i64, _i64, _i64]
         L.sfa_batch2d_plan_bytes.restype = _i64
@@ -123,7 +126,8 @@ def lib():
 
 def exported_symbols():
     return sorted(list(_SIGS) + ["sfa_version", "sfa_last_error", "sfa_helmholtz3d_workspace",
-                                 "sfa_helmholtz3d_plan_bytes", "sfa_batch2d_plan_bytes",
+                                 "sfa_helmholtz3d_plan_bytes", "sfa_helmholtz3d_noalloc_max_bytes",
+                                 "sfa_batch2d_plan_bytes",
                                  "sfa_batch2d_workspace", "sfa_batch2d_variants",
                                  "sfa_batch2d_leg_elems", "sfa_batch2d_dft_elems",
                                  "sfa_batch2d_plan_ext_bytes", "sfa_batch2d_wide_plan_bytes",

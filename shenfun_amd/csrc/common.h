@@ -106,6 +106,19 @@ int make_geom(int ndim, const int64_t* shape, const int64_t* strides, int axis, 
 const double2* fft_table(int n);
 const double2* quarter_table(int N);
 
+// While a NoAllocScope lives on this host thread, the lazily built device
+// tables (twiddles, Galerkin diagonals, stencils) are looked up only: a table
+// that does not exist yet makes its getter return NULL (and record the error)
+// instead of calling hipMalloc.  Used by the no-allocation entry points.
+bool alloc_forbidden();
+struct NoAllocScope {
+    NoAllocScope();
+    ~NoAllocScope();
+    NoAllocScope(const NoAllocScope&) = delete;
+    NoAllocScope& operator=(const NoAllocScope&) = delete;
+    bool prev;
+};
+
 int device_cu_count();
 
 }  // namespace sfa

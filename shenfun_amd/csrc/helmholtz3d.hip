@@ -38,6 +38,7 @@ static const double* helm_tables(int N, int neumann)
     auto key = std::make_pair(N * 2 + neumann, dev);
     auto it = g_helm_tabs.find(key);
     if (it != g_helm_tabs.end()) return it->second;
+    if (alloc_forbidden()) return nullptr;
     const int M = N - 2;
     const double pi = 3.14159265358979323846;
     std::vector<double> h(6 * (size_t)M, 0.0);
@@ -215,6 +216,7 @@ static const double* cheb_k2(int N, int neumann)
     auto key = std::make_pair(N * 2 + neumann, dev);
     auto it = g_k2.find(key);
     if (it != g_k2.end()) return it->second;
+    if (alloc_forbidden()) return nullptr;
     std::vector<double> h(N - 2);
     for (int k = 0; k < N - 2; ++k) {
         const double kk = k;
@@ -568,6 +570,11 @@ extern "C" int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double a
                        (int)L.E, neumann, alpha, (const double*)(pb + L.off_s), L.nu,
                        (double4*)(pb + L.off_bs));
     SFA_LAUNCH_CHECK();
+    // the device tables of the pipeline's kernels are built here, at set-up,
+    // so that the no-allocation solve (which only looks them up) finds them
+    SFA_CHECK(fft_table((int)Nx) && quarter_table((int)Nx) && fft_table((int)Ny) &&
+                  fft_table((int)Nz) && cheb_k2((int)Nx, neumann),
+              SFA_ERR_HIP, "table allocation failed");
     // the host vectors are copied before they go out of scope
     SFA_HIP(hipStreamSynchronize(s));
     {
@@ -612,6 +619,80 @@ extern "C" int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t N
 {
     return sfa_helmholtz3d_many(f, u, 1, Nx, Ny, Nz, alpha, neumann, precision, fp64_planes, plan,
                                 work, work_bytes, stream);
+}
+
+// ------------------------------------------------------------- no-allocation solve
+// The workspace of sfa_helmholtz3d_noalloc: one file-scope device array per
+// GPU, held until the process ends.  Host code never passes its name to a
+// kernel (in host code the name is a host-side stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, once per
+// device, and only that address is used.
+static constexpr int64_t kNoAllocScratchBytes = 32ll << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::map<int, void*> g_scratch_ptr;  // device -> address of its `scratch`
+
+static void* scratch_address(int* rc)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        *rc = fail(SFA_ERR_HIP, "hipGetDevice failed: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        *rc = fail(SFA_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                   hipGetErrorString(e));
+        return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+    return p;
+}
+
+extern "C" int64_t sfa_helmholtz3d_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+extern "C" int sfa_helmholtz3d_noalloc(const double* f, double* u, int64_t Nx, int64_t Ny,
+                                       int64_t Nz, double alpha, int neumann, int precision,
+                                       int fp64_planes, const void* plan, void* stream)
+{
+    SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
+              "need Nx >= 8, Ny >= 1, even Nz >= 2");
+    SFA_CHECK(f != nullptr && u != nullptr, SFA_ERR_ARG, "NULL f or u");
+    SFA_CHECK(precision == 0 || precision == 1, SFA_ERR_ARG, "precision must be 0 or 1");
+    SFA_CHECK(plan != nullptr && plan_ok(plan, Nx, Ny, Nz, alpha, neumann), SFA_ERR_ARG,
+              "the no-allocation solve needs a plan built by sfa_helmholtz3d_plan for these "
+              "sizes/alpha/bc");
+    auto pow2 = [](int64_t n) { return n > 0 && (n & (n - 1)) == 0; };
+    SFA_CHECK((Nx == 64 || Nx == 128 || Nx == 256 || Nx == 512) && pow2(Ny) && Ny >= 16 &&
+                  Ny <= 512 && pow2(Nz) && Nz >= 32 && Nz <= 512,
+              SFA_ERR_UNSUPPORTED,
+              "the no-allocation solve takes Nx in {64, 128, 256, 512} and power-of-two Ny in "
+              "[16, 512], Nz in [32, 512]");
+    const int64_t need = core_workspace(Nx, Ny, Nz);
+    SFA_CHECK(need <= kNoAllocScratchBytes, SFA_ERR_ARG,
+              "%lld x %lld x %lld needs %lld bytes of workspace; the no-allocation solve holds "
+              "%lld bytes (use sfa_helmholtz3d with a caller workspace)",
+              (long long)Nx, (long long)Ny, (long long)Nz, (long long)need,
+              (long long)kNoAllocScratchBytes);
+    int rc = SFA_OK;
+    void* w = scratch_address(&rc);
+    if (w == nullptr) return rc;
+    NoAllocScope guard;
+    // every table the kernels read was built by sfa_helmholtz3d_plan
+    SFA_CHECK(fft_table((int)Nx) && quarter_table((int)Nx) && fft_table((int)Ny) &&
+                  fft_table((int)Nz) && cheb_k2((int)Nx, neumann),
+              SFA_ERR_ARG, "device tables missing: call sfa_helmholtz3d_plan on this device first");
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == 1)
+        return helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann, mixed_planes(Nz, fp64_planes),
+                                 plan, w, s, nullptr, 1);
+    return helmholtz3d_fp64(f, u, Nx, Ny, Nz, alpha, neumann, plan, w, s, nullptr, 1);
 }
 
 namespace sfa {

@@ -61,6 +61,13 @@ int make_geom(int ndim, const int64_t* shape, const int64_t* strides, int axis, 
     return SFA_OK;
 }
 
+// ------------------------------------------------------------- allocation guard
+static thread_local bool g_no_alloc = false;
+
+bool alloc_forbidden() { return g_no_alloc; }
+NoAllocScope::NoAllocScope() : prev(g_no_alloc) { g_no_alloc = true; }
+NoAllocScope::~NoAllocScope() { g_no_alloc = prev; }
+
 // ------------------------------------------------------------- twiddles
 static std::mutex g_tab_mu;
 static std::map<std::pair<int, int>, std::pair<int, double2*>> g_tabs;  // (kind, n) -> (device, ptr)
@@ -73,6 +80,10 @@ static const double2* get_table(int kind, int n)
     auto key = std::make_pair(kind * 64 + dev, n);
     auto it = g_tabs.find(key);
     if (it != g_tabs.end()) return it->second.second;
+    if (g_no_alloc) {
+        fail(SFA_ERR_ARG, "twiddle table of length %d was not built before a no-allocation call", n);
+        return nullptr;
+    }
     std::vector<double2> h(n);
     const long double pi = 3.141592653589793238462643383279502884L;
     for (int m = 0; m < n; ++m) {
