@@ -168,6 +168,42 @@ def batched_leg(sfa, torch, n=4096, seed=0):
     }
 
 
+def cheb_kinds_leg(sfa, torch, reps=50):
+    """Chebyshev-SD scalar product / backward along axis 0 of (N, N, N/2+1)
+    complex arrays: the DCT kernels (kind 'fast') against the dense
+    contraction on FP64 MFMA (kind 'vandermonde'), microseconds per call from
+    HIP events, and their largest relative difference."""
+    import numpy as np
+    out = []
+    for N in (32, 64, 128, 256):
+        V = sfa.FunctionSpace(N, "chebyshev", bc=(0, 0))
+        rng = np.random.default_rng(0)
+        shp = (N, N, N // 2 + 1)
+        x = torch.from_numpy(rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).cuda()
+        res = {"N": N, "lines": shp[1] * shp[2]}
+        ys = {}
+        for kind in ("fast", "vandermonde"):
+            y = V.scalar_product(x, torch.empty_like(x), kind=kind)
+            z = V.backward(y, torch.empty_like(x), kind=kind)
+            torch.cuda.synchronize()
+            for op in ("scalar_product", "backward"):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    if op == "scalar_product":
+                        V.scalar_product(x, y, kind=kind)
+                    else:
+                        V.backward(y, z, kind=kind)
+                e1.record()
+                torch.cuda.synchronize()
+                res[f"{kind}_{op}_us"] = e0.elapsed_time(e1) / reps * 1e3
+            ys[kind] = y
+        res["max_rel_diff"] = float((ys["fast"] - ys["vandermonde"]).abs().max() / ys["fast"].abs().max())
+        out.append(res)
+    return out
+
+
 def c3_leg(sfa, torch, steps, warmup, N=512):
     """The mixed-precision configuration at its own shape, 512^3: guarded
     mixed and FP64 solves/s (HIP-event stage times on the launch stream),
@@ -261,7 +297,7 @@ def main():
                     help="skip the 4096-problem batched 2D Legendre x Fourier leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the 512^3 leg")
-    ap.add_argument("--only", choices=("batched", "served"), default=None,
+    ap.add_argument("--only", choices=("batched", "served", "cheb_kinds"), default=None,
                     help="run one leg alone and print its JSON (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
@@ -290,6 +326,9 @@ def main():
 
     if args.only == "batched":
         print(json.dumps({"batched2d": batched_leg(sfa, torch)}), flush=True)
+        return
+    if args.only == "cheb_kinds":
+        print(json.dumps({"chebyshev_kinds": cheb_kinds_leg(sfa, torch)}), flush=True)
         return
     if args.only == "served":
         print(json.dumps({"served": serving_leg(torch),

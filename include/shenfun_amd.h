@@ -367,8 +367,11 @@ int sfa_poisson_add_solve(const void* b, void* u, int dtype, int ndim, const int
  * fills a device buffer of sfa_helmholtz3d_plan_bytes() bytes with, for every
  * distinct beta, the LU recurrence state at 16 segment starts of each parity
  * chain (the solve regenerates the factors of a segment from its start
- * state), and the line -> entry map, synchronising `stream`.  plan may be NULL in sfa_helmholtz3d, in which case the factors
- * are computed on the fly (wavefront scan of the LU recurrence).
+ * state), and the line -> entry map, asynchronously on `stream` (the line
+ * map and the distinct ky^2 + kz^2 of a (Ny, Nz) are built on the host and
+ * uploaded once per process and device, a blocking step of the first plan
+ * of that shape).  plan may be NULL in sfa_helmholtz3d, in which case the
+ * factors are computed on the fly (wavefront scan of the LU recurrence).
  * precision: 0 = FP64 throughout.  1 = mixed: after the FP64 R2C of f, the
  * kz planes [0, fp64_planes) stay complex128 with FP64 transforms and the
  * planes [fp64_planes, Nz/2] are stored and transformed as complex64; the
@@ -404,9 +407,9 @@ int sfa_helmholtz3d_many(const double* f, double* u, int64_t nbatch, int64_t Nx,
                          const void* plan, void* work, int64_t work_bytes, void* stream);
 
 /* Guarded mixed precision: the mixed pipeline of sfa_helmholtz3d_many with
- * the first complex64 plane kh_hint (<= 0: 17), whose z R2C and x-pass also
- * sum, per band of 16 kz planes, the energy of f's z spectrum and of u's
- * Chebyshev coefficients.  After the x-pass the host reads these sums (the
+ * the first complex64 plane kh_hint (<= 0: 17), whose x-pass also sums, per
+ * band of 16 kz planes, the energy of its input lines (f after the z and y
+ * transforms) and of u's Chebyshev coefficients.  After the x-pass the host reads these sums (the
  * inverse passes are already queued) and predicts the relative max error of
  * the complex64 part: 64 eps32 sqrt((w^2 E_f,hi + E_u,hi) / E_u), E_hi the
  * energies of the planes at and above the cut, w = 1 / (alpha + Kh^2) the

@@ -11,7 +11,10 @@ path reads, with the HIP backend as the only executor.
 * ``config['transforms']['kind'][family]`` -- transform flavour per family
   (shenfun: 'fast' | 'recursive' | 'vandermonde', config.py:16-30).  'fast'
   runs the LDS FFT/DCT kernels, 'recursive' and 'vandermonde' run the dense
-  per-line contraction (the Legendre default, like shenfun's 'recursive').
+  per-line contraction on FP64 MFMA (the Legendre default, like shenfun's
+  'recursive'); 'auto' (the Chebyshev default here) takes the dense
+  contraction up to 128 points and the DCT kernels above (the faster of the
+  two on the MI355X at each size).
 * ``config['precision']`` -- 'fp64' or 'mixed': the default precision of
   ``HelmholtzSolver3D`` when none is given.
 """
@@ -19,7 +22,7 @@ config = {
     "optimization": {"mode": "hip", "verbose": False},
     "transforms": {
         "kind": {
-            "chebyshev": "fast",
+            "chebyshev": "auto",
             "legendre": "recursive",
             "fourier": "fast",
         }

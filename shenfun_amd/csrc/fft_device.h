@@ -8,11 +8,12 @@ namespace sfa {
 // Spectrum statistics of the guarded mixed-precision solve: per band of 16
 // kz planes (band 0 = kz 0..16, band b >= 1 = kz 16b+1 .. 16b+16, so the
 // band edges are the admissible first complex64 planes 16m+1), the energy of
-// f's z spectrum (accumulators 0 .. kStatBands-1) and that of u's Chebyshev
-// coefficients (kStatBands .. 2 kStatBands-1).  Each accumulator is spread
+// the x-pass input lines (f after the z and y transforms, accumulators 0 ..
+// kStatBands-1) and that of u's Chebyshev coefficients (kStatBands ..
+// 2 kStatBands-1), both summed by the x-pass.  Each accumulator is spread
 // over kStatSlots slots of one 128-byte line each, picked by workgroup index:
 // thousands of waves adding into a handful of addresses serialise on them
-// (the z R2C at 256^3 took 117 us instead of 50).  stat_reduce_kernel sums the
+// (a z R2C summing per wave at 256^3 took 117 us instead of 50).  stat_reduce_kernel sums the
 // slots into the 2 kStatBands doubles at kStatSumOff.
 constexpr int kStatBands = 16;
 constexpr int kStatSlots = 64;

@@ -435,12 +435,29 @@ struct XPass2Args {
     double* stats;       // STATS kernels: band energies of the solution's coefficients
 };
 
+// Statistics of the guarded mixed form: lane jb == 0 of mapping-B line tb of
+// block bid adds w_kz e to accumulator first + band(kz) (slot bid % kStatSlots).
+template <int N>
+__device__ __forceinline__ void xpass_stat_add(const XPass2Args& a, int64_t bid, int tb, int jb, int first,
+                                               double e)
+{
+    const int64_t Ls = bid * TL + tb;
+    if (jb != 0 || Ls >= a.nl) return;
+    const int64_t Lr = Ls % a.nlb;
+    const int64_t ky = Lr / a.Kp, kz = a.kz0 + (Lr - ky * a.Kp);
+    if (kz >= a.Nzh) return;  // a pitch-padding line
+    const double wz = (kz == 0 || kz == a.Nzh - 1) ? 1.0 : 2.0;
+    atomicAdd(a.stats + stat_index(first + stat_band((int)kz), (int)(bid % kStatSlots)), wz * e);
+}
+
 // C2 = double2: FP64 everywhere.  C2 = float2: complex64 storage and DCT/FFT
 // arithmetic, the Helmholtz solve still FP64 (values widened on entry).
-// STATS: also accumulate, per band of 16 kz planes, the energy of the
-// solution's Chebyshev coefficients t (after to_ortho),
-// w_kz (|t_0|^2 + 1/2 sum_{k>=1} |t_k|^2) per line (w_kz = 2 for 0 < kz < Nz/2,
-// else 1), into accumulator kStatBands + band of a.stats (slot bid % kStatSlots).
+// STATS: also accumulate, per band of 16 kz planes (w_kz = 2 for 0 < kz < Nz/2,
+// else 1; slot bid % kStatSlots), the energy of the line's input,
+// w_kz sum_x |F_x|^2, into accumulator band of a.stats (from the forward FFT
+// by Parseval, sum |V_k|^2 = N sum |F_x|^2), and that of the solution's
+// Chebyshev coefficients t (after to_ortho), w_kz (|t_0|^2 + 1/2 sum_{k>=1}
+// |t_k|^2), into accumulator kStatBands + band.
 template <int N, typename C2, bool STATS = false>
 __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
 {
@@ -513,6 +530,17 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     __syncthreads();  // exchange, and the switch to mapping B
     C2* ln = ex + tb * LA;
     pow2_fft_p2<N, false>(v, jb, ln);
+    if (STATS) {  // the line's input energy, added at once (nothing held through the solve)
+        double ef = 0.0;
+#pragma unroll
+        for (int r = 0; r < P; ++r) {
+            const double2 c = to_c64(v[r]);
+            ef += c.x * c.x + c.y * c.y;
+        }
+#pragma unroll
+        for (int off = Q / 2; off >= 1; off >>= 1) ef += __shfl_xor(ef, off);
+        xpass_stat_add<N>(a, bid, tb, jb, 0, ef * (1.0 / N));
+    }
     wave_sync();
 #pragma unroll
     for (int h = 0; h < P / Q; ++h)
@@ -601,16 +629,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         }
 #pragma unroll
         for (int off = Q / 2; off >= 1; off >>= 1) el += __shfl_xor(el, off);
-        const int64_t Ls = bid * TL + tb;
-        if (jb == 0 && Ls < a.nl) {
-            const int64_t Lr = Ls % a.nlb;
-            const int64_t ky = Lr / a.Kp, kz = a.kz0 + (Lr - ky * a.Kp);
-            if (kz < a.Nzh) {
-                const double wz = (kz == 0 || kz == a.Nzh - 1) ? 1.0 : 2.0;
-                atomicAdd(a.stats + stat_index(kStatBands + stat_band((int)kz), (int)(bid % kStatSlots)),
-                          wz * el);
-            }
-        }
+        xpass_stat_add<N>(a, bid, tb, jb, kStatBands, el);
     }
     wave_sync();
 #pragma unroll
@@ -782,15 +801,10 @@ namespace sfa {
 // ----------------------------------------------------------------- kz-split z passes
 // R2C of contiguous rows of 2H reals (FP64 arithmetic), output split by kz:
 // k < Kh -> o64 (row pitch Kh, complex128), k >= Kh -> o32 (row pitch H+1-Kh, complex64).
-// STATS: also accumulate the energy sum_k w_k |X_k|^2 of the output per band
-// of 16 kz planes (w_k = 2 for 0 < k < H, else 1: the R2C half spectrum)
-// into accumulator band of stats (slot: the row block modulo kStatSlots): one
-// vector atomic per wave.
-template <int H, bool STATS, int NR = TL>
+template <int H, int NR = TL>
 __device__ __forceinline__ void r2c_split_body(const double* __restrict__ in, double2* __restrict__ o64,
                                                float2* __restrict__ o32, int64_t nrows, int Kh,
-                                               double scale, const double2* __restrict__ twg,
-                                               double* __restrict__ stats, int64_t R0)
+                                               double scale, const double2* __restrict__ twg, int64_t R0)
 {
     using Pl = Pow2<H>;
     constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
@@ -823,9 +837,6 @@ __device__ __forceinline__ void r2c_split_body(const double* __restrict__ in, do
 #pragma unroll
         for (int k2 = 0; k2 < Q; ++k2) ln[pow2_out_index<H>(j, h, k2)] = a[h * Q + k2];
     __syncthreads();
-    double en[P + 2];
-#pragma unroll
-    for (int i = 0; i < P + 2; ++i) en[i] = 0.0;
     if (valid) {
 #pragma unroll
         for (int i = 0; i <= P; ++i) {
@@ -837,44 +848,17 @@ __device__ __forceinline__ void r2c_split_body(const double* __restrict__ in, do
             const double2 X = make_double2(0.5 * (zk.x + zc.x + d.y) * scale, 0.5 * (zk.y + zc.y - d.x) * scale);
             if (k < Kh) o64[R * Kh + k] = X;
             else o32[R * Kl + (k - Kh)] = from_c64<float2>(X);
-            if (STATS) en[i] = ((k == 0 || k == H) ? 1.0 : 2.0) * (X.x * X.x + X.y * X.y);
         }
-    }
-    if (STATS) {
-        // thread j holds k = j + Q i.  With G = 16 / Q elements per band,
-        // element i lies in band i / G for j >= 1; for j == 0 (k = Q i) in
-        // band (i - 1) / G, so its elements are shifted down by one and k = 0
-        // is added to band 0.
-        constexpr int G = 16 / Q;
-        constexpr int NBL = P / G + 1;
-        constexpr int NB = (H - 1) / 16 + 1;
-        static_assert(NB <= kStatBands && NB <= 64, "band count");
-        double acc[NBL];
-#pragma unroll
-        for (int b = 0; b < NBL; ++b) acc[b] = 0.0;
-#pragma unroll
-        for (int i = 0; i <= P; ++i) acc[i / G] += (j == 0) ? en[i + 1] : en[i];
-        if (j == 0) acc[0] += en[0];
-#pragma unroll
-        for (int b = 0; b < NBL; ++b)
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) acc[b] += __shfl_xor(acc[b], off);
-        const int lane = threadIdx.x & 63;
-        double mine = 0.0;
-#pragma unroll
-        for (int b = 0; b < NBL; ++b)
-            if (lane == b) mine = acc[b];
-        if (lane < NB) atomicAdd(stats + stat_index(lane, (int)((R0 / NR) % kStatSlots)), mine);
     }
 }
 
-template <int H, bool STATS>
+template <int H>
 __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     r2c_split_kernel(const double* __restrict__ in, double2* __restrict__ o64,
                      float2* __restrict__ o32, int64_t nrows, int Kh, double scale,
-                     const double2* __restrict__ twg, double* __restrict__ stats)
+                     const double2* __restrict__ twg)
 {
-    r2c_split_body<H, STATS>(in, o64, o32, nrows, Kh, scale, twg, stats, (int64_t)blockIdx.x * TL);
+    r2c_split_body<H>(in, o64, o32, nrows, Kh, scale, twg, (int64_t)blockIdx.x * TL);
 }
 
 // C2R (FP64 arithmetic) of rows merged from o64 (k < Kh) and o32 (k >= Kh).
@@ -983,7 +967,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 
 template <int H, bool SPLIT>
 static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int Kh, double scale,
-                        hipStream_t s, double* stats)
+                        hipStream_t s)
 {
     using Pl = Pow2<H>;
     const double2* tw = fft_table(2 * H);
@@ -991,11 +975,11 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
     const size_t lds = sizeof(double2) * ((size_t)TL * zrow_stride<H>() + H);
     dim3 grid((unsigned)((nrows + TL - 1) / TL)), block(TL * Pl::Q);
     if (SPLIT) {
-        auto k = stats ? r2c_split_kernel<H, true> : r2c_split_kernel<H, false>;
+        auto k = r2c_split_kernel<H>;
         if (lds > 64 * 1024)
             hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double*)in, (double2*)o64, (float2*)o32,
-                           nrows, Kh, scale, tw, stats);
+                           nrows, Kh, scale, tw);
     } else {
         auto k = Kh <= 3 * Pl::Q ? c2r_merge_kernel<H, true> : c2r_merge_kernel<H, false>;
         if (lds > 64 * 1024)
@@ -1008,14 +992,14 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
 }
 
 // split (r2c=true: in -> o64/o32) or merge (r2c=false: o64/o32 -> in) of
-// contiguous rows; stats (split only, may be NULL): band energies of the output
+// contiguous rows
 int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
-             double scale, hipStream_t s, int* rc, double* stats)
+             double scale, hipStream_t s, int* rc)
 {
 #define SFA_SP(HH)                                                                            \
     case HH:                                                                                  \
-        *rc = r2c ? launch_split<HH, true>(in, o64, o32, nrows, Kh, scale, s, stats)          \
-                  : launch_split<HH, false>(in, o64, o32, nrows, Kh, scale, s, nullptr);      \
+        *rc = r2c ? launch_split<HH, true>(in, o64, o32, nrows, Kh, scale, s)                 \
+                  : launch_split<HH, false>(in, o64, o32, nrows, Kh, scale, s);               \
         return 1;
     switch (H) {
         SFA_SP(16) SFA_SP(32) SFA_SP(64) SFA_SP(128) SFA_SP(256)

@@ -11,9 +11,12 @@
 //   z: C2R                         W -> u
 #include <math.h>
 
+#include <string.h>
+
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "fft_device.h"
@@ -246,7 +249,7 @@ int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int 
 int fast_c2c_mixed(const Geom& g64, void* W64, const Geom& g32, void* W32, int sign, double scale,
                    hipStream_t s, int* rc);
 int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
-             double scale, hipStream_t s, int* rc, double* stats);
+             double scale, hipStream_t s, int* rc);
 
 // ------------------------------------------------------------------ plan
 // Device plan of the fused solve, the counterpart of chebyshev.la.Helmholtz's
@@ -290,6 +293,48 @@ static void plan_shape(int64_t Nx, int64_t Ny, int64_t Nz, PlanLayout* L, std::v
         svals->resize(us.size());
         for (size_t i = 0; i < us.size(); ++i) (*svals)[i] = (double)us[i];
     }
+}
+
+// The plan's line map and distinct s = ky^2 + kz^2 depend on (Ny, Nz) only:
+// per device and (Ny, Nz) they are built on the host and uploaded once (the
+// first plan of a shape synchronises), then every plan copies them on its
+// stream, so that planning a new alpha is asynchronous.
+struct PlanHead {
+    void* d;        // [linemap | s] as in the plan, off_bs bytes
+    PlanLayout L;   // layout at Nx = 8 (E, K and bytes are recomputed per Nx)
+};
+static std::mutex g_head_mu;
+static std::map<std::tuple<int, int64_t, int64_t>, PlanHead> g_heads;
+
+static int plan_head(int64_t Ny, int64_t Nz, PlanHead* out)
+{
+    int dev = 0;
+    SFA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_head_mu);
+    const auto key = std::make_tuple(dev, Ny, Nz);
+    auto it = g_heads.find(key);
+    if (it != g_heads.end()) {
+        *out = it->second;
+        return SFA_OK;
+    }
+    SFA_CHECK(!alloc_forbidden(), SFA_ERR_ARG, "plan head of %lld x %lld was not built",
+              (long long)Ny, (long long)Nz);
+    PlanHead h;
+    std::vector<int> map;
+    std::vector<double> sv;
+    plan_shape(8, Ny, Nz, &h.L, &map, &sv);
+    std::vector<char> hb((size_t)h.L.off_bs, 0);
+    memcpy(hb.data(), map.data(), map.size() * sizeof(int));
+    memcpy(hb.data() + h.L.off_s, sv.data(), sv.size() * sizeof(double));
+    SFA_HIP(hipMalloc(&h.d, hb.size()));
+    const hipError_t e = hipMemcpy(h.d, hb.data(), hb.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(h.d);
+        return fail(SFA_ERR_HIP, "plan head upload failed: %s", hipGetErrorString(e));
+    }
+    g_heads[key] = h;
+    *out = h;
+    return SFA_OK;
 }
 
 struct PlanInfo {
@@ -410,7 +455,7 @@ static int y_pass_mixed(const Geom& g64, const Geom& g32, double2* W64, float2* 
 // nb arrays (Nx, Ny, Nz) back to back in f and u; the workspace holds the nb
 // arrays' complex128 planes, then their complex64 planes.
 // stats (may be NULL): zeroed band-energy accumulators (kStatBytes) that the
-// z R2C and the x-pass add to; evx (may be NULL): recorded after the x-pass.
+// x-pass adds to; evx (may be NULL): recorded after the x-pass.
 static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                              double alpha, int neumann, int Kh, const void* plan, void* work,
                              hipStream_t s, hipEvent_t* ev, int64_t nb = 1, double* stats = nullptr,
@@ -442,7 +487,7 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
     };
     const Geom g64 = geom(Kh), g32 = geom(Kl);
     mark(0);
-    if (!split_rc(true, (int)(Nz / 2), f, W64, W32, rows, Kh, 1.0 / (double)Nz, s, &rc, stats))
+    if (!split_rc(true, (int)(Nz / 2), f, W64, W32, rows, Kh, 1.0 / (double)Nz, s, &rc))
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz=%lld unsupported", (long long)Nz);
     if (rc) return rc;
     mark(1);
@@ -456,7 +501,7 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
     if (evx) SFA_HIP(hipEventRecord(evx, s));
     if ((rc = y_pass_mixed(g64, g32, W64, W32, +1, 1.0, s))) return rc;
     mark(4);
-    if (!split_rc(false, (int)(Nz / 2), u, W64, W32, rows, Kh, 1.0, s, &rc, nullptr))
+    if (!split_rc(false, (int)(Nz / 2), u, W64, W32, rows, Kh, 1.0, s, &rc))
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz unsupported");
     mark(5);
     return rc;
@@ -554,16 +599,17 @@ extern "C" int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double a
     SFA_CHECK(Nx >= 8 && Ny >= 1 && Nz >= 2 && Nz % 2 == 0, SFA_ERR_ARG,
               "need Nx >= 8, Ny >= 1, even Nz >= 2");
     SFA_CHECK(plan != nullptr, SFA_ERR_ARG, "plan buffer is NULL");
-    PlanLayout L;
-    std::vector<int> map;
-    std::vector<double> sv;
-    plan_shape(Nx, Ny, Nz, &L, &map, &sv);
+    PlanHead head;
+    int rc = plan_head(Ny, Nz, &head);
+    if (rc) return rc;
+    PlanLayout L = head.L;
+    L.K = (Nx - 2 + 1) / 2;
+    L.E = (L.K + 15) / 16;
     SFA_CHECK(plan_bytes >= L.bytes, SFA_ERR_ARG, "plan buffer too small (%lld < %lld)",
               (long long)plan_bytes, (long long)L.bytes);
     hipStream_t s = (hipStream_t)stream;
     char* pb = (char*)plan;
-    SFA_HIP(hipMemcpyAsync(pb, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    SFA_HIP(hipMemcpyAsync(pb + L.off_s, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    SFA_HIP(hipMemcpyAsync(pb, head.d, (size_t)L.off_bs, hipMemcpyDeviceToDevice, s));
     const int M = (int)Nx - 2;
     const int64_t nth = 2 * L.nu;
     hipLaunchKernelGGL(helm_bstate_kernel, dim3((unsigned)((nth + 127) / 128)), dim3(128), 0, s, M,
@@ -575,8 +621,6 @@ extern "C" int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double a
     SFA_CHECK(fft_table((int)Nx) && quarter_table((int)Nx) && fft_table((int)Ny) &&
                   fft_table((int)Nz) && cheb_k2((int)Nx, neumann),
               SFA_ERR_HIP, "table allocation failed");
-    // the host vectors are copied before they go out of scope
-    SFA_HIP(hipStreamSynchronize(s));
     {
         std::lock_guard<std::mutex> lk(g_plan_mu);
         g_plans[plan] = PlanInfo{Nx, Ny, Nz, plan_bytes, alpha, neumann, L.off_s, L.off_bs};
@@ -702,14 +746,19 @@ __global__ void zero_doubles_kernel(double* p, int n)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
-// the slots of each statistics accumulator summed into stats[kStatSumOff + acc]
-__global__ void stat_reduce_kernel(double* stats)
+// the slots of each statistics accumulator summed into stats[kStatSumOff + acc]:
+// 8 lanes per accumulator, each summing every 8th slot, then a butterfly
+constexpr int kStatReduceThreads = kStatAcc * 8;
+__global__ void __launch_bounds__(kStatReduceThreads) stat_reduce_kernel(double* stats)
 {
-    const int acc = threadIdx.x;
-    if (acc >= kStatAcc) return;
+    const int acc = threadIdx.x >> 3, part = threadIdx.x & 7;
     double s = 0.0;
-    for (int sl = 0; sl < kStatSlots; ++sl) s += stats[stat_index(acc, sl)];
-    stats[kStatSumOff + acc] = s;
+#pragma unroll
+    for (int sl = part; sl < kStatSlots; sl += 8) s += stats[stat_index(acc, sl)];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    s += __shfl_xor(s, 4);
+    if (part == 0) stats[kStatSumOff + acc] = s;
 }
 
 // Error model of the mixed form.  Complex64 storage and FP32 transforms of
@@ -719,9 +768,10 @@ __global__ void stat_reduce_kernel(double* stats)
 // 1 / (alpha + Kh^2), and backward (DCT-III, inverse y FFT, the stored
 // spectrum) on the solution's own values.  Rounding errors add like noise, so
 // the rms error of u is about eps32 sqrt(w^2 E_f,hi + E_u,hi) with E the grid
-// energies of f and u in those planes (Parseval: E_f = Nz sum w_kz |X|^2 over
-// f's z spectrum X; E_u = Nx Ny Nz sum w_kz (|t_0|^2 + 1/2 sum |t_k|^2) over
-// u's Chebyshev coefficients t, the DCT-II being orthogonal on the
+// energies of f and u in those planes (Parseval: E_f = Ny Nz sum w_kz |F_x|^2
+// over the x-pass input lines F (kz, ky, x), the z spectrum scaled by 1/Nz and
+// the y transform by 1/Ny; E_u = Nx Ny Nz sum w_kz (|t_0|^2 + 1/2 sum |t_k|^2)
+// over u's Chebyshev coefficients t, the DCT-II being orthogonal on the
 // Gauss-Chebyshev points), and max|u| >= rms(u) = sqrt(E_u / (Nx Ny Nz)).
 // kGuardK (64) covers the number of rounding stages (about 30, adding like a
 // random walk) and the ratio of the largest to the rms error.  Bands
@@ -744,7 +794,7 @@ static double guard_predict(const double* h, int Kh, int64_t Nx, int64_t Ny, int
         }
     }
     const double vol = (double)Nx * (double)Ny * (double)Nz;
-    ef_hi *= (double)Nz;
+    ef_hi *= (double)Ny * (double)Nz;
     eu_hi *= vol;
     eu *= vol;
     const double den = alpha + (double)Kh * (double)Kh;
@@ -813,7 +863,7 @@ extern "C" int sfa_helmholtz3d_guarded(const double* f, double* u, int64_t nbatc
     // the statistics are complete after the x-pass: copy them on the side
     // stream while the inverse y and z passes run
     SFA_HIP(hipStreamWaitEvent(g->side, g->evx, 0));
-    hipLaunchKernelGGL(stat_reduce_kernel, dim3(1), dim3(64), 0, g->side, stats);
+    hipLaunchKernelGGL(stat_reduce_kernel, dim3(1), dim3(kStatReduceThreads), 0, g->side, stats);
     SFA_LAUNCH_CHECK();
     SFA_HIP(hipMemcpyAsync(g->host, stats + kStatSumOff, kStatAcc * sizeof(double),
                            hipMemcpyDeviceToHost, g->side));

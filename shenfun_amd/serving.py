@@ -185,6 +185,23 @@ def _largest_prime_factor(n):
     return max(f, n)
 
 
+class _RecentSet:
+    """The last `cap` keys added (a bounded set)."""
+
+    def __init__(self, cap):
+        self.cap = int(cap)
+        self.keys = collections.OrderedDict()
+
+    def __contains__(self, k):
+        return k in self.keys
+
+    def add(self, k):
+        self.keys[k] = None
+        self.keys.move_to_end(k)
+        while len(self.keys) > self.cap:
+            self.keys.popitem(last=False)
+
+
 def _use(stream, obj_ev):
     """The cached object, with `stream` ordered after its build."""
     obj, ev = obj_ev
@@ -224,31 +241,43 @@ class Server:
         self.batches = collections.Counter()
         self._work3d = {}   # slot -> workspace tensor (grown on demand)
         self._stage = {}    # slot -> pinned host staging tensor (grown on demand)
+        self._seen1d = _RecentSet(4096)  # 1D (N, bc, alpha) factor keys seen once
         self._now = lambda: 0.0
 
     # ------------------------------------------------------------ engines
+    # host inputs at least this many elements, already in pinned memory, are
+    # copied to the device directly; smaller ones are gathered in the slot's
+    # pinned staging buffer and copied together (one copy call per batch)
+    DIRECT_UPLOAD = 1 << 16
+
     def _stage_upload(self, reqs, slot, stream):
         """One float64 device array holding the requests' f back to back
-        (flattened), uploaded on `stream`: host inputs through the slot's
-        pinned staging buffer (one host-to-device copy), device inputs by a
-        device copy.  Returns (device array, element offsets)."""
+        (flattened), uploaded on `stream`: large pinned host inputs by direct
+        host-to-device copies, other host inputs through the slot's pinned
+        staging buffer, device inputs by a device copy.  Returns (device
+        array, element offsets)."""
         sizes = [r.f.numel() for r in reqs]
         off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
         total = int(off[-1])
-        host = [i for i, r in enumerate(reqs) if not r.f.is_cuda]
+        direct = [i for i, r in enumerate(reqs)
+                  if not r.f.is_cuda and sizes[i] >= self.DIRECT_UPLOAD and r.f.is_pinned()]
+        dset = set(direct)
+        staged = [i for i, r in enumerate(reqs) if not r.f.is_cuda and i not in dset]
         with torch.cuda.stream(stream):
             D = torch.empty(total, dtype=torch.float64, device=self.device)
-            if host:
+            for i in direct:
+                D[off[i]:off[i + 1]].copy_(reqs[i].f.reshape(-1), non_blocking=True)
+            if staged:
                 st = self._stage.get(slot)
                 if st is None or st.numel() < total:
                     st = torch.empty(max(total, 1 << 16), dtype=torch.float64).pin_memory()
                     self._stage[slot] = st
-                for i in host:
+                for i in staged:
                     st[off[i]:off[i + 1]].copy_(reqs[i].f.reshape(-1))
-                if len(host) == len(reqs):
+                if len(staged) == len(reqs):
                     D.copy_(st[:total], non_blocking=True)
                 else:
-                    for i in host:
+                    for i in staged:
                         D[off[i]:off[i + 1]].copy_(st[off[i]:off[i + 1]], non_blocking=True)
             for i, r in enumerate(reqs):
                 if r.f.is_cuda:
@@ -315,20 +344,39 @@ class Server:
         shp = _lib.arr([B, N])
         st = _lib.arr([N, 1])
         F, _ = self._stage_upload(reqs, slot, stream)
+        alphas = {float(r.key[2]) for r in reqs}
+        keys = [ckey]
+        abh = ab = None
+        if len(alphas) == 1:
+            # one alpha seen before: the cached factors of (N, bc, alpha),
+            # broadcast to every line (an alpha seen once is factorised with
+            # the batch, so requests with alphas of their own fill no cache)
+            fkey = ("cheb1d_lu", N, bc, next(iter(alphas)))
+            if fkey in self.cache.items or fkey in self._seen1d:
+                fac, fkey = self._cheb1d_factors(N, bc, next(iter(alphas)), diags, stream)
+                fst = _lib.arr([0, 1])
+                keys.append(fkey)
+            else:
+                self._seen1d.add(fkey)
         with torch.cuda.stream(stream):
             F = F.view(B, N)
-            # the batch's factors, one line per request (chebyshev/la.py:223-276)
-            ab = torch.tensor([-1.0] * B + [float(r.key[2]) for r in reqs], dtype=torch.float64)
-            ab = ab.to(self.device, non_blocking=False)
-            fac = torch.zeros((4, B, N), dtype=torch.float64, device=self.device)
             fh = torch.empty_like(F)
             U = torch.empty_like(F)
             sp = _lib.stream_ptr(stream)
-            A0, A2, A4, Bm2, B0, B2 = diags
-            _lib.call("sfa_helmholtz_lu", neumann, N, _lib.ptr(A0), _lib.ptr(A2), _lib.ptr(A4),
-                      _lib.ptr(Bm2), _lib.ptr(B0), _lib.ptr(B2), _lib.ptr(ab), _lib.ptr(ab[B:]), 1,
-                      _lib.ptr(fac[0]), _lib.ptr(fac[1]), _lib.ptr(fac[2]), _lib.ptr(fac[3]), 2, shp,
-                      st, 1, sp)
+            if len(keys) == 1:
+                # the batch's factors, one line per request (chebyshev/la.py:223-276):
+                # alfa = -1 and beta = alpha per request, staged in pinned memory and
+                # copied without a host wait (the host array is kept with the batch)
+                abh = torch.tensor([-1.0] * B + [float(r.key[2]) for r in reqs],
+                                   dtype=torch.float64).pin_memory()
+                ab = abh.to(self.device, non_blocking=True)
+                fac = torch.zeros((4, B, N), dtype=torch.float64, device=self.device)
+                fst = st
+                A0, A2, A4, Bm2, B0, B2 = diags
+                _lib.call("sfa_helmholtz_lu", neumann, N, _lib.ptr(A0), _lib.ptr(A2), _lib.ptr(A4),
+                          _lib.ptr(Bm2), _lib.ptr(B0), _lib.ptr(B2), _lib.ptr(ab), _lib.ptr(ab[B:]), 1,
+                          _lib.ptr(fac[0]), _lib.ptr(fac[1]), _lib.ptr(fac[2]), _lib.ptr(fac[3]), 2,
+                          shp, st, 1, sp)
             if dense is None:
                 _lib.call("sfa_cheb_scalar_product", _lib.ptr(F), _lib.ptr(fh), _lib.F64, 2, shp, st, st,
                           1, _lib.ptr(K2), sp)
@@ -336,7 +384,7 @@ class Server:
                 _lib.call("sfa_line_matmul", _lib.ptr(F), _lib.ptr(fh), _lib.F64, 2, shp, st, st, 1, N,
                           _lib.ptr(dense[0]), sp)
             _lib.call("sfa_helmholtz_solve", _lib.ptr(fh), _lib.ptr(fh), _lib.F64, 2, shp, st, st, 1,
-                      neumann, _lib.ptr(fac[0]), _lib.ptr(fac[1]), _lib.ptr(fac[2]), _lib.ptr(fac[3]), st,
+                      neumann, _lib.ptr(fac[0]), _lib.ptr(fac[1]), _lib.ptr(fac[2]), _lib.ptr(fac[3]), fst,
                       sp)
             if dense is None:
                 _lib.call("sfa_cheb_backward", _lib.ptr(fh), _lib.ptr(U), _lib.F64, 2, shp, st, st, 1,
@@ -346,7 +394,24 @@ class Server:
                           _lib.ptr(dense[1]), sp)
         for i, r in enumerate(reqs):
             r.u = U[i]
-        return (F, fh, U, fac, ab), [ckey]
+        return (F, fh, U, fac, ab, abh), keys
+
+    def _cheb1d_factors(self, N, bc, alpha, diags, stream):
+        """Helmholtz LU factors (4, 1, N) of the class (N, bc) at one alpha
+        (chebyshev/la.py:223-276), cached."""
+        def build():
+            abh = torch.tensor([-1.0, float(alpha)], dtype=torch.float64).pin_memory()
+            abd = abh.to(self.device, non_blocking=True)
+            fac = torch.zeros((4, 1, N), dtype=torch.float64, device=self.device)
+            A0, A2, A4, Bm2, B0, B2 = diags
+            _lib.call("sfa_helmholtz_lu", int(bc == "N"), N, _lib.ptr(A0), _lib.ptr(A2),
+                      _lib.ptr(A4), _lib.ptr(Bm2), _lib.ptr(B0), _lib.ptr(B2), _lib.ptr(abd),
+                      _lib.ptr(abd[1:]), 1, _lib.ptr(fac[0]), _lib.ptr(fac[1]), _lib.ptr(fac[2]),
+                      _lib.ptr(fac[3]), 2, _lib.arr([1, N]), _lib.arr([N, 1]), 1, _lib.stream_ptr())
+            fac.pinned_source = abh  # the pinned copy source lives as long as the factors
+            return fac, fac.numel() * 8
+        key = ("cheb1d_lu", N, bc, float(alpha))
+        return _use(stream, self.cache.lookup(key, build, stream)), key
 
     def _wide_2d(self, Nx, Ny):
         """Tile-parallel mode when one problem would dominate the persistent
@@ -674,9 +739,11 @@ class Server:
         if ok:
             lat = np.array([r.t_done - r.arrival for r in ok]) * 1e3
             span = max(r.t_done for r in ok) - min(r.arrival for r in ok)
+            up = sum(r.f.numel() * 8 for r in ok if not r.f.is_cuda)
             out.update({"p50_ms": float(np.percentile(lat, 50)),
                         "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
-                        "solves_per_s": len(ok) / span if span > 0 else float("inf")})
+                        "solves_per_s": len(ok) / span if span > 0 else float("inf"),
+                        "uploaded_GB_per_s": up / span / 1e9 if span > 0 else 0.0})
         return out
 
 
