@@ -801,7 +801,9 @@ namespace sfa {
 // ----------------------------------------------------------------- kz-split z passes
 // R2C of contiguous rows of 2H reals (FP64 arithmetic), output split by kz:
 // k < Kh -> o64 (row pitch Kh, complex128), k >= Kh -> o32 (row pitch H+1-Kh, complex64).
-template <int H, int NR = TL>
+// KSMALL: Kh <= 3Q, so only a thread's first three outputs (k = j + Q i,
+// i < 3) can go to o64 and the others are stored without a per-element branch.
+template <int H, bool KSMALL, int NR = TL>
 __device__ __forceinline__ void r2c_split_body(const double* __restrict__ in, double2* __restrict__ o64,
                                                float2* __restrict__ o32, int64_t nrows, int Kh,
                                                double scale, const double2* __restrict__ twg, int64_t R0)
@@ -846,19 +848,20 @@ __device__ __forceinline__ void r2c_split_body(const double* __restrict__ in, do
             const double2 zc = cconj(ln[k == 0 ? 0 : H - k]);
             const double2 d = cmul(twr[i], csub(zk, zc));
             const double2 X = make_double2(0.5 * (zk.x + zc.x + d.y) * scale, 0.5 * (zk.y + zc.y - d.x) * scale);
-            if (k < Kh) o64[R * Kh + k] = X;
+            if (KSMALL && i >= 3) o32[R * Kl + (k - Kh)] = from_c64<float2>(X);
+            else if (k < Kh) o64[R * Kh + k] = X;
             else o32[R * Kl + (k - Kh)] = from_c64<float2>(X);
         }
     }
 }
 
-template <int H>
+template <int H, bool KSMALL>
 __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     r2c_split_kernel(const double* __restrict__ in, double2* __restrict__ o64,
                      float2* __restrict__ o32, int64_t nrows, int Kh, double scale,
                      const double2* __restrict__ twg)
 {
-    r2c_split_body<H>(in, o64, o32, nrows, Kh, scale, twg, (int64_t)blockIdx.x * TL);
+    r2c_split_body<H, KSMALL>(in, o64, o32, nrows, Kh, scale, twg, (int64_t)blockIdx.x * TL);
 }
 
 // C2R (FP64 arithmetic) of rows merged from o64 (k < Kh) and o32 (k >= Kh).
@@ -975,7 +978,7 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
     const size_t lds = sizeof(double2) * ((size_t)TL * zrow_stride<H>() + H);
     dim3 grid((unsigned)((nrows + TL - 1) / TL)), block(TL * Pl::Q);
     if (SPLIT) {
-        auto k = r2c_split_kernel<H>;
+        auto k = Kh <= 3 * Pl::Q ? r2c_split_kernel<H, true> : r2c_split_kernel<H, false>;
         if (lds > 64 * 1024)
             hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double*)in, (double2*)o64, (float2*)o32,

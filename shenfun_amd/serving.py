@@ -7,7 +7,9 @@ device memory free up, so work of several requests overlaps on the device:
 * 1D Chebyshev (SD/SN, N) requests of one class (N, bc) are stacked into one
   (B, N) array with their own alphas: one LU launch for the batch's factors
   (one line per request), then one scalar-product, one Helmholtz-solve and one
-  backward launch.  Lengths with a prime factor above 31 (whose mixed-radix
+  backward launch.  A batch whose requests share one alpha that has been seen
+  before uses the cached factors of (N, bc, alpha) instead, broadcast to every
+  line; an alpha seen once fills no cache entry.  Lengths with a prime factor above 31 (whose mixed-radix
   DCT degenerates to an O(N p) DFT) use the cached dense transform matrices
   instead (the matrix kind of spectralbase.py:484-491, 550-558).  Reference
   path per request: chebyshev.bases scalar_product
@@ -23,8 +25,9 @@ device memory free up, so work of several requests overlaps on the device:
   launches (sfa_helmholtz3d_many) when several wait.
 
 A request's right-hand side may be a host tensor (uploaded on the slot's
-stream when its batch is launched, through a pinned staging buffer for 1D/2D
-batches, so the upload is inside its latency) or a device tensor.
+stream when its batch is launched, so the upload is inside its latency: pinned
+inputs of 64 Ki elements or more by direct asynchronous copies, the others
+through the slot's pinned staging buffer) or a device tensor.
 Factorisations (1D Galerkin diagonals and stencils, Legendre/DFT matrices, 3D
 plans) live in an LRU cache keyed by (basis, sizes, bc[, alpha]) with a byte
 budget; each entry is built on the stream of the batch that missed and
