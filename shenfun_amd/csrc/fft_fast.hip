@@ -458,7 +458,7 @@ __device__ __forceinline__ void xpass_stat_add(const XPass2Args& a, int64_t bid,
 // by Parseval, sum |V_k|^2 = N sum |F_x|^2), and that of the solution's
 // Chebyshev coefficients t (after to_ortho), w_kz (|t_0|^2 + 1/2 sum_{k>=1}
 // |t_k|^2), into accumulator kStatBands + band.
-template <int N, typename C2, bool STATS = false>
+template <int N, typename C2, bool STATS, bool NM>
 __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
 {
     using Pl = Pow2<N>;
@@ -595,7 +595,7 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
                 f[e] = make_double2(0.0, 0.0);
             }
         }
-        helm_chain_solve_bs<E, 16, C2, PADW>(f, X, p, M, be, a.neumann != 0, b4, lane);
+        helm_chain_solve_bs<E, 16, C2, PADW>(f, X, p, M, be, NM, b4, lane);
         // boundary slots M, M+1 (u carries no value there).  The odd chain
         // runs after the even one and its stencil reads X[M+1], so it clears
         // them once its own reads (same group, earlier in program order) are done.
@@ -664,28 +664,34 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
 }
 
 // workgroups per CU the x-pass is compiled for: at N = 512 one block holds
-// 16 lines of 512 complex values in LDS (~156 KB in FP64)
+// 16 lines of 512 complex values in LDS (~156 KB in FP64); complex64 storage
+// halves the lines' LDS, and without the Neumann row scalings (IEEE divides)
+// the Dirichlet solve fits three blocks' registers per CU at N <= 256
 template <int N>
 constexpr int xpass_minb() { return N >= 512 ? 1 : 2; }
+template <int N, typename C2, bool NM>
+constexpr int xpass_minb_c() { return (N <= 256 && sizeof(C2) == 8 && !NM) ? 3 : xpass_minb<N>(); }
 
-template <int N, typename C2>
-__global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_kernel(XPass2Args a)
+// NM: Neumann (Chebyshev SN) rows, else Dirichlet (SD); one kernel each so
+// that the Dirichlet kernels carry no Neumann code
+template <int N, typename C2, bool NM>
+__global__ void __launch_bounds__(TL * Pow2<N>::Q, (xpass_minb_c<N, C2, NM>())) xpass2_kernel(XPass2Args a)
 {
-    xpass2_body<N, C2>(a, blockIdx.x);
+    xpass2_body<N, C2, false, NM>(a, blockIdx.x);
 }
 
 // Mixed precision: the complex128 planes' blocks [0, nb64) and the complex64
 // planes' blocks after them in one grid, so the few FP64 blocks run beside the
 // FP32 ones instead of in a launch of their own.
-template <int N, bool STATS>
+template <int N, bool STATS, bool NM>
 __global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_dual_kernel(XPass2Args a64, XPass2Args a32,
                                                                         int64_t nb64)
 {
     const int64_t b = blockIdx.x;
     if (b < nb64)
-        xpass2_body<N, double2, STATS>(a64, b);
+        xpass2_body<N, double2, STATS, NM>(a64, b);
     else
-        xpass2_body<N, float2, STATS>(a32, b - nb64);
+        xpass2_body<N, float2, STATS, NM>(a32, b - nb64);
 }
 
 template <int N, typename C2>
@@ -693,7 +699,7 @@ static int launch_xpass2(XPass2Args& a, hipStream_t s)
 {
     using Pl = Pow2<N>;
     const size_t lds = XLay<N>::lds(sizeof(C2));
-    auto k = xpass2_kernel<N, C2>;
+    auto k = a.neumann ? xpass2_kernel<N, C2, true> : xpass2_kernel<N, C2, false>;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k, dim3((unsigned)((a.nl + TL - 1) / TL)), dim3(TL * Pl::Q), lds, s, a);
@@ -706,7 +712,8 @@ static int launch_xpass2_dual(XPass2Args& a64, XPass2Args& a32, hipStream_t s)
 {
     using Pl = Pow2<N>;
     const size_t lds = XLay<N>::lds(sizeof(double2));
-    auto k = a64.stats ? xpass2_dual_kernel<N, true> : xpass2_dual_kernel<N, false>;
+    auto k = a64.neumann ? (a64.stats ? xpass2_dual_kernel<N, true, true> : xpass2_dual_kernel<N, false, true>)
+                         : (a64.stats ? xpass2_dual_kernel<N, true, false> : xpass2_dual_kernel<N, false, false>);
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const int64_t nb64 = (a64.nl + TL - 1) / TL, nb32 = (a32.nl + TL - 1) / TL;

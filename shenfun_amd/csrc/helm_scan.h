@@ -451,24 +451,32 @@ __device__ inline void helm_chain_solve_bs(const double2 (&f)[E], C2* X, int p, 
     const bool zb = nm && fabs(be) < 1e-8;
     const int m0 = lane * E;
     double r1 = bs.x, r2 = bs.y, pinv = bs.z;
-    double inv[E], e1[E], e2[E], am[E];
+    // Forward elimination y_m = f_m - (g_m / d0_{m-1}) y_{m-1} in the scaled
+    // unknowns z_m = y_m / d0_m:  z_m = f_m / d0_m + am_m z_{m-1} with
+    // am_m = -g_m / d0_m, so a row keeps f / d0, am, d1/d0 and d2/d0 and no
+    // separate pivot for the back substitution's y / d0.
+    double2 yp[E];
+    double e1[E], e2[E], am[E];
     bool gauge = false;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int i = 2 * (m0 + e) + p;
         const HelmRow r = helm_row_cf(i, M, be, nm, zb);
-        helm_lu_row(r, i, M, i < 2, r1, r2, pinv, inv[e], am[e]);
+        double iv, amr;
+        helm_lu_row(r, i, M, i < 2, r1, r2, pinv, iv, amr);
+        am[e] = (i < 2) ? 0.0 : -r.g * iv;
+        yp[e] = make_double2(f[e].x * iv, f[e].y * iv);
         e1[e] = r1;
         e2[e] = r2;
         if (nm && i == 0) gauge = (r.h - 1.0) * (r.h - 1.0) < 1e-16;
     }
-    // forward elimination y_m = f_m + am_m y_{m-1}: affine prefix scan over lanes
+    // affine prefix scan over lanes
     double A = 1.0;
     double2 B = make_double2(0.0, 0.0);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         A = am[e] * A;
-        B = make_double2(am[e] * B.x + f[e].x, am[e] * B.y + f[e].y);
+        B = make_double2(am[e] * B.x + yp[e].x, am[e] * B.y + yp[e].y);
     }
 #pragma unroll
     for (int k = 1; k < W; k <<= 1) {
@@ -480,11 +488,10 @@ __device__ inline void helm_chain_solve_bs(const double2 (&f)[E], C2* X, int p, 
     }
     double2 y = make_double2(shup<W>(B.x, 1), shup<W>(B.y, 1));
     if (lane == 0) y = make_double2(0.0, 0.0);
-    double2 yp[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        y = make_double2(f[e].x + am[e] * y.x, f[e].y + am[e] * y.y);
-        yp[e] = make_double2(y.x * inv[e], y.y * inv[e]);
+        y = make_double2(yp[e].x + am[e] * y.x, yp[e].y + am[e] * y.y);
+        yp[e] = y;
     }
     double P00 = 1.0, P01 = 0.0, P10 = 0.0, P11 = 1.0;
     double2 c0 = make_double2(0.0, 0.0), c1 = make_double2(0.0, 0.0);
