@@ -441,10 +441,10 @@ template <int N>
 __device__ __forceinline__ void xpass_stat_add(const XPass2Args& a, int64_t bid, int tb, int jb, int first,
                                                double e)
 {
-    const int64_t Ls = bid * TL + tb;
-    if (jb != 0 || Ls >= a.nl) return;
-    const int64_t Lr = Ls % a.nlb;
-    const int64_t ky = Lr / a.Kp, kz = a.kz0 + (Lr - ky * a.Kp);
+    const uint32_t Ls = (uint32_t)bid * TL + tb;  // lines < 2^31 (checked at launch)
+    if (jb != 0 || Ls >= (uint32_t)a.nl) return;
+    const uint32_t Lr = Ls % (uint32_t)a.nlb;
+    const uint32_t ky = Lr / (uint32_t)a.Kp, kz = a.kz0 + (Lr - ky * (uint32_t)a.Kp);
     if (kz >= a.Nzh) return;  // a pitch-padding line
     const double wz = (kz == 0 || kz == a.Nzh - 1) ? 1.0 : 2.0;
     atomicAdd(a.stats + stat_index(first + stat_band((int)kz), (int)(bid % kStatSlots)), wz * e);
@@ -488,42 +488,69 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     // x row is one contiguous TL-line segment.  Mapping B (everything between
     // the two FFT exchanges): a line's Q threads are adjacent lanes of one
     // wave, so those phases synchronise per wave, not per workgroup.
+    // line t of group g in mapping A: element x = 0 at Lo
+    // line t of block g in mapping A, element x = 0 at Lo (line indices in 32
+    // bits: fewer than 2^31 lines, checked at launch)
+    auto line_of = [&](int64_t g, int t, int64_t& Lo) {
+        const uint32_t L = (uint32_t)g * TL + t;
+        const uint32_t Lb = L / (uint32_t)a.nlb;
+        Lo = (int64_t)Lb * a.bstride + (L - Lb * (uint32_t)a.nlb);
+        return L < (uint32_t)a.nl;
+    };
+    // 1. load with the DCT-II even/odd permutation: v[m] = x[2m] (m < N/2), x[2(N-1-m)+1]
+    // A line past the end loads line 0 (in bounds) and its results are never
+    // stored or counted: the loads stay outside any branch, so the waits after
+    // them can count them instead of draining every load.
+    auto load = [&](int64_t g, int t, int j, C2 (&w)[P]) {
+        int64_t Lo;
+        const int64_t Ls = line_of(g, t, Lo) ? Lo : 0;
+#pragma unroll
+        for (int r = 0; r < P; ++r) {
+            const int m = j + Q * r;
+            const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
+            w[r] = Wd[(int64_t)x * a.sa + Ls];
+        }
+    };
+    // The block's loads go out together before any wait: this thread's table
+    // entries (clamped indices, no branch), its line's values, the plan lookups.
+    constexpr int TPI = (N + NTH - 1) / NTH;
+    double2 twr[TPI], tqr[TPI];
+    double k2r[TPI];
+#pragma unroll
+    for (int e = 0; e < TPI; ++e) {
+        const int i = min(tid + e * NTH, N - 1);
+        twr[e] = a.tw[(i % Q) * (i / Q)];  // k1-major: tw[k1 Q + j] = w^(j k1)
+        tqr[e] = a.tq[i];
+        k2r[e] = a.K2[min(i, N - 3)];
+    }
     const int t = tid % TL, j = tid / TL;
     const int tb = tid / Q, jb = tid % Q;
-    const int64_t L = bid * TL + t;
-    const bool valid = L < a.nl;
-    const int64_t Lb = L / a.nlb;
-    const int64_t Lo = Lb * a.bstride + (L - Lb * a.nlb);  // this line's element x = 0
-    for (int i = tid; i < N; i += NTH) {
-        tw[i] = from_c64<C2>(a.tw[(i % Q) * (i / Q)]);  // k1-major: tw[k1 Q + j] = w^(j k1)
-        tqs[i] = from_c64<C2>(a.tq[i]);
-        k2s[pd(i)] = (i < N - 2) ? a.K2[i] : 0.0;
-    }
-    // 1. load with the DCT-II even/odd permutation: v[m] = x[2m] (m < N/2), x[2(N-1-m)+1]
     C2 v[P];
+    load(bid, t, j, v);
 #pragma unroll
-    for (int r = 0; r < P; ++r) {
-        const int m = j + Q * r;
-        const int x = (m < N / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
-        v[r] = valid ? Wd[(int64_t)x * a.sa + Lo] : cmk<C2>(0.0, 0.0);
+    for (int e = 0; e < TPI; ++e) {  // (threads past N rewrite entry N - 1 with its own value)
+        const int i = min(tid + e * NTH, N - 1);
+        tw[i] = from_c64<C2>(twr[e]);
+        tqs[i] = from_c64<C2>(tqr[e]);
+        k2s[pd(i)] = (i < N - 2) ? k2r[e] : 0.0;
     }
-    // plan entries of this group's solve lines (the group's chains are lines
-    // ct0 .. ct0 + ROUNDS/2 - 1, each with both parities), looked up here so the
-    // line-map latency overlaps the loads and FFT; before the solve only the
-    // independent svals / restart-state loads remain
+    // plan entries of this block's solve lines (the 16-lane group's chains are
+    // lines ct0 .. ct0 + ROUNDS/2 - 1, each with both parities), looked up here
+    // so the line-map latency overlaps the loads and FFT; before the solve only
+    // the independent svals / restart-state loads remain
     static_assert(ROUNDS % 2 == 0, "chains per group must pair both parities");
     constexpr int NLR = ROUNDS / 2;
     int ug[NLR];
 #pragma unroll
     for (int q = 0; q < NLR; ++q) {
-        const int64_t Lc = bid * TL + (tid / 16) * NLR + q;
-        int uq = 0;
-        if (Lc < a.nl) {
-            const int64_t Lr = Lc % a.nlb;  // line within its array
-            const int64_t ky = Lr / a.Kp, kz = a.kz0 + (Lr - ky * a.Kp);
-            if (kz < a.Nzh) uq = a.pl.linemap[ky * a.Nzh + kz];  // else a pitch-padding line
-        }
-        ug[q] = uq;
+        // branch-free, so that no wait for this load is placed here
+        const uint32_t Lc = (uint32_t)bid * TL + (tid / 16) * NLR + q;
+        const bool inl = Lc < (uint32_t)a.nl;
+        const uint32_t Lr = (inl ? Lc : 0u) % (uint32_t)a.nlb;  // line within its array
+        const uint32_t ky = Lr / (uint32_t)a.Kp, kz = a.kz0 + (Lr - ky * (uint32_t)a.Kp);
+        const bool ok = inl && kz < (uint32_t)a.Nzh;  // else past the end or a pitch-padding line
+        const int uq = a.pl.linemap[ok ? ky * a.Nzh + kz : 0];
+        ug[q] = ok ? uq : 0;
     }
     __syncthreads();  // tables
     pow2_fft_p1t<N, false>(v, j, ex + t * LA, tw);
@@ -649,7 +676,8 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     pow2_fft_p2<N, true>(v, j, ex + t * LA);
     const C2 t0a = t0s[t];
     // 4. store 0.5 y + t_0/2 with the inverse permutation
-    if (valid) {
+    int64_t Lo;
+    if (line_of(bid, t, Lo)) {
         const R half = (R)0.5;
 #pragma unroll
         for (int h = 0; h < P / Q; ++h)
@@ -664,20 +692,16 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
 }
 
 // workgroups per CU the x-pass is compiled for: at N = 512 one block holds
-// 16 lines of 512 complex values in LDS (~156 KB in FP64); complex64 storage
-// halves the lines' LDS, and without the Neumann row scalings (IEEE divides)
-// the Dirichlet solve fits three blocks' registers per CU at N <= 256
+// 16 lines of 512 complex values in LDS (~156 KB in FP64)
 template <int N>
 constexpr int xpass_minb() { return N >= 512 ? 1 : 2; }
-template <int N, typename C2, bool NM>
-constexpr int xpass_minb_c() { return (N <= 256 && sizeof(C2) == 8 && !NM) ? 3 : xpass_minb<N>(); }
 
-// NM: Neumann (Chebyshev SN) rows, else Dirichlet (SD); one kernel each so
-// that the Dirichlet kernels carry no Neumann code
-template <int N, typename C2, bool NM>
-__global__ void __launch_bounds__(TL * Pow2<N>::Q, (xpass_minb_c<N, C2, NM>())) xpass2_kernel(XPass2Args a)
+// FP64 x-pass.  NM: Neumann (Chebyshev SN) rows, else Dirichlet (SD); one
+// kernel each so that the Dirichlet kernels carry no Neumann code
+template <int N, bool NM>
+__global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_kernel(XPass2Args a)
 {
-    xpass2_body<N, C2, false, NM>(a, blockIdx.x);
+    xpass2_body<N, double2, false, NM>(a, blockIdx.x);
 }
 
 // Mixed precision: the complex128 planes' blocks [0, nb64) and the complex64
@@ -694,12 +718,12 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_dual_
         xpass2_body<N, float2, STATS, NM>(a32, b - nb64);
 }
 
-template <int N, typename C2>
+template <int N>
 static int launch_xpass2(XPass2Args& a, hipStream_t s)
 {
     using Pl = Pow2<N>;
-    const size_t lds = XLay<N>::lds(sizeof(C2));
-    auto k = a.neumann ? xpass2_kernel<N, C2, true> : xpass2_kernel<N, C2, false>;
+    const size_t lds = XLay<N>::lds(sizeof(double2));
+    auto k = a.neumann ? xpass2_kernel<N, true> : xpass2_kernel<N, false>;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k, dim3((unsigned)((a.nl + TL - 1) / TL)), dim3(TL * Pl::Q), lds, s, a);
@@ -722,8 +746,8 @@ static int launch_xpass2_dual(XPass2Args& a64, XPass2Args& a32, hipStream_t s)
     return SFA_OK;
 }
 
-// Fused x-pass on the kz planes [kz0, kz0 + Kp) held (line pitch Kp) in W.
-// f32 selects complex64 storage/arithmetic.  Returns 1 if handled.
+// Fused FP64 x-pass on the kz planes [kz0, kz0 + Kp) held (line pitch Kp)
+// in W.  Returns 1 if handled.
 static bool xpass_args(void* W, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha, int neumann,
                        const double* K2, const XPlan& pl, int64_t nb, XPass2Args* a)
 {
@@ -746,8 +770,8 @@ static bool xpass_args(void* W, int Nx, int Ny, int Nzh, int Kp, int kz0, double
     return a->tw && a->tq;
 }
 
-int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha,
-                   int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc)
+static int fast_xpass_sub(void* W, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha, int neumann,
+                          const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc)
 {
     if (!(Nx == 64 || Nx == 128 || Nx == 256 || Nx == 512) || pl.linemap == nullptr || pl.bs == nullptr)
         return 0;
@@ -756,14 +780,16 @@ int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, 
         *rc = fail(SFA_ERR_HIP, "table allocation failed");
         return 1;
     }
-#define SFA_XP(NN)                                                                     \
-    case NN:                                                                           \
-        *rc = f32 ? launch_xpass2<NN, float2>(a, s) : launch_xpass2<NN, double2>(a, s); \
-        break;
-    switch (Nx) {
-        SFA_XP(64) SFA_XP(128) SFA_XP(256) SFA_XP(512)
+    if (a.nl >= (int64_t(1) << 31)) {
+        *rc = fail(SFA_ERR_UNSUPPORTED, "x-pass: 2^31 or more lines");
+        return 1;
     }
-#undef SFA_XP
+    switch (Nx) {
+        case 64: *rc = launch_xpass2<64>(a, s); break;
+        case 128: *rc = launch_xpass2<128>(a, s); break;
+        case 256: *rc = launch_xpass2<256>(a, s); break;
+        default: *rc = launch_xpass2<512>(a, s); break;
+    }
     return 1;
 }
 
@@ -782,6 +808,10 @@ int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, doub
         *rc = fail(SFA_ERR_HIP, "table allocation failed");
         return 1;
     }
+    if (a64.nl + a32.nl >= (int64_t(1) << 31)) {
+        *rc = fail(SFA_ERR_UNSUPPORTED, "x-pass: 2^31 or more lines");
+        return 1;
+    }
     a64.stats = a32.stats = stats;
     switch (Nx) {
         case 64: *rc = launch_xpass2_dual<64>(a64, a32, s); break;
@@ -798,7 +828,7 @@ int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, doub
 int fast_xpass(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int neumann,
                const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc)
 {
-    return fast_xpass_sub(W, false, Nx, Ny, Nzh, Kp, 0, alpha, neumann, K2, pl, nb, s, rc);
+    return fast_xpass_sub(W, Nx, Ny, Nzh, Kp, 0, alpha, neumann, K2, pl, nb, s, rc);
 }
 
 }  // namespace sfa

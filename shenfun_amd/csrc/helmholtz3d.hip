@@ -237,8 +237,6 @@ static const double* cheb_k2(int N, int neumann)
 
 int fast_xpass(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int neumann,
                const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc);
-int fast_xpass_sub(void* W, bool f32, int Nx, int Ny, int Nzh, int Kp, int kz0, double alpha,
-                   int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc);
 int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, double alpha,
                      int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc,
                      double* stats);

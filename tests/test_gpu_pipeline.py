@@ -45,7 +45,7 @@ def test_poisson2d_legendre_manufactured(sfa, seed):
 
 @pytest.mark.parametrize("bc", ["D", "N"])
 @pytest.mark.parametrize("shape", [(16, 12, 10), (32, 24, 16), (64, 64, 64), (128, 24, 20),
-                                   (256, 12, 10), (256, 32, 64), (512, 16, 32)])
+                                   (256, 12, 10), (256, 32, 64), (512, 16, 32), (512, 128, 64)])
 def test_helmholtz3d_dropin_and_fused(sfa, bc, shape):
     """(alpha - lap) u = f on Chebyshev x C2C x R2C: the drop-in chain
     T.scalar_product -> Helmholtz -> T.backward and the fused pipeline agree
