@@ -434,6 +434,7 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_batched:
         out["batched2d"] = batched_leg(sfa, torch)
+        torch.cuda.empty_cache()
     if not args.no_serving:
         out["served"] = serving_leg(torch)
         out["served_resident"] = serving_leg(torch, rates=(1000, 50000), upload=False,

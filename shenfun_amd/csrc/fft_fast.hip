@@ -555,6 +555,17 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
     __syncthreads();  // tables
     pow2_fft_p1t<N, false>(v, j, ex + t * LA, tw);
     __syncthreads();  // exchange, and the switch to mapping B
+    // the solve rounds' shifts and restart states, loaded now (the line map has
+    // arrived with the line values) so they arrive during the FFT's second half
+    // and the DCT post-processing
+    double beR[ROUNDS];
+    double4 b4R[ROUNDS];
+#pragma unroll
+    for (int rho = 0; rho < ROUNDS; ++rho) {
+        const int u = ug[rho >> 1];
+        beR[rho] = a.alpha + a.pl.svals[u];
+        b4R[rho] = a.pl.bs[((int64_t)2 * u + (rho & 1)) * 16 + (tid & 15)];
+    }
     C2* ln = ex + tb * LA;
     pow2_fft_p2<N, false>(v, jb, ln);
     if (STATS) {  // the line's input energy, added at once (nothing held through the solve)
@@ -604,12 +615,14 @@ __device__ __forceinline__ void xpass2_body(const XPass2Args& a, int64_t bid)
         const int c = (tid / 16) * ROUNDS + rho;
         const int ct = c >> 1, p = c & 1, lane = tid & 15;
         C2* X = ex + ct * LA;
-        int u = ug[0];
+        double be = beR[0];
+        double4 b4 = b4R[0];
 #pragma unroll
-        for (int q = 1; q < NLR; ++q)
-            if ((rho >> 1) == q) u = ug[q];
-        const double be = a.alpha + a.pl.svals[u];
-        const double4 b4 = a.pl.bs[((int64_t)2 * u + p) * 16 + lane];
+        for (int q = 1; q < ROUNDS; ++q)
+            if (rho == q) {
+                be = beR[q];
+                b4 = b4R[q];
+            }
         double2 f[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {

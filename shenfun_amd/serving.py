@@ -43,6 +43,7 @@ admission).  Latency of a request is the host-observed completion of its
 batch minus its arrival time.
 """
 import collections
+import gc
 import heapq
 import itertools
 import math
@@ -708,6 +709,21 @@ class Server:
             r.t_admit = r.t_done = r.u = r.status = None
         self.done = []
         self._sync()
+        # the objects alive before the run are moved out of the collector's
+        # reach for its duration, so a collection pauses the loop only for the
+        # objects the run itself creates (a full collection over a large heap
+        # stalls every request in flight)
+        freeze = gc.isenabled()
+        if freeze:
+            gc.collect()
+            gc.freeze()
+        try:
+            return self._loop(reqs, realtime)
+        finally:
+            if freeze:
+                gc.unfreeze()
+
+    def _loop(self, reqs, realtime):
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         self._now = now
