@@ -905,12 +905,16 @@ __device__ __forceinline__ void r2c_split_body(const double* __restrict__ in, do
     }
 }
 
+// zero[0, nzero): cleared on the side (grid-stride), for the accumulators that
+// later kernels of the same pass add into, so no launch of its own is needed
 template <int H, bool KSMALL>
 __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     r2c_split_kernel(const double* __restrict__ in, double2* __restrict__ o64,
                      float2* __restrict__ o32, int64_t nrows, int Kh, double scale,
-                     const double2* __restrict__ twg)
+                     const double2* __restrict__ twg, double* __restrict__ zero, int nzero)
 {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nzero; i += gridDim.x * blockDim.x)
+        zero[i] = 0.0;
     r2c_split_body<H, KSMALL>(in, o64, o32, nrows, Kh, scale, twg, (int64_t)blockIdx.x * TL);
 }
 
@@ -1020,7 +1024,7 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 
 template <int H, bool SPLIT>
 static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int Kh, double scale,
-                        hipStream_t s)
+                        hipStream_t s, double* zero, int nzero)
 {
     using Pl = Pow2<H>;
     const double2* tw = fft_table(2 * H);
@@ -1032,7 +1036,7 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
         if (lds > 64 * 1024)
             hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double*)in, (double2*)o64, (float2*)o32,
-                           nrows, Kh, scale, tw);
+                           nrows, Kh, scale, tw, zero, nzero);
     } else {
         auto k = Kh <= 3 * Pl::Q ? c2r_merge_kernel<H, true> : c2r_merge_kernel<H, false>;
         if (lds > 64 * 1024)
@@ -1045,14 +1049,14 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
 }
 
 // split (r2c=true: in -> o64/o32) or merge (r2c=false: o64/o32 -> in) of
-// contiguous rows
+// contiguous rows; the split also clears zero[0, nzero)
 int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
-             double scale, hipStream_t s, int* rc)
+             double scale, hipStream_t s, int* rc, double* zero, int nzero)
 {
 #define SFA_SP(HH)                                                                            \
     case HH:                                                                                  \
-        *rc = r2c ? launch_split<HH, true>(in, o64, o32, nrows, Kh, scale, s)                 \
-                  : launch_split<HH, false>(in, o64, o32, nrows, Kh, scale, s);               \
+        *rc = r2c ? launch_split<HH, true>(in, o64, o32, nrows, Kh, scale, s, zero, nzero)    \
+                  : launch_split<HH, false>(in, o64, o32, nrows, Kh, scale, s, nullptr, 0);   \
         return 1;
     switch (H) {
         SFA_SP(16) SFA_SP(32) SFA_SP(64) SFA_SP(128) SFA_SP(256)

@@ -247,7 +247,7 @@ int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int 
 int fast_c2c_mixed(const Geom& g64, void* W64, const Geom& g32, void* W32, int sign, double scale,
                    hipStream_t s, int* rc);
 int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
-             double scale, hipStream_t s, int* rc);
+             double scale, hipStream_t s, int* rc, double* zero = nullptr, int nzero = 0);
 
 // ------------------------------------------------------------------ plan
 // Device plan of the fused solve, the counterpart of chebyshev.la.Helmholtz's
@@ -452,8 +452,9 @@ static int y_pass_mixed(const Geom& g64, const Geom& g32, double2* W64, float2* 
 // Helmholtz solve are FP64.
 // nb arrays (Nx, Ny, Nz) back to back in f and u; the workspace holds the nb
 // arrays' complex128 planes, then their complex64 planes.
-// stats (may be NULL): zeroed band-energy accumulators (kStatBytes) that the
-// x-pass adds to; evx (may be NULL): recorded after the x-pass.
+// stats (may be NULL): band-energy accumulators (kStatBytes), cleared by the z
+// R2C split and then added to by the x-pass; evx (may be NULL): recorded after
+// the x-pass.
 static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                              double alpha, int neumann, int Kh, const void* plan, void* work,
                              hipStream_t s, hipEvent_t* ev, int64_t nb = 1, double* stats = nullptr,
@@ -485,7 +486,9 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
     };
     const Geom g64 = geom(Kh), g32 = geom(Kl);
     mark(0);
-    if (!split_rc(true, (int)(Nz / 2), f, W64, W32, rows, Kh, 1.0 / (double)Nz, s, &rc))
+    // the split also clears the spectrum statistics that the x-pass adds into
+    if (!split_rc(true, (int)(Nz / 2), f, W64, W32, rows, Kh, 1.0 / (double)Nz, s, &rc, stats,
+                  stats ? kStatBytes / 8 : 0))
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz=%lld unsupported", (long long)Nz);
     if (rc) return rc;
     mark(1);
@@ -739,11 +742,6 @@ extern "C" int sfa_helmholtz3d_noalloc(const double* f, double* u, int64_t Nx, i
 
 namespace sfa {
 
-__global__ void zero_doubles_kernel(double* p, int n)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
-}
-
 // the slots of each statistics accumulator summed into stats[kStatSumOff + acc]:
 // 8 lanes per accumulator, each summing every 8th slot, then a butterfly
 constexpr int kStatReduceThreads = kStatAcc * 8;
@@ -851,9 +849,7 @@ extern "C" int sfa_helmholtz3d_guarded(const double* f, double* u, int64_t nbatc
     hipStream_t s = (hipStream_t)stream;
     GuardCtx* g = guard_ctx();
     SFA_CHECK(g != nullptr, SFA_ERR_HIP, "guard stream/event/pinned buffer creation failed");
-    double* stats = (double*)((char*)work + nbatch * core);
-    hipLaunchKernelGGL(zero_doubles_kernel, dim3(32), dim3(256), 0, s, stats, kStatBytes / 8);
-    SFA_LAUNCH_CHECK();
+    double* stats = (double*)((char*)work + nbatch * core);  // cleared by the z R2C split
     const int Kh = mixed_planes(Nz, kh_hint);
     int rc = helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann, Kh, plan, work, s, nullptr, nbatch,
                                stats, g->evx);
