@@ -10,8 +10,9 @@ tensor each (``pack``/``unpack``).  ``autotune`` times the kernel's block-tile
 variants and workgroup counts on the actual batch and keeps the fastest.
 
 ``mode="wide"`` runs the tile-parallel form (sfa_batch2d_wide_solve: each
-contraction one launch over all (problem, tile) tasks) for batches with fewer
-problems than CUs; its outputs are bitwise equal to the persistent form's.
+contraction one launch over all (problem, tile) tasks, consecutive tasks
+sharing a problem's operands in the L2); its outputs are bitwise equal to the
+persistent form's, and ``autotune`` times it beside the persistent variants.
 """
 import json
 import os
@@ -97,6 +98,9 @@ class BatchedHelmholtz2D:
         """Matrix arena (one Phi/P block per distinct (Nx, bc), one Df/Di block
         per distinct Ny) and the tile-parallel plan."""
         L = _lib.lib()
+        if self.plan is not None:
+            L.sfa_batch2d_release(_lib.ptr(self.plan))
+            self.plan = None
         ptr = lambda a: _lib.ctypes.c_void_p(a.ctypes.data)
         if L.sfa_batch2d_plan_bytes(self.nprob, ptr(self.Nx), ptr(self.Ny), ptr(self.bc), 1) < 0:
             raise ValueError("invalid problem sizes (even Nx in [8, 1024], even Ny in [4, 1024])")
@@ -168,13 +172,24 @@ class BatchedHelmholtz2D:
         return U
 
     # ------------------------------------------------------------ tuning
-    def autotune(self, F, blocks_per_cu=(1, 2), variants=None, reps=3, cache=None):
-        """Time every (tile variant, workgroups per CU) pair on this batch and
-        keep the fastest (persistent mode only).  Returns {"best": {...}, "table": [...]}.  With
-        ``cache`` (a JSON path) a previous result for the same batch signature
-        is reused and new results are stored."""
-        if self.mode != "persistent":
-            raise ValueError("autotune applies to the persistent mode")
+    def _use(self, cfg):
+        """Switch to a tuned configuration: {"mode", "nblocks", "variant"}."""
+        if cfg.get("mode", "persistent") == "wide":
+            if self.mode != "wide" or self.plan is None:
+                self.mode = "wide"
+                self._build_wide()
+        else:
+            if self.mode != "persistent" or cfg["nblocks"] != self.nblocks:
+                self.mode = "persistent"
+                self.build(cfg["nblocks"])
+            self.variant = cfg["variant"]
+
+    def autotune(self, F, blocks_per_cu=(1, 2), variants=None, reps=3, cache=None, wide=True):
+        """Time every (tile variant, workgroups per CU) pair of the persistent
+        form, and the tile-parallel form (``wide``), on this batch and keep the
+        fastest; both forms give the same bits.  Returns {"best": {...},
+        "table": [...]}.  With ``cache`` (a JSON path) a previous result for the
+        same batch signature is reused and new results are stored."""
         key = self.signature()
         if cache and os.path.exists(cache):
             try:
@@ -183,32 +198,39 @@ class BatchedHelmholtz2D:
             except (OSError, ValueError):
                 hit = None
             if hit:
-                if hit["nblocks"] != self.nblocks:
-                    self.build(hit["nblocks"])
-                self.variant = hit["variant"]
+                self._use(hit)
                 return {"best": hit, "table": [], "cached": True}
         U = torch.empty_like(F)
+
+        def timed():
+            self(F, U)
+            torch.cuda.synchronize()
+            best = float("inf")
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                self(F, U)
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+            return best * 1e3
+
         tiles = tile_variants()
         variants = range(len(tiles)) if variants is None else variants
         table = []
+        self.mode = "persistent"
         for bpc in blocks_per_cu:
             self.build(bpc * self.cus)
             for v in variants:
                 self.variant = v
-                self(F, U)
-                torch.cuda.synchronize()
-                best = float("inf")
-                for _ in range(reps):
-                    t0 = time.perf_counter()
-                    self(F, U)
-                    torch.cuda.synchronize()
-                    best = min(best, time.perf_counter() - t0)
-                table.append({"variant": v, "tile": list(tiles[v]), "nblocks": self.nblocks,
-                              "ms": best * 1e3})
+                table.append({"mode": "persistent", "variant": v, "tile": list(tiles[v]),
+                              "nblocks": self.nblocks, "ms": timed(),
+                              "label": "x".join(map(str, tiles[v])) + f"@{self.nblocks}"})
+        if wide:
+            self.mode = "wide"
+            self._build_wide()
+            table.append({"mode": "wide", "variant": None, "tile": None, "nblocks": None,
+                          "ms": timed(), "label": "wide"})
         win = min(table, key=lambda r: r["ms"])
-        if win["nblocks"] != self.nblocks:
-            self.build(win["nblocks"])
-        self.variant = win["variant"]
+        self._use(win)
         if cache:
             data = {}
             if os.path.exists(cache):

@@ -82,16 +82,22 @@ __device__ inline void leg_rows(int i, int bc, double& Akk, double& Bkk, double&
 }
 
 // Thomas algorithm per (ky, parity chain) on the complex columns of W0 (M rows,
-// row pitch ldw doubles, column ky at 2 ky).  cp: scratch, M * Nyh doubles.
-// Neumann with beta = 0 (ky = 0, alpha = 0): the singular row 0 is replaced by
-// u_0 = 0 (the constraint (0, 0) of the reference's mode-0 solver).
+// row pitch ldw doubles, column ky at 2 ky).  cp: scratch, M * Nyh doubles,
+// row-major (row i, column ky), so a wave's stores of one step are contiguous
+// (ky-major, a chain's own column, they were 32 separate lines per wave: the
+// solve launch of the tile-parallel form took 1.21 ms instead of 0.83 ms for
+// the 4096-problem batch).  Staging row chunks in LDS between barriers measured
+// slower still (1.43 ms): the many resident workgroups already cover a row's
+// load latency.  Neumann with beta = 0 (ky = 0, alpha = 0): the singular row 0
+// is replaced by u_0 = 0 (the constraint (0, 0) of the reference's mode-0
+// solver).
 __device__ void batch_solve(double* __restrict__ W0, int64_t ldw, int M, int Nyh, int bc,
                             double alpha, double* __restrict__ cp)
 {
     for (int c = threadIdx.x; c < 2 * Nyh; c += blockDim.x) {
         const int ky = c >> 1, p = c & 1;
         const double beta = alpha + (double)ky * (double)ky;
-        double* cpc = cp + (int64_t)ky * M;
+        double* cpc = cp + ky;  // cpc[i * Nyh]
         double cprev = 0.0, rprev_re = 0.0, rprev_im = 0.0, lprev_up = 0.0;
         int last = -1;
         for (int i = p; i < M; i += 2) {
@@ -112,7 +118,7 @@ __device__ void batch_solve(double* __restrict__ W0, int64_t ldw, int M, int Nyh
                 rr_re = (re - lo * rprev_re) * dd;
                 rr_im = (im - lo * rprev_im) * dd;
             }
-            cpc[i] = cc;
+            cpc[(int64_t)i * Nyh] = cc;
             x[0] = rr_re;
             x[1] = rr_im;
             cprev = cc;
@@ -124,8 +130,9 @@ __device__ void batch_solve(double* __restrict__ W0, int64_t ldw, int M, int Nyh
         double xr = 0.0, xi = 0.0;
         for (int i = last; i >= 0; i -= 2) {
             double* x = W0 + (int64_t)i * ldw + 2 * ky;
-            xr = x[0] - cpc[i] * xr;
-            xi = x[1] - cpc[i] * xi;
+            const double c = cpc[(int64_t)i * Nyh];
+            xr = x[0] - c * xr;
+            xi = x[1] - c * xi;
             x[0] = xr;
             x[1] = xi;
         }
@@ -162,16 +169,28 @@ __global__ void __launch_bounds__(B2_THREADS) batch2d_kernel(B2Args a)
         const double* P = a.mats + pr.p_off;
         const double* Df = a.mats + pr.df_off;
         const double* Di = a.mats + pr.di_off;
-        // T1 = P f (M x Ny), held in u
-        block_gemm<BM, BN, BK>(P, Nx, f, Ny, u, Ny, M, Ny, Nx, sA, sB);
-        // T2 = T1 Df (M x L) in W0
-        block_gemm<BM, BN, BK>(u, Ny, Df, L, W0, L, M, L, Ny, sA, sB);
-        // per-ky solve in place, W1 as the Thomas scratch
-        batch_solve(W0, L, M, Nyh, pr.bc, pr.alpha, W1);
-        // U1 = Phi u_hat (Nx x L) in W1
-        block_gemm<BM, BN, BK>(Phi, M, W0, L, W1, L, Nx, L, M, sA, sB);
-        // u = U1 Di (Nx x Ny)
-        block_gemm<BM, BN, BK>(W1, L, Di, Ny, u, Ny, Nx, Ny, L, sA, sB);
+        // The four contractions through one GEMM call site (a loop over the
+        // stages, not unrolled): with four inlined copies the compiler held the
+        // address setup of all of them, 163 VGPRs against 67 for one tile.
+#pragma unroll 1
+        for (int st = 0; st < 4; ++st) {
+            const double *A, *B;
+            double* C;
+            int64_t lda, ldb, ldc;
+            int m, n, k;
+            if (st == 0) {  // T1 = P f (M x Ny), held in u
+                A = P, lda = Nx, B = f, ldb = Ny, C = u, ldc = Ny, m = M, n = Ny, k = Nx;
+            } else if (st == 1) {  // T2 = T1 Df (M x L) in W0
+                A = u, lda = Ny, B = Df, ldb = L, C = W0, ldc = L, m = M, n = L, k = Ny;
+            } else if (st == 2) {  // U1 = Phi u_hat (Nx x L) in W1
+                // per-ky solve in place first, W1 as the Thomas scratch
+                batch_solve(W0, L, M, Nyh, pr.bc, pr.alpha, W1);
+                A = Phi, lda = M, B = W0, ldb = L, C = W1, ldc = L, m = Nx, n = L, k = M;
+            } else {  // u = U1 Di (Nx x Ny)
+                A = W1, lda = L, B = Di, ldb = Ny, C = u, ldc = Ny, m = Nx, n = Ny, k = L;
+            }
+            block_gemm<BM, BN, BK>(A, lda, B, ldb, C, ldc, m, n, k, sA, sB);
+        }
     }
 }
 
@@ -182,7 +201,7 @@ __global__ void __launch_bounds__(B2_THREADS) batch2d_kernel(B2Args a)
 // Per-problem intermediates live at work + woff[p] (W0: (Nx-2) x (Ny+2),
 // then W1: Nx x (Ny+2)); T1 is held in u as in the persistent kernel.
 struct B2Task {
-    int32_t prob, m0, n0, pad;
+    int32_t prob, m0, n0, tile;  // tile: 0 = 32 x 32, 1 = 64 x 64 block tile
 };
 
 struct B2WideArgs {
@@ -195,28 +214,52 @@ struct B2WideArgs {
     double* work;
 };
 
-constexpr int B2W_BM = 32, B2W_BN = 32, B2W_BK = 16;
+constexpr int B2W_BK = 16;
+// block tiles of the tile classes (a 128 x 64 class spills in this kernel)
+constexpr int kB2WClasses = 2;
+constexpr int kB2WTile[kB2WClasses][2] = {{32, 32}, {64, 64}};
 
-template <int STAGE>
-__global__ void __launch_bounds__(B2_THREADS) batch2d_tile_kernel(B2WideArgs a)
+template <int BM, int BN>
+__device__ void wide_gemm(const B2WideArgs& a, const B2Task& tk, int stage, double* sA, double* sB)
 {
-    __shared__ double sA[B2W_BK * (B2W_BM + 1)];
-    __shared__ double sB[B2W_BK * B2W_BN];
-    const B2Task tk = a.tasks[blockIdx.x];
     const B2Prob pr = a.probs[tk.prob];
     const int Nx = pr.Nx, Ny = pr.Ny, M = Nx - 2, L = Ny + 2;
     double* u = a.U + pr.u_off;
     double* W0 = a.work + a.woff[tk.prob];
     double* W1 = W0 + (int64_t)M * L;
-    if (STAGE == 0)  // T1 = P f (M x Ny) in u
-        gemm_tile<B2W_BM, B2W_BN, B2W_BK>(a.mats + pr.p_off, Nx, a.F + pr.f_off, Ny, u, Ny, M, Ny, Nx,
-                                          tk.m0, tk.n0, sA, sB);
-    else if (STAGE == 1)  // T2 = T1 Df (M x L) in W0
-        gemm_tile<B2W_BM, B2W_BN, B2W_BK>(u, Ny, a.mats + pr.df_off, L, W0, L, M, L, Ny, tk.m0, tk.n0, sA, sB);
-    else if (STAGE == 2)  // U1 = Phi u_hat (Nx x L) in W1
-        gemm_tile<B2W_BM, B2W_BN, B2W_BK>(a.mats + pr.phi_off, M, W0, L, W1, L, Nx, L, M, tk.m0, tk.n0, sA, sB);
+    if (stage == 0)  // T1 = P f (M x Ny) in u
+        gemm_tile<BM, BN, B2W_BK>(a.mats + pr.p_off, Nx, a.F + pr.f_off, Ny, u, Ny, M, Ny, Nx, tk.m0, tk.n0,
+                                  sA, sB);
+    else if (stage == 1)  // T2 = T1 Df (M x L) in W0
+        gemm_tile<BM, BN, B2W_BK>(u, Ny, a.mats + pr.df_off, L, W0, L, M, L, Ny, tk.m0, tk.n0, sA, sB);
+    else if (stage == 2)  // U1 = Phi u_hat (Nx x L) in W1
+        gemm_tile<BM, BN, B2W_BK>(a.mats + pr.phi_off, M, W0, L, W1, L, Nx, L, M, tk.m0, tk.n0, sA, sB);
     else  // u = U1 Di (Nx x Ny)
-        gemm_tile<B2W_BM, B2W_BN, B2W_BK>(W1, L, a.mats + pr.di_off, Ny, u, Ny, Nx, Ny, L, tk.m0, tk.n0, sA, sB);
+        gemm_tile<BM, BN, B2W_BK>(W1, L, a.mats + pr.di_off, Ny, u, Ny, Nx, Ny, L, tk.m0, tk.n0, sA, sB);
+}
+
+// One task per workgroup, all of a launch's tasks of tile class CLS; every
+// output keeps its one MFMA chain over K, so the results do not depend on the
+// class.
+template <int STAGE, int CLS>
+__global__ void __launch_bounds__(B2_THREADS) batch2d_tile_kernel(B2WideArgs a)
+{
+    constexpr int BM = kB2WTile[CLS][0], BN = kB2WTile[CLS][1];
+    __shared__ double sA[B2W_BK * (BM + 1)];
+    __shared__ double sB[B2W_BK * BN];
+    wide_gemm<BM, BN>(a, a.tasks[blockIdx.x], STAGE, sA, sB);
+}
+
+template <int STAGE>
+static int launch_wide_stage(const B2WideArgs& a, int cls, int64_t ntask, hipStream_t s)
+{
+    const dim3 blk(B2_THREADS), grid((unsigned)ntask);
+    if (cls == 1)
+        hipLaunchKernelGGL((batch2d_tile_kernel<STAGE, 1>), grid, blk, 0, s, a);
+    else
+        hipLaunchKernelGGL((batch2d_tile_kernel<STAGE, 0>), grid, blk, 0, s, a);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
 }
 
 __global__ void __launch_bounds__(B2_THREADS) batch2d_wide_solve_kernel(B2WideArgs a)
@@ -453,6 +496,7 @@ static std::map<const void*, B2Info> g_b2;
 // tile-parallel plan: problem table | tasks of the four GEMM stages | woff
 struct B2WideInfo {
     int nprob;
+    int cls[4];  // tile class of each stage
     int64_t ntask[4], off_task[4];
     int64_t off_probs, off_woff, work_doubles;
     const double* mats;
@@ -555,18 +599,31 @@ static int b2_plan_core(int nprob, const int* Nx, const int* Ny, const double* a
 }
 
 
-static void b2_wide_shape(int nprob, const int* Nx, const int* Ny, int64_t ntask[4], int64_t* work)
+// Tile class of each stage: the largest whose task count still gives every
+// one of the kWideCus compute units a few workgroups (more operand reuse per
+// tile where the batch is large, more tiles where a few problems must fill the
+// device); ntask[st] is that stage's task count.
+constexpr int kWideCus = 256;  // compute units of an MI355X
+
+static void b2_wide_shape(int nprob, const int* Nx, const int* Ny, int64_t ntask[4], int64_t* work,
+                          int cls[4] = nullptr)
 {
     auto cdiv = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
-    for (int st = 0; st < 4; ++st) ntask[st] = 0;
     *work = 0;
+    int64_t cnt[4][kB2WClasses] = {};
     for (int i = 0; i < nprob; ++i) {
         const int64_t M = Nx[i] - 2, L = Ny[i] + 2, X = Nx[i], Y = Ny[i];
-        ntask[0] += cdiv(M, B2W_BM) * cdiv(Y, B2W_BN);
-        ntask[1] += cdiv(M, B2W_BM) * cdiv(L, B2W_BN);
-        ntask[2] += cdiv(X, B2W_BM) * cdiv(L, B2W_BN);
-        ntask[3] += cdiv(X, B2W_BM) * cdiv(Y, B2W_BN);
+        const int64_t mm[4] = {M, M, X, X}, nn[4] = {Y, L, L, Y};
+        for (int st = 0; st < 4; ++st)
+            for (int c = 0; c < kB2WClasses; ++c)
+                cnt[st][c] += cdiv(mm[st], kB2WTile[c][0]) * cdiv(nn[st], kB2WTile[c][1]);
         *work += (M * L + X * L + 31) / 32 * 32;
+    }
+    for (int st = 0; st < 4; ++st) {
+        int c = kB2WClasses - 1;
+        while (c > 0 && cnt[st][c] < 4 * kWideCus) --c;
+        ntask[st] = cnt[st][c];
+        if (cls) cls[st] = c;
     }
 }
 
@@ -784,7 +841,8 @@ extern "C" int sfa_batch2d_wide_plan(int nprob, const int* Nx, const int* Ny, co
         return rc;
     B2WideInfo wi;
     wi.nprob = nprob;
-    b2_wide_shape(nprob, Nx, Ny, wi.ntask, &wi.work_doubles);
+    int* cls = wi.cls;
+    b2_wide_shape(nprob, Nx, Ny, wi.ntask, &wi.work_doubles, cls);
     const int64_t bytes = b2_wide_bytes(nprob, wi.ntask, wi.off_task, &wi.off_woff);
     SFA_CHECK(plan_bytes >= bytes, SFA_ERR_ARG, "plan buffer too small (%lld < %lld)",
               (long long)plan_bytes, (long long)bytes);
@@ -796,9 +854,11 @@ extern "C" int sfa_batch2d_wide_plan(int nprob, const int* Nx, const int* Ny, co
     for (int i = 0; i < nprob; ++i) {
         const int M = Nx[i] - 2, L = Ny[i] + 2, X = Nx[i], Y = Ny[i];
         const int mm[4] = {M, M, X, X}, nn[4] = {Y, L, L, Y};
-        for (int st = 0; st < 4; ++st)
-            for (int m0 = 0; m0 < mm[st]; m0 += B2W_BM)
-                for (int n0 = 0; n0 < nn[st]; n0 += B2W_BN) tasks[st].push_back(B2Task{i, m0, n0, 0});
+        for (int st = 0; st < 4; ++st) {
+            const int bm = kB2WTile[cls[st]][0], bn = kB2WTile[cls[st]][1];
+            for (int m0 = 0; m0 < mm[st]; m0 += bm)
+                for (int n0 = 0; n0 < nn[st]; n0 += bn) tasks[st].push_back(B2Task{i, m0, n0, cls[st]});
+        }
         woff[i] = w;
         w += ((int64_t)M * L + (int64_t)X * L + 31) / 32 * 32;
     }
@@ -839,21 +899,17 @@ extern "C" int sfa_batch2d_wide_solve(const double* F, double* U, const void* pl
     a.work = (double*)work;
     hipStream_t s = (hipStream_t)stream;
     const dim3 blk(B2_THREADS);
+    int rc;
     a.tasks = (const B2Task*)(pb + wi.off_task[0]);
-    hipLaunchKernelGGL(batch2d_tile_kernel<0>, dim3((unsigned)wi.ntask[0]), blk, 0, s, a);
-    SFA_LAUNCH_CHECK();
+    if ((rc = launch_wide_stage<0>(a, wi.cls[0], wi.ntask[0], s))) return rc;
     a.tasks = (const B2Task*)(pb + wi.off_task[1]);
-    hipLaunchKernelGGL(batch2d_tile_kernel<1>, dim3((unsigned)wi.ntask[1]), blk, 0, s, a);
-    SFA_LAUNCH_CHECK();
+    if ((rc = launch_wide_stage<1>(a, wi.cls[1], wi.ntask[1], s))) return rc;
     hipLaunchKernelGGL(batch2d_wide_solve_kernel, dim3((unsigned)wi.nprob), blk, 0, s, a);
     SFA_LAUNCH_CHECK();
     a.tasks = (const B2Task*)(pb + wi.off_task[2]);
-    hipLaunchKernelGGL(batch2d_tile_kernel<2>, dim3((unsigned)wi.ntask[2]), blk, 0, s, a);
-    SFA_LAUNCH_CHECK();
+    if ((rc = launch_wide_stage<2>(a, wi.cls[2], wi.ntask[2], s))) return rc;
     a.tasks = (const B2Task*)(pb + wi.off_task[3]);
-    hipLaunchKernelGGL(batch2d_tile_kernel<3>, dim3((unsigned)wi.ntask[3]), blk, 0, s, a);
-    SFA_LAUNCH_CHECK();
-    return SFA_OK;
+    return launch_wide_stage<3>(a, wi.cls[3], wi.ntask[3], s);
 }
 
 extern "C" int sfa_mfma_f64_layout_probe(double* out, void* stream)

@@ -34,7 +34,9 @@ __device__ void gemm_tile(const double* __restrict__ A, int64_t lda, const doubl
     constexpr int TI = BM / 32, TJ = BN / 32;
     constexpr int LA = BM * BK / GEMM_THREADS, LB = BK * BN / GEMM_THREADS;
     static_assert(LA >= 1 && LB >= 1 && TI >= 1 && TJ >= 1, "tile too small");
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));  // per-tile indices: nothing hoisted across the caller's tile loop
+    const int lane = tid & 63, wv = tid >> 6;
     const int wi = wv >> 1, wj = wv & 1;
     const int lr = lane & 15, lk = lane >> 4;
     v4d acc[TI][TJ];

@@ -152,7 +152,10 @@ def test_batch2d_wide_equals_persistent(sfa, batch):
     W = sfa.BatchedHelmholtz2D(H.Nx, H.Ny, H.alpha, ["D" if b == 0 else "N" for b in H.bc], mode="wide")
     for rep in range(2):
         assert torch.equal(W(F), ref), f"call {rep}"
-    for nx, ny, al, bc in [(512, 512, 3.0, "D"), (390, 72, 0.75, "N")]:
+    # (96, 400), (70, 600), (512, 512): more solve chains (2 Ny/2 + 2) than
+    # the workgroup's 256 threads
+    for nx, ny, al, bc in [(512, 512, 3.0, "D"), (390, 72, 0.75, "N"), (96, 400, 2.0, "N"),
+                           (70, 600, 1.5, "D")]:
         print(f"[batch2d wide: lone {nx}x{ny} {bc}]", flush=True)
         f = batch2d_rhs(nx, ny, np.random.default_rng(nx))
         P1 = sfa.BatchedHelmholtz2D([nx], [ny], [al], bc)
@@ -163,3 +166,27 @@ def test_batch2d_wide_equals_persistent(sfa, batch):
         err = _rel(u.view(nx, ny), S.legendre_fourier_2d(f, al, bc)[0])
         print(f"  max rel err {err:.2e}", flush=True)
         assert err < 1e-11
+
+
+def test_batch2d_wide_large_tiles_and_autotune(sfa):
+    """A batch large enough for the tile-parallel form's 64 x 64 tiles (at
+    least 4 per CU of them per stage): the same bits as the persistent kernel,
+    a sample of problems against the oracle, and the autotuner, which times
+    both forms, leaves a configuration that still gives those bits."""
+    from shenfun_amd.workloads import batch2d_problems, batch2d_rhs
+    print("[batch2d wide 64x64 tiles: 600 problems]", flush=True)
+    nx, ny, al, rng = batch2d_problems(600, 3)
+    fs = [batch2d_rhs(int(a), int(b), rng) for a, b in zip(nx, ny)]
+    P = sfa.BatchedHelmholtz2D(nx, ny, al, "D")
+    F = P.pack(fs)
+    ref = P(F).clone()
+    W = sfa.BatchedHelmholtz2D(nx, ny, al, "D", mode="wide")
+    assert torch.equal(W(F), ref)
+    for i in range(0, 600, 97):
+        err = _rel(W.unpack(ref)[i], S.legendre_fourier_2d(fs[i], float(al[i]), "D")[0])
+        assert err < 1e-11, (i, err)
+    tune = P.autotune(F, blocks_per_cu=(2,), variants=[0, 3], reps=1)
+    modes = {r["mode"] for r in tune["table"]}
+    print(f"  autotune: {tune['best']['label']} ({sorted(modes)})", flush=True)
+    assert modes == {"persistent", "wide"}
+    assert torch.equal(P(F), ref)
