@@ -87,54 +87,105 @@ __device__ inline void leg_rows(int i, int bc, double& Akk, double& Bkk, double&
 // (ky-major, a chain's own column, they were 32 separate lines per wave: the
 // solve launch of the tile-parallel form took 1.21 ms instead of 0.83 ms for
 // the 4096-problem batch).  Staging row chunks in LDS between barriers measured
-// slower still (1.43 ms): the many resident workgroups already cover a row's
-// load latency.  Neumann with beta = 0 (ky = 0, alpha = 0): the singular row 0
+// slower still (1.43 ms).  Neumann with beta = 0 (ky = 0, alpha = 0): the singular row 0
 // is replaced by u_0 = 0 (the constraint (0, 0) of the reference's mode-0
 // solver).
-__device__ void batch_solve(double* __restrict__ W0, int64_t ldw, int M, int Nyh, int bc,
-                            double alpha, double* __restrict__ cp)
+// One row of the forward elimination (the reference's TDMA order), x = (re, im)
+// of row i in, its reduced values out.
+__device__ __forceinline__ void thomas_row(int i, int M, int bc, double beta, double2& x, double& cc,
+                                           double& cprev, double& rprev_re, double& rprev_im,
+                                           double& lprev_up)
 {
+    double Akk, Bkk, Bup;
+    leg_rows(i, bc, Akk, Bkk, Bup);
+    const double d = Akk + beta * Bkk;
+    const double up = (i + 2 < M) ? beta * Bup : 0.0;
+    const double lo = (i >= 2) ? lprev_up : 0.0;  // B symmetric: B_{i,i-2} = B_{i-2,i}
+    double rr_re, rr_im;
+    if (bc == 1 && i == 0 && fabs(beta) < 1e-12) {
+        cc = 0.0;
+        rr_re = rr_im = 0.0;
+    } else {
+        const double dd = 1.0 / (d - lo * cprev);
+        cc = up * dd;
+        rr_re = (x.x - lo * rprev_re) * dd;
+        rr_im = (x.y - lo * rprev_im) * dd;
+    }
+    x = make_double2(rr_re, rr_im);
+    cprev = cc;
+    rprev_re = rr_re;
+    rprev_im = rr_im;
+    lprev_up = up;
+}
+
+__device__ __forceinline__ void batch_solve(double* __restrict__ W0, int64_t ldw, int M, int Nyh, int bc,
+                                            double alpha, double* __restrict__ cp)
+{
+    // A chain's row loads do not depend on its recurrence: the sweeps take the
+    // rows D at a time from one of two register buffers while the next D rows
+    // load into the other, so a step does not wait out a load of its own
+    // (clamped row indices keep the loads unconditional).
+    constexpr int D = 8;
     for (int c = threadIdx.x; c < 2 * Nyh; c += blockDim.x) {
         const int ky = c >> 1, p = c & 1;
         const double beta = alpha + (double)ky * (double)ky;
         double* cpc = cp + ky;  // cpc[i * Nyh]
+        double2* xc = reinterpret_cast<double2*>(W0 + 2 * ky);  // row i at xc[i * ldw / 2]
+        const int64_t lx = ldw / 2;
         double cprev = 0.0, rprev_re = 0.0, rprev_im = 0.0, lprev_up = 0.0;
         int last = -1;
-        for (int i = p; i < M; i += 2) {
-            double Akk, Bkk, Bup;
-            leg_rows(i, bc, Akk, Bkk, Bup);
-            const double d = Akk + beta * Bkk;
-            const double up = (i + 2 < M) ? beta * Bup : 0.0;
-            const double lo = (i >= 2) ? lprev_up : 0.0;  // B symmetric: B_{i,i-2} = B_{i-2,i}
-            double* x = W0 + (int64_t)i * ldw + 2 * ky;
-            double re = x[0], im = x[1];
-            double cc, rr_re, rr_im;
-            if (bc == 1 && i == 0 && fabs(beta) < 1e-12) {
-                cc = 0.0;
-                rr_re = rr_im = 0.0;
-            } else {
-                const double dd = 1.0 / (d - lo * cprev);
-                cc = up * dd;
-                rr_re = (re - lo * rprev_re) * dd;
-                rr_im = (im - lo * rprev_im) * dd;
+        double2 bA[D], bB[D];
+        auto fwd = [&](int i0, double2 (&cur)[D], double2 (&nxt)[D]) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) nxt[d] = xc[min(i0 + 2 * (D + d), M - 1) * lx];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int i = i0 + 2 * d;
+                if (i < M) {
+                    double cc;
+                    thomas_row(i, M, bc, beta, cur[d], cc, cprev, rprev_re, rprev_im, lprev_up);
+                    cpc[(int64_t)i * Nyh] = cc;
+                    xc[i * lx] = cur[d];
+                    last = i;
+                }
             }
-            cpc[(int64_t)i * Nyh] = cc;
-            x[0] = rr_re;
-            x[1] = rr_im;
-            cprev = cc;
-            rprev_re = rr_re;
-            rprev_im = rr_im;
-            lprev_up = up;
-            last = i;
+        };
+#pragma unroll
+        for (int d = 0; d < D; ++d) bA[d] = xc[min(p + 2 * d, M - 1) * lx];
+        for (int i0 = p; i0 < M; i0 += 4 * D) {
+            fwd(i0, bA, bB);
+            if (i0 + 2 * D < M) fwd(i0 + 2 * D, bB, bA);
         }
+        // back substitution over the same rows, last to first
         double xr = 0.0, xi = 0.0;
-        for (int i = last; i >= 0; i -= 2) {
-            double* x = W0 + (int64_t)i * ldw + 2 * ky;
-            const double c = cpc[(int64_t)i * Nyh];
-            xr = x[0] - c * xr;
-            xi = x[1] - c * xi;
-            x[0] = xr;
-            x[1] = xi;
+        double2 cA[D], cB[D];
+        double kA[D], kB[D];
+        auto bwd = [&](int i0, double2 (&cx)[D], double (&ck)[D], double2 (&nx)[D], double (&nk)[D]) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int i = max(i0 - 2 * (D + d), 0);
+                nx[d] = xc[i * lx];
+                nk[d] = cpc[(int64_t)i * Nyh];
+            }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int i = i0 - 2 * d;
+                if (i >= 0) {
+                    xr = cx[d].x - ck[d] * xr;
+                    xi = cx[d].y - ck[d] * xi;
+                    xc[i * lx] = make_double2(xr, xi);
+                }
+            }
+        };
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int i = max(last - 2 * d, 0);
+            cA[d] = xc[i * lx];
+            kA[d] = cpc[(int64_t)i * Nyh];
+        }
+        for (int i0 = last; i0 >= 0; i0 -= 4 * D) {
+            bwd(i0, cA, kA, cB, kB);
+            if (i0 - 2 * D >= 0) bwd(i0 - 2 * D, cB, kB, cA, kA);
         }
     }
     __syncthreads();
@@ -210,6 +261,7 @@ struct B2WideArgs {
     const B2Prob* probs;
     const B2Task* tasks;   // this stage's tasks
     const int64_t* woff;   // per problem, in doubles
+    const int* order;      // the solve launch's problem per workgroup
     const double* mats;
     double* work;
 };
@@ -264,7 +316,7 @@ static int launch_wide_stage(const B2WideArgs& a, int cls, int64_t ntask, hipStr
 
 __global__ void __launch_bounds__(B2_THREADS) batch2d_wide_solve_kernel(B2WideArgs a)
 {
-    const int p = blockIdx.x;
+    const int p = a.order[blockIdx.x];
     const B2Prob pr = a.probs[p];
     const int Nx = pr.Nx, Ny = pr.Ny, M = Nx - 2, L = Ny + 2;
     double* W0 = a.work + a.woff[p];
@@ -627,6 +679,8 @@ static void b2_wide_shape(int nprob, const int* Nx, const int* Ny, int64_t ntask
     }
 }
 
+// plan layout: problem table | tasks of each stage | per-problem workspace
+// offsets | solve order (problems by decreasing rows, the longest chains first)
 static int64_t b2_wide_bytes(int nprob, const int64_t ntask[4], int64_t off_task[4], int64_t* off_woff)
 {
     int64_t o = r256((int64_t)nprob * sizeof(B2Prob));
@@ -635,7 +689,8 @@ static int64_t b2_wide_bytes(int nprob, const int64_t ntask[4], int64_t off_task
         o += r256(ntask[st] * (int64_t)sizeof(B2Task));
     }
     *off_woff = o;
-    return o + r256((int64_t)nprob * 8);
+    o += r256((int64_t)nprob * 8);
+    return o + r256((int64_t)nprob * 4);  // the solve order after the offsets
 }
 
 template <int BM, int BN, int BK>
@@ -869,6 +924,13 @@ extern "C" int sfa_batch2d_wide_plan(int nprob, const int* Nx, const int* Ny, co
         SFA_HIP(hipMemcpyAsync(pb + wi.off_task[st], tasks[st].data(), tasks[st].size() * sizeof(B2Task),
                                hipMemcpyHostToDevice, s));
     SFA_HIP(hipMemcpyAsync(pb + wi.off_woff, woff.data(), woff.size() * 8, hipMemcpyHostToDevice, s));
+    // the solve's chains run sequentially over a problem's rows: the problems
+    // with the most rows start first so they do not finish last
+    std::vector<int> order(nprob);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return Nx[x] > Nx[y]; });
+    SFA_HIP(hipMemcpyAsync(pb + wi.off_woff + r256((int64_t)nprob * 8), order.data(), order.size() * 4,
+                           hipMemcpyHostToDevice, s));
     SFA_HIP(hipStreamSynchronize(s));  // host vectors are copied before they go out of scope
     std::lock_guard<std::mutex> lk(g_b2_mu);
     g_b2.erase(plan);  // the address may have held a persistent-mode plan
@@ -895,6 +957,7 @@ extern "C" int sfa_batch2d_wide_solve(const double* F, double* U, const void* pl
     a.U = U;
     a.probs = (const B2Prob*)pb;
     a.woff = (const int64_t*)(pb + wi.off_woff);
+    a.order = (const int*)(pb + wi.off_woff + r256((int64_t)wi.nprob * 8));
     a.mats = wi.mats;
     a.work = (double*)work;
     hipStream_t s = (hipStream_t)stream;
