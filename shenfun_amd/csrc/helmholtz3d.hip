@@ -862,7 +862,17 @@ extern "C" int sfa_helmholtz3d_guarded(const double* f, double* u, int64_t nbatc
     SFA_HIP(hipMemcpyAsync(g->host, stats + kStatSumOff, kStatAcc * sizeof(double),
                            hipMemcpyDeviceToHost, g->side));
     SFA_HIP(hipEventRecord(g->evc, g->side));
-    SFA_HIP(hipEventSynchronize(g->evc));
+    // Wait by polling: the statistics are ready about 20 us after the x-pass,
+    // while the inverse passes still have ~70 us to run, and the next call
+    // must be enqueued before those end; a blocking wait's wake-up latency had
+    // let the device idle between solves (~16 us per guarded solve at 256^3).
+    for (;;) {
+        const hipError_t q = hipEventQuery(g->evc);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) {
+            SFA_HIP(q);
+        }
+    }
     double e = guard_predict(g->host, Kh, Nx, Ny, Nz, alpha);
     // admissible first complex64 planes 16m + 1 (the band edges)
     int next = 0;
