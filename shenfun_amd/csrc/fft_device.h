@@ -13,18 +13,42 @@ namespace sfa {
 // 2 kStatBands-1), both summed by the x-pass.  Each accumulator is spread
 // over kStatSlots slots of one 128-byte line each, picked by workgroup index:
 // thousands of waves adding into a handful of addresses serialise on them
-// (a z R2C summing per wave at 256^3 took 117 us instead of 50).  stat_reduce_kernel sums the
-// slots into the 2 kStatBands doubles at kStatSumOff.
+// (a z R2C summing per wave at 256^3 took 117 us instead of 50).  A one-workgroup
+// kernel after the x-pass sums the slots into the 2 kStatBands results.
 constexpr int kStatBands = 16;
 constexpr int kStatSlots = 64;
 constexpr int kStatLine = 16;  // doubles per slot
 constexpr int kStatAcc = 2 * kStatBands;
-constexpr int kStatSumOff = kStatAcc * kStatSlots * kStatLine;  // doubles
-constexpr int kStatBytes = (kStatSumOff + kStatAcc) * 8;
+constexpr int kStatBytes = kStatAcc * kStatSlots * kStatLine * 8;
 __host__ __device__ constexpr int stat_band(int kz) { return kz == 0 ? 0 : (kz - 1) / 16; }
 __host__ __device__ constexpr int stat_index(int acc, int slot)
 {
     return (acc * kStatSlots + slot) * kStatLine;
+}
+
+// Where the summed statistics go: kStatAcc doubles, then a flag set to seq
+// once they are written (device addresses of pinned, fine-grained host memory
+// that the host spins on).  flag == NULL: no delivery.
+struct StatOut {
+    double* sums;
+    unsigned* flag;
+    unsigned seq;
+};
+
+// One workgroup (every thread calls it) sums the kStatSlots slots of each
+// accumulator of a finished statistics buffer, writes the kStatAcc sums and
+// then out.seq to out.flag.
+__device__ inline void stat_deliver(const double* __restrict__ stats, const StatOut& out)
+{
+    for (int acc = threadIdx.x; acc < kStatAcc; acc += blockDim.x) {
+        double s = 0.0;
+#pragma unroll 16
+        for (int sl = 0; sl < kStatSlots; ++sl) s += stats[stat_index(acc, sl)];
+        out.sums[acc] = s;
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(out.flag, out.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 struct FFTPlan {

@@ -66,13 +66,22 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q)
 
 // the mixed form's two sub-arrays in one grid: the complex128 planes' blocks
 // first, then the complex64 planes' blocks (two launches left the GPU
-// under-filled during the few complex128 blocks)
+// under-filled during the few complex128 blocks).  out.flag != NULL: block 0
+// (dispatched first) delivers the statistics in stats, which the previous
+// kernel finished, while the other blocks transform.
 template <int N, bool INV>
 __global__ void __launch_bounds__(TL * Pow2<N>::Q)
     c2c_strided_dual_kernel(StridedLines g64, StridedLines g32, int64_t nblk64,
-                            const double2* __restrict__ twg)
+                            const double2* __restrict__ twg, const double* stats, StatOut out)
 {
-    const int64_t b = blockIdx.x;
+    int64_t b = blockIdx.x;
+    if (out.flag != nullptr) {
+        if (b == 0) {
+            stat_deliver(stats, out);
+            return;
+        }
+        --b;
+    }
     if (b < nblk64)
         c2c_strided_body<N, INV, double2>(g64, twg, b);
     else
@@ -272,20 +281,21 @@ static int launch_c2c_strided(const Geom& gi, const Geom& go, const void* in, vo
 
 template <int N, bool INV>
 static int launch_c2c_strided_dual(const Geom& g64, void* W64, const Geom& g32, void* W32,
-                                   double scale, hipStream_t s)
+                                   double scale, hipStream_t s, const double* stats, StatOut out)
 {
     using Pl = Pow2<N>;
     const double2* tw = fft_table(N);
     SFA_CHECK(tw, SFA_ERR_HIP, "twiddle table allocation failed");
     const int64_t nb64 = (g64.d1 * g64.d2 + TL - 1) / TL, nb32 = (g32.d1 * g32.d2 + TL - 1) / TL;
-    SFA_CHECK(nb64 + nb32 < (1ll << 31), SFA_ERR_UNSUPPORTED, "too many lines");
+    SFA_CHECK(nb64 + nb32 < (1ll << 31) - 1, SFA_ERR_UNSUPPORTED, "too many lines");
     const size_t lds = sizeof(double2) * ((size_t)N + (size_t)TL * Pl::LS);  // the larger body
     auto k = c2c_strided_dual_kernel<N, INV>;
     if (lds > 64 * 1024)
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k, dim3((unsigned)(nb64 + nb32)), dim3(TL * Pl::Q), lds, s,
+    const int64_t nblk = nb64 + nb32 + (out.flag != nullptr ? 1 : 0);
+    hipLaunchKernelGGL(k, dim3((unsigned)nblk), dim3(TL * Pl::Q), lds, s,
                        strided_lines(g64, g64, W64, W64, scale), strided_lines(g32, g32, W32, W32, scale),
-                       nb64, tw);
+                       nb64, tw, stats, out);
     SFA_LAUNCH_CHECK();
     return SFA_OK;
 }
@@ -295,15 +305,16 @@ static int launch_c2c_strided_dual(const Geom& g64, void* W64, const Geom& g32, 
 // takes 38.5 -> 34.2 us per pass: the 272 complex128 blocks alone leave a
 // tail on 256 CUs.  Not at Ny = 512, where the complex128 body's 256 VGPRs
 // would hold the complex64 blocks to one wave per SIMD (217 -> 261 us).
+// out.flag != NULL: the launch also delivers the statistics in stats.
 int fast_c2c_mixed(const Geom& g64, void* W64, const Geom& g32, void* W32, int sign, double scale,
-                   hipStream_t s, int* rc)
+                   hipStream_t s, int* rc, const double* stats, StatOut out)
 {
     if (!(g64.s2 == 1 && g32.s2 == 1 && g64.sa != 1 && g32.sa != 1 && g64.n == g32.n)) return 0;
     const bool inv = sign > 0;
 #define SFA_C2CM(NN)                                                                      \
     case NN:                                                                              \
-        *rc = inv ? launch_c2c_strided_dual<NN, true>(g64, W64, g32, W32, scale, s)       \
-                  : launch_c2c_strided_dual<NN, false>(g64, W64, g32, W32, scale, s);     \
+        *rc = inv ? launch_c2c_strided_dual<NN, true>(g64, W64, g32, W32, scale, s, stats, out)   \
+                  : launch_c2c_strided_dual<NN, false>(g64, W64, g32, W32, scale, s, stats, out); \
         return 1;
     switch (g64.n) {
         SFA_C2CM(16) SFA_C2CM(32) SFA_C2CM(64) SFA_C2CM(128) SFA_C2CM(256)

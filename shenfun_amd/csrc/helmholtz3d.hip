@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -240,12 +241,20 @@ int fast_xpass(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int ne
 int fast_xpass_mixed(void* W64, void* W32, int Nx, int Ny, int Nzh, int Kh, double alpha,
                      int neumann, const double* K2, const XPlan& pl, int64_t nb, hipStream_t s, int* rc,
                      double* stats);
+
+// statistics delivery on its own, where the inverse y pass cannot carry it
+__global__ void __launch_bounds__(64) stat_deliver_kernel(const double* stats, StatOut out)
+{
+    stat_deliver(stats, out);
+}
+
 int fast_c2c(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
              hipStream_t s, int* rc);
 int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
                  hipStream_t s, int* rc);
 int fast_c2c_mixed(const Geom& g64, void* W64, const Geom& g32, void* W32, int sign, double scale,
-                   hipStream_t s, int* rc);
+                   hipStream_t s, int* rc, const double* stats = nullptr,
+                   StatOut out = StatOut{nullptr, nullptr, 0u});
 int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
              double scale, hipStream_t s, int* rc, double* zero = nullptr, int nzero = 0);
 
@@ -435,11 +444,18 @@ int xpass_launch(double2* W, int Nx, int Ny, int Nzh, int Kp, double alpha, int 
 // y C2C of both sub-arrays of the mixed workspace: one launch whose first
 // blocks take the complex128 planes (running the two on two streams measured
 // slower); two launches for the lengths the dual kernel does not cover.
+// out.flag != NULL: also deliver the statistics in stats (a block of the
+// one launch, else a kernel of its own first).
 static int y_pass_mixed(const Geom& g64, const Geom& g32, double2* W64, float2* W32, int sign,
-                        double scale, hipStream_t s)
+                        double scale, hipStream_t s, const double* stats = nullptr,
+                        StatOut out = StatOut{nullptr, nullptr, 0u})
 {
     int rc = 0;
-    if (fast_c2c_mixed(g64, W64, g32, W32, sign, scale, s, &rc)) return rc;
+    if (fast_c2c_mixed(g64, W64, g32, W32, sign, scale, s, &rc, stats, out)) return rc;
+    if (out.flag) {
+        hipLaunchKernelGGL(stat_deliver_kernel, dim3(1), dim3(64), 0, s, stats, out);
+        SFA_LAUNCH_CHECK();
+    }
     if (!fast_c2c(g64, g64, W64, W64, sign, scale, s, &rc) || rc ||
         !fast_c2c_f32(g32, g32, W32, W32, sign, scale, s, &rc) || rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Ny=%lld unsupported", (long long)g64.n);
@@ -453,12 +469,12 @@ static int y_pass_mixed(const Geom& g64, const Geom& g32, double2* W64, float2* 
 // nb arrays (Nx, Ny, Nz) back to back in f and u; the workspace holds the nb
 // arrays' complex128 planes, then their complex64 planes.
 // stats (may be NULL): band-energy accumulators (kStatBytes), cleared by the z
-// R2C split and then added to by the x-pass; evx (may be NULL): recorded after
-// the x-pass.
+// R2C split and then added to by the x-pass; out.flag (may be NULL): where the
+// inverse y pass delivers the summed accumulators.
 static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                              double alpha, int neumann, int Kh, const void* plan, void* work,
                              hipStream_t s, hipEvent_t* ev, int64_t nb = 1, double* stats = nullptr,
-                             hipEvent_t evx = nullptr)
+                             StatOut out = StatOut{nullptr, nullptr, 0u})
 {
     auto mark = [&](int i) {
         if (ev) hipEventRecord(ev[i], s);
@@ -499,8 +515,7 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
         rc)
         return rc ? rc : fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nx=%lld unsupported", (long long)Nx);
     mark(3);
-    if (evx) SFA_HIP(hipEventRecord(evx, s));
-    if ((rc = y_pass_mixed(g64, g32, W64, W32, +1, 1.0, s))) return rc;
+    if ((rc = y_pass_mixed(g64, g32, W64, W32, +1, 1.0, s, stats, out))) return rc;
     mark(4);
     if (!split_rc(false, (int)(Nz / 2), u, W64, W32, rows, Kh, 1.0, s, &rc))
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz unsupported");
@@ -742,21 +757,6 @@ extern "C" int sfa_helmholtz3d_noalloc(const double* f, double* u, int64_t Nx, i
 
 namespace sfa {
 
-// the slots of each statistics accumulator summed into stats[kStatSumOff + acc]:
-// 8 lanes per accumulator, each summing every 8th slot, then a butterfly
-constexpr int kStatReduceThreads = kStatAcc * 8;
-__global__ void __launch_bounds__(kStatReduceThreads) stat_reduce_kernel(double* stats)
-{
-    const int acc = threadIdx.x >> 3, part = threadIdx.x & 7;
-    double s = 0.0;
-#pragma unroll
-    for (int sl = part; sl < kStatSlots; sl += 8) s += stats[stat_index(acc, sl)];
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
-    s += __shfl_xor(s, 4);
-    if (part == 0) stats[kStatSumOff + acc] = s;
-}
-
 // Error model of the mixed form.  Complex64 storage and FP32 transforms of
 // the planes kz >= Kh add relative rounding errors of a few eps32 per stage to
 // those planes' values: forward (the stored z spectrum of f, the y FFT, the
@@ -800,13 +800,18 @@ static double guard_predict(const double* h, int Kh, int64_t Nx, int64_t Ny, int
     return r >= 0.0 ? kGuardK * kEps32 * sqrt(r) : INFINITY;
 }
 
-// Per host thread and device: a side stream, two events and a pinned buffer
-// that carry the statistics to the host while the backward kernels run.
-// They live until the process ends.
+// Per host thread and device: a pinned, device-mapped, fine-grained host
+// buffer into which the inverse y pass writes the summed statistics and then
+// the call's sequence number.  It lives until the process ends.
+struct GuardHost {
+    double sums[kStatAcc];
+    volatile unsigned flag;
+};
+
 struct GuardCtx {
-    hipStream_t side;
-    hipEvent_t evx, evc;
-    double* host;
+    GuardHost* host;
+    GuardHost* dev;  // device address of host
+    unsigned seq;
 };
 
 static GuardCtx* guard_ctx()
@@ -817,12 +822,40 @@ static GuardCtx* guard_ctx()
     auto it = ctxs.find(dev);
     if (it != ctxs.end()) return &it->second;
     GuardCtx c{};
-    if (hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess) return nullptr;
-    if (hipEventCreateWithFlags(&c.evx, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c.evc, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void**)&c.host, kStatAcc * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&c.host, sizeof(GuardHost), hipHostMallocMapped | hipHostMallocCoherent) !=
+        hipSuccess)
         return nullptr;
+    if (hipHostGetDevicePointer((void**)&c.dev, c.host, 0) != hipSuccess) {
+        hipHostFree(c.host);
+        return nullptr;
+    }
+    c.host->flag = 0u;
     return &(ctxs[dev] = c);
+}
+
+// Spin until call seq's sums are published.  Once per millisecond of waiting
+// the stream is queried (a query can append a marker to the stream's queue,
+// which measured as a ~6 us stall before the next call's first kernel when
+// issued every few microseconds): an error ends the wait with that error, and
+// a stream that has drained without the flag set is reported as a failure.
+static int guard_wait(const GuardCtx* g, unsigned seq, hipStream_t s)
+{
+    using clk = std::chrono::steady_clock;
+    auto next = clk::now() + std::chrono::milliseconds(1);
+    for (unsigned it = 1;; ++it) {
+        if (g->host->flag == seq) break;
+        if ((it & 1023u) == 0 && clk::now() >= next) {
+            const hipError_t q = hipStreamQuery(s);
+            if (q == hipSuccess) {
+                if (g->host->flag == seq) break;
+                return fail(SFA_ERR_HIP, "guarded solve: statistics were not delivered");
+            }
+            if (q != hipErrorNotReady) SFA_HIP(q);
+            next = clk::now() + std::chrono::milliseconds(1);
+        }
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);  // the sums were written before the flag
+    return SFA_OK;
 }
 
 }  // namespace sfa
@@ -848,37 +881,25 @@ extern "C" int sfa_helmholtz3d_guarded(const double* f, double* u, int64_t nbatc
               "workspace too small (need nbatch * %lld + %d bytes)", (long long)core, kStatBytes);
     hipStream_t s = (hipStream_t)stream;
     GuardCtx* g = guard_ctx();
-    SFA_CHECK(g != nullptr, SFA_ERR_HIP, "guard stream/event/pinned buffer creation failed");
+    SFA_CHECK(g != nullptr, SFA_ERR_HIP, "guard pinned buffer creation failed");
     double* stats = (double*)((char*)work + nbatch * core);  // cleared by the z R2C split
     const int Kh = mixed_planes(Nz, kh_hint);
+    const unsigned seq = ++g->seq == 0u ? ++g->seq : g->seq;  // 0 is the buffer's initial flag
+    // The statistics are complete when the x-pass ends; the first workgroup
+    // of the inverse y pass sums them into the host buffer, and the host's
+    // wait for that overlaps the inverse y and z passes.  The wait spins: the next call must be
+    // enqueued before those ~70 us (256^3) run out, and a blocking wait's
+    // wake-up latency had let the device idle between solves.
     int rc = helmholtz3d_mixed(f, u, Nx, Ny, Nz, alpha, neumann, Kh, plan, work, s, nullptr, nbatch,
-                               stats, g->evx);
+                               stats, StatOut{g->dev->sums, const_cast<unsigned*>(&g->dev->flag), seq});
     if (rc) return rc;
-    // the statistics are complete after the x-pass: copy them on the side
-    // stream while the inverse y and z passes run
-    SFA_HIP(hipStreamWaitEvent(g->side, g->evx, 0));
-    hipLaunchKernelGGL(stat_reduce_kernel, dim3(1), dim3(kStatReduceThreads), 0, g->side, stats);
-    SFA_LAUNCH_CHECK();
-    SFA_HIP(hipMemcpyAsync(g->host, stats + kStatSumOff, kStatAcc * sizeof(double),
-                           hipMemcpyDeviceToHost, g->side));
-    SFA_HIP(hipEventRecord(g->evc, g->side));
-    // Wait by polling: the statistics are ready about 20 us after the x-pass,
-    // while the inverse passes still have ~70 us to run, and the next call
-    // must be enqueued before those end; a blocking wait's wake-up latency had
-    // let the device idle between solves (~16 us per guarded solve at 256^3).
-    for (;;) {
-        const hipError_t q = hipEventQuery(g->evc);
-        if (q == hipSuccess) break;
-        if (q != hipErrorNotReady) {
-            SFA_HIP(q);
-        }
-    }
-    double e = guard_predict(g->host, Kh, Nx, Ny, Nz, alpha);
+    if ((rc = guard_wait(g, seq, s))) return rc;
+    double e = guard_predict(g->host->sums, Kh, Nx, Ny, Nz, alpha);
     // admissible first complex64 planes 16m + 1 (the band edges)
     int next = 0;
     double enext = 0.0;
     for (int m = 1; 16 * m + 1 <= Nzh - 2; ++m) {
-        const double em = guard_predict(g->host, 16 * m + 1, Nx, Ny, Nz, alpha);
+        const double em = guard_predict(g->host->sums, 16 * m + 1, Nx, Ny, Nz, alpha);
         if (em <= tol) {
             next = 16 * m + 1;
             enext = em;

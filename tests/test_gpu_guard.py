@@ -136,3 +136,29 @@ def test_guarded_solve_many(sfa):
     assert H2.last_kh == 33
     for b in range(3):
         assert rel(U[b], S.helmholtz3d_cheb_fourier(fs[b], 1.5, "N")[0]) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 64), (64, 512, 64)])
+def test_guard_statistics_follow_each_call(sfa, shape):
+    """The predicted error comes from the current call's statistics: calls
+    alternating between two right-hand sides reproduce each one's prediction
+    from a fresh solver (up to the order of the atomic sums), so no call reads
+    the previous call's sums.  Ny = 512 takes the inverse y pass's two-launch
+    form, whose statistics go through a kernel of their own."""
+    print("[guard sequence]", shape, flush=True)
+    T = _space(sfa, shape, "D")
+    fa, fb = dev(rhs("decay2", shape, seed=3)), dev(rhs("decay2", shape, seed=4))
+    ref = []
+    for f in (fa, fb):
+        H = sfa.HelmholtzSolver3D(T, 1.0, precision="mixed")
+        H(f)
+        assert H.last_kh == 17
+        ref.append(H.last_predicted_error)
+    assert abs(ref[0] - ref[1]) > 1e-6 * ref[0]
+    H = sfa.HelmholtzSolver3D(T, 1.0, precision="mixed")
+    for i in range(6):
+        u = H(fa if i % 2 == 0 else fb)
+        assert H.last_kh == 17
+        assert abs(H.last_predicted_error - ref[i % 2]) <= 1e-9 * ref[i % 2]
+    f = (fa, fb)[1]
+    assert rel(u, S.helmholtz3d_cheb_fourier(f.cpu().numpy(), 1.0, "D")[0]) < 1e-10
