@@ -135,6 +135,14 @@ int sfa_boundary_update(const void* src, void* dst, int dtype, int ndim, const i
  * R2C/C2C.convolve(fast=True) fourier/bases.py:437-475, 600-640). */
 int sfa_pointwise_mul(const void* a, const void* b, void* out, int dtype, int64_t n, void* stream);
 
+/* y[r n + i] += a x[i] for i < n, r < nrep (float64, contiguous).  No single
+ * reference counterpart: the fused 3D solve's nonhomogeneous boundary values
+ * enter as u = u_0 + L, u_0 the solution with homogeneous values and L the
+ * lift ((alpha - lap) L = 0 with the given values), which the reference forms
+ * through its bc matrices and the boundary basis in backward
+ * (chebyshev/la.py:303-307, tensorproductspace.py:2070-2089). */
+int sfa_axpy(const double* x, double* y, double a, int64_t n, int64_t nrep, void* stream);
+
 /* Direct convolution sums of two Fourier coefficient vectors (complex128,
  * contiguous): real = 0, C2C, N coefficients in FFT order, out 2N entries
  * (convolve_1D, optimization/cython/convolve.pyx:9-30); real = 1, R2C, N/2+1

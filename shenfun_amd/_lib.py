@@ -40,6 +40,7 @@ _SIGS = {
     "sfa_pad_axis": "vviiaaaaiiillllv",
     "sfa_boundary_update": "vviiaaaailllvvlv",
     "sfa_pointwise_mul": "vvvilv",
+    "sfa_axpy": "vvdllv",
     "sfa_convolve_direct": "vvvliv",
     "sfa_band_matvec": "vviiaaaaillivvviv",
     "sfa_tensor_eval": "vlllvvvvlv",

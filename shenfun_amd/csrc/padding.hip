@@ -340,6 +340,45 @@ extern "C" int sfa_pointwise_mul(const void* a, const void* b, void* out, int dt
     return SFA_OK;
 }
 
+// y[r n + i] += a x[i] for r < nrep: the lift of nonhomogeneous boundary
+// values added to each solution of a stack.  Two doubles per thread (16-byte
+// accesses) where x and y are 16-byte aligned and n is even, else one.
+template <int V>
+__global__ void __launch_bounds__(256) axpy_kernel(const double* __restrict__ x, double* __restrict__ y,
+                                                   double a, int64_t nv, int64_t nrep)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv * nrep; i += stride) {
+        const int64_t k = i % nv;
+        if constexpr (V == 2) {
+            const double2 xv = reinterpret_cast<const double2*>(x)[k];
+            double2 yv = reinterpret_cast<double2*>(y)[i];
+            yv.x += a * xv.x;
+            yv.y += a * xv.y;
+            reinterpret_cast<double2*>(y)[i] = yv;
+        } else {
+            y[i] += a * x[k];
+        }
+    }
+}
+
+extern "C" int sfa_axpy(const double* x, double* y, double a, int64_t n, int64_t nrep, void* stream)
+{
+    SFA_CHECK(x != nullptr && y != nullptr && n >= 0 && nrep >= 0, SFA_ERR_ARG, "axpy: bad arguments");
+    if (n == 0 || nrep == 0) return SFA_OK;
+    const bool v2 = n % 2 == 0 && ((uintptr_t)x | (uintptr_t)y) % 16 == 0;
+    const int64_t nv = v2 ? n / 2 : n, tot = nv * nrep;
+    const int64_t blocks = std::min<int64_t>((tot + 255) / 256, 8192);
+    if (v2)
+        hipLaunchKernelGGL(axpy_kernel<2>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, x, y, a,
+                           nv, nrep);
+    else
+        hipLaunchKernelGGL(axpy_kernel<1>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, x, y, a,
+                           nv, nrep);
+    SFA_LAUNCH_CHECK();
+    return SFA_OK;
+}
+
 extern "C" int sfa_convolve_direct(const void* u, const void* v, void* out, int64_t N, int real,
                                    void* stream)
 {

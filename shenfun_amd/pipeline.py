@@ -17,6 +17,11 @@ after the x-pass, and a prediction above ``tol`` redoes the solve with more
 complex128 planes or in FP64.  ``last_kh`` / ``last_predicted_error`` report
 the cut that produced the returned u.  guard=False (or precision='mixed' with
 an explicit ``fp64_planes`` and guard=False) runs the unchecked form.
+
+Nonhomogeneous boundary values on the Chebyshev axis (constant, varying along
+y and z, or time-dependent) give u = u_0 + L: u_0 the fused solve with
+homogeneous values, L the lift computed once by the drop-in chain on f = 0
+(see ``lift``) and added on the device (sfa_axpy).
 """
 import ctypes
 
@@ -45,10 +50,12 @@ class HelmholtzSolver3D:
         if b[0].quad != "GC":
             raise NotImplementedError("the fused solver is built for Chebyshev-Gauss (GC) points; "
                                       "use T.scalar_product -> la.Helmholtz -> T.backward for GL")
-        if b[0].has_nonhomogeneous_bcs or any(x.is_padded for x in b):
-            raise NotImplementedError("the fused solver takes homogeneous boundary values on the "
-                                      "quadrature grid; use T.scalar_product -> la.Helmholtz -> "
-                                      "T.backward")
+        if any(x.is_padded for x in b):
+            raise NotImplementedError("the fused solver takes unpadded spaces; use "
+                                      "T.scalar_product -> la.Helmholtz -> T.backward")
+        if b[0].has_nonhomogeneous_bcs and b[0].bc_kind == "N" and float(alpha) == 0.0:
+            raise NotImplementedError("nonhomogeneous Neumann values need alpha > 0 in the fused "
+                                      "solver; use T.scalar_product -> la.Helmholtz -> T.backward")
         self.T = T
         self.alpha = float(alpha)
         self.neumann = b[0].bc_kind == "N"
@@ -60,6 +67,8 @@ class HelmholtzSolver3D:
         self._fp64_left = 0
         self.last_kh = None
         self.last_predicted_error = None
+        self._lift = None
+        self._lift_key = None
         self.shape = T.shape(False)
         Nx, Ny, Nz = self.shape
         self._ws_bytes = int(_lib.lib().sfa_helmholtz3d_workspace(Nx, Ny, Nz, self.precision))
@@ -101,6 +110,37 @@ class HelmholtzSolver3D:
         return u
 
     def _run(self, f, u, nb, w):
+        self._solve(f, u, nb, w)
+        if self.T.bases[0].has_nonhomogeneous_bcs:
+            L = self.lift()
+            _lib.call("sfa_axpy", _lib.ptr(L), _lib.ptr(u), 1.0, L.numel(), nb, _lib.stream_ptr())
+
+    def lift(self):
+        """The lift L of the x-basis' boundary values on the grid (FP64):
+        (alpha - lap) L = 0 with L (Dirichlet) or L_x (Neumann) equal to the
+        given values at x = -1, 1.  A solve returns u = u_0 + L, u_0 the fused
+        solve of f with homogeneous values: the same discrete solution as the
+        drop-in chain, where the bc matrices move the boundary basis' part to
+        the right-hand side (chebyshev/la.py:303-307) and backward adds the
+        boundary basis (tensorproductspace.py:2070-2089).  L is that chain on
+        f = 0, computed when first needed and again when the values change
+        (time-dependent values after ``bc.update``).  The guard's error
+        prediction covers u_0."""
+        b0 = self.T.bases[0]
+        key = repr(b0.bc_values)
+        if self._lift is None or key != self._lift_key:
+            from . import la
+            from .matrices import helmholtz_tpmatrices
+            from .tensorproductspace import Function
+            H = la.Helmholtz(*helmholtz_tpmatrices(self.T, self.alpha))
+            zero = torch.zeros(tuple(self.T.shape(True)), dtype=torch.complex128,
+                               device=self.T.device)
+            L = torch.empty(tuple(self.shape), dtype=torch.float64, device=self.T.device)
+            self._lift = self.T.backward(H(zero, Function(self.T)), L)
+            self._lift_key = key
+        return self._lift
+
+    def _solve(self, f, u, nb, w):
         Nx, Ny, Nz = self.shape
         if self.guard and self._fp64_left == 0:
             used, nxt, err = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
