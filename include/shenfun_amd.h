@@ -426,8 +426,12 @@ int sfa_helmholtz3d_many(const double* f, double* u, int64_t nbatch, int64_t Nx,
  * (f is left unchanged).  Out (each may be NULL): kh_used = first complex64
  * plane of the returned u (Nz/2 + 1: all FP64), kh_next = the smallest
  * passing cut for this f (0: none), err_pred = the prediction for kh_used (0
- * for FP64).  Synchronises with the host once per call, after the x-pass.
- * work: at least nbatch x the complex workspace + the statistics area
+ * for FP64).  Synchronises with the host once per call, after the x-pass:
+ * the first workgroup of the inverse y pass writes the sums and a sequence
+ * number to pinned host memory, and the calling thread spins on it (querying
+ * the stream once per millisecond for errors) while the inverse passes run.
+ * Calls from one host thread to one device share that buffer and run one at
+ * a time (each returns after its wait).  work: at least nbatch x the complex workspace + the statistics area
  * (sfa_helmholtz3d_workspace() bytes for nbatch = 1).  Same reference sequence
  * as sfa_helmholtz3d; the guard has no reference counterpart (shenfun is FP64
  * only). */
