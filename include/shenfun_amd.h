@@ -400,6 +400,17 @@ int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double alpha, int n
  * memory is freed or reused: a plan is recognised by its address).  No
  * reference counterpart (chebyshev.la.Helmholtz's factors are Python-owned). */
 int sfa_helmholtz3d_plan_release(const void* plan);
+/* Register with a plan a lift L (Nx * Ny * Nz doubles on the device, the
+ * physical grid, 16-byte aligned; NULL clears it) that every later solve with
+ * that plan (sfa_helmholtz3d, _many, _guarded, _noalloc, _stage_times) adds
+ * to u, inside the final z C2R's stores where its fast kernel runs, else by
+ * a pass over u: nonhomogeneous boundary values as u = u_0 + L, u_0 the
+ * solve with homogeneous values and L the solution of (alpha - lap) L = 0
+ * with the given values, the same discrete solution as the reference's bc
+ * matrices (chebyshev/la.py:303-307) with the boundary basis added in
+ * backward (tensorproductspace.py:2070-2089).  The caller keeps L alive while
+ * it is registered. */
+int sfa_helmholtz3d_set_lift(const void* plan, const double* lift);
 int64_t sfa_helmholtz3d_workspace(int64_t Nx, int64_t Ny, int64_t Nz, int precision);
 int sfa_helmholtz3d(const double* f, double* u, int64_t Nx, int64_t Ny, int64_t Nz,
                     double alpha, int neumann, int precision, int fp64_planes, const void* plan,

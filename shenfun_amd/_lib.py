@@ -41,6 +41,7 @@ _SIGS = {
     "sfa_boundary_update": "vviiaaaailllvvlv",
     "sfa_pointwise_mul": "vvvilv",
     "sfa_axpy": "vvdllv",
+    "sfa_helmholtz3d_set_lift": "vv",
     "sfa_convolve_direct": "vvvliv",
     "sfa_band_matvec": "vviiaaaaillivvviv",
     "sfa_tensor_eval": "vlllvvvvlv",

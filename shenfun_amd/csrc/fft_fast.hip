@@ -166,12 +166,47 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
                        (int64_t)blockIdx.x * TL);
 }
 
-template <int H, int NR = TL>
+// Row store of the C2R kernels: out = scale z, plus, with ADD, the same row
+// of add (the rows of add are contiguous, 2H doubles each); all of a
+// thread's add loads are issued before its first store.
+template <int H, bool ADD>
+__device__ __forceinline__ void c2r_store(const double2 (&a)[Pow2<H>::P], int j, double* __restrict__ dst,
+                                          const double* __restrict__ arow, double scale)
+{
+    using Pl = Pow2<H>;
+    constexpr int P = Pl::P, Q = Pl::Q;
+    double2 ad[P];
+    if constexpr (ADD) {
+#pragma unroll
+        for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+            for (int k2 = 0; k2 < Q; ++k2)
+                ad[h * Q + k2] = *reinterpret_cast<const double2*>(arow + 2 * pow2_out_index<H>(j, h, k2));
+    }
+#pragma unroll
+    for (int h = 0; h < P / Q; ++h)
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) {
+            const int m = pow2_out_index<H>(j, h, k2);
+            const double2 z = a[h * Q + k2];
+            double2 o = make_double2(z.x * scale, z.y * scale);
+            if constexpr (ADD) {
+                o.x += ad[h * Q + k2].x;
+                o.y += ad[h * Q + k2].y;
+            }
+            *reinterpret_cast<double2*>(dst + 2 * m) = o;
+        }
+}
+
+// ADD: output rows are contiguous (row R at out + 2H R) and row R also gets
+// row R % add_rows of add.
+template <int H, bool ADD = false, int NR = TL>
 __device__ __forceinline__ void c2r_contig_body(const double2* __restrict__ in, double* __restrict__ out,
                                                 int64_t in_rs, int64_t out_rs, int64_t d2, int64_t in_s1,
                                                 int64_t out_s1, int64_t nrows, double scale,
                                                 const double2* __restrict__ twg /* 2H entries */,
-                                                int64_t R0)
+                                                int64_t R0, const double* __restrict__ add = nullptr,
+                                                int64_t add_rows = 1)
 {
     using Pl = Pow2<H>;
     constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
@@ -219,27 +254,20 @@ __device__ __forceinline__ void c2r_contig_body(const double2* __restrict__ in, 
     }
     __syncthreads();
     pow2_fft<H, true>(a, j, ln, twH);
-    if (valid) {
-        double* dst = out + i1 * out_s1 + i2 * out_rs;
-#pragma unroll
-        for (int h = 0; h < P / Q; ++h)
-#pragma unroll
-            for (int k2 = 0; k2 < Q; ++k2) {
-                const int m = pow2_out_index<H>(j, h, k2);
-                const double2 z = a[h * Q + k2];
-                *reinterpret_cast<double2*>(dst + 2 * m) = make_double2(z.x * scale, z.y * scale);
-            }
-    }
+    if (valid)
+        c2r_store<H, ADD>(a, j, out + i1 * out_s1 + i2 * out_rs, ADD ? add + (R % add_rows) * N : nullptr,
+                          scale);
 }
 
-template <int H>
+template <int H, bool ADD = false>
 __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     c2r_contig_kernel(const double2* __restrict__ in, double* __restrict__ out, int64_t in_rs,
                       int64_t out_rs, int64_t d2, int64_t in_s1, int64_t out_s1, int64_t nrows,
-                      double scale, const double2* __restrict__ twg /* 2H entries */)
+                      double scale, const double2* __restrict__ twg /* 2H entries */,
+                      const double* __restrict__ add, int64_t add_rows)
 {
-    c2r_contig_body<H>(in, out, in_rs, out_rs, d2, in_s1, out_s1, nrows, scale, twg,
-                       (int64_t)blockIdx.x * TL);
+    c2r_contig_body<H, ADD>(in, out, in_rs, out_rs, d2, in_s1, out_s1, nrows, scale, twg,
+                            (int64_t)blockIdx.x * TL, add, add_rows);
 }
 
 // ----------------------------------------------------------------- host dispatch
@@ -363,7 +391,7 @@ int fast_c2c_f32(const Geom& gi, const Geom& go, const void* in, void* out, int 
 
 template <int H, bool R2C>
 static int launch_rc(const Geom& gi, const Geom& go, const void* in, void* out, double scale,
-                     hipStream_t s)
+                     hipStream_t s, const double* add, int64_t add_rows)
 {
     using Pl = Pow2<H>;
     const double2* tw = fft_table(2 * H);
@@ -378,21 +406,26 @@ static int launch_rc(const Geom& gi, const Geom& go, const void* in, void* out, 
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double*)in, (double2*)out, gi.s2, go.s2,
                            gi.d2, gi.s1, go.s1, nr, scale, tw);
     } else {
-        auto k = c2r_contig_kernel<H>;
+        auto k = add ? c2r_contig_kernel<H, true> : c2r_contig_kernel<H, false>;
         if (lds > 64 * 1024)
             hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double2*)in, (double*)out, gi.s2, go.s2,
-                           gi.d2, gi.s1, go.s1, nr, scale, tw);
+                           gi.d2, gi.s1, go.s1, nr, scale, tw, add, add_rows);
     }
     SFA_LAUNCH_CHECK();
     return SFA_OK;
 }
 
-// real length n along a contiguous axis (sa == 1 on both sides)
+// real length n along a contiguous axis (sa == 1 on both sides).  add (C2R
+// only, may be NULL): add_rows contiguous rows of n doubles, row R % add_rows
+// added to output row R; needs contiguous output rows.
 int fast_rc(bool r2c, const Geom& gi, const Geom& go, const void* in, void* out, double scale,
-            hipStream_t s, int* rc)
+            hipStream_t s, int* rc, const double* add, int64_t add_rows)
 {
     if (!(gi.sa == 1 && go.sa == 1)) return 0;
+    if (add && (r2c || add_rows < 1 || go.s2 != gi.n || go.s1 != go.d2 * gi.n ||
+                (reinterpret_cast<uintptr_t>(add) & 15)))
+        return 0;
     // rows must not overlap the packed complex loads: real rows 16-byte aligned
     const int64_t H = gi.n / 2;
     const int64_t real_rs = r2c ? gi.s2 : go.s2;
@@ -400,8 +433,8 @@ int fast_rc(bool r2c, const Geom& gi, const Geom& go, const void* in, void* out,
     if ((real_rs & 1) || (real_s1 & 1)) return 0;
 #define SFA_RC(HH)                                                                            \
     case HH:                                                                                  \
-        *rc = r2c ? launch_rc<HH, true>(gi, go, in, out, scale, s)                            \
-                  : launch_rc<HH, false>(gi, go, in, out, scale, s);                          \
+        *rc = r2c ? launch_rc<HH, true>(gi, go, in, out, scale, s, nullptr, 1)                \
+                  : launch_rc<HH, false>(gi, go, in, out, scale, s, add, add_rows);           \
         return 1;
     switch (H) {
         SFA_RC(16) SFA_RC(32) SFA_RC(64) SFA_RC(128) SFA_RC(256)
@@ -934,10 +967,12 @@ __global__ void __launch_bounds__(TL * Pow2<H>::Q)
 // i < 3) can lie in the complex128 planes; every other load is a plain
 // complex64 load, and all loads are issued before any conversion (a
 // per-element select of a converted complex64 load waited on each load in turn).
-template <int H, bool KSMALL, int NR = TL>
+// ADD: row R also gets row R % add_rows of add.
+template <int H, bool KSMALL, bool ADD = false, int NR = TL>
 __device__ __forceinline__ void c2r_merge_body(const double2* __restrict__ i64, const float2* __restrict__ i32,
                                                double* __restrict__ out, int64_t nrows, int Kh,
-                                               double scale, const double2* __restrict__ twg, int64_t R0)
+                                               double scale, const double2* __restrict__ twg, int64_t R0,
+                                               const double* __restrict__ add = nullptr, int64_t add_rows = 1)
 {
     using Pl = Pow2<H>;
     constexpr int P = Pl::P, Q = Pl::Q, LS = zrow_stride<H>();
@@ -1011,31 +1046,23 @@ __device__ __forceinline__ void c2r_merge_body(const double2* __restrict__ i64, 
     }
     __syncthreads();
     pow2_fft<H, true>(a, j, ln, twH);
-    if (valid) {
-        double* dst = out + R * N;
-#pragma unroll
-        for (int h = 0; h < P / Q; ++h)
-#pragma unroll
-            for (int k2 = 0; k2 < Q; ++k2) {
-                const int m = pow2_out_index<H>(j, h, k2);
-                const double2 z = a[h * Q + k2];
-                *reinterpret_cast<double2*>(dst + 2 * m) = make_double2(z.x * scale, z.y * scale);
-            }
-    }
+    if (valid)
+        c2r_store<H, ADD>(a, j, out + R * N, ADD ? add + (R % add_rows) * N : nullptr, scale);
 }
 
-template <int H, bool KSMALL>
+template <int H, bool KSMALL, bool ADD = false>
 __global__ void __launch_bounds__(TL * Pow2<H>::Q)
     c2r_merge_kernel(const double2* __restrict__ i64, const float2* __restrict__ i32,
                      double* __restrict__ out, int64_t nrows, int Kh, double scale,
-                     const double2* __restrict__ twg)
+                     const double2* __restrict__ twg, const double* __restrict__ add, int64_t add_rows)
 {
-    c2r_merge_body<H, KSMALL>(i64, i32, out, nrows, Kh, scale, twg, (int64_t)blockIdx.x * TL);
+    c2r_merge_body<H, KSMALL, ADD>(i64, i32, out, nrows, Kh, scale, twg, (int64_t)blockIdx.x * TL, add,
+                                   add_rows);
 }
 
 template <int H, bool SPLIT>
 static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int Kh, double scale,
-                        hipStream_t s, double* zero, int nzero)
+                        hipStream_t s, double* zero, int nzero, const double* add, int64_t add_rows)
 {
     using Pl = Pow2<H>;
     const double2* tw = fft_table(2 * H);
@@ -1049,11 +1076,12 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double*)in, (double2*)o64, (float2*)o32,
                            nrows, Kh, scale, tw, zero, nzero);
     } else {
-        auto k = Kh <= 3 * Pl::Q ? c2r_merge_kernel<H, true> : c2r_merge_kernel<H, false>;
+        auto k = Kh <= 3 * Pl::Q ? (add ? c2r_merge_kernel<H, true, true> : c2r_merge_kernel<H, true, false>)
+                                 : (add ? c2r_merge_kernel<H, false, true> : c2r_merge_kernel<H, false, false>);
         if (lds > 64 * 1024)
             hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k, grid, block, lds, s, (const double2*)o64, (const float2*)o32,
-                           (double*)in, nrows, Kh, scale, tw);
+                           (double*)in, nrows, Kh, scale, tw, add, add_rows);
     }
     SFA_LAUNCH_CHECK();
     return SFA_OK;
@@ -1062,12 +1090,14 @@ static int launch_split(const void* in, void* o64, void* o32, int64_t nrows, int
 // split (r2c=true: in -> o64/o32) or merge (r2c=false: o64/o32 -> in) of
 // contiguous rows; the split also clears zero[0, nzero)
 int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
-             double scale, hipStream_t s, int* rc, double* zero, int nzero)
+             double scale, hipStream_t s, int* rc, double* zero, int nzero, const double* add,
+             int64_t add_rows)
 {
-#define SFA_SP(HH)                                                                            \
-    case HH:                                                                                  \
-        *rc = r2c ? launch_split<HH, true>(in, o64, o32, nrows, Kh, scale, s, zero, nzero)    \
-                  : launch_split<HH, false>(in, o64, o32, nrows, Kh, scale, s, nullptr, 0);   \
+    if (add && (r2c || add_rows < 1 || (reinterpret_cast<uintptr_t>(add) & 15))) return 0;
+#define SFA_SP(HH)                                                                                       \
+    case HH:                                                                                             \
+        *rc = r2c ? launch_split<HH, true>(in, o64, o32, nrows, Kh, scale, s, zero, nzero, nullptr, 1)  \
+                  : launch_split<HH, false>(in, o64, o32, nrows, Kh, scale, s, nullptr, 0, add, add_rows); \
         return 1;
     switch (H) {
         SFA_SP(16) SFA_SP(32) SFA_SP(64) SFA_SP(128) SFA_SP(256)

@@ -23,6 +23,10 @@
 #include "fft_device.h"
 #include "helm_scan.h"
 
+int fft_c2r_add(const void* in, double* out, int ndim, const int64_t* real_shape, const int64_t* in_strides,
+                const int64_t* out_strides, int axis, double scale, void* stream, const double* add,
+                int64_t add_rows);
+
 namespace sfa {
 
 struct HelmTables {
@@ -256,7 +260,8 @@ int fast_c2c_mixed(const Geom& g64, void* W64, const Geom& g32, void* W32, int s
                    hipStream_t s, int* rc, const double* stats = nullptr,
                    StatOut out = StatOut{nullptr, nullptr, 0u});
 int split_rc(bool r2c, int H, const void* in, void* o64, void* o32, int64_t nrows, int Kh,
-             double scale, hipStream_t s, int* rc, double* zero = nullptr, int nzero = 0);
+             double scale, hipStream_t s, int* rc, double* zero = nullptr, int nzero = 0,
+             const double* add = nullptr, int64_t add_rows = 1);
 
 // ------------------------------------------------------------------ plan
 // Device plan of the fused solve, the counterpart of chebyshev.la.Helmholtz's
@@ -349,6 +354,7 @@ struct PlanInfo {
     double alpha;
     int neumann;
     int64_t off_s, off_bs;
+    const double* lift;  // added to every solution of this plan (NULL: none)
 };
 static std::mutex g_plan_mu;
 static std::map<const void*, PlanInfo> g_plans;
@@ -381,6 +387,15 @@ static bool plan_view(const void* plan, int64_t Nx, int64_t Ny, int64_t Nz, doub
 static bool plan_ok(const void* plan, int64_t Nx, int64_t Ny, int64_t Nz, double alpha, int neumann)
 {
     return plan_view(plan, Nx, Ny, Nz, alpha, neumann, nullptr);
+}
+
+// The lift registered with a plan (sfa_helmholtz3d_set_lift), NULL if none.
+static const double* plan_lift(const void* plan)
+{
+    if (plan == nullptr) return nullptr;
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(plan);
+    return it == g_plans.end() ? nullptr : it->second.lift;
 }
 
 // W: nb arrays (Nx, Ny, Nzh) of complex lines at kz pitch Kp >= Nzh, back to back.
@@ -517,7 +532,9 @@ static int helmholtz3d_mixed(const double* f, double* u, int64_t Nx, int64_t Ny,
     mark(3);
     if ((rc = y_pass_mixed(g64, g32, W64, W32, +1, 1.0, s, stats, out))) return rc;
     mark(4);
-    if (!split_rc(false, (int)(Nz / 2), u, W64, W32, rows, Kh, 1.0, s, &rc))
+    // the plan's lift (nonhomogeneous boundary values) is added by the C2R's stores
+    const double* lift = plan_lift(plan);
+    if (!split_rc(false, (int)(Nz / 2), u, W64, W32, rows, Kh, 1.0, s, &rc, nullptr, 0, lift, Nx * Ny))
         return fail(SFA_ERR_UNSUPPORTED, "mixed precision: Nz unsupported");
     mark(5);
     return rc;
@@ -564,7 +581,8 @@ static int helmholtz3d_fp64(const double* f, double* u, int64_t Nx, int64_t Ny, 
     mark(3);
     if ((rc = sfa_fft_c2c(W, W, 3, yshape, cstr, cstr, 1, +1, 1.0, st))) return rc;
     mark(4);
-    if ((rc = sfa_fft_c2r(W, u, 3, rshape, cstr, rstr, 2, 1.0, st))) return rc;
+    // the plan's lift (nonhomogeneous boundary values) is added by the C2R
+    if ((rc = fft_c2r_add(W, u, 3, rshape, cstr, rstr, 2, 1.0, st, plan_lift(plan), Nx * Ny))) return rc;
     mark(5);
     return SFA_OK;
 }
@@ -639,8 +657,19 @@ extern "C" int sfa_helmholtz3d_plan(int64_t Nx, int64_t Ny, int64_t Nz, double a
               SFA_ERR_HIP, "table allocation failed");
     {
         std::lock_guard<std::mutex> lk(g_plan_mu);
-        g_plans[plan] = PlanInfo{Nx, Ny, Nz, plan_bytes, alpha, neumann, L.off_s, L.off_bs};
+        g_plans[plan] = PlanInfo{Nx, Ny, Nz, plan_bytes, alpha, neumann, L.off_s, L.off_bs, nullptr};
     }
+    return SFA_OK;
+}
+
+extern "C" int sfa_helmholtz3d_set_lift(const void* plan, const double* lift)
+{
+    SFA_CHECK(lift == nullptr || (reinterpret_cast<uintptr_t>(lift) & 15) == 0, SFA_ERR_ARG,
+              "lift must be 16-byte aligned");
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(plan);
+    SFA_CHECK(it != g_plans.end(), SFA_ERR_ARG, "plan was not built by sfa_helmholtz3d_plan");
+    it->second.lift = lift;
     return SFA_OK;
 }
 

@@ -388,7 +388,7 @@ static int launch_line(int op, bool inv, LineArgs& a, hipStream_t stream)
 int fast_c2c(const Geom& gi, const Geom& go, const void* in, void* out, int sign, double scale,
              hipStream_t s, int* rc);
 int fast_rc(bool r2c, const Geom& gi, const Geom& go, const void* in, void* out, double scale,
-            hipStream_t s, int* rc);
+            hipStream_t s, int* rc, const double* add = nullptr, int64_t add_rows = 1);
 
 static inline bool aligned16(const void* p) { return (((uintptr_t)p) & 15) == 0; }
 
@@ -473,9 +473,24 @@ extern "C" int sfa_fft_r2c(const double* in, void* out, int ndim, const int64_t*
     return launch_line(OP_R2C, false, a, (hipStream_t)stream);
 }
 
+int fft_c2r_add(const void* in, double* out, int ndim, const int64_t* real_shape, const int64_t* in_strides,
+                const int64_t* out_strides, int axis, double scale, void* stream, const double* add,
+                int64_t add_rows);
+
 extern "C" int sfa_fft_c2r(const void* in, double* out, int ndim, const int64_t* real_shape,
                            const int64_t* in_strides, const int64_t* out_strides, int axis,
                            double scale, void* stream)
+{
+    return fft_c2r_add(in, out, ndim, real_shape, in_strides, out_strides, axis, scale, stream, nullptr, 1);
+}
+
+// sfa_fft_c2r that also adds add (add_rows contiguous rows of the transform
+// length, row R % add_rows to output row R, the output rows contiguous) to
+// the result: inside the transform's stores where its fast kernel runs, else
+// by a pass over the output afterwards.
+int fft_c2r_add(const void* in, double* out, int ndim, const int64_t* real_shape, const int64_t* in_strides,
+                const int64_t* out_strides, int axis, double scale, void* stream, const double* add,
+                int64_t add_rows)
 {
     Geom gi, go;
     int rc;
@@ -485,19 +500,27 @@ extern "C" int sfa_fft_c2r(const void* in, double* out, int ndim, const int64_t*
     SFA_CHECK(n >= 1, SFA_ERR_ARG, "C2R length must be >= 1");
     SFA_CHECK(in != nullptr && out != nullptr, SFA_ERR_ARG, "C2R: null array");
     if (n % 2) {
+        SFA_CHECK(add == nullptr, SFA_ERR_UNSUPPORTED, "C2R with add: odd length");
         LineArgs a;
         if ((rc = setup_line(a, in, out, gi, go, n, n, 1, 0))) return rc;
         a.scale = scale;
         return launch_line(OP_C2RF, true, a, (hipStream_t)stream);
     }
-    if (aligned16(in) && aligned16(out) && fast_rc(false, gi, go, in, out, scale, (hipStream_t)stream, &rc))
+    if (aligned16(in) && aligned16(out) &&
+        fast_rc(false, gi, go, in, out, scale, (hipStream_t)stream, &rc, add, add_rows))
         return rc;
     LineArgs a;
     if ((rc = setup_line(a, in, out, gi, go, n, n / 2, 1, 0))) return rc;
     a.tw2 = fft_table(n);
     SFA_CHECK(a.tw2 != nullptr, SFA_ERR_HIP, "twiddle table allocation failed");
     a.scale = scale;
-    return launch_line(OP_C2R, true, a, (hipStream_t)stream);
+    if ((rc = launch_line(OP_C2R, true, a, (hipStream_t)stream))) return rc;
+    if (add) {
+        SFA_CHECK(go.s2 == n && go.s1 == go.d2 * n && (go.d1 * go.d2) % add_rows == 0, SFA_ERR_ARG,
+                  "C2R with add: output rows must be contiguous");
+        return sfa_axpy(add, out, 1.0, add_rows * n, go.d1 * go.d2 / add_rows, stream);
+    }
+    return SFA_OK;
 }
 
 static int cheb_common(int op, const void* in, void* out, int dtype, int ndim, const int64_t* shape,

@@ -262,7 +262,7 @@ def _per_line(s, shape, axis):
 
 def _helm_diags(A, B):
     M = A.shape[0]
-    return [torch.from_numpy(d).cuda() for d in
+    return [torch.from_numpy(np.array(d, dtype=float)).cuda() for d in
             (_diag(A, 0, M), _diag(A, 2, M - 2), _diag(A, 4, M - 4), _diag(B, -2, M - 2),
              _diag(B, 0, M), _diag(B, 2, M - 2))], M
 

@@ -21,7 +21,7 @@ an explicit ``fp64_planes`` and guard=False) runs the unchecked form.
 Nonhomogeneous boundary values on the Chebyshev axis (constant, varying along
 y and z, or time-dependent) give u = u_0 + L: u_0 the fused solve with
 homogeneous values, L the lift computed once by the drop-in chain on f = 0
-(see ``lift``) and added on the device (sfa_axpy).
+(see ``lift``) and added by the solve's final z C2R as it stores u.
 """
 import ctypes
 
@@ -110,10 +110,9 @@ class HelmholtzSolver3D:
         return u
 
     def _run(self, f, u, nb, w):
-        self._solve(f, u, nb, w)
         if self.T.bases[0].has_nonhomogeneous_bcs:
-            L = self.lift()
-            _lib.call("sfa_axpy", _lib.ptr(L), _lib.ptr(u), 1.0, L.numel(), nb, _lib.stream_ptr())
+            self.lift()  # registered with the plan: the solve's last kernel adds it
+        self._solve(f, u, nb, w)
 
     def lift(self):
         """The lift L of the x-basis' boundary values on the grid (FP64):
@@ -124,8 +123,9 @@ class HelmholtzSolver3D:
         the right-hand side (chebyshev/la.py:303-307) and backward adds the
         boundary basis (tensorproductspace.py:2070-2089).  L is that chain on
         f = 0, computed when first needed and again when the values change
-        (time-dependent values after ``bc.update``).  The guard's error
-        prediction covers u_0."""
+        (time-dependent values after ``bc.update``), and registered with the
+        plan (sfa_helmholtz3d_set_lift) so that the solve's final z C2R adds
+        it as it stores u.  The guard's error prediction covers u_0."""
         b0 = self.T.bases[0]
         key = repr(b0.bc_values)
         if self._lift is None or key != self._lift_key:
@@ -136,7 +136,9 @@ class HelmholtzSolver3D:
             zero = torch.zeros(tuple(self.T.shape(True)), dtype=torch.complex128,
                                device=self.T.device)
             L = torch.empty(tuple(self.shape), dtype=torch.float64, device=self.T.device)
-            self._lift = self.T.backward(H(zero, Function(self.T)), L)
+            L = self.T.backward(H(zero, Function(self.T)), L)
+            _lib.call("sfa_helmholtz3d_set_lift", _lib.ptr(self.plan), _lib.ptr(L))
+            self._lift = L
             self._lift_key = key
         return self._lift
 

@@ -195,18 +195,19 @@ def test_bc_time_dependent_helmholtz2d(sfa):
         assert rel(T.forward(dev(ue.copy())), u_hat.cpu().numpy()) < 1e-11, tt
 
 
+@pytest.mark.parametrize("N", [(64, 16, 32), (64, 16, 256)])
 @pytest.mark.parametrize("kind", ["D", "N"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_bc_fused_solver_matches_chain(sfa, kind, precision):
+def test_bc_fused_solver_matches_chain(sfa, kind, precision, N):
     """The fused HelmholtzSolver3D with nonhomogeneous boundary values
-    (u = u_0 + lift) returns the drop-in chain's solution
-    (scalar_product -> la.Helmholtz with its bc matrices -> backward) for a
-    random smooth f, recovers a manufactured solution of the discrete space,
-    and adds the lift to every member of a stack."""
-    print(f"[bc fused {kind} {precision}]", flush=True)
+    (u = u_0 + lift, the lift added by the final z C2R's stores) returns the
+    drop-in chain's solution (scalar_product -> la.Helmholtz with its bc
+    matrices -> backward) for a random smooth f, recovers a manufactured
+    solution of the discrete space, and adds the lift to every member of a
+    stack; clearing the registered lift gives the homogeneous solve back."""
+    print(f"[bc fused {kind} {precision} {N}]", flush=True)
     a, b = 0.6, -0.4
     alpha = 2.0
-    N = (64, 16, 32)
     SD = sfa.FunctionSpace(N[0], "C", bc=_bc(kind, a, b))
     T = sfa.TensorProductSpace(None, (SD, sfa.FunctionSpace(N[1], "F", dtype="D"),
                                       sfa.FunctionSpace(N[2], "F", dtype="d")))
@@ -240,6 +241,13 @@ def test_bc_fused_solver_matches_chain(sfa, kind, precision):
     U = Hf.solve_many(dev(np.stack([f, f_ex * np.ones(T.shape(False))])))
     assert rel(U[0], u_chain) < (1e-12 if precision == "fp64" else 1e-10)
     assert rel(U[1], np.broadcast_to(u_ex, T.shape(False))) < 1e-11
+    # without the lift: u_0 = u - L
+    L = Hf.lift()
+    sfa._lib.call("sfa_helmholtz3d_set_lift", sfa._lib.ptr(Hf.plan), None)
+    u0 = torch.empty_like(L)
+    Hf._solve(dev(f), u0, 1, Hf.work)
+    assert rel(u0 + L, u_chain) < (1e-12 if precision == "fp64" else 1e-10)
+    sfa._lib.call("sfa_helmholtz3d_set_lift", sfa._lib.ptr(Hf.plan), sfa._lib.ptr(L))
 
 
 def test_bc_fused_solver_time_dependent(sfa):
