@@ -274,3 +274,19 @@ def test_bc_fused_solver_time_dependent(sfa):
         lap = -np.sin(tt) * np.cos(Y) * (1 - x) / 2 + q.deriv(2)(x) * cz - 5 * q(x) * cz
         f = np.broadcast_to(alpha * u - lap, T.shape(False)).copy()
         assert rel(Hf(dev(f)), np.broadcast_to(u, T.shape(False))) < 1e-11, tt
+
+
+@pytest.mark.parametrize("n,nrep,offset", [(1000, 3, 0), (777, 2, 0), (64, 5, 1)])
+def test_axpy_stack(sfa, n, nrep, offset):
+    """sfa_axpy: y[r n + i] += a x[i] over a stack, the 16-byte form (even n,
+    aligned) and the scalar form (odd n, or a misaligned view)."""
+    print(f"[axpy {n} {nrep} {offset}]", flush=True)
+    g = torch.Generator().manual_seed(n + nrep)
+    xb = torch.randn(n + offset, dtype=torch.float64, generator=g).cuda()
+    yb = torch.randn(n * nrep + offset, dtype=torch.float64, generator=g).cuda()
+    x, y = xb[offset:], yb[offset:]
+    ref = (y.cpu().view(nrep, n) + 0.75 * x.cpu()[None, :]).reshape(-1)
+    sfa._lib.call("sfa_axpy", sfa._lib.ptr(x), sfa._lib.ptr(y), 0.75, n, nrep, sfa._lib.stream_ptr())
+    torch.cuda.synchronize()
+    # the device may fuse the multiply-add (one rounding): within an ulp
+    assert torch.allclose(y.cpu(), ref, rtol=4e-16, atol=4e-16)
