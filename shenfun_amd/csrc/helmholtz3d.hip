@@ -873,6 +873,7 @@ static int guard_wait(const GuardCtx* g, unsigned seq, hipStream_t s)
     auto next = clk::now() + std::chrono::milliseconds(1);
     for (unsigned it = 1;; ++it) {
         if (g->host->flag == seq) break;
+        __builtin_ia32_pause();  // spin-wait hint: yields the core's pipeline to a sibling thread
         if ((it & 1023u) == 0 && clk::now() >= next) {
             const hipError_t q = hipStreamQuery(s);
             if (q == hipSuccess) {
