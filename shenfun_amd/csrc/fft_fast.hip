@@ -764,6 +764,39 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_kerne
 // Mixed precision: the complex128 planes' blocks [0, nb64) and the complex64
 // planes' blocks after them in one grid, so the few FP64 blocks run beside the
 // FP32 ones instead of in a launch of their own.
+// Block order of the complex64 part of the dual x-pass: where a ky row is a
+// whole number of blocks (bpk) and the launch holds one array, dispatch slot
+// d runs the block of ky and slot d + 8 the block of Ny - ky with the same kz
+// range, 8 such pairs per 16 slots.  Workgroups go to the 8 XCDs round-robin,
+// so the two land on one XCD at about the same time, and their lines, which
+// share beta = alpha + ky^2 + kz^2, read one set of plan restart states
+// through that XCD's L2 (mixed x-pass at 256^3 76.3 -> 75.5 us, the same
+// bits).  ky = 0 and Ny/2, their own partners, come last.  A permutation of
+// [0, Ny bpk) (checked exhaustively on the host for Ny <= 512, bpk <= 16).
+__device__ __forceinline__ int64_t xpass_pair_order(const XPass2Args& a, int64_t d)
+{
+    if (a.Kp % TL != 0 || a.nl != a.nlb) return d;
+    const int64_t bpk = a.Kp / TL, Ny = a.Ny;
+    const int64_t E = (Ny / 2 - 1) * bpk;  // blocks of ky in [1, Ny/2)
+    if (d >= 2 * E) {
+        const int64_t r = d - 2 * E;
+        return r < bpk ? r : (Ny / 2) * bpk + (r - bpk);
+    }
+    const int64_t G = E / 8, rem = E % 8;
+    int64_t e;
+    bool upper;
+    if (d < 16 * G) {
+        e = 8 * (d / 16) + d % 8;
+        upper = (d % 16) >= 8;
+    } else {  // the last, partial group: rem blocks, then their partners
+        const int64_t r2 = d - 16 * G;
+        upper = r2 >= rem;
+        e = 8 * G + (upper ? r2 - rem : r2);
+    }
+    const int64_t ky = 1 + e / bpk, c = e % bpk;
+    return (upper ? Ny - ky : ky) * bpk + c;
+}
+
 template <int N, bool STATS, bool NM>
 __global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_dual_kernel(XPass2Args a64, XPass2Args a32,
                                                                         int64_t nb64)
@@ -772,7 +805,7 @@ __global__ void __launch_bounds__(TL * Pow2<N>::Q, xpass_minb<N>()) xpass2_dual_
     if (b < nb64)
         xpass2_body<N, double2, STATS, NM>(a64, b);
     else
-        xpass2_body<N, float2, STATS, NM>(a32, b - nb64);
+        xpass2_body<N, float2, STATS, NM>(a32, xpass_pair_order(a32, b - nb64));
 }
 
 template <int N>
